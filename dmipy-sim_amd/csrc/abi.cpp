@@ -1,0 +1,533 @@
+// abi.cpp -- host side of the C ABI declared in include/dmipy_walk.h.
+//
+// Uploads a walk (waveform, geometry tables, optional start positions and
+// injected increments), launches the walk kernel over walker tiles and
+// measurement blocks, reduces the per-wave partial sums in float64 and copies
+// the requested outputs back.  One HIP stream per calling thread; no state is
+// kept between calls except inside a dmw_plan.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dmipy_walk.h"
+#include "walk_kernel.h"
+#include "walk_launch.h"
+#include "batch.h"
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int g_device = 0;
+thread_local hipStream_t g_stream = nullptr;
+thread_local int g_stream_device = -1;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+    m += " (out of memory)";
+    return fail(DMW_ENOMEM, m);
+  }
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) return fail(DMW_ENODEV, m);
+  return fail(DMW_EHIP, m);
+}
+
+#define HIP_TRY(expr, what)                        \
+  do {                                             \
+    hipError_t e_ = (expr);                        \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+int ensure_stream(hipStream_t* out) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(DMW_ENODEV, "no HIP device visible");
+  if (g_device < 0 || g_device >= n) return fail(DMW_ENODEV, "device index out of range");
+  HIP_TRY(hipSetDevice(g_device), "hipSetDevice");
+  if (g_stream == nullptr || g_stream_device != g_device) {
+    HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
+    g_stream_device = g_device;
+  }
+  *out = g_stream;
+  return DMW_OK;
+}
+
+// Device allocation owned by a plan.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t n) {
+    if (n <= bytes && p) return DMW_OK;
+    release();
+    if (n == 0) return DMW_OK;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess) {
+      p = nullptr;
+      return hip_fail(e, "hipMalloc");
+    }
+    bytes = n;
+    return DMW_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+int upload(DevBuf& b, const void* src, size_t n, hipStream_t st) {
+  int rc = b.alloc(n);
+  if (rc) return rc;
+  if (n) HIP_TRY(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+  return DMW_OK;
+}
+
+// Periodic neighbour grid of a packed substrate: every cell lists, in ascending
+// object order, the objects whose disc/ball inflated by the largest single-step
+// reach intersects the cell.
+struct Grid {
+  int n = 1;
+  float h = 0.0f;
+  std::vector<int> start, items;
+  bool exact_minimage = false;
+};
+
+Grid build_grid(const dmw_geometry& g, float step_len, int dim) {
+  Grid G;
+  const int n_obj = g.n_obj;
+  const double L = g.cell;
+  double rmean = 0.0, rmax = 0.0;
+  for (int k = 0; k < n_obj; ++k) {
+    rmean += g.radii[k];
+    rmax = std::max(rmax, (double)g.radii[k]);
+  }
+  rmean /= std::max(1, n_obj);
+  const int nmax = dim == 2 ? 64 : 24;
+  int n = (int)std::lround(L / std::max(0.8 * rmean, 1e-12));
+  n = std::max(1, std::min(nmax, n));
+  for (;;) {
+    const double h = L / n;
+    const double tol = 1e-3 * h + 1e-12;
+    const double inflate = 1.001 * step_len + 4.0 * g.nudge + tol;
+    const int n_cells = dim == 2 ? n * n : n * n * n;
+    std::vector<std::vector<int>> lists(n_cells);
+    for (int k = 0; k < n_obj; ++k) {
+      double c[3] = {0, 0, 0};
+      for (int d = 0; d < dim; ++d) {
+        c[d] = g.centers[k * dim + d];
+        c[d] -= L * std::floor(c[d] / L + 0.5);
+      }
+      const double reach = g.radii[k] + inflate;
+      for (int cz = 0; cz < (dim == 3 ? n : 1); ++cz)
+        for (int cy = 0; cy < n; ++cy)
+          for (int cx = 0; cx < n; ++cx) {
+            const int ci[3] = {cx, cy, cz};
+            bool touch = false;
+            for (int sz = (dim == 3 ? -1 : 0); sz <= (dim == 3 ? 1 : 0) && !touch; ++sz)
+              for (int sy = -1; sy <= 1 && !touch; ++sy)
+                for (int sx = -1; sx <= 1 && !touch; ++sx) {
+                  const int sh[3] = {sx, sy, sz};
+                  double d2 = 0.0;
+                  for (int d = 0; d < dim; ++d) {
+                    const double lo = -0.5 * L + ci[d] * h - tol, hi = lo + h + 2 * tol;
+                    const double cc = c[d] + sh[d] * L;
+                    const double e = cc < lo ? lo - cc : (cc > hi ? cc - hi : 0.0);
+                    d2 += e * e;
+                  }
+                  touch = d2 <= reach * reach;
+                }
+            if (touch) lists[(cz * n + cy) * n + cx].push_back(k);
+          }
+    }
+    G.n = n;
+    G.h = (float)h;
+    G.start.assign(n_cells + 1, 0);
+    G.items.clear();
+    for (int i = 0; i < n_cells; ++i) {
+      G.start[i] = (int)G.items.size();
+      G.items.insert(G.items.end(), lists[i].begin(), lists[i].end());
+    }
+    G.start[n_cells] = (int)G.items.size();
+    const size_t lds = (size_t)n_obj * 16 + (G.start.size() + G.items.size()) * 4;
+    G.exact_minimage = (rmax + inflate + h * std::sqrt((double)dim) + tol) >= 0.45 * L;
+    if (lds <= 60 * 1024 || n == 1) return G;
+    n = std::max(1, n * 3 / 4);
+  }
+}
+
+}  // namespace
+
+namespace dmw {
+int batch_fail(int code, const char* msg) { return fail(code, msg); }
+int batch_hip_fail(hipError_t e, const char* what) { return hip_fail(e, what); }
+}  // namespace dmw
+
+// ---------------------------------------------------------------------------
+struct dmw_plan {
+  dmw_geometry geom{};
+  dmw_run run{};
+  int n_meas = 0, n_t = 0, n_echo = 0;
+  int mb = 4, n_chunks = 1, n_meas_pad = 4;
+  bool kahan = false;
+  long long n_waves = 0;
+  unsigned grid_x = 0;
+  size_t lds = 0;
+  dmw::WalkArgs args{};
+  DevBuf Gs, chi, echo, objs, cstart, citems, r0, inj_n, inj_u;
+  DevBuf partials, echo_partials, sig;
+  DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~dmw_plan() {
+    for (DevBuf* b : {&Gs, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &partials,
+                      &echo_partials, &sig, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
+                      &o_logw, &o_phi, &o_counters})
+      b->release();
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+extern "C" {
+
+int dmw_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int dmw_set_device(int device) {
+  g_device = device;
+  return DMW_OK;
+}
+
+const char* dmw_last_error(void) { return g_err.c_str(); }
+
+static int pick_meas_block(int n_meas, int requested, bool kahan) {
+  if (requested > 0) {
+    for (int mb : dmw::kMeasBlocks)
+      if (mb == requested) return mb;
+    return -1;
+  }
+  if (n_meas <= 4) return 4;
+  if (n_meas <= 16) return 16;
+  if (n_meas <= 32) return 32;
+  if (n_meas <= 64) return 64;
+  return kahan ? 64 : 128;
+}
+
+int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform* wave,
+                    const dmw_run* run) {
+  if (!out || !geom || !wave || !run) return fail(DMW_EINVAL, "null argument");
+  *out = nullptr;
+  if (run->n_walkers < 1 || run->n_walkers > (1LL << 31)) return fail(DMW_EINVAL, "n_walkers must be in [1, 2^31]");
+  if (wave->n_meas < 1 || wave->n_t < 1 || !wave->G) return fail(DMW_EINVAL, "empty waveform");
+  if (run->n_sub < 1) return fail(DMW_EINVAL, "n_sub must be >= 1");
+  if ((long long)wave->n_t * run->n_sub >= (1LL << 32)) return fail(DMW_EINVAL, "n_t * n_sub too large");
+  if (geom->kind < DMW_FREE || geom->kind > DMW_PACKED_SPHERES) return fail(DMW_EINVAL, "unknown geometry kind");
+  if (geom->wall < DMW_WALL_REFLECT || geom->wall > DMW_WALL_PERMEATE) return fail(DMW_EINVAL, "unknown wall mode");
+  if (geom->kind == DMW_FREE && geom->wall != DMW_WALL_REFLECT)
+    return fail(DMW_EUNSUPPORTED, "FreeDiffusion has no wall");
+  if (geom->kind == DMW_BOX1D && geom->wall == DMW_WALL_PERMEATE)
+    return fail(DMW_EUNSUPPORTED, "Box1D has no permeable wall");
+  const bool packed = geom->kind == DMW_PACKED_CYLINDERS || geom->kind == DMW_PACKED_SPHERES;
+  if (packed && (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f)))
+    return fail(DMW_EINVAL, "packed substrate needs objects and a positive cell side");
+  if (wave->n_echo < 0 || (wave->n_echo > 0 && !wave->echo_idx)) return fail(DMW_EINVAL, "bad echo list");
+  for (int e = 0; e < wave->n_echo; ++e) {
+    const int v = wave->echo_idx[e];
+    if (v < 0 || v >= wave->n_t || (e > 0 && v <= wave->echo_idx[e - 1]))
+      return fail(DMW_EINVAL, "echo indices must be ascending and inside [0, n_t)");
+  }
+  const long long N = run->n_walkers;
+  const long long S = (long long)wave->n_t * run->n_sub;
+  if (run->inj_normals && geom->wall == DMW_WALL_PERMEATE && !run->inj_uniforms)
+    return fail(DMW_EINVAL, "injected walk on a permeable wall needs injected uniforms");
+
+  hipStream_t st;
+  int rc = ensure_stream(&st);
+  if (rc) return rc;
+
+  dmw_plan* P = new dmw_plan();
+  P->geom = *geom;
+  P->run = *run;
+  P->n_meas = wave->n_meas;
+  P->n_t = wave->n_t;
+  P->n_echo = wave->n_echo;
+  P->kahan = run->kahan != 0;
+  P->mb = pick_meas_block(wave->n_meas, run->meas_block, P->kahan);
+  if (P->mb < 0) {
+    delete P;
+    return fail(DMW_EINVAL, "meas_block must be 0, 4, 16, 32, 64 or 128");
+  }
+  P->n_chunks = (wave->n_meas + P->mb - 1) / P->mb;
+  P->n_meas_pad = P->n_chunks * P->mb;
+  P->grid_x = (unsigned)((N + dmw::kBlock - 1) / dmw::kBlock);
+  P->n_waves = (long long)P->grid_x * (dmw::kBlock / dmw::kWave);
+
+#define TRY(x)        \
+  do {                \
+    int r_ = (x);     \
+    if (r_) {         \
+      delete P;       \
+      return r_;      \
+    }                 \
+  } while (0)
+
+  // Waveform: (n_meas, n_t, 3) -> (n_t, n_meas_pad, 3) scaled by gamma*dt_sub.
+  {
+    std::vector<float> gs((size_t)wave->n_t * P->n_meas_pad * 3, 0.0f);
+    const double gdt = run->gamma_dt_sub;
+    for (int m = 0; m < wave->n_meas; ++m)
+      for (int t = 0; t < wave->n_t; ++t)
+        for (int c = 0; c < 3; ++c)
+          gs[((size_t)t * P->n_meas_pad + m) * 3 + c] =
+              (float)(gdt * (double)wave->G[((size_t)m * wave->n_t + t) * 3 + c]);
+    TRY(upload(P->Gs, gs.data(), gs.size() * 4, st));
+  }
+  if (wave->chi) TRY(upload(P->chi, wave->chi, (size_t)wave->n_t * 4, st));
+  if (wave->n_echo > 0) TRY(upload(P->echo, wave->echo_idx, (size_t)wave->n_echo * 4, st));
+
+  dmw::WalkArgs& a = P->args;
+  a.size = geom->size;
+  a.cell = geom->cell;
+  a.inv_cell = geom->cell > 0.0f ? 1.0f / geom->cell : 0.0f;
+  a.eps = geom->eps;
+  a.nudge = geom->nudge;
+  a.rhoD = geom->rho_over_D;
+  a.kappaD = geom->kappa_over_D;
+  a.n_obj = 0;
+  if (packed) {
+    const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
+    Grid G = build_grid(*geom, run->step_len, dim);
+    std::vector<float> objs((size_t)geom->n_obj * 4, 0.0f);
+    for (int k = 0; k < geom->n_obj; ++k) {
+      for (int d = 0; d < dim; ++d) objs[k * 4 + d] = geom->centers[k * dim + d];
+      objs[k * 4 + 3] = geom->radii[k];
+    }
+    TRY(upload(P->objs, objs.data(), objs.size() * 4, st));
+    TRY(upload(P->cstart, G.start.data(), G.start.size() * 4, st));
+    TRY(upload(P->citems, G.items.data(), std::max<size_t>(1, G.items.size()) * 4, st));
+    a.n_obj = geom->n_obj;
+    a.grid_n = G.n;
+    a.grid_h = G.h;
+    a.grid_inv_h = 1.0f / G.h;
+    a.n_items = (int)G.items.size();
+    a.exact_minimage = G.exact_minimage ? 1 : 0;
+    P->lds = (size_t)geom->n_obj * 16 + (G.start.size() + G.items.size()) * 4;
+  }
+  a.objs = P->objs.as<float4>();
+  a.cell_start = P->cstart.as<int>();
+  a.cell_items = P->citems.as<int>();
+  a.n_sub = run->n_sub;
+  a.step_len = run->step_len;
+  a.dt_sub = run->dt_sub;
+  a.inv_T2 = run->inv_T2;
+  a.inv_T1 = run->inv_T1;
+  a.has_t2 = run->has_t2;
+  a.has_t1 = run->has_t1;
+  a.n_t = wave->n_t;
+  a.n_meas = wave->n_meas;
+  a.n_meas_pad = P->n_meas_pad;
+  a.Gs = P->Gs.as<float>();
+  a.chi = wave->chi ? P->chi.as<float>() : nullptr;
+  a.n_echo = wave->n_echo;
+  a.echo_idx = wave->n_echo > 0 ? P->echo.as<int>() : nullptr;
+  a.n_walkers = N;
+  a.instance = 0;
+
+  if (run->r0) {
+    std::vector<float> soa((size_t)N * 3);
+    for (long long w = 0; w < N; ++w)
+      for (int c = 0; c < 3; ++c) soa[(size_t)c * N + w] = run->r0[w * 3 + c];
+    TRY(upload(P->r0, soa.data(), soa.size() * 4, st));
+  }
+  a.r0 = run->r0 ? P->r0.as<float>() : nullptr;
+  if (run->inj_normals) {
+    std::vector<float> t((size_t)S * 3 * N);
+    for (long long w = 0; w < N; ++w)
+      for (long long s = 0; s < S; ++s)
+        for (int c = 0; c < 3; ++c) t[((size_t)s * 3 + c) * N + w] = run->inj_normals[((size_t)w * S + s) * 3 + c];
+    TRY(upload(P->inj_n, t.data(), t.size() * 4, st));
+    if (run->inj_uniforms) {
+      std::vector<float> u((size_t)S * N);
+      for (long long w = 0; w < N; ++w)
+        for (long long s = 0; s < S; ++s) u[(size_t)s * N + w] = run->inj_uniforms[(size_t)w * S + s];
+      TRY(upload(P->inj_u, u.data(), u.size() * 4, st));
+    }
+  }
+  a.inj_n = run->inj_normals ? P->inj_n.as<float>() : nullptr;
+  a.inj_u = (run->inj_normals && run->inj_uniforms) ? P->inj_u.as<float>() : nullptr;
+
+  TRY(P->partials.alloc((size_t)P->n_waves * P->n_meas_pad * 2 * sizeof(double)));
+  if (wave->n_echo > 0)
+    TRY(P->echo_partials.alloc((size_t)P->n_waves * wave->n_echo * P->n_meas_pad * 2 * sizeof(double)));
+  TRY(P->sig.alloc((size_t)std::max(1, wave->n_echo) * P->n_meas_pad * 2 * sizeof(double)));
+  a.partials = P->partials.as<double>();
+  a.echo_partials = wave->n_echo > 0 ? P->echo_partials.as<double>() : nullptr;
+  hipError_t e = hipEventCreate(&P->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&P->ev1);
+  if (e != hipSuccess) {
+    delete P;
+    return hip_fail(e, "hipEventCreate");
+  }
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    delete P;
+    return hip_fail(e, "upload");
+  }
+#undef TRY
+  *out = P;
+  return DMW_OK;
+}
+
+static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid, size_t lds,
+                              hipStream_t st, const dmw::WalkArgs& a) {
+  switch (kind) {
+    case DMW_FREE: return dmw::launch_walk_free(wall, mb, kahan, grid, lds, st, a);
+    case DMW_BOX1D: return dmw::launch_walk_box1d(wall, mb, kahan, grid, lds, st, a);
+    case DMW_CYLINDER: return dmw::launch_walk_cyl(wall, mb, kahan, grid, lds, st, a);
+    case DMW_SPHERE: return dmw::launch_walk_sph(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PACKED_CYLINDERS: return dmw::launch_walk_pcyl(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PACKED_SPHERES: return dmw::launch_walk_psph(wall, mb, kahan, grid, lds, st, a);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
+  if (!P || !out || !out->signal) return fail(DMW_EINVAL, "null argument");
+  hipStream_t st;
+  int rc = ensure_stream(&st);
+  if (rc) return rc;
+  const long long N = P->run.n_walkers;
+  dmw::WalkArgs a = P->args;
+  a.seed_lo = (uint32_t)(seed & 0xFFFFFFFFu);
+  a.seed_hi = (uint32_t)(seed >> 32);
+
+#define OUTBUF(field, buf, bytes)                   \
+  if (out->field) {                                 \
+    rc = P->buf.alloc(bytes);                       \
+    if (rc) return rc;                              \
+  }
+  OUTBUF(final_pos, o_pos, (size_t)N * 3 * 4);
+  OUTBUF(comp_origin, o_c0, (size_t)N * 4);
+  OUTBUF(comp_final, o_cf, (size_t)N * 4);
+  OUTBUF(comp_full, o_cfull, (size_t)N * P->n_t * 4);
+  OUTBUF(pos_full, o_pfull, (size_t)N * P->n_t * 3 * 4);
+  OUTBUF(n_cross, o_ncross, (size_t)N * 4);
+  OUTBUF(log_w, o_logw, (size_t)N * 4);
+  OUTBUF(phi, o_phi, (size_t)N * P->n_meas_pad * 4);
+#undef OUTBUF
+  a.final_pos = out->final_pos ? P->o_pos.as<float>() : nullptr;
+  a.comp_origin = out->comp_origin ? P->o_c0.as<int>() : nullptr;
+  a.comp_final = out->comp_final ? P->o_cf.as<int>() : nullptr;
+  a.comp_full = out->comp_full ? P->o_cfull.as<int>() : nullptr;
+  a.pos_full = out->pos_full ? P->o_pfull.as<float>() : nullptr;
+  a.n_cross = out->n_cross ? P->o_ncross.as<uint32_t>() : nullptr;
+  a.log_w = out->log_w ? P->o_logw.as<float>() : nullptr;
+  a.phi = out->phi ? P->o_phi.as<float>() : nullptr;
+  a.counters = nullptr;
+  if (out->want_counters) {
+    rc = P->o_counters.alloc(16);
+    if (rc) return rc;
+    HIP_TRY(hipMemsetAsync(P->o_counters.p, 0, 16, st), "hipMemsetAsync");
+    a.counters = P->o_counters.as<unsigned long long>();
+  }
+
+  const dim3 grid(P->grid_x, P->n_chunks);
+  HIP_TRY(hipEventRecord(P->ev0, st), "hipEventRecord");
+  HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a), "walk kernel launch");
+  HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
+  // CPMG returns the echo rows; a plain walk returns the signal after the last sample.
+  const double inv_n = 1.0 / (double)N;
+  double* sig = P->sig.as<double>();
+  if (P->n_echo > 0) {
+    HIP_TRY(dmw::launch_reduce_partials(P->echo_partials.as<double>(), P->n_waves,
+                                        P->n_echo * P->n_meas_pad * 2, inv_n, sig, st),
+            "reduce launch");
+  } else {
+    HIP_TRY(dmw::launch_reduce_partials(P->partials.as<double>(), P->n_waves, P->n_meas_pad * 2, inv_n,
+                                        sig, st),
+            "reduce launch");
+  }
+  const int rows = std::max(1, P->n_echo);
+  std::vector<double> hs((size_t)rows * P->n_meas_pad * 2);
+  HIP_TRY(hipMemcpyAsync(hs.data(), sig, hs.size() * 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
+  HIP_TRY(hipStreamSynchronize(st), "walk");
+  float ms = 0.0f;
+  HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
+  out->kernel_ms = ms;
+  for (int r = 0; r < rows; ++r)
+    for (int m = 0; m < P->n_meas; ++m) {
+      out->signal[(size_t)r * P->n_meas + m] = hs[((size_t)r * P->n_meas_pad + m) * 2];
+      if (out->signal_im) out->signal_im[(size_t)r * P->n_meas + m] = hs[((size_t)r * P->n_meas_pad + m) * 2 + 1];
+    }
+
+  auto d2h = [&](void* dst, const DevBuf& b, size_t bytes) -> int {
+    HIP_TRY(hipMemcpy(dst, b.p, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+    return DMW_OK;
+  };
+  if (out->final_pos) {
+    std::vector<float> t((size_t)N * 3);
+    if ((rc = d2h(t.data(), P->o_pos, t.size() * 4))) return rc;
+    for (long long w = 0; w < N; ++w)
+      for (int c = 0; c < 3; ++c) out->final_pos[w * 3 + c] = t[(size_t)c * N + w];
+  }
+  if (out->comp_origin && (rc = d2h(out->comp_origin, P->o_c0, (size_t)N * 4))) return rc;
+  if (out->comp_final && (rc = d2h(out->comp_final, P->o_cf, (size_t)N * 4))) return rc;
+  if (out->n_cross && (rc = d2h(out->n_cross, P->o_ncross, (size_t)N * 4))) return rc;
+  if (out->log_w && (rc = d2h(out->log_w, P->o_logw, (size_t)N * 4))) return rc;
+  if (out->comp_full) {
+    std::vector<int> t((size_t)N * P->n_t);
+    if ((rc = d2h(t.data(), P->o_cfull, t.size() * 4))) return rc;
+    for (int s = 0; s < P->n_t; ++s)
+      for (long long w = 0; w < N; ++w) out->comp_full[(size_t)w * P->n_t + s] = t[(size_t)s * N + w];
+  }
+  if (out->pos_full) {
+    std::vector<float> t((size_t)N * P->n_t * 3);
+    if ((rc = d2h(t.data(), P->o_pfull, t.size() * 4))) return rc;
+    for (int s = 0; s < P->n_t; ++s)
+      for (int c = 0; c < 3; ++c)
+        for (long long w = 0; w < N; ++w)
+          out->pos_full[((size_t)w * P->n_t + s) * 3 + c] = t[((size_t)s * 3 + c) * N + w];
+  }
+  if (out->phi) {
+    std::vector<float> t((size_t)N * P->n_meas_pad);
+    if ((rc = d2h(t.data(), P->o_phi, t.size() * 4))) return rc;
+    for (int m = 0; m < P->n_meas; ++m)
+      for (long long w = 0; w < N; ++w) out->phi[(size_t)w * P->n_meas + m] = t[(size_t)m * N + w];
+  }
+  if (out->want_counters) {
+    unsigned long long c[2];
+    if ((rc = d2h(c, P->o_counters, 16))) return rc;
+    out->n_hits = c[0];
+    out->n_cand = c[1];
+  }
+  return DMW_OK;
+}
+
+int dmw_plan_destroy(dmw_plan* P) {
+  delete P;
+  return DMW_OK;
+}
+
+int dmw_walk(const dmw_geometry* geom, const dmw_waveform* wave, const dmw_run* run, dmw_result* out) {
+  if (!run || !out) return fail(DMW_EINVAL, "null argument");
+  dmw_plan* P = nullptr;
+  int rc = dmw_plan_create(&P, geom, wave, run);
+  if (rc) return rc;
+  rc = dmw_plan_execute(P, run->seed, out);
+  dmw_plan_destroy(P);
+  return rc;
+}
+
+}  // extern "C"

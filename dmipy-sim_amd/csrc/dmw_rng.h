@@ -1,0 +1,99 @@
+// dmw_rng.h -- counter-based random numbers and the isotropic step direction.
+//
+// The reference draws, per sub-step, a JAX threefry key split and a normal(3)
+// triple, and normalises it to a unit vector (dmipy_sim/physics.py:298-301); the
+// only observable is an isotropic direction.  Here every walker owns a Philox
+// stream (no per-walker key state in memory): the counter is
+// (walker, instance, call, tag) and the key is the 64-bit seed.
+//
+// Philox4x32-10 follows Salmon, Moraes, Dror & Shaw, "Parallel random numbers:
+// as easy as 1, 2, 3", SC'11: ten rounds of two 32x32->64 multiplies with the
+// round multipliers 0xD2511F53 / 0xCD9E8D57 and the Weyl key increments
+// 0x9E3779B9 / 0xBB67AE85.
+//
+// The direction is sampled with Archimedes' theorem (z uniform on [-1, 1],
+// azimuth uniform) using only IEEE-exact operations (multiply, add, sqrt) and a
+// Taylor polynomial for sin/cos(pi*f) on |f| <= 1/4, evaluated with explicit
+// fused multiply-adds, so a CPU build and the GPU build produce identical bits.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define DMW_HD __host__ __device__ __forceinline__
+#else
+#define DMW_HD static inline
+#endif
+
+namespace dmw {
+
+struct u32x4 { uint32_t x, y, z, w; };
+
+DMW_HD uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t* hi) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  return (uint32_t)p;
+}
+
+// Philox4x32-10 block: 4 x 32 random bits for counter c under key (k0, k1).
+DMW_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, hi1;
+    uint32_t lo0 = mulhilo32(0xD2511F53u, c.x, &hi0);
+    uint32_t lo1 = mulhilo32(0xCD9E8D57u, c.z, &hi1);
+    u32x4 n;
+    n.x = hi1 ^ c.y ^ k0;
+    n.y = lo1;
+    n.z = hi0 ^ c.w ^ k1;
+    n.w = lo0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Uniform float in [0, 1) on a 2^-24 grid (exact conversion).
+DMW_HD float u01(uint32_t b) { return (float)(b >> 8) * 0x1p-24f; }
+
+// Counter tags separating the independent streams of one walker.
+enum : uint32_t { TAG_STEP = 0u, TAG_INIT = 1u, TAG_POWLES = 2u };
+
+// sin(pi f), cos(pi f) for |f| <= 1/4 from the Taylor series of sin and cos
+// (coefficients (-1)^k pi^(2k+1)/(2k+1)! and (-1)^k pi^(2k)/(2k)!, rounded to
+// float32); truncation error < 2e-9.
+DMW_HD void sincospi_quarter(float f, float* s, float* c) {
+  const float x = f * f;
+  float ps = fmaf(x, 0x1.507834p-4f, -0x1.32d2ccp-1f);
+  ps = fmaf(x, ps, 0x1.466bc6p+1f);
+  ps = fmaf(x, ps, -0x1.4abbcep+2f);
+  ps = fmaf(x, ps, 0x1.921fb6p+1f);
+  *s = f * ps;
+  float pc = fmaf(x, -0x1.a6d1f2p-6f, 0x1.e1f506p-3f);
+  pc = fmaf(x, pc, -0x1.55d3c8p+0f);
+  pc = fmaf(x, pc, 0x1.03c1f0p+2f);
+  pc = fmaf(x, pc, -0x1.3bd3ccp+2f);
+  *c = fmaf(x, pc, 1.0f);
+}
+
+// Unit vector uniformly distributed on the sphere from two 32-bit words.
+DMW_HD void iso_direction(uint32_t a, uint32_t b, float* ux, float* uy, float* uz) {
+  const float z = 1.0f - 2.0f * u01(a);           // exact, in (-1, 1]
+  const float s = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+  const float t = 2.0f * u01(b);                  // azimuth / pi in [0, 2)
+  const float q = rintf(2.0f * t);                // quadrant 0..4
+  const float f = t - 0.5f * q;                   // exact, |f| <= 1/4
+  float sf, cf;
+  sincospi_quarter(f, &sf, &cf);
+  const int qi = ((int)q) & 3;
+  float c = cf, sn = sf;
+  if (qi == 1) { c = -sf; sn = cf; }
+  else if (qi == 2) { c = -cf; sn = -sf; }
+  else if (qi == 3) { c = sf; sn = -cf; }
+  *ux = s * c;
+  *uy = s * sn;
+  *uz = z;
+}
+
+}  // namespace dmw

@@ -1,0 +1,819 @@
+// walk_kernel.h -- the wavefront-resident Monte-Carlo walk for gfx950.
+//
+// One lane = one walker for the whole walk: position, phase, compensation,
+// log-weight, compartment and crossing count live in VGPRs across all n_t
+// waveform samples and n_sub sub-steps; nothing per step touches HBM.  The
+// gradient G(t) is wave-uniform and is read with scalar loads; packed-substrate
+// object tables and their periodic neighbour grid are staged in LDS.
+//
+// Semantics follow the fused engine of the reference:
+//   sub-step       physics.py:296-314 (surface 337-355, plain 371-387, 402-412)
+//   FreeDiffusion  geometries.py:40-41
+//   Box1D          geometries.py:93-139
+//   Sphere         geometries.py:197-317 (reflect*), 320-424 (permeate)
+//   Cylinder       geometries.py:493-654 (reflect*), 656-801 (permeate)
+//   PackedCyl.     geometries.py:1746-2016 (reflect*), 2018-2183 (permeate),
+//                  2185-2214 (classify)
+//   PackedSph.     geometries.py:2445-2631, 2633-2756, 2758-2771
+//   ensemble       core.py:426-429, stimulated echo 734-735 (host)
+// with these deliberate differences, none of which changes a position or a wall
+// decision:
+//   * the bounce loop stops at the first iteration without a hit (the
+//     reference's fixed 10 iterations are no-ops after that);
+//   * packed substrates test only the objects listed in the walker's periodic
+//     grid cell (the others cannot be reached within one step; first-index
+//     argmin order is preserved by the ascending cell lists);
+//   * packed positions are an integer cell index plus a float32 offset, folded
+//     once per waveform sample;
+//   * the phase is advanced once per waveform sample with the sum of the n_sub
+//     sub-step positions (G is constant across the sub-steps of a sample,
+//     physics.py:287-289), optionally Kahan-compensated;
+//   * the ensemble sum is float64, reduced in a fixed order.
+// All float32 wall arithmetic is written operation by operation in the
+// reference's order and compiled without contraction.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dmw_rng.h"
+
+#pragma clang fp contract(off)
+
+namespace dmw {
+
+enum GeomKind { G_FREE = 0, G_BOX1D = 1, G_CYL = 2, G_SPH = 3, G_PCYL = 4, G_PSPH = 5 };
+enum WallKind { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
+
+typedef __attribute__((address_space(4))) const float cfloat;
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+struct WalkArgs {
+  // geometry
+  float size, cell, inv_cell, eps, nudge, rhoD, kappaD;
+  int n_obj;
+  const float4* objs;        // device (cx, cy, cz, R), native frame
+  int grid_n;                // packed: grid cells per axis
+  float grid_h, grid_inv_h;  // packed: grid cell side
+  const int* cell_start;     // device (grid_n^d + 1)
+  const int* cell_items;     // device
+  int n_items;
+  int exact_minimage;        // 1: min-image folds by true division
+  // step
+  int n_sub;
+  float step_len, dt_sub, inv_T2, inv_T1;
+  int has_t2, has_t1;
+  // waveform
+  int n_t, n_meas, n_meas_pad;
+  const float* Gs;           // device (n_t, n_meas_pad, 3), scaled by gamma*dt_sub
+  const float* chi;          // device (n_t) or null
+  int n_echo;
+  const int* echo_idx;       // device (n_echo)
+  // walkers
+  long long n_walkers;
+  uint32_t seed_lo, seed_hi, instance;
+  const float* r0;           // device (3, N) or null
+  const float* inj_n;        // device (S, 3, N) or null, S = n_t * n_sub
+  const float* inj_u;        // device (S, N) or null
+  // outputs
+  double* partials;          // (n_waves, n_meas_pad, 2)
+  double* echo_partials;     // (n_waves, n_echo, n_meas_pad, 2)
+  float* final_pos;          // (3, N)
+  int* comp_origin;          // (N)
+  int* comp_final;           // (N)
+  int* comp_full;            // (n_t, N)
+  float* pos_full;           // (n_t, 3, N)
+  uint32_t* n_cross;         // (N)
+  float* log_w;              // (N)
+  float* phi;                // (n_meas_pad, N)
+  unsigned long long* counters;  // [2]: wall events, candidates tested
+};
+
+// ---------------------------------------------------------------------------
+// Walker position: offset (float32) + periodic cell index (packed substrates).
+// ---------------------------------------------------------------------------
+struct Pos {
+  float x, y, z;
+  int cx, cy, cz;
+};
+
+// Source of the Powles uniform: injected, or a Philox draw made only when a
+// wall is actually hit (the reference draws one every sub-step and uses it only
+// on a hit, geometries.py:2122-2125).
+struct PowlesSrc {
+  uint32_t walker, instance, gstep, seed_lo, seed_hi;
+  float injected;
+  bool inject;
+  __device__ __forceinline__ float get() const {
+    if (inject) return injected;
+    u32x4 r = philox4x32_10({walker, instance, gstep, TAG_POWLES}, seed_lo, seed_hi);
+    return u01(r.x);
+  }
+};
+
+// floor(q / L + 0.5) as the reference computes it (true division).  For
+// objects listed in a walker's grid cell the quotient is never near a rounding
+// boundary (checked on the host), so the reciprocal form gives the same integer.
+__device__ __forceinline__ float img_shift(float q, const WalkArgs& a) {
+  if (a.exact_minimage) return floorf(q / a.cell + 0.5f);
+  return floorf(q * a.inv_cell + 0.5f);
+}
+
+// Exact fold of a position into the primary cell: x - L*floor(x/L + 0.5)
+// (geometries.py:1847).  Uses true division only when the quotient is close to
+// a rounding boundary.
+__device__ __forceinline__ float fold_shift(float x, const WalkArgs& a) {
+  const float app = x * a.inv_cell + 0.5f;
+  const float k = floorf(app);
+  if (fabsf(app - rintf(app)) < 1e-5f) return floorf(x / a.cell + 0.5f);
+  return k;
+}
+
+__device__ __forceinline__ int grid_index(float v, const WalkArgs& a) {
+  const float f = v - a.cell * floorf(v * a.inv_cell + 0.5f);
+  int i = (int)floorf((f + 0.5f * a.cell) * a.grid_inv_h);
+  i = i < 0 ? 0 : i;
+  return i >= a.grid_n ? a.grid_n - 1 : i;
+}
+
+// ---------------------------------------------------------------------------
+// Box1D (geometries.py:93-139)
+// ---------------------------------------------------------------------------
+template <int WALL>
+__device__ __forceinline__ float box_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                          uint32_t& hits) {
+  const float L = a.size;
+  const float L2 = 2.0f * L;
+  const float xr = p.x + sx;
+  float dlog = 0.0f;
+  if (WALL == W_SURFACE) {
+    const float dperp = fmaxf(xr - L, 0.0f) + fmaxf(-xr, 0.0f);
+    if (dperp > 0.0f) { dlog = (-2.0f * a.rhoD) * dperp; hits++; }
+  }
+  float m = fmodf(xr, L2);
+  if (m != 0.0f && ((m < 0.0f) != (L2 < 0.0f))) m = m + L2;
+  if (m > L) m = L2 - m;
+  p.x = m;
+  p.y = p.y + sy;
+  p.z = p.z + sz;
+  return dlog;
+}
+
+// ---------------------------------------------------------------------------
+// Cylinder, interior reflection (geometries.py:516-654).  Returns sum d_perp.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float sy, const WalkArgs& a,
+                                             uint32_t& hits) {
+  const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
+  const float RR = R * R;
+  const float sl = sqrtf(sx * sx + sy * sy);
+  float dx = 0.0f, dy = 0.0f;
+  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; }
+  float rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {
+    const float dp = dx * x + dy * y;
+    const float disc = fmaxf(dp * dp - ((x * x + y * y) - RR), 0.0f);
+    const float sq = sqrtf(disc);
+    const float d = -dp + sq;
+    if (!((d > EPS) && (d < rem) && (sl > 0.0f))) break;
+    const float hx = x + d * dx, hy = y + d * dy;
+    const float nx = hx / R, ny = hy / R;
+    const float tdd = 2.0f * (dx * nx + dy * ny);
+    float rx = dx - tdd * nx, ry = dy - tdd * ny;
+    const float rn = sqrtf(rx * rx + ry * ry);
+    rx = rx / rn; ry = ry / rn;
+    dsum = dsum + (rem - d) * (sq / R);
+    x = hx - NUDGE * nx;
+    y = hy - NUDGE * ny;
+    dx = rx; dy = ry;
+    rem = (rem - d) - NUDGE;
+    hits++;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float fx = x + dx * rm, fy = y + dy * rm;
+  const float fn = sqrtf(fx * fx + fy * fy);
+  if (fn >= R) {
+    const float s = R - NUDGE;
+    fx = (fx * s) / fn;
+    fy = (fy * s) / fn;
+  }
+  x = fx; y = fy;
+  return dsum;
+}
+
+// Cylinder, Powles permeation (geometries.py:656-801).
+__device__ __forceinline__ float cyl_permeate(float& x, float& y, float sx, float sy, const WalkArgs& a,
+                                              const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
+  const float RR = R * R;
+  const float sl = sqrtf(sx * sx + sy * sy);
+  float dx = 0.0f, dy = 0.0f;
+  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; }
+  const float dp = dx * x + dy * y;
+  const float rr = x * x + y * y;
+  const float disc = dp * dp - (rr - RR);
+  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const bool inside = rr < RR;
+  const float t_hit = inside ? (-dp + sq) : (-dp - sq);
+  const bool any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
+  if (!any_hit) { x = x + sx; y = y + sy; return 0.0f; }
+  hits++;
+  const float hx = x + t_hit * dx, hy = y + t_hit * dy;
+  const float hl = fmaxf(sqrtf(hx * hx + hy * hy), 1e-30f);
+  const float nx = hx / hl, ny = hy / hl;
+  const float remaining = sl - t_hit;
+  const float dperp = remaining * (sq / R);
+  const float p = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+  if (ps.get() < p) {
+    x = x + sx; y = y + sy; ncross++;
+    return 0.0f;
+  }
+  const float tdd = 2.0f * (dx * nx + dy * ny);
+  float rx = dx - tdd * nx, ry = dy - tdd * ny;
+  const float rn = sqrtf(rx * rx + ry * ry);
+  rx = rx / rn; ry = ry / rn;
+  const float sxh = R * nx, syh = R * ny;
+  const float qx = inside ? sxh - NUDGE * nx : sxh + NUDGE * nx;
+  const float qy = inside ? syh - NUDGE * ny : syh + NUDGE * ny;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float fx = qx + rx * go, fy = qy + ry * go;
+  const float fnorm = sqrtf(fx * fx + fy * fy);
+  const float fsafe = fmaxf(fnorm, 1e-30f);
+  const float target = inside ? R - NUDGE : R + NUDGE;
+  const bool wrong = inside ? (fnorm >= R) : (fnorm <= R);
+  if (wrong) { fx = (fx * target) / fsafe; fy = (fy * target) / fsafe; }
+  x = fx; y = fy;
+  return (-2.0f * a.rhoD) * dperp;
+}
+
+// ---------------------------------------------------------------------------
+// Sphere, interior reflection (geometries.py:197-317).  Returns sum d_perp.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                             uint32_t& hits) {
+  const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
+  const float RR = R * R;
+  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  float x = p.x, y = p.y, z = p.z;
+  float rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {
+    const float dp = (dx * x + dy * y) + dz * z;
+    const float disc = fmaxf(dp * dp - (((x * x + y * y) + z * z) - RR), 0.0f);
+    const float sq = sqrtf(disc);
+    const float d = -dp + sq;
+    if (!((d > EPS) && (d < rem))) break;
+    const float hx = x + d * dx, hy = y + d * dy, hz = z + d * dz;
+    const float nx = hx / R, ny = hy / R, nz = hz / R;
+    const float tdd = 2.0f * ((dx * nx + dy * ny) + dz * nz);
+    float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
+    const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
+    rx = rx / rn; ry = ry / rn; rz = rz / rn;
+    dsum = dsum + (rem - d) * (sq / R);
+    x = hx - NUDGE * nx; y = hy - NUDGE * ny; z = hz - NUDGE * nz;
+    dx = rx; dy = ry; dz = rz;
+    rem = (rem - d) - NUDGE;
+    hits++;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float fx = x + dx * rm, fy = y + dy * rm, fz = z + dz * rm;
+  const float fn = sqrtf((fx * fx + fy * fy) + fz * fz);
+  if (fn >= R) {
+    const float s = R - NUDGE;
+    fx = (fx * s) / fn; fy = (fy * s) / fn; fz = (fz * s) / fn;
+  }
+  p.x = fx; p.y = fy; p.z = fz;
+  return dsum;
+}
+
+// Sphere, Powles permeation (geometries.py:320-424): no snap, no side clamp.
+__device__ __forceinline__ float sph_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                              const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
+  const float RR = R * R;
+  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float x = p.x, y = p.y, z = p.z;
+  const float dp = (dx * x + dy * y) + dz * z;
+  const float rr = (x * x + y * y) + z * z;
+  const float disc = dp * dp - (rr - RR);
+  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const bool inside = rr < RR;
+  const float t_hit = inside ? (-dp + sq) : (-dp - sq);
+  const bool any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
+  if (!any_hit) { p.x = x + sx; p.y = y + sy; p.z = z + sz; return 0.0f; }
+  hits++;
+  const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
+  const float nx = hx / R, ny = hy / R, nz = hz / R;
+  const float remaining = sl - t_hit;
+  const float dperp = remaining * (sq / R);
+  const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+  if (ps.get() < pr) {
+    p.x = x + sx; p.y = y + sy; p.z = z + sz; ncross++;
+    return 0.0f;
+  }
+  const float tdd = 2.0f * ((dx * nx + dy * ny) + dz * nz);
+  float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
+  const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
+  rx = rx / rn; ry = ry / rn; rz = rz / rn;
+  const float qx = inside ? hx - NUDGE * nx : hx + NUDGE * nx;
+  const float qy = inside ? hy - NUDGE * ny : hy + NUDGE * ny;
+  const float qz = inside ? hz - NUDGE * nz : hz + NUDGE * nz;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  p.x = qx + rx * go; p.y = qy + ry * go; p.z = qz + rz * go;
+  return (-2.0f * a.rhoD) * dperp;
+}
+
+// ---------------------------------------------------------------------------
+// Packed cylinders (2-D cell) and packed spheres (3-D cell).
+// ---------------------------------------------------------------------------
+struct Tables {
+  const float4* objs;
+  const int* start;
+  const int* items;
+};
+
+template <int DIM>
+__device__ __forceinline__ int cell_of(const Pos& p, const WalkArgs& a) {
+  const int ix = grid_index(p.x, a), iy = grid_index(p.y, a);
+  if (DIM == 2) return iy * a.grid_n + ix;
+  const int iz = grid_index(p.z, a);
+  return (iz * a.grid_n + iy) * a.grid_n + ix;
+}
+
+// Minimum-image displacement of the walker from object o (geometries.py:2067-2068).
+template <int DIM>
+__device__ __forceinline__ void min_image(const Pos& p, const float4 o, const WalkArgs& a,
+                                          float& qx, float& qy, float& qz) {
+  const float L = a.cell;
+  qx = p.x - o.x; qx = qx - L * img_shift(qx, a);
+  qy = p.y - o.y; qy = qy - L * img_shift(qy, a);
+  if (DIM == 3) { qz = p.z - o.z; qz = qz - L * img_shift(qz, a); }
+  else qz = 0.0f;
+}
+
+template <int DIM>
+__device__ __forceinline__ float dot_d(float ax, float ay, float az, float bx, float by, float bz) {
+  if (DIM == 2) return ax * bx + ay * by;
+  return (ax * bx + ay * by) + az * bz;
+}
+
+// Exterior reflection off the nearest object + fold-first clamp pass
+// (PackedCylinders.reflect / reflect_with_log_weight geometries.py:1746-2016,
+// PackedSpheres geometries.py:2445-2631).  Returns d_perp of the hit (0 if none).
+template <int DIM>
+__device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                                const Tables& tb, uint32_t& hits, uint32_t& ncand) {
+  const float EPS = a.eps, NUDGE = a.nudge, L = a.cell;
+  if (DIM == 2) (void)sz;
+  const float sl = (DIM == 2) ? sqrtf(sx * sx + sy * sy) : sqrtf((sx * sx + sy * sy) + sz * sz);
+  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; if (DIM == 3) dz = sz / sl; }
+  const int gc = cell_of<DIM>(p, a);
+  const int beg = tb.start[gc], end = tb.start[gc + 1];
+  ncand += (uint32_t)(end - beg);
+  float best_t = __builtin_inff(), best_disc = 0.0f, bqx = 0.0f, bqy = 0.0f, bqz = 0.0f, bR = 1.0f;
+  bool hit = false;
+  for (int i = beg; i < end; ++i) {
+    const float4 o = tb.objs[tb.items[i]];
+    float qx, qy, qz;
+    min_image<DIM>(p, o, a, qx, qy, qz);
+    const float RR = o.w * o.w;
+    const float dp = dot_d<DIM>(dx, dy, dz, qx, qy, qz);
+    const float disc = dp * dp - (dot_d<DIM>(qx, qy, qz, qx, qy, qz) - RR);
+    const float t = -dp - sqrtf(fmaxf(disc, 0.0f));
+    const bool valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+    if (valid && t < best_t) {
+      best_t = t; best_disc = disc; bqx = qx; bqy = qy; bqz = qz; bR = o.w; hit = true;
+    }
+  }
+  float fx = p.x + sx, fy = p.y + sy, fz = p.z + (DIM == 3 ? sz : 0.0f);
+  float dperp = 0.0f;
+  if (hit) {
+    hits++;
+    const float ts = best_t;
+    const float qhx = bqx + ts * dx, qhy = bqy + ts * dy, qhz = bqz + ts * dz;
+    const float nx = qhx / bR, ny = qhy / bR, nz = (DIM == 3) ? qhz / bR : 0.0f;
+    const float tdd = 2.0f * dot_d<DIM>(dx, dy, dz, nx, ny, nz);
+    float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = (DIM == 3) ? dz - tdd * nz : 0.0f;
+    const float rn = sqrtf(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
+    if (rn > 0.0f) {
+      const float rs = fmaxf(rn, 1e-30f);
+      rx = rx / rs; ry = ry / rs; rz = rz / rs;
+    } else {
+      rx = 0.0f; ry = 0.0f; rz = 0.0f;
+    }
+    const float remaining = sl - ts;
+    const float go = fmaxf(remaining - NUDGE, 0.0f);
+    fx = ((p.x + ts * dx) + NUDGE * nx) + rx * go;
+    fy = ((p.y + ts * dy) + NUDGE * ny) + ry * go;
+    if (DIM == 3) fz = ((p.z + ts * dz) + NUDGE * nz) + rz * go;
+    dperp = remaining * (sqrtf(fmaxf(best_disc, 0.0f)) / bR);
+  }
+  // Safety clamp on the folded end point, over the same candidates
+  // (geometries.py:1858-1877).
+  const float gx = fx - L * fold_shift(fx, a);
+  const float gy = fy - L * fold_shift(fy, a);
+  const float gz = (DIM == 3) ? fz - L * fold_shift(fz, a) : 0.0f;
+  float best_pen = 1.0f, cqx = 0.0f, cqy = 0.0f, cqz = 0.0f, cR = 1.0f;
+  for (int i = beg; i < end; ++i) {
+    const float4 o = tb.objs[tb.items[i]];
+    float qx = gx - o.x; qx = qx - L * img_shift(qx, a);
+    float qy = gy - o.y; qy = qy - L * img_shift(qy, a);
+    float qz = 0.0f;
+    if (DIM == 3) { qz = gz - o.z; qz = qz - L * img_shift(qz, a); }
+    const float d2 = dot_d<DIM>(qx, qy, qz, qx, qy, qz);
+    const float RR = o.w * o.w;
+    if (d2 < RR) {
+      const float pen = d2 / RR;
+      if (pen < best_pen) { best_pen = pen; cqx = qx; cqy = qy; cqz = qz; cR = o.w; }
+    }
+  }
+  if (best_pen < 1.0f) {
+    const float dcl = sqrtf(dot_d<DIM>(cqx, cqy, cqz, cqx, cqy, cqz));
+    const float scale = (cR + NUDGE) / fmaxf(dcl, NUDGE);
+    const float sm1 = scale - 1.0f;
+    fx = fx + cqx * sm1;
+    fy = fy + cqy * sm1;
+    if (DIM == 3) fz = fz + cqz * sm1;
+  }
+  p.x = fx; p.y = fy;
+  p.z = (DIM == 3) ? fz : p.z + sz;
+  return dperp;
+}
+
+// Powles permeation against the nearest object wall, from inside or outside
+// (PackedCylinders.permeate geometries.py:2018-2183, PackedSpheres 2633-2756).
+template <int DIM>
+__device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                                 const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
+                                                 uint32_t& ncross, uint32_t& ncand) {
+  const float EPS = a.eps, NUDGE = a.nudge;
+  const float sl = (DIM == 2) ? sqrtf(sx * sx + sy * sy) : sqrtf((sx * sx + sy * sy) + sz * sz);
+  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; if (DIM == 3) dz = sz / sl; }
+  const int gc = cell_of<DIM>(p, a);
+  const int beg = tb.start[gc], end = tb.start[gc + 1];
+  ncand += (uint32_t)(end - beg);
+  float best_t = __builtin_inff(), best_disc = 0.0f, bqx = 0.0f, bqy = 0.0f, bqz = 0.0f, bR = 1.0f;
+  bool hit = false, hit_inside = false;
+  for (int i = beg; i < end; ++i) {
+    const float4 o = tb.objs[tb.items[i]];
+    float qx, qy, qz;
+    min_image<DIM>(p, o, a, qx, qy, qz);
+    const float RR = o.w * o.w;
+    const float dist2 = dot_d<DIM>(qx, qy, qz, qx, qy, qz);
+    const bool inside = dist2 < RR;
+    const float dp = dot_d<DIM>(dx, dy, dz, qx, qy, qz);
+    const float disc = dp * dp - (dist2 - RR);
+    const float sq = sqrtf(fmaxf(disc, 0.0f));
+    const float t = inside ? (-dp + sq) : (-dp - sq);
+    const bool valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+    if (valid && t < best_t) {
+      best_t = t; best_disc = disc; bqx = qx; bqy = qy; bqz = qz; bR = o.w; hit = true;
+      hit_inside = inside;
+    }
+  }
+  if (!hit) {
+    p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz;
+    return 0.0f;
+  }
+  hits++;
+  const float ts = best_t;
+  const float hx = bqx + ts * dx, hy = bqy + ts * dy, hz = (DIM == 3) ? bqz + ts * dz : 0.0f;
+  const float hl = fmaxf(sqrtf(dot_d<DIM>(hx, hy, hz, hx, hy, hz)), 1e-30f);
+  const float nx = hx / hl, ny = hy / hl, nz = (DIM == 3) ? hz / hl : 0.0f;
+  const float remaining = sl - ts;
+  const float dperp = remaining * (sqrtf(fmaxf(best_disc, 0.0f)) / bR);
+  const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+  if (ps.get() < pr) {
+    p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; ncross++;
+    return 0.0f;
+  }
+  const float tdd = 2.0f * dot_d<DIM>(dx, dy, dz, nx, ny, nz);
+  float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = (DIM == 3) ? dz - tdd * nz : 0.0f;
+  const float rn = sqrtf(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
+  if (rn > 0.0f) {
+    const float rs = fmaxf(rn, 1e-30f);
+    rx = rx / rs; ry = ry / rs; rz = rz / rs;
+  } else {
+    rx = 0.0f; ry = 0.0f; rz = 0.0f;
+  }
+  const float shx = bR * nx, shy = bR * ny, shz = bR * nz;
+  const float ux = hit_inside ? shx - NUDGE * nx : shx + NUDGE * nx;
+  const float uy = hit_inside ? shy - NUDGE * ny : shy + NUDGE * ny;
+  const float uz = hit_inside ? shz - NUDGE * nz : shz + NUDGE * nz;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float fx = ux + rx * go, fy = uy + ry * go, fz = (DIM == 3) ? uz + rz * go : 0.0f;
+  const float fn = sqrtf(dot_d<DIM>(fx, fy, fz, fx, fy, fz));
+  const float fs = fmaxf(fn, 1e-30f);
+  const float target = hit_inside ? bR - NUDGE : bR + NUDGE;
+  const bool wrong = hit_inside ? (fn >= bR) : (fn <= bR);
+  if (wrong) { fx = (fx * target) / fs; fy = (fy * target) / fs; if (DIM == 3) fz = (fz * target) / fs; }
+  p.x = p.x + (fx - bqx);
+  p.y = p.y + (fy - bqy);
+  p.z = (DIM == 3) ? p.z + (fz - bqz) : p.z + sz;
+  return (-2.0f * a.rhoD) * dperp;
+}
+
+// Compartment of a position (classify_position): 0/1 for Cylinder and Sphere
+// (geometries.py:180-184, 803-813), 0 or the largest 1-based index of a
+// containing object for packed substrates (geometries.py:2185-2214, 2758-2771).
+template <int GEOM>
+__device__ __forceinline__ int classify(const Pos& p, const WalkArgs& a, const Tables& tb) {
+  if (GEOM == G_CYL) {
+    const float RR = a.size * a.size;
+    return (p.x * p.x + p.y * p.y) < RR ? 0 : 1;
+  } else if (GEOM == G_SPH) {
+    const float RR = a.size * a.size;
+    return ((p.x * p.x + p.y * p.y) + p.z * p.z) < RR ? 0 : 1;
+  } else if (GEOM == G_PCYL || GEOM == G_PSPH) {
+    constexpr int DIM = (GEOM == G_PCYL) ? 2 : 3;
+    const int gc = cell_of<DIM>(p, a);
+    int id = 0;
+    for (int i = tb.start[gc]; i < tb.start[gc + 1]; ++i) {
+      const int k = tb.items[i];
+      const float4 o = tb.objs[k];
+      float qx, qy, qz;
+      min_image<DIM>(p, o, a, qx, qy, qz);
+      if (dot_d<DIM>(qx, qy, qz, qx, qy, qz) < o.w * o.w) id = k + 1 > id ? k + 1 : id;
+    }
+    return id;
+  }
+  return 0;
+}
+
+// Fold a packed position into the primary cell, carrying the cell index.
+template <int DIM>
+__device__ __forceinline__ void fold(Pos& p, const WalkArgs& a) {
+  const float L = a.cell;
+  float k = fold_shift(p.x, a);
+  if (k != 0.0f) { p.x = p.x - L * k; p.cx += (int)k; }
+  k = fold_shift(p.y, a);
+  if (k != 0.0f) { p.y = p.y - L * k; p.cy += (int)k; }
+  if (DIM == 3) {
+    k = fold_shift(p.z, a);
+    if (k != 0.0f) { p.z = p.z - L * k; p.cz += (int)k; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Start positions sampled in-kernel (reference: host rejection sampling,
+// geometries.py:87-91, 186-195, 477-491, 1717-1744, 2421-2443).
+// ---------------------------------------------------------------------------
+template <int GEOM>
+__device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const Tables& tb) {
+  Pos p{0.0f, 0.0f, 0.0f, 0, 0, 0};
+  if (GEOM == G_FREE) return p;
+  for (uint32_t attempt = 0; attempt < 4096u; ++attempt) {
+    const u32x4 r = philox4x32_10({w, a.instance, attempt, TAG_INIT}, a.seed_lo, a.seed_hi);
+    const float u0 = u01(r.x), u1 = u01(r.y), u2 = u01(r.z);
+    if (GEOM == G_BOX1D) {
+      p.x = u0 * a.size;
+      return p;
+    } else if (GEOM == G_CYL) {
+      p.x = (2.0f * u0 - 1.0f) * a.size;
+      p.y = (2.0f * u1 - 1.0f) * a.size;
+      if (p.x * p.x + p.y * p.y < a.size * a.size) return p;
+    } else if (GEOM == G_SPH) {
+      p.x = (2.0f * u0 - 1.0f) * a.size;
+      p.y = (2.0f * u1 - 1.0f) * a.size;
+      p.z = (2.0f * u2 - 1.0f) * a.size;
+      if ((p.x * p.x + p.y * p.y) + p.z * p.z < a.size * a.size) return p;
+    } else {
+      p.x = (u0 - 0.5f) * a.cell;
+      p.y = (u1 - 0.5f) * a.cell;
+      p.z = (GEOM == G_PSPH) ? (u2 - 0.5f) * a.cell : 0.0f;
+      if (classify<GEOM>(p, a, tb) == 0) return p;
+    }
+  }
+  return p;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Wave-reduced (Re, Im) of w e^{i(phi - c)} for one measurement block, written
+// by lane 0 to dst[m*2 + {0,1}].
+template <int MB, bool KAHAN>
+__device__ __forceinline__ void emit_partials(const float (&phi)[MB], const float (&kc)[MB], float wgt,
+                                              bool active, int lane, double* dst) {
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    float sn, cs;
+    sincosf(phi[m], &sn, &cs);
+    if (KAHAN) {
+      const float c0 = cs;
+      cs = fmaf(kc[m], sn, cs);
+      sn = fmaf(-kc[m], c0, sn);
+    }
+    const double re = wave_sum(active ? (double)(wgt * cs) : 0.0);
+    const double im = wave_sum(active ? (double)(wgt * sn) : 0.0);
+    if (lane == 0) { dst[2 * m] = re; dst[2 * m + 1] = im; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The walk kernel.  Grid: x = walker tiles of 256, y = measurement blocks.
+// ---------------------------------------------------------------------------
+template <int GEOM, int WALL, int MB, bool KAHAN>
+__global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
+  constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
+  constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
+  extern __shared__ float4 lds[];
+
+  Tables tb{nullptr, nullptr, nullptr};
+  if (PACKED) {
+    const int n_cells = (DIM == 2) ? a.grid_n * a.grid_n : a.grid_n * a.grid_n * a.grid_n;
+    float4* objs = lds;
+    int* start = reinterpret_cast<int*>(lds + a.n_obj);
+    int* items = start + n_cells + 1;
+    for (int i = threadIdx.x; i < a.n_obj; i += kBlock) objs[i] = a.objs[i];
+    for (int i = threadIdx.x; i <= n_cells; i += kBlock) start[i] = a.cell_start[i];
+    for (int i = threadIdx.x; i < a.n_items; i += kBlock) items[i] = a.cell_items[i];
+    __syncthreads();
+    tb = Tables{objs, start, items};
+  }
+
+  const int lane = threadIdx.x & (kWave - 1);
+  const long long gid = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long wave_id = gid / kWave;
+  const bool active = gid < a.n_walkers;
+  const long long N = a.n_walkers;
+  const uint32_t w = (uint32_t)(active ? gid : N - 1);
+  const int m0 = blockIdx.y * MB;
+  const bool primary = (blockIdx.y == 0);
+  const bool inject = (a.inj_n != nullptr);
+
+  Pos p;
+  if (a.r0) {
+    p = Pos{a.r0[w], a.r0[N + w], a.r0[2 * N + w], 0, 0, 0};
+    if (PACKED) fold<DIM>(p, a);
+  } else {
+    p = sample_start<GEOM>(w, a, tb);
+  }
+  int comp = classify<GEOM>(p, a, tb);
+  if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
+
+  float phi[MB], kc[MB];
+#pragma unroll
+  for (int m = 0; m < MB; ++m) { phi[m] = 0.0f; kc[m] = 0.0f; }
+  float logw = 0.0f;
+  uint32_t hits = 0, ncross = 0, ncand = 0;
+  const float L = a.cell;
+  const int n_sub = a.n_sub;
+  const float fsub = (float)n_sub;
+  u32x4 rnd{0u, 0u, 0u, 0u};
+  int next_echo = 0;
+
+  for (int t = 0; t < a.n_t; ++t) {
+    const float chi = a.chi ? a.chi[t] : 1.0f;
+    const float x0 = p.x, y0 = p.y, z0 = p.z;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (int s = 0; s < n_sub; ++s) {
+      const uint32_t gs = (uint32_t)t * (uint32_t)n_sub + (uint32_t)s;
+      float ux, uy, uz, uinj = 0.0f;
+      if (inject) {
+        const size_t base = (size_t)gs * 3 * (size_t)N + w;
+        const float nx = a.inj_n[base], ny = a.inj_n[base + N], nz = a.inj_n[base + 2 * N];
+        const float nrm = sqrtf((nx * nx + ny * ny) + nz * nz);
+        ux = nx / nrm; uy = ny / nrm; uz = nz / nrm;
+        if (WALL == W_PERMEATE && a.inj_u) uinj = a.inj_u[(size_t)gs * N + w];
+      } else {
+        if (!(gs & 1u) || (t == 0 && s == 0)) {
+          rnd = philox4x32_10({w, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
+        }
+        const uint32_t ra = (gs & 1u) ? rnd.z : rnd.x;
+        const uint32_t rb = (gs & 1u) ? rnd.w : rnd.y;
+        iso_direction(ra, rb, &ux, &uy, &uz);
+      }
+      const float sx = ux * a.step_len, sy = uy * a.step_len, sz = uz * a.step_len;
+      const PowlesSrc ps{w, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
+      float dlog = 0.0f;
+      if (GEOM == G_FREE) {
+        p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz;
+      } else if (GEOM == G_BOX1D) {
+        dlog = box_step<WALL>(p, sx, sy, sz, a, hits);
+      } else if (GEOM == G_CYL) {
+        if (WALL == W_PERMEATE) {
+          dlog = cyl_permeate(p.x, p.y, sx, sy, a, ps, hits, ncross);
+        } else {
+          const float ds = cyl_reflect(p.x, p.y, sx, sy, a, hits);
+          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
+        }
+        p.z = p.z + sz;
+      } else if (GEOM == G_SPH) {
+        if (WALL == W_PERMEATE) {
+          dlog = sph_permeate(p, sx, sy, sz, a, ps, hits, ncross);
+        } else {
+          const float ds = sph_reflect(p, sx, sy, sz, a, hits);
+          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
+        }
+      } else {
+        if (WALL == W_PERMEATE) {
+          dlog = packed_permeate<DIM>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+        } else {
+          const float dp = packed_reflect<DIM>(p, sx, sy, sz, a, tb, hits, ncand);
+          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * dp;
+        }
+      }
+      // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
+      if (WALL != W_REFLECT) dlog = dlog * chi;
+      if (a.has_t2) dlog = dlog - (a.dt_sub * a.inv_T2) * chi;
+      if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
+      logw = logw + dlog;
+      if (n_sub > 1) {
+        ax = ax + (p.x - x0);
+        ay = ay + (p.y - y0);
+        az = az + (p.z - z0);
+      }
+    }
+    // Sum of the n_sub sub-step positions (unfolded), then the phase update.
+    float Px, Py, Pz;
+    {
+      float ox = x0, oy = y0, oz = z0;
+      float ex = p.x, ey = p.y, ez = p.z;
+      if (PACKED) {
+        ox = ox + L * (float)p.cx; oy = oy + L * (float)p.cy; oz = oz + L * (float)p.cz;
+        ex = ex + L * (float)p.cx; ey = ey + L * (float)p.cy; ez = ez + L * (float)p.cz;
+      }
+      if (n_sub > 1) {
+        Px = fmaf(fsub, ox, ax); Py = fmaf(fsub, oy, ay); Pz = fmaf(fsub, oz, az);
+      } else {
+        Px = ex; Py = ey; Pz = ez;
+      }
+    }
+    // G(t) is wave-uniform: read it through the constant address space so the
+    // loads are scalar (s_load) and the values arrive in SGPRs.
+    const cfloat* g = (const cfloat*)(a.Gs + ((size_t)t * a.n_meas_pad + m0) * 3);
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+      const float dphi = fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
+      if (KAHAN) {
+        const float yk = dphi - kc[m];
+        const float tk = phi[m] + yk;
+        kc[m] = (tk - phi[m]) - yk;
+        phi[m] = tk;
+      } else {
+        phi[m] = phi[m] + dphi;
+      }
+    }
+    if (PACKED) fold<DIM>(p, a);
+    if (a.comp_full) comp = classify<GEOM>(p, a, tb);
+    if (primary && active) {
+      if (a.comp_full) a.comp_full[(size_t)t * N + w] = comp;
+      if (a.pos_full) {
+        a.pos_full[((size_t)t * 3 + 0) * N + w] = PACKED ? p.x + L * (float)p.cx : p.x;
+        a.pos_full[((size_t)t * 3 + 1) * N + w] = PACKED ? p.y + L * (float)p.cy : p.y;
+        a.pos_full[((size_t)t * 3 + 2) * N + w] = PACKED ? p.z + L * (float)p.cz : p.z;
+      }
+    }
+    if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
+      const float wgt = expf(logw);
+      emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane,
+                               a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
+      ++next_echo;
+    }
+  }
+
+  if (a.comp_final && !a.comp_full) comp = classify<GEOM>(p, a, tb);
+  const float wgt = expf(logw);
+  emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane,
+                           a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
+  if (active) {
+    if (a.phi) {
+#pragma unroll
+      for (int m = 0; m < MB; ++m)
+        if (m0 + m < a.n_meas) a.phi[(size_t)(m0 + m) * N + w] = phi[m] - kc[m];
+    }
+    if (primary) {
+      if (a.final_pos) {
+        a.final_pos[w] = PACKED ? p.x + L * (float)p.cx : p.x;
+        a.final_pos[N + w] = PACKED ? p.y + L * (float)p.cy : p.y;
+        a.final_pos[2 * N + w] = PACKED ? p.z + L * (float)p.cz : p.z;
+      }
+      if (a.comp_final) a.comp_final[w] = comp;
+      if (a.n_cross) a.n_cross[w] = ncross;
+      if (a.log_w) a.log_w[w] = logw;
+    }
+  }
+  if (a.counters && primary) {
+    unsigned long long h = active ? hits : 0u, c = active ? ncand : 0u;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      h += __shfl_xor(h, off, kWave);
+      c += __shfl_xor(c, off, kWave);
+    }
+    if (lane == 0) {
+      atomicAdd(&a.counters[0], h);
+      atomicAdd(&a.counters[1], c);
+    }
+  }
+}
+
+}  // namespace dmw

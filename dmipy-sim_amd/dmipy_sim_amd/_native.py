@@ -1,0 +1,136 @@
+"""ctypes binding of the HIP walk library (include/dmipy_walk.h).
+
+The library ``libdmw.so`` is built in-tree next to this file.  There is no
+fallback: when the library or a GPU is missing every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdmw.so")
+_lib = None
+
+DMW_OK, DMW_EINVAL, DMW_EUNSUPPORTED, DMW_ENOMEM, DMW_ENODEV, DMW_EHIP = range(6)
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+
+
+class Geometry_t(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("wall", C.c_int32), ("n_obj", C.c_int32),
+                ("size", C.c_float), ("cell", C.c_float), ("eps", C.c_float),
+                ("nudge", C.c_float), ("rho_over_D", C.c_float), ("kappa_over_D", C.c_float),
+                ("centers", _fp), ("radii", _fp)]
+
+
+class Waveform_t(C.Structure):
+    _fields_ = [("n_meas", C.c_int32), ("n_t", C.c_int32), ("G", _fp), ("chi", _fp),
+                ("n_echo", C.c_int32), ("echo_idx", _ip)]
+
+
+class Run_t(C.Structure):
+    _fields_ = [("n_walkers", C.c_int64), ("seed", C.c_uint64), ("n_sub", C.c_int32),
+                ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
+                ("inv_T2", C.c_float), ("inv_T1", C.c_float), ("has_t2", C.c_int32),
+                ("has_t1", C.c_int32), ("kahan", C.c_int32), ("meas_block", C.c_int32),
+                ("r0", _fp), ("inj_normals", _fp), ("inj_uniforms", _fp)]
+
+
+class Result_t(C.Structure):
+    _fields_ = [("signal", _dp), ("signal_im", _dp), ("final_pos", _fp),
+                ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
+                ("pos_full", _fp), ("n_cross", _up), ("log_w", _fp), ("phi", _fp),
+                ("kernel_ms", C.c_double), ("n_hits", C.c_uint64), ("n_cand", C.c_uint64),
+                ("want_counters", C.c_int32)]
+
+
+class Instance_t(C.Structure):
+    _fields_ = [("geom", Geometry_t), ("n_walkers", C.c_int64), ("n_sub", C.c_int32),
+                ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
+                ("inv_T2", C.c_float), ("has_t2", C.c_int32)]
+
+
+class BatchStats_t(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double),
+                ("walker_steps", C.c_uint64), ("n_units", C.c_int32), ("grid", C.c_int32)]
+
+
+#: Every symbol include/dmipy_walk.h declares, with (restype, argtypes).
+SIGNATURES = {
+    "dmw_device_count": (C.c_int, []),
+    "dmw_set_device": (C.c_int, [C.c_int]),
+    "dmw_last_error": (C.c_char_p, []),
+    "dmw_walk": (C.c_int, [C.POINTER(Geometry_t), C.POINTER(Waveform_t), C.POINTER(Run_t),
+                           C.POINTER(Result_t)]),
+    "dmw_plan_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Geometry_t),
+                                  C.POINTER(Waveform_t), C.POINTER(Run_t)]),
+    "dmw_plan_execute": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Result_t)]),
+    "dmw_plan_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Instance_t),
+                                   C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
+    "dmw_batch_execute": (C.c_int, [C.c_void_p, C.c_uint64, _dp, C.POINTER(BatchStats_t)]),
+    "dmw_batch_destroy": (C.c_int, [C.c_void_p]),
+}
+
+
+class NativeLibraryMissing(RuntimeError):
+    """The HIP walk library has not been built (run __graft_entry__.build())."""
+
+
+def lib():
+    """Load ``libdmw.so`` once; raise if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise NativeLibraryMissing(
+                f"HIP walk library not found at {_LIB_PATH}; build it with "
+                "`python -c 'import __graft_entry__ as g; g.build()'`")
+        handle = C.CDLL(_LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def device_count():
+    return int(lib().dmw_device_count())
+
+
+def check(rc):
+    """Map a dmw_status to the reference's exception types (ValueError,
+    NotImplementedError, RuntimeError with "out of memory")."""
+    if rc == DMW_OK:
+        return
+    msg = lib().dmw_last_error().decode(errors="replace")
+    if rc == DMW_EINVAL:
+        raise ValueError(msg)
+    if rc == DMW_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    if rc == DMW_ENOMEM:
+        raise RuntimeError(msg if "out of memory" in msg else msg + " (out of memory)")
+    raise RuntimeError(msg)
+
+
+def fptr(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def dptr(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def iptr(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def uptr(a):
+    return None if a is None else a.ctypes.data_as(_up)
+
+
+def f32c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)

@@ -1,0 +1,401 @@
+"""Substrate geometries: descriptors of the walls the walk reflects from.
+
+The classes keep the constructor signatures and attributes of
+dmipy_sim/geometries.py (``radius``, ``orientation``, ``surface_relaxivity_t2``,
+``permeability``, ``_radii_np``, ``_L_float``, ``min_gap``, ``init_positions``,
+``volume``/``surface_area``/``volume_fraction``) so that code written against the
+reference builds the same substrate.  The wall physics itself (specular
+reflection, Brownstein-Tarr weights, Powles crossing, periodic images) runs only in
+the HIP walk kernel; these objects carry the parameters to it.
+
+* ``FreeDiffusion``   -- dmipy_sim/geometries.py:34-45
+* ``Box1D``           -- dmipy_sim/geometries.py:48-140
+* ``Sphere``          -- dmipy_sim/geometries.py:143-424
+* ``Cylinder``        -- dmipy_sim/geometries.py:427-844
+* ``pack_cylinders``  -- dmipy_sim/geometries.py:1502-1601
+* ``PackedCylinders`` -- dmipy_sim/geometries.py:1604-2242
+* ``pack_spheres``    -- dmipy_sim/geometries.py:2249-2340
+* ``PackedSpheres``   -- dmipy_sim/geometries.py:2343-2786
+"""
+import numpy as np
+
+# Geometry kind codes shared with include/dmipy_walk.h (enum dmw_geom_kind).
+KIND_FREE, KIND_BOX1D, KIND_CYLINDER, KIND_SPHERE, KIND_PACKED_CYL, KIND_PACKED_SPH = range(6)
+
+
+def _rng_from_key(key):
+    """numpy Generator for host-side placement: an int seed, a Generator, or any
+    object convertible to an int (the reference derives its numpy seed from a JAX
+    key, dmipy_sim/geometries.py:188)."""
+    if isinstance(key, np.random.Generator):
+        return key
+    if key is None:
+        key = 0
+    arr = np.asarray(key).ravel()
+    seed = [int(v) & 0xFFFFFFFF for v in arr] if arr.size > 1 else int(arr[0]) & (2**63 - 1)
+    return np.random.default_rng(seed)
+
+
+def _rotation_to_z(v):
+    """Rotation matrix R with R @ v = [0, 0, 1] (Rodrigues' formula; the parallel
+    and anti-parallel cases are handled explicitly, dmipy_sim/geometries.py:1466-1495)."""
+    v = np.asarray(v, dtype=np.float64)
+    v = v / np.linalg.norm(v)
+    k = np.array([0.0, 0.0, 1.0])
+    c = float(np.dot(v, k))
+    if abs(c - 1.0) < 1e-10:
+        return np.eye(3)
+    if abs(c + 1.0) < 1e-10:
+        return np.diag([1.0, -1.0, -1.0])
+    axis = np.cross(v, k)
+    axis = axis / np.linalg.norm(axis)
+    ang = np.arccos(np.clip(c, -1.0, 1.0))
+    K = np.array([[0.0, -axis[2], axis[1]],
+                  [axis[2], 0.0, -axis[0]],
+                  [-axis[1], axis[0], 0.0]])
+    return np.eye(3) + np.sin(ang) * K + (1.0 - np.cos(ang)) * (K @ K)
+
+
+class Geometry:
+    """Base of the walker substrates: ``init_positions(n_walkers, key)`` gives the
+    reference start distribution (dmipy_sim/geometries.py:14-31)."""
+    kind = None
+    surface_relaxivity_t2 = None
+    permeability = None
+
+    def init_positions(self, n_walkers, key=0):
+        raise NotImplementedError
+
+
+def _opt_float(x):
+    return float(x) if x is not None else None
+
+
+class FreeDiffusion(Geometry):
+    """Unbounded free diffusion; walkers start at the origin, compartment 0."""
+    kind = KIND_FREE
+
+    def init_positions(self, n_walkers, key=0):
+        return np.zeros((n_walkers, 3), dtype=np.float32)
+
+
+class Box1D(Geometry):
+    """Reflecting slab between x=0 and x=length, free along y and z.
+
+    ``surface_relaxivity_t2`` (m/s) applies the Brownstein-Tarr weight
+    exp(-2 rho d_perp / D) with d_perp the overshoot past the wall.
+    """
+    kind = KIND_BOX1D
+
+    def __init__(self, length, surface_relaxivity_t2=None):
+        self.length = float(length)
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+
+    def volume(self):
+        return self.length
+
+    def surface_area(self):
+        return 2.0
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        x = rng.uniform(0.0, self.length, (n_walkers, 1))
+        return np.concatenate([x, np.zeros((n_walkers, 2))], axis=1).astype(np.float32)
+
+
+class Sphere(Geometry):
+    """Sphere of ``radius`` at the origin, reflecting (optionally relaxing via
+    ``surface_relaxivity_t2``) or permeable (``permeability``, Powles crossing)."""
+    kind = KIND_SPHERE
+
+    def __init__(self, radius, surface_relaxivity_t2=None, permeability=None):
+        self.radius = float(radius)
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.permeability = _opt_float(permeability)
+
+    def volume(self):
+        return (4.0 / 3.0) * np.pi * self.radius ** 3
+
+    def surface_area(self):
+        return 4.0 * np.pi * self.radius ** 2
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        out, have = [], 0
+        while have < n_walkers:
+            pts = rng.uniform(-self.radius, self.radius, (n_walkers * 4, 3))
+            pts = pts[np.linalg.norm(pts, axis=1) < self.radius]
+            out.append(pts)
+            have += len(pts)
+        return np.concatenate(out)[:n_walkers].astype(np.float32)
+
+
+class Cylinder(Geometry):
+    """Infinite cylinder of ``radius`` along ``orientation`` through the origin.
+
+    The walk runs in the cylinder's native frame (axis = z); ``_R`` maps lab
+    vectors into it (dmipy_sim/geometries.py:458-468).
+    """
+    kind = KIND_CYLINDER
+
+    def __init__(self, radius, orientation, surface_relaxivity_t2=None, permeability=None):
+        self.radius = float(radius)
+        o = np.asarray(orientation, dtype=np.float64)
+        self.orientation = (o / np.linalg.norm(o)).astype(np.float32)
+        R = _rotation_to_z(self.orientation)
+        self._R = R.astype(np.float32)
+        self._R_inv = R.T.astype(np.float32)
+        self._is_identity_rotation = bool(np.allclose(R, np.eye(3)))
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.permeability = _opt_float(permeability)
+
+    def volume(self, L=1.0):
+        return np.pi * self.radius ** 2 * float(L)
+
+    def surface_area(self, L=1.0, include_caps=False):
+        a = 2.0 * np.pi * self.radius * float(L)
+        if include_caps:
+            a += 2.0 * np.pi * self.radius ** 2
+        return a
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        out, have = [], 0
+        while have < n_walkers:
+            xy = rng.uniform(-self.radius, self.radius, (n_walkers * 2, 2))
+            xy = xy[np.linalg.norm(xy, axis=1) < self.radius]
+            out.append(xy)
+            have += len(xy)
+        xy = np.concatenate(out)[:n_walkers].astype(np.float32)
+        r_cyl = np.stack([xy[:, 0], xy[:, 1], np.zeros(n_walkers, np.float32)], axis=1)
+        return (self._R_inv.astype(np.float64) @ r_cyl.T).T.astype(np.float32)
+
+
+def _check_pack_inputs(radii, target_vf, L, dim):
+    radii = np.asarray(radii, dtype=np.float64).ravel()
+    if len(radii) == 0:
+        raise ValueError("radii must contain at least one element.")
+    if np.any(radii <= 0):
+        raise ValueError("All radii must be positive.")
+    if (target_vf is None) == (L is None):
+        raise ValueError("Provide exactly one of target_vf or L.")
+    if target_vf is not None:
+        if not 0.0 < float(target_vf) < 1.0:
+            raise ValueError(f"target_vf must be in (0, 1), got {target_vf}.")
+        if dim == 2:
+            L = float(np.sqrt(np.pi * np.sum(radii ** 2) / float(target_vf)))
+        else:
+            L = float(((4.0 / 3.0) * np.pi * np.sum(radii ** 3) / float(target_vf)) ** (1.0 / 3.0))
+    else:
+        L = float(L)
+    if np.any(radii > L / 2.0):
+        raise ValueError(
+            f"Largest radius ({np.max(radii) * 1e6:.2f} µm) exceeds L/2 "
+            f"({L / 2.0 * 1e6:.2f} µm).  Reduce target_vf or supply a larger L.")
+    return radii, L
+
+
+def _rsa(radii, L, seed, max_attempts, dim, what):
+    """Random sequential addition in a periodic box, largest objects first, with a
+    minimum-image overlap test (dmipy_sim/geometries.py:1569-1595, 2309-2334)."""
+    rng = np.random.default_rng(int(seed))
+    order = np.argsort(radii)[::-1]
+    rs = radii[order]
+    cs = np.zeros((len(radii), dim))
+    for i, r_new in enumerate(rs):
+        for _ in range(max_attempts):
+            c_new = rng.uniform(-L / 2.0, L / 2.0, dim)
+            ok = True
+            for j in range(i):
+                dq = c_new - cs[j]
+                dq -= L * np.round(dq / L)
+                if np.linalg.norm(dq) < r_new + rs[j]:
+                    ok = False
+                    break
+            if ok:
+                cs[i] = c_new
+                break
+        else:
+            raise RuntimeError(
+                f"RSA failed after {max_attempts} attempts placing {what} {i} "
+                f"(r = {r_new * 1e6:.2f} µm).  "
+                f"The target packing fraction may exceed what RSA can achieve; "
+                f"try reducing target_vf or increasing max_attempts.")
+    out = np.empty_like(cs)
+    out[order] = cs
+    return out
+
+
+def pack_cylinders(radii, target_vf=None, L=None, seed=0, max_attempts=100_000):
+    """Pack parallel cylinders in a periodic square by RSA
+    (dmipy_sim/geometries.py:1502-1601).  Returns ``(centers (N,2), L, achieved_vf)``."""
+    radii, L = _check_pack_inputs(radii, target_vf, L, 2)
+    centers = _rsa(radii, L, seed, max_attempts, 2, "cylinder")
+    return centers, L, float(np.pi * np.sum(radii ** 2) / L ** 2)
+
+
+def pack_spheres(radii, target_vf=None, L=None, seed=0, max_attempts=100_000):
+    """Pack spheres in a periodic cube by RSA (dmipy_sim/geometries.py:2249-2340).
+    Returns ``(centers (N,3), L, achieved_vf)``."""
+    radii, L = _check_pack_inputs(radii, target_vf, L, 3)
+    centers = _rsa(radii, L, seed, max_attempts, 3, "sphere")
+    return centers, L, float((4.0 / 3.0) * np.pi * np.sum(radii ** 3) / L ** 3)
+
+
+def pack_dense(radii, target_vf, seed=0, dim=2, gap=0.02, max_iter=200_000):
+    """Non-overlapping periodic packing at volume fractions beyond RSA's reach.
+
+    Random start in the final box, then repeated pairwise overlap relaxation: every
+    pair closer than ``r_i + r_j + gap*min(r)`` (minimum image) is pushed apart
+    along its centre line, each by half the deficit, until no pair overlaps.  The
+    reference's RSA stops near VF 0.55 for polydisperse disks
+    (dmipy_sim/geometries.py:1508-1509); the relaxation reaches the VF 0.6 packing
+    the PackedCylinders workload asks for.  Returns ``(centers, L, achieved_vf)``.
+    """
+    radii, L = _check_pack_inputs(radii, target_vf, None, dim)
+    rng = np.random.default_rng(int(seed))
+    n = len(radii)
+    c = rng.uniform(-L / 2.0, L / 2.0, (n, dim))
+    sep = radii[:, None] + radii[None, :] + gap * float(radii.min())
+    iu = np.triu_indices(n, 1)
+    for it in range(max_iter):
+        d = c[:, None, :] - c[None, :, :]
+        d -= L * np.round(d / L)
+        dist = np.sqrt(np.sum(d * d, axis=2))
+        deficit = sep - dist
+        np.fill_diagonal(deficit, -1.0)
+        if deficit[iu].max() <= 0.0:
+            break
+        push = np.where(deficit > 0, 0.5 * deficit * 1.0001 + 1e-12, 0.0)
+        unit = d / np.maximum(dist, 1e-30)[:, :, None]
+        c += np.sum(push[:, :, None] * unit, axis=1)
+        c -= L * np.floor(c / L + 0.5)
+    else:
+        raise RuntimeError("pack_dense: overlap relaxation did not converge; "
+                           "reduce target_vf or gap")
+    if dim == 2:
+        vf = float(np.pi * np.sum(radii ** 2) / L ** 2)
+    else:
+        vf = float((4.0 / 3.0) * np.pi * np.sum(radii ** 3) / L ** 3)
+    return c, L, vf
+
+
+def _min_gap(centers, radii, L):
+    """Smallest clear gap between two object surfaces, periodic images included
+    (dmipy_sim/geometries.py:1700-1715)."""
+    n = len(radii)
+    g = float('inf')
+    for i in range(n):
+        for j in range(i + 1, n):
+            dq = centers[i] - centers[j]
+            dq -= L * np.round(dq / L)
+            g = min(g, float(np.linalg.norm(dq) - radii[i] - radii[j]))
+    for i in range(n):
+        g = min(g, L - 2.0 * radii[i])
+    return float(g)
+
+
+def _outside_all(pts, centers, radii, L):
+    ok = np.ones(len(pts), dtype=bool)
+    for k in range(len(radii)):
+        d = pts - centers[k]
+        d -= L * np.round(d / L)
+        ok &= np.sum(d * d, axis=1) > radii[k] ** 2
+    return ok
+
+
+class PackedCylinders(Geometry):
+    """Parallel cylinders in a periodic square cell of side ``L``.
+
+    Walkers start in the extra-axonal space; positions stay unfolded for the phase
+    while wall tests use the minimum image.  Compartments: 0 extra-axonal, k inside
+    cylinder k (1-based).
+    """
+    kind = KIND_PACKED_CYL
+
+    def __init__(self, radii, centers, L, orientation=(0., 0., 1.),
+                 surface_relaxivity_t2=None, permeability=None):
+        radii = np.asarray(radii, dtype=np.float64).ravel()
+        centers = np.asarray(centers, dtype=np.float64)
+        if centers.shape != (len(radii), 2):
+            raise ValueError(
+                f"centers shape {centers.shape} does not match "
+                f"({len(radii)}, 2) for {len(radii)} cylinders.")
+        if np.any(radii <= 0):
+            raise ValueError("All radii must be positive.")
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.permeability = _opt_float(permeability)
+        self._L_float = float(L)
+        self._radii_np = radii.copy()
+        self._centers_np = centers.copy()
+        o = np.asarray(orientation, dtype=np.float64)
+        self.orientation = (o / np.linalg.norm(o)).astype(np.float32)
+        R = _rotation_to_z(self.orientation)
+        self._R = R.astype(np.float32)
+        self._R_inv = R.T.astype(np.float32)
+        self._is_identity_rotation = bool(np.allclose(R, np.eye(3)))
+        self.min_gap = _min_gap(centers, radii, float(L))
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        L = self._L_float
+        out, have = [], 0
+        while have < n_walkers:
+            xy = rng.uniform(-L / 2.0, L / 2.0, (max(n_walkers * 4, 1024), 2))
+            xy = xy[_outside_all(xy, self._centers_np, self._radii_np, L)]
+            out.append(xy)
+            have += len(xy)
+        xy = np.concatenate(out)[:n_walkers].astype(np.float32)
+        r_cyl = np.concatenate([xy, np.zeros((n_walkers, 1), np.float32)], axis=1)
+        return (self._R_inv.astype(np.float64) @ r_cyl.T).T.astype(np.float32)
+
+    def volume(self, L=1.0):
+        return float(np.pi * np.sum(self._radii_np ** 2) * float(L))
+
+    def surface_area(self, L=1.0):
+        return float(2.0 * np.pi * np.sum(self._radii_np) * float(L))
+
+    def volume_fraction(self):
+        return float(np.pi * np.sum(self._radii_np ** 2) / self._L_float ** 2)
+
+
+class PackedSpheres(Geometry):
+    """Spheres in a periodic cubic cell of side ``L`` (3-D analogue of
+    ``PackedCylinders``).  Compartments: 0 outside, k inside sphere k (1-based)."""
+    kind = KIND_PACKED_SPH
+
+    def __init__(self, radii, centers, L, surface_relaxivity_t2=None, permeability=None):
+        radii = np.asarray(radii, dtype=np.float64).ravel()
+        centers = np.asarray(centers, dtype=np.float64)
+        if centers.shape != (len(radii), 3):
+            raise ValueError(
+                f"centers shape {centers.shape} does not match "
+                f"({len(radii)}, 3) for {len(radii)} spheres.")
+        if np.any(radii <= 0):
+            raise ValueError("All radii must be positive.")
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.permeability = _opt_float(permeability)
+        self._L_float = float(L)
+        self._radii_np = radii.copy()
+        self._centers_np = centers.copy()
+        self.min_gap = _min_gap(centers, radii, float(L))
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        L = self._L_float
+        out, have = [], 0
+        while have < n_walkers:
+            pts = rng.uniform(-L / 2.0, L / 2.0, (max(n_walkers * 4, 1024), 3))
+            pts = pts[_outside_all(pts, self._centers_np, self._radii_np, L)]
+            out.append(pts)
+            have += len(pts)
+        return np.concatenate(out)[:n_walkers].astype(np.float32)
+
+    def volume(self):
+        return float((4.0 / 3.0) * np.pi * np.sum(self._radii_np ** 3))
+
+    def surface_area(self):
+        return float(4.0 * np.pi * np.sum(self._radii_np ** 2))
+
+    def volume_fraction(self):
+        return float((4.0 / 3.0) * np.pi * np.sum(self._radii_np ** 3) / self._L_float ** 3)

@@ -1,0 +1,219 @@
+"""Sub-step rules and the per-run constants of one walk.
+
+``make_walk_plan`` is the counterpart of ``make_step_fn``
+(dmipy_sim/physics.py:191-417): it decides which wall routine the walk uses
+(plain reflection, reflection with the Brownstein-Tarr weight, or Powles
+permeation), how many physical sub-steps resolve one waveform sample, and the
+float32 constants of the step.  Each constant is rounded exactly where the
+reference rounds it, so the kernel and a restatement of the reference step agree
+bit for bit on identical increments:
+
+* step length  ``sqrt(f32(6*D) * f32(dt/n_sub))``      (physics.py:256-260)
+* gamma*dt_sub ``f32(GAMMA*dt/n_sub)``                  (physics.py:288-290)
+* rho/D        ``f32(rho)/f32(D)`` on the surface branch (physics.py:322-328),
+               ``f32(rho/D)`` on the permeable branch    (physics.py:279-281)
+* kappa/D      ``f32(kappa/D)``                         (physics.py:279)
+* EPS, NUDGE   ``f32(1e-7*R)``, ``f32(1e-4*R)`` with R the radius, or the
+               smallest radius of a packing (geometries.py:503-504, 1694-1696)
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .constants import GAMMA
+from . import geometries as geo
+
+WALL_REFLECT, WALL_SURFACE, WALL_PERMEATE = 0, 1, 2
+
+f32 = np.float32
+
+
+def _geometry_radius(geometry):
+    """Smallest geometric radius, or None (dmipy_sim/physics.py:15-32)."""
+    R = getattr(geometry, 'radius', None)
+    if R is None:
+        R = getattr(geometry, 'sphere_radius', None)
+    if R is None:
+        radii = getattr(geometry, '_radii_np', None)
+        if radii is not None and len(radii) > 0:
+            R = float(np.min(radii))
+    if R is None:
+        inner = getattr(geometry, '_inner_radii_np', None)
+        if inner is not None and len(inner) > 0 and np.any(inner > 0):
+            R = float(np.min(inner[inner > 0]))
+    return float(R) if R is not None else None
+
+
+def permeable_sub_steps(geometry, diffusivity, dt):
+    """Sub-steps so the step is about R/25 (dt_max = R^2/(3750 D)); 1 without a
+    radius (dmipy_sim/physics.py:35-48)."""
+    R = _geometry_radius(geometry)
+    if R is None:
+        return 1
+    dt_max = R ** 2 / (3750.0 * diffusivity)
+    return max(1, int(np.ceil(dt / dt_max)))
+
+
+def _surface_char_radius(geometry):
+    """Pore scale for the surface rule: the extra-axonal pore area/perimeter when
+    the geometry describes one, else the geometric radius
+    (dmipy_sim/physics.py:51-69)."""
+    outer = getattr(geometry, '_outer_radii_np', None)
+    cell = getattr(geometry, '_cell_size', None)
+    if outer is not None and cell is not None:
+        outer = np.asarray(outer, float)
+        outer = outer[outer > 0]
+        area_ext = float(cell) ** 2 - float(np.sum(np.pi * outer ** 2))
+        perim = float(np.sum(2.0 * np.pi * outer))
+        if perim > 0 and area_ext > 0:
+            return area_ext / perim
+    return _geometry_radius(geometry)
+
+
+def surface_sub_steps(geometry, diffusivity, dt, frac=8.0):
+    """Sub-steps so the step is about R_char/frac; ``surface_substep_frac`` on the
+    geometry overrides ``frac`` and 0/None disables sub-stepping
+    (dmipy_sim/physics.py:163-188)."""
+    g_frac = getattr(geometry, 'surface_substep_frac', None)
+    if g_frac is not None:
+        frac = g_frac
+    if not frac or frac <= 0:
+        return 1
+    Rc = _surface_char_radius(geometry)
+    if Rc is None:
+        return 1
+    dt_max = (Rc / frac) ** 2 / (6.0 * diffusivity)
+    return max(1, int(np.ceil(dt / dt_max)))
+
+
+def collision_sub_steps(geometry, diffusivity, dt, frac=0.9):
+    """Sub-steps keeping a step inside a collision grid cell; 1 for geometries
+    without ``cell_size`` -- every analytic geometry here
+    (dmipy_sim/physics.py:72-95)."""
+    cs = getattr(geometry, "cell_size", None)
+    if cs is None or not np.isfinite(cs) or cs <= 0:
+        return 1
+    L = float(np.sqrt(6.0 * float(diffusivity) * float(dt)))
+    ratio = L / (float(frac) * float(cs))
+    return int(max(1, np.ceil(ratio ** 2))) if ratio > 1.0 else 1
+
+
+def replay_sub_steps(geometry, diffusivity, dt, permeable=False):
+    """Sub-step rule of the walk-once producer: step <= R/6 (dt_max = R^2/(216 D)),
+    R/25 when permeable (dmipy_sim/core.py:1089-1118)."""
+    R = _geometry_radius(geometry)
+    if R is None:
+        return 1
+    div = 3750.0 if permeable else 216.0
+    return max(1, int(np.ceil(dt / (R ** 2 / (div * diffusivity)))))
+
+
+@dataclass
+class WalkPlan:
+    """Everything the kernel needs besides the waveform and the walkers."""
+    kind: int
+    wall: int
+    n_sub: int
+    size: float            # radius (Cylinder/Sphere), length (Box1D)
+    cell: float            # periodic cell side (packed)
+    eps: float
+    nudge: float
+    rho_over_D: float
+    kappa_over_D: float
+    step_len: float
+    gamma_dt_sub: float
+    dt_sub: float
+    inv_T2: float
+    inv_T1: float
+    has_t2: bool
+    has_t1: bool
+    has_weight: bool
+    centers: np.ndarray = None   # native frame, float32
+    radii: np.ndarray = None     # float32
+    rot: np.ndarray = None       # lab -> native rotation (float32 3x3) or None
+    extra: dict = field(default_factory=dict)
+
+
+def _rotation_of(geometry):
+    R = getattr(geometry, '_R', None)
+    if R is None or getattr(geometry, '_is_identity_rotation', True):
+        return None
+    return np.asarray(R, dtype=np.float32)
+
+
+def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fused"):
+    """Resolve wall routine, sub-steps and float32 constants for one walk.
+
+    ``substep_rule="fused"`` follows ``make_step_fn`` (R/25 permeable, R_char/8
+    with surface relaxivity, collision rule otherwise); ``"replay"`` follows the
+    walk-once producer (R/6, or R/25 when permeable).
+    """
+    kind = getattr(geometry, 'kind', None)
+    if kind is None:
+        raise NotImplementedError(
+            f"geometry {type(geometry).__name__} is not supported by the HIP walk")
+    if diffusivity is None:
+        raise ValueError("diffusivity is required for standard geometries")
+    D = float(diffusivity)
+    dt = float(dt)
+    has_surf = getattr(geometry, 'surface_relaxivity_t2', None) is not None
+    has_perm = getattr(geometry, 'permeability', None) is not None
+    has_t2 = T2 is not None
+    has_t1 = T1 is not None
+
+    rho_over_D = 0.0
+    kappa_over_D = 0.0
+    if has_perm:
+        wall = WALL_PERMEATE
+        kappa_over_D = float(f32(geometry.permeability / D))
+        if has_surf:
+            rho_over_D = float(f32(geometry.surface_relaxivity_t2 / D))
+        n_sub = permeable_sub_steps(geometry, D, dt)
+    elif has_surf:
+        wall = WALL_SURFACE
+        rho_over_D = float(f32(geometry.surface_relaxivity_t2) / f32(D))
+        n_sub = surface_sub_steps(geometry, D, dt)
+    else:
+        wall = WALL_REFLECT
+        n_sub = collision_sub_steps(geometry, D, dt)
+    if substep_rule == "replay":
+        n_sub = replay_sub_steps(geometry, D, dt, permeable=has_perm)
+    elif substep_rule != "fused":
+        raise ValueError(f"unknown substep_rule {substep_rule!r}")
+    if kind == geo.KIND_FREE:
+        wall = WALL_REFLECT
+    if kind == geo.KIND_BOX1D and wall == WALL_PERMEATE:
+        raise NotImplementedError("Box1D has no permeable wall")
+
+    dt_sub = f32(dt / n_sub)
+    step_len = float(np.sqrt(f32(6.0 * D) * dt_sub, dtype=np.float32))
+    gamma_dt_sub = float(f32(GAMMA * (dt / n_sub)))
+    inv_T2 = float(f32(1.0 / T2)) if has_t2 else 0.0
+    inv_T1 = float(f32(1.0 / T1)) if has_t1 else 0.0
+
+    size, cell = 0.0, 0.0
+    centers = radii = None
+    if kind == geo.KIND_BOX1D:
+        size = geometry.length
+        eps = nudge = 0.0
+    elif kind in (geo.KIND_CYLINDER, geo.KIND_SPHERE):
+        size = geometry.radius
+        eps = float(f32(1e-7 * geometry.radius))
+        nudge = float(f32(1e-4 * geometry.radius))
+    elif kind in (geo.KIND_PACKED_CYL, geo.KIND_PACKED_SPH):
+        min_r = float(np.min(geometry._radii_np))
+        eps = float(f32(1e-7 * min_r))
+        nudge = float(f32(1e-4 * min_r))
+        cell = geometry._L_float
+        centers = np.ascontiguousarray(geometry._centers_np, dtype=np.float32)
+        radii = np.ascontiguousarray(geometry._radii_np, dtype=np.float32)
+    else:
+        eps = nudge = 0.0
+
+    return WalkPlan(kind=kind, wall=wall, n_sub=int(n_sub), size=float(f32(size)),
+                    cell=float(f32(cell)), eps=eps, nudge=nudge,
+                    rho_over_D=rho_over_D, kappa_over_D=kappa_over_D,
+                    step_len=step_len, gamma_dt_sub=gamma_dt_sub, dt_sub=float(dt_sub),
+                    inv_T2=inv_T2, inv_T1=inv_T1, has_t2=has_t2, has_t1=has_t1,
+                    has_weight=bool(has_surf or has_perm or has_t2 or has_t1),
+                    centers=centers, radii=radii, rot=_rotation_of(geometry))

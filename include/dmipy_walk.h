@@ -1,0 +1,190 @@
+/*
+ * dmipy_walk.h -- C ABI of the MI355X Monte-Carlo spin walk.
+ *
+ * The reference has no FFI: its seam is the Python call
+ *   simulate(n_walkers, diffusivity, waveform, geometry, seed, T2, T1, ...)
+ *     dmipy_sim/core.py:193-762
+ *   simulate_cpmg(n_walkers, diffusivity, waveform, geometry, T2, seed, ...)
+ *     dmipy_sim/core.py:861-969
+ * whose hot part is the jax.vmap(walker) . lax.scan(n_t) . lax.scan(n_sub) walk
+ * (core.py:700-727, body physics.py:293-415) over the geometry methods
+ * reflect / reflect_with_log_weight / permeate / classify_position
+ * (geometries.py:93-139, 197-424, 516-813, 1746-2214, 2445-2771) and the ensemble
+ * reduction _ens (core.py:426-429).  Everything from the walker keys to _ens is
+ * replaced by the entry points below; the Python layer (dmipy_sim_amd) keeps the
+ * reference's call signatures and calls them through ctypes.
+ *
+ * Conventions
+ *  - Plain C types only; all pointers in the descriptor structs are HOST pointers
+ *    owned by the caller for the duration of the call, except where a field says
+ *    "device".
+ *  - Vectors are in the geometry's native frame (cylinder axis = z); the caller
+ *    rotates lab vectors in and out.
+ *  - Every entry point returns 0 on success or a dmw_status code; the message of
+ *    the last failure on the calling thread is dmw_last_error().  DMW_ENOMEM
+ *    messages contain "out of memory" so the reference's OOM back-off
+ *    (core.py:366-394) keys on them unchanged.
+ *  - Calls on one thread are serialised on one HIP stream per thread; the library
+ *    allocates device memory itself and frees it before returning (plans keep
+ *    theirs until dmw_plan_destroy).
+ */
+#ifndef DMIPY_WALK_H
+#define DMIPY_WALK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum dmw_geom_kind {
+  DMW_FREE = 0,              /* FreeDiffusion      geometries.py:34-45     */
+  DMW_BOX1D = 1,             /* Box1D              geometries.py:48-140    */
+  DMW_CYLINDER = 2,          /* Cylinder           geometries.py:427-844   */
+  DMW_SPHERE = 3,            /* Sphere             geometries.py:143-424   */
+  DMW_PACKED_CYLINDERS = 4,  /* PackedCylinders    geometries.py:1604-2242 */
+  DMW_PACKED_SPHERES = 5     /* PackedSpheres      geometries.py:2343-2786 */
+};
+
+enum dmw_wall {
+  DMW_WALL_REFLECT = 0,      /* geometry.reflect                 physics.py:360-415 */
+  DMW_WALL_SURFACE = 1,      /* geometry.reflect_with_log_weight physics.py:319-358 */
+  DMW_WALL_PERMEATE = 2      /* geometry.permeate                physics.py:276-317 */
+};
+
+enum dmw_status {
+  DMW_OK = 0,
+  DMW_EINVAL = 1,            /* bad argument              -> ValueError          */
+  DMW_EUNSUPPORTED = 2,      /* unsupported combination   -> NotImplementedError */
+  DMW_ENOMEM = 3,            /* hipErrorOutOfMemory       -> RuntimeError("out of memory") */
+  DMW_ENODEV = 4,            /* no usable GPU             -> RuntimeError        */
+  DMW_EHIP = 5               /* any other HIP failure     -> RuntimeError        */
+};
+
+/* Geometry and wall constants (float32 values rounded as the reference rounds
+ * them, see make_step_fn physics.py:229-290 and the EPS/NUDGE definitions
+ * geometries.py:503-504, 1694-1696). */
+typedef struct dmw_geometry {
+  int32_t kind;              /* enum dmw_geom_kind */
+  int32_t wall;              /* enum dmw_wall */
+  int32_t n_obj;             /* packed substrates: number of objects */
+  float size;                /* Cylinder/Sphere radius, Box1D length */
+  float cell;                /* packed substrates: periodic cell side L */
+  float eps;                 /* EPS_detect */
+  float nudge;               /* NUDGE */
+  float rho_over_D;          /* surface relaxivity / D (0: none) */
+  float kappa_over_D;        /* permeability / D */
+  const float* centers;      /* (n_obj, 2) cylinders or (n_obj, 3) spheres */
+  const float* radii;        /* (n_obj) */
+} dmw_geometry;
+
+/* Gradient waveform, Waveform.G of dmipy_sim/waveforms.py:39-90. */
+typedef struct dmw_waveform {
+  int32_t n_meas;
+  int32_t n_t;
+  const float* G;            /* (n_meas, n_t, 3) T/m */
+  const float* chi;          /* (n_t) transverse-coherence flag, or NULL = all 1 */
+  int32_t n_echo;            /* > 0: sample the ensemble at echo_idx (CPMG) */
+  const int32_t* echo_idx;   /* (n_echo) sample indices, ascending */
+} dmw_waveform;
+
+/* Walker set and step constants of one walk. */
+typedef struct dmw_run {
+  int64_t n_walkers;
+  uint64_t seed;             /* Philox4x32-10 key */
+  int32_t n_sub;             /* physical sub-steps per waveform sample */
+  float step_len;            /* sqrt(6 D dt_sub) */
+  float gamma_dt_sub;        /* GAMMA * dt / n_sub */
+  float dt_sub;
+  float inv_T2;              /* 1/T2 (has_t2) */
+  float inv_T1;              /* 1/T1 (has_t1) */
+  int32_t has_t2;
+  int32_t has_t1;
+  int32_t kahan;             /* 1: compensated (Kahan) phase accumulation */
+  int32_t meas_block;        /* measurements per kernel pass: 0 = automatic */
+  const float* r0;           /* (n_walkers, 3) start positions, or NULL = sample in-kernel */
+  const float* inj_normals;  /* (n_walkers, n_t*n_sub, 3) raw normal triples, or NULL = Philox */
+  const float* inj_uniforms; /* (n_walkers, n_t*n_sub) Powles uniforms (permeable walls) */
+} dmw_run;
+
+/* Outputs.  signal is required; every other array is optional (NULL). */
+typedef struct dmw_result {
+  double* signal;            /* (n_meas) or (n_echo, n_meas): Re <w e^{i phi}> */
+  double* signal_im;         /* same shape: Im <w e^{i phi}> */
+  float* final_pos;          /* (n_walkers, 3) unfolded, native frame */
+  int32_t* comp_origin;      /* (n_walkers) compartment at t = 0 */
+  int32_t* comp_final;       /* (n_walkers) compartment after the last sample */
+  int32_t* comp_full;        /* (n_walkers, n_t) compartment after every sample */
+  float* pos_full;           /* (n_walkers, n_t, 3) position after every sample */
+  uint32_t* n_cross;         /* (n_walkers) membrane transmissions */
+  float* log_w;              /* (n_walkers) accumulated log-weight */
+  float* phi;                /* (n_walkers, n_meas) accumulated phase */
+  double kernel_ms;          /* out: time of the walk kernel(s) on the device */
+  uint64_t n_hits;           /* out: wall events (reflections + transmissions) */
+  uint64_t n_cand;           /* out: candidate objects tested (packed substrates) */
+  int32_t want_counters;     /* in: 1 = fill n_hits / n_cand */
+} dmw_result;
+
+/* Number of visible HIP devices (0 when none). */
+int dmw_device_count(void);
+
+/* Select the device used by this thread's calls (default 0). */
+int dmw_set_device(int device);
+
+/* Message of the last failure on the calling thread. */
+const char* dmw_last_error(void);
+
+/* One complete walk: upload, walk, reduce, download.
+ * Replaces the fused engine of simulate() (core.py:572-762) and, with
+ * wave->n_echo > 0, the echo sampling of simulate_cpmg() (core.py:937-969). */
+int dmw_walk(const dmw_geometry* geom, const dmw_waveform* wave,
+             const dmw_run* run, dmw_result* out);
+
+/* A walk whose inputs stay resident in device memory across executions. */
+typedef struct dmw_plan dmw_plan;
+
+int dmw_plan_create(dmw_plan** plan, const dmw_geometry* geom,
+                    const dmw_waveform* wave, const dmw_run* run);
+/* Run the planned walk with a new seed; inputs are already in HBM. */
+int dmw_plan_execute(dmw_plan* plan, uint64_t seed, dmw_result* out);
+int dmw_plan_destroy(dmw_plan* plan);
+
+/* ---- Batched persistent launch (many substrate instances, one waveform) ----
+ * Has no single counterpart in the reference, where each instance is a separate
+ * simulate() call (core.py:193) -- e.g. the serial loop of simulate_mixture
+ * (core.py:821-858).  Instances may be Cylinder or Sphere (walls reflect, relax or
+ * permeate); every instance shares the waveform. */
+typedef struct dmw_instance {
+  dmw_geometry geom;         /* kind DMW_CYLINDER or DMW_SPHERE */
+  int64_t n_walkers;
+  int32_t n_sub;
+  float step_len;
+  float gamma_dt_sub;
+  float dt_sub;
+  float inv_T2;
+  int32_t has_t2;
+} dmw_instance;
+
+typedef struct dmw_batch dmw_batch;
+
+typedef struct dmw_batch_stats {
+  double kernel_ms;          /* walk kernel time */
+  double total_ms;           /* whole graph launch */
+  uint64_t walker_steps;     /* sum over instances of N * n_t * n_sub */
+  int32_t n_units;           /* work units (64-walker tiles) */
+  int32_t grid;              /* persistent workgroups */
+} dmw_batch_stats;
+
+int dmw_batch_create(dmw_batch** batch, int32_t n_inst, const dmw_instance* inst,
+                     const dmw_waveform* shared, int32_t waves_per_simd,
+                     int32_t use_graph);
+/* signals: (n_inst, n_meas) Re <w e^{i phi}> per instance. */
+int dmw_batch_execute(dmw_batch* batch, uint64_t seed, double* signals,
+                      dmw_batch_stats* stats);
+int dmw_batch_destroy(dmw_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMIPY_WALK_H */

@@ -1,0 +1,203 @@
+"""Python side of the CPU oracle (test infrastructure only).
+
+The oracle checks the HIP walk; it is imported only by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+
+``oracle_plan`` restates, independently of the dmipy_sim_amd package, how the
+reference turns (geometry, D, dt, T2, T1) into the step constants:
+  sub-step rules     dmipy_sim/physics.py:15-48 (R/25 permeable), 51-69 + 163-188
+                     (R_char/8 surface), 72-95 (collision, 1 here)
+  step length        physics.py:256-260   sqrt(f32(6 D) * f32(dt/n_sub))
+  gamma*dt_sub       physics.py:288-290   f32(GAMMA*dt/n_sub)
+  rho/D, kappa/D     physics.py:279-281 (permeable: f32(x/D)), 322-328
+                     (surface: f32(rho)/f32(D))
+  EPS / NUDGE        geometries.py:503-504, 1694-1696
+``run_oracle`` calls walk_oracle.c, which restates the step itself.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+GAMMA = 267.513e6
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle_walk.so")
+
+KIND = {"FreeDiffusion": 0, "Box1D": 1, "Cylinder": 2, "Sphere": 3,
+        "PackedCylinders": 4, "PackedSpheres": 5}
+W_REFLECT, W_SURFACE, W_PERMEATE = 0, 1, 2
+
+_fp = C.POINTER(C.c_float)
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+
+
+class OracleArgs(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("wall", C.c_int32), ("n_obj", C.c_int32),
+                ("size", C.c_float), ("cell", C.c_float), ("eps", C.c_float),
+                ("nudge", C.c_float), ("rho_over_D", C.c_float), ("kappa_over_D", C.c_float),
+                ("centers", _fp), ("radii", _fp), ("n_sub", C.c_int32),
+                ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
+                ("inv_T2", C.c_float), ("inv_T1", C.c_float), ("has_t2", C.c_int32),
+                ("has_t1", C.c_int32), ("n_t", C.c_int32), ("n_meas", C.c_int32),
+                ("G", _fp), ("chi", _fp), ("n_echo", C.c_int32), ("echo_idx", _ip),
+                ("n_walkers", C.c_int64), ("seed", C.c_uint64), ("r0", _fp),
+                ("inj_n", _fp), ("inj_u", _fp), ("cell_form", C.c_int32),
+                ("phase64", C.c_int32), ("threads", C.c_int32),
+                ("walker_begin", C.c_int64), ("walker_end", C.c_int64)]
+
+
+class OracleOut(C.Structure):
+    _fields_ = [("signal", _dp), ("signal_im", _dp), ("final_pos", _fp),
+                ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
+                ("n_cross", _up), ("log_w", _fp), ("phi", _fp), ("n_hits", C.c_uint64)]
+
+
+_lib = None
+
+
+def build():
+    """Compile walk_oracle.c (make -B in oracle/)."""
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        h = C.CDLL(LIB)
+        h.oracle_walk.restype = C.c_int
+        h.oracle_walk.argtypes = [C.POINTER(OracleArgs), C.POINTER(OracleOut)]
+        h.oracle_philox4x32_10.restype = None
+        h.oracle_philox4x32_10.argtypes = [_up, _up, _up]
+        _lib = h
+    return _lib
+
+
+def philox4x32_10(ctr, key):
+    c = np.ascontiguousarray(ctr, dtype=np.uint32)
+    k = np.ascontiguousarray(key, dtype=np.uint32)
+    o = np.zeros(4, np.uint32)
+    lib().oracle_philox4x32_10(c.ctypes.data_as(_up), k.ctypes.data_as(_up), o.ctypes.data_as(_up))
+    return o
+
+
+f32 = np.float32
+
+
+def _radius(g):
+    """Smallest geometric radius (physics.py:15-32)."""
+    for attr in ("radius", "sphere_radius"):
+        v = getattr(g, attr, None)
+        if v is not None:
+            return float(v)
+    radii = getattr(g, "_radii_np", None)
+    if radii is not None and len(radii) > 0:
+        return float(np.min(radii))
+    return None
+
+
+def oracle_plan(geom, D, dt, T2=None, T1=None):
+    """Step constants of one walk, restated from the reference (see module doc)."""
+    name = type(geom).__name__
+    kind = KIND[name]
+    has_perm = getattr(geom, "permeability", None) is not None
+    has_surf = getattr(geom, "surface_relaxivity_t2", None) is not None
+    R = _radius(geom)
+    if has_perm:
+        wall = W_PERMEATE
+        n_sub = 1 if R is None else max(1, math.ceil(dt / (R * R / (3750.0 * D))))
+        kD = float(f32(geom.permeability / D))
+        rD = float(f32(geom.surface_relaxivity_t2 / D)) if has_surf else 0.0
+    elif has_surf:
+        wall = W_SURFACE
+        frac = getattr(geom, "surface_substep_frac", None)
+        frac = 8.0 if frac is None else frac
+        if not frac or frac <= 0 or R is None:
+            n_sub = 1
+        else:
+            n_sub = max(1, math.ceil(dt / ((R / frac) ** 2 / (6.0 * D))))
+        kD = 0.0
+        rD = float(f32(f32(geom.surface_relaxivity_t2) / f32(D)))
+    else:
+        wall, n_sub, kD, rD = W_REFLECT, 1, 0.0, 0.0
+    if kind == 0:
+        wall = W_REFLECT
+    dt_sub = f32(dt / n_sub)
+    step = float(np.sqrt(f32(f32(6.0 * D) * dt_sub)))
+    p = dict(kind=kind, wall=wall, n_sub=int(n_sub), rho_over_D=rD, kappa_over_D=kD,
+             step_len=step, gamma_dt_sub=float(f32(GAMMA * (dt / n_sub))), dt_sub=float(dt_sub),
+             inv_T2=float(f32(1.0 / T2)) if T2 is not None else 0.0,
+             inv_T1=float(f32(1.0 / T1)) if T1 is not None else 0.0,
+             has_t2=T2 is not None, has_t1=T1 is not None, size=0.0, cell=0.0,
+             eps=0.0, nudge=0.0, centers=None, radii=None)
+    if kind == 1:
+        p["size"] = float(f32(geom.length))
+    elif kind in (2, 3):
+        p["size"] = float(f32(geom.radius))
+        p["eps"] = float(f32(1e-7 * geom.radius))
+        p["nudge"] = float(f32(1e-4 * geom.radius))
+    elif kind in (4, 5):
+        rmin = float(np.min(geom._radii_np))
+        p["eps"] = float(f32(1e-7 * rmin))
+        p["nudge"] = float(f32(1e-4 * rmin))
+        p["cell"] = float(f32(geom._L_float))
+        p["centers"] = np.ascontiguousarray(geom._centers_np, dtype=np.float32)
+        p["radii"] = np.ascontiguousarray(geom._radii_np, dtype=np.float32)
+    return p
+
+
+def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
+               inject_normals=None, inject_uniforms=None, cell_form=1, phase64=False,
+               threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None):
+    """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame)."""
+    p = plan if plan is not None else oracle_plan(geom, D, waveform.dt, T2, T1)
+    G = np.ascontiguousarray(waveform.G if G_native is None else G_native, dtype=np.float32)
+    n_meas, n_t, _ = G.shape
+    N = int(n_walkers)
+    chi = getattr(waveform, "chi_perp", None)
+    chi = None if chi is None else np.ascontiguousarray(chi, dtype=np.float32).reshape(n_t)
+    echo = None if echo_indices is None else np.ascontiguousarray(echo_indices, dtype=np.int32)
+    r0 = None if r0 is None else np.ascontiguousarray(r0, dtype=np.float32).reshape(N, 3)
+    inj_n = None if inject_normals is None else np.ascontiguousarray(inject_normals, dtype=np.float32)
+    inj_u = None if inject_uniforms is None else np.ascontiguousarray(inject_uniforms, dtype=np.float32)
+    fp = lambda a: None if a is None else a.ctypes.data_as(_fp)
+    w0, w1 = (0, 0) if walker_range is None else walker_range
+    A = OracleArgs(kind=p["kind"], wall=p["wall"],
+                   n_obj=0 if p["radii"] is None else len(p["radii"]),
+                   size=p["size"], cell=p["cell"], eps=p["eps"], nudge=p["nudge"],
+                   rho_over_D=p["rho_over_D"], kappa_over_D=p["kappa_over_D"],
+                   centers=fp(p["centers"]), radii=fp(p["radii"]), n_sub=p["n_sub"],
+                   step_len=p["step_len"], gamma_dt_sub=p["gamma_dt_sub"], dt_sub=p["dt_sub"],
+                   inv_T2=p["inv_T2"], inv_T1=p["inv_T1"], has_t2=int(p["has_t2"]),
+                   has_t1=int(p["has_t1"]), n_t=n_t, n_meas=n_meas, G=fp(G), chi=fp(chi),
+                   n_echo=0 if echo is None else len(echo),
+                   echo_idx=None if echo is None else echo.ctypes.data_as(_ip),
+                   n_walkers=N, seed=int(seed) & (2**64 - 1), r0=fp(r0), inj_n=fp(inj_n),
+                   inj_u=fp(inj_u), cell_form=int(cell_form), phase64=int(bool(phase64)),
+                   threads=int(threads), walker_begin=int(w0), walker_end=int(w1))
+    rows = 1 if echo is None else len(echo)
+    res = {"signal": np.zeros(rows * n_meas), "signal_im": np.zeros(rows * n_meas),
+           "final_pos": np.zeros((N, 3), np.float32), "comp_origin": np.zeros(N, np.int32),
+           "comp_final": np.zeros(N, np.int32), "n_cross": np.zeros(N, np.uint32),
+           "log_w": np.zeros(N, np.float32), "phi": np.zeros((N, n_meas), np.float32)}
+    O = OracleOut(signal=res["signal"].ctypes.data_as(_dp),
+                  signal_im=res["signal_im"].ctypes.data_as(_dp),
+                  final_pos=fp(res["final_pos"]),
+                  comp_origin=res["comp_origin"].ctypes.data_as(_ip),
+                  comp_final=res["comp_final"].ctypes.data_as(_ip), comp_full=None,
+                  n_cross=res["n_cross"].ctypes.data_as(_up), log_w=fp(res["log_w"]),
+                  phi=fp(res["phi"]))
+    rc = lib().oracle_walk(C.byref(A), C.byref(O))
+    if rc != 0:
+        raise RuntimeError("oracle_walk failed")
+    if echo is not None:
+        res["signal"] = res["signal"].reshape(rows, n_meas)
+        res["signal_im"] = res["signal_im"].reshape(rows, n_meas)
+    res["n_hits"] = int(O.n_hits)
+    res["n_sub"] = p["n_sub"]
+    return res

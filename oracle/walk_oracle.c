@@ -1,0 +1,678 @@
+/*
+ * walk_oracle.c -- CPU restatement of dmipy-sim's fused Monte-Carlo walk.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker of the HIP walk: it is
+ * built and called by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+ * of bench.py, never by the dmipy_sim_amd package.
+ *
+ * It restates, operation by operation in float32, the reference's step:
+ *   sub-step body          dmipy_sim/physics.py:296-314 (permeable),
+ *                          337-355 (surface), 371-387 (T2/T1), 402-412 (plain)
+ *   FreeDiffusion          dmipy_sim/geometries.py:40-41
+ *   Box1D reflect(_w)      dmipy_sim/geometries.py:93-139
+ *   Sphere reflect(_w)     dmipy_sim/geometries.py:197-317
+ *   Sphere permeate        dmipy_sim/geometries.py:320-424
+ *   Cylinder reflect(_w)   dmipy_sim/geometries.py:516-654
+ *   Cylinder permeate      dmipy_sim/geometries.py:656-801
+ *   classify_position      dmipy_sim/geometries.py:180-184, 803-813, 2185-2214, 2758-2771
+ *   PackedCyl. reflect(_w) dmipy_sim/geometries.py:1746-2016
+ *   PackedCyl. permeate    dmipy_sim/geometries.py:2018-2183
+ *   PackedSph. reflect(_w) dmipy_sim/geometries.py:2445-2631
+ *   PackedSph. permeate    dmipy_sim/geometries.py:2633-2756
+ *   ensemble               dmipy_sim/core.py:426-429
+ * with the reference's cost structure: the fixed 10-iteration bounce scan,
+ * every object of a packing tested on every sub-step (plus the all-object
+ * safety-clamp pass), and the phase of every measurement advanced on every
+ * sub-step with the float32 right-endpoint rule phi += gamma*dt_sub*(g.r_new).
+ *
+ * Choices the reference leaves open (its XLA lowering decides them) are fixed
+ * here: 3-term sums are ((a+b)+c), the per-bounce d_perp values are summed in
+ * iteration order, no contraction into FMA (build with -ffp-contract=off).
+ *
+ * Random numbers: the reference's threefry stream cannot be reproduced without
+ * JAX; the oracle consumes either injected raw normal triples / Powles uniforms
+ * (the reference's increments), or a Philox4x32-10 stream (Salmon et al., SC'11)
+ * with the isotropic-direction construction documented in oracle_direction().
+ *
+ * Position forms for packed substrates: 0 = unfolded float32 coordinates, as
+ * the reference keeps them (geometries.py:1843-1849); 1 = integer cell index +
+ * float32 offset folded to the primary cell after every waveform sample.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct {
+  int32_t kind, wall, n_obj;
+  float size, cell, eps, nudge, rho_over_D, kappa_over_D;
+  const float* centers;
+  const float* radii;
+  int32_t n_sub;
+  float step_len, gamma_dt_sub, dt_sub, inv_T2, inv_T1;
+  int32_t has_t2, has_t1;
+  int32_t n_t, n_meas;
+  const float* G;          /* (n_meas, n_t, 3) */
+  const float* chi;        /* (n_t) or NULL */
+  int32_t n_echo;
+  const int32_t* echo_idx; /* (n_echo) */
+  int64_t n_walkers;
+  uint64_t seed;
+  const float* r0;         /* (N, 3) or NULL: Philox start sampler */
+  const float* inj_n;      /* (N, S, 3) or NULL */
+  const float* inj_u;      /* (N, S) or NULL */
+  int32_t cell_form;
+  int32_t phase64;         /* 1: phase accumulated in float64 instead of float32 */
+  int32_t threads;
+  int64_t walker_begin;    /* sub-range of walkers to run (cpu timing slices) */
+  int64_t walker_end;      /* 0 = all */
+} oracle_args;
+
+typedef struct {
+  double* signal;          /* (rows, n_meas) */
+  double* signal_im;
+  float* final_pos;        /* (N, 3) unfolded */
+  int32_t* comp_origin;
+  int32_t* comp_final;
+  int32_t* comp_full;      /* (N, n_t) */
+  uint32_t* n_cross;
+  float* log_w;
+  float* phi;              /* (N, n_meas) */
+  uint64_t n_hits;
+} oracle_out;
+
+enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5 };
+enum { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
+
+/* ---------------------------------------------------------------- Philox */
+static void philox10(const uint32_t in[4], uint32_t k0, uint32_t k1, uint32_t out[4]) {
+  uint32_t v[4] = {in[0], in[1], in[2], in[3]};
+  for (int round = 0; round < 10; ++round) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * v[0];
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * v[2];
+    const uint32_t a = (uint32_t)(p1 >> 32) ^ v[1] ^ k0;
+    const uint32_t b = (uint32_t)p1;
+    const uint32_t c = (uint32_t)(p0 >> 32) ^ v[3] ^ k1;
+    const uint32_t d = (uint32_t)p0;
+    v[0] = a; v[1] = b; v[2] = c; v[3] = d;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  memcpy(out, v, sizeof v);
+}
+
+void oracle_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  philox10(ctr, key[0], key[1], out);
+}
+
+static float unit_u(uint32_t b) { return (float)(b >> 8) * (1.0f / 16777216.0f); }
+
+/* Isotropic unit vector from two 32-bit words: z = 1 - 2u (uniform on (-1,1]),
+ * azimuth = 2*pi*v; sin/cos(pi*f) on |f| <= 1/4 by their Taylor polynomials
+ * (degree 9 / 10, float32 coefficients (-1)^k pi^n / n!), Horner with fmaf. */
+static void oracle_direction(uint32_t a, uint32_t b, float d[3]) {
+  const float z = 1.0f - 2.0f * unit_u(a);
+  const float s = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+  const float t = 2.0f * unit_u(b);
+  const float q = rintf(2.0f * t);
+  const float f = t - 0.5f * q;
+  const float x = f * f;
+  float ps = fmaf(x, 0.08214588463306427f, -0.5992645025253296f);
+  ps = fmaf(x, ps, 2.550163984298706f);
+  ps = fmaf(x, ps, -5.167712688446045f);
+  ps = fmaf(x, ps, 3.1415927410125732f);
+  const float sf = f * ps;
+  float pc = fmaf(x, -0.025806890800595284f, 0.23533062636852264f);
+  pc = fmaf(x, pc, -1.3352627754211426f);
+  pc = fmaf(x, pc, 4.058712005615234f);
+  pc = fmaf(x, pc, -4.934802055358887f);
+  const float cf = fmaf(x, pc, 1.0f);
+  float c, sn;
+  switch (((int)q) & 3) {
+    case 0: c = cf; sn = sf; break;
+    case 1: c = -sf; sn = cf; break;
+    case 2: c = -cf; sn = -sf; break;
+    default: c = sf; sn = -cf; break;
+  }
+  d[0] = s * c;
+  d[1] = s * sn;
+  d[2] = z;
+}
+
+/* ------------------------------------------------------------- helpers */
+static float dot3(const float a[3], const float b[3]) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static float norm3(const float a[3]) { return sqrtf(dot3(a, a)); }
+
+/* x - L*floor(x/L + 0.5) component (min-image / fold), reference form. */
+static float mi(float q, float L) { return q - L * floorf(q / L + 0.5f); }
+
+/* -------------------------------------------------------------- Box1D */
+static float box_step(float r[3], const float st[3], float L, float rhoD, int surface, uint32_t* hits) {
+  const float xr = r[0] + st[0];
+  float dlog = 0.0f;
+  if (surface) {
+    const float dperp = fmaxf(xr - L, 0.0f) + fmaxf(-xr, 0.0f);
+    if (dperp > 0.0f) { dlog = -2.0f * rhoD * dperp; (*hits)++; }
+  }
+  const float L2 = 2.0f * L;
+  float m = fmodf(xr, L2);                      /* jnp.mod = lax.rem + sign fix */
+  if (m != 0.0f && ((m < 0.0f) != (L2 < 0.0f))) m = m + L2;
+  if (m > L) m = L2 - m;
+  r[0] = m;
+  r[1] = r[1] + st[1];
+  r[2] = r[2] + st[2];
+  return dlog;
+}
+
+/* ---------------------------------------------- Cylinder (interior, xy) */
+static float cyl_reflect(float r[3], const float st[3], const oracle_args* A, uint32_t* hits) {
+  const float R = A->size, EPS = A->eps, NUDGE = A->nudge;
+  const float sxy[2] = {st[0], st[1]};
+  const float sl = sqrtf(sxy[0] * sxy[0] + sxy[1] * sxy[1]);
+  float d2[2] = {0.0f, 0.0f};
+  if (sl > 0.0f) { d2[0] = sxy[0] / sl; d2[1] = sxy[1] / sl; }
+  float p2[2] = {r[0], r[1]};
+  float rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {             /* fixed-length scan */
+    const float dp = d2[0] * p2[0] + d2[1] * p2[1];
+    const float disc = fmaxf(dp * dp - ((p2[0] * p2[0] + p2[1] * p2[1]) - R * R), 0.0f);
+    const float d = -dp + sqrtf(disc);
+    const int hit = (d > EPS) && (d < rem) && (sl > 0.0f);
+    const float h[2] = {p2[0] + d * d2[0], p2[1] + d * d2[1]};
+    const float n[2] = {h[0] / R, h[1] / R};
+    const float dd = 2.0f * (d2[0] * n[0] + d2[1] * n[1]);
+    float rf[2] = {d2[0] - dd * n[0], d2[1] - dd * n[1]};
+    const float rn = sqrtf(rf[0] * rf[0] + rf[1] * rf[1]);
+    rf[0] = rf[0] / rn; rf[1] = rf[1] / rn;
+    const float cosa = sqrtf(disc) / R;
+    const float dperp = hit ? (rem - d) * cosa : 0.0f;
+    if (hit) {
+      p2[0] = h[0] - NUDGE * n[0]; p2[1] = h[1] - NUDGE * n[1];
+      d2[0] = rf[0]; d2[1] = rf[1];
+      rem = rem - d - NUDGE;
+      (*hits)++;
+    }
+    dsum = dsum + dperp;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float xy[2] = {p2[0] + d2[0] * rm, p2[1] + d2[1] * rm};
+  const float nn = sqrtf(xy[0] * xy[0] + xy[1] * xy[1]);
+  if (nn >= R) { xy[0] = xy[0] * (R - NUDGE) / nn; xy[1] = xy[1] * (R - NUDGE) / nn; }
+  r[0] = xy[0]; r[1] = xy[1]; r[2] = r[2] + st[2];
+  return dsum;
+}
+
+static float cyl_permeate(float r[3], const float st[3], float u, const oracle_args* A,
+                          uint32_t* hits, uint32_t* ncross) {
+  const float R = A->size, EPS = A->eps, NUDGE = A->nudge;
+  const float p2[2] = {r[0], r[1]};
+  const float sxy[2] = {st[0], st[1]};
+  const float sl = sqrtf(sxy[0] * sxy[0] + sxy[1] * sxy[1]);
+  float dh[2] = {0.0f, 0.0f};
+  if (sl > 0.0f) { dh[0] = sxy[0] / sl; dh[1] = sxy[1] / sl; }
+  const float dp = dh[0] * p2[0] + dh[1] * p2[1];
+  const float disc = dp * dp - ((p2[0] * p2[0] + p2[1] * p2[1]) - R * R);
+  const float disc_s = fmaxf(disc, 0.0f);
+  const int inside = (p2[0] * p2[0] + p2[1] * p2[1]) < R * R;
+  const float t_exit = -dp + sqrtf(disc_s), t_entry = -dp - sqrtf(disc_s);
+  const float t_hit = inside ? t_exit : t_entry;
+  const int any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
+  const float ts = any_hit ? t_hit : 0.0f;
+  const float hr[2] = {p2[0] + ts * dh[0], p2[1] + ts * dh[1]};
+  const float hl = sqrtf(hr[0] * hr[0] + hr[1] * hr[1]);
+  const float n[2] = {hr[0] / fmaxf(hl, 1e-30f), hr[1] / fmaxf(hl, 1e-30f)};
+  const float hs[2] = {R * n[0], R * n[1]};
+  const float remaining = sl - ts;
+  const float cosa = sqrtf(disc_s) / R;
+  const float dperp = any_hit ? remaining * cosa : 0.0f;
+  const float pt = fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+  const int transmit = any_hit && (u < pt);
+  const float straight[2] = {p2[0] + sxy[0], p2[1] + sxy[1]};
+  const float dd = 2.0f * (dh[0] * n[0] + dh[1] * n[1]);
+  float rf[2] = {dh[0] - dd * n[0], dh[1] - dd * n[1]};
+  const float rn = sqrtf(rf[0] * rf[0] + rf[1] * rf[1]);
+  rf[0] = rf[0] / rn; rf[1] = rf[1] / rn;
+  const float nd[2] = {inside ? -n[0] : n[0], inside ? -n[1] : n[1]};
+  const float nud[2] = {hs[0] + NUDGE * nd[0], hs[1] + NUDGE * nd[1]};
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float rr[2] = {nud[0] + rf[0] * go, nud[1] + rf[1] * go};
+  const float rrn = sqrtf(rr[0] * rr[0] + rr[1] * rr[1]);
+  const float rrs = fmaxf(rrn, 1e-30f);
+  const float target = inside ? R - NUDGE : R + NUDGE;
+  const int wrong = inside ? (rrn >= R) : (rrn <= R);
+  if (wrong) { rr[0] = rr[0] * target / rrs; rr[1] = rr[1] * target / rrs; }
+  float xy[2];
+  if (any_hit) {
+    xy[0] = transmit ? straight[0] : rr[0];
+    xy[1] = transmit ? straight[1] : rr[1];
+    (*hits)++;
+    if (transmit) (*ncross)++;
+  } else {
+    xy[0] = p2[0] + sxy[0]; xy[1] = p2[1] + sxy[1];
+  }
+  r[0] = xy[0]; r[1] = xy[1]; r[2] = r[2] + st[2];
+  return (any_hit && !transmit) ? -2.0f * A->rho_over_D * dperp : 0.0f;
+}
+
+/* ------------------------------------------------ Sphere (interior, 3D) */
+static float sph_reflect(float r[3], const float st[3], const oracle_args* A, uint32_t* hits) {
+  const float R = A->size, EPS = A->eps, NUDGE = A->nudge;
+  const float sl = norm3(st);
+  float dh[3] = {st[0] / sl, st[1] / sl, st[2] / sl};
+  float p[3] = {r[0], r[1], r[2]};
+  float rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {
+    const float dp = dot3(dh, p);
+    const float disc = fmaxf(dp * dp - (dot3(p, p) - R * R), 0.0f);
+    const float d = -dp + sqrtf(disc);
+    const int hit = (d > EPS) && (d < rem);
+    const float h[3] = {p[0] + d * dh[0], p[1] + d * dh[1], p[2] + d * dh[2]};
+    const float n[3] = {h[0] / R, h[1] / R, h[2] / R};
+    const float dd = 2.0f * dot3(dh, n);
+    float rf[3] = {dh[0] - dd * n[0], dh[1] - dd * n[1], dh[2] - dd * n[2]};
+    const float rn = norm3(rf);
+    rf[0] = rf[0] / rn; rf[1] = rf[1] / rn; rf[2] = rf[2] / rn;
+    const float dperp = hit ? (rem - d) * (sqrtf(disc) / R) : 0.0f;
+    if (hit) {
+      for (int c = 0; c < 3; ++c) { p[c] = h[c] - NUDGE * n[c]; dh[c] = rf[c]; }
+      rem = rem - d - NUDGE;
+      (*hits)++;
+    }
+    dsum = dsum + dperp;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float o[3] = {p[0] + dh[0] * rm, p[1] + dh[1] * rm, p[2] + dh[2] * rm};
+  const float nn = norm3(o);
+  if (nn >= R) for (int c = 0; c < 3; ++c) o[c] = o[c] * (R - NUDGE) / nn;
+  r[0] = o[0]; r[1] = o[1]; r[2] = o[2];
+  return dsum;
+}
+
+static float sph_permeate(float r[3], const float st[3], float u, const oracle_args* A,
+                          uint32_t* hits, uint32_t* ncross) {
+  const float R = A->size, EPS = A->eps, NUDGE = A->nudge;
+  const float sl = norm3(st);
+  const float dh[3] = {st[0] / sl, st[1] / sl, st[2] / sl};
+  const float dp = dot3(dh, r);
+  const float disc = dp * dp - (dot3(r, r) - R * R);
+  const float disc_s = fmaxf(disc, 0.0f);
+  const int inside = dot3(r, r) < R * R;
+  const float t_hit = inside ? (-dp + sqrtf(disc_s)) : (-dp - sqrtf(disc_s));
+  const int any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
+  const float ts = any_hit ? t_hit : 0.0f;
+  const float h[3] = {r[0] + ts * dh[0], r[1] + ts * dh[1], r[2] + ts * dh[2]};
+  const float n[3] = {h[0] / R, h[1] / R, h[2] / R};
+  const float remaining = sl - ts;
+  const float dperp = any_hit ? remaining * (sqrtf(disc_s) / R) : 0.0f;
+  const float pt = fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+  const int transmit = any_hit && (u < pt);
+  const float dd = 2.0f * dot3(dh, n);
+  float rf[3] = {dh[0] - dd * n[0], dh[1] - dd * n[1], dh[2] - dd * n[2]};
+  const float rn = norm3(rf);
+  for (int c = 0; c < 3; ++c) rf[c] = rf[c] / rn;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float out[3];
+  for (int c = 0; c < 3; ++c) {
+    const float nd = inside ? -n[c] : n[c];
+    const float refl = (h[c] + NUDGE * nd) + rf[c] * go;
+    const float straight = r[c] + st[c];
+    out[c] = any_hit ? (transmit ? straight : refl) : r[c] + st[c];
+  }
+  if (any_hit) { (*hits)++; if (transmit) (*ncross)++; }
+  r[0] = out[0]; r[1] = out[1]; r[2] = out[2];
+  return (any_hit && !transmit) ? -2.0f * A->rho_over_D * dperp : 0.0f;
+}
+
+/* ------------------------------------- packed substrates (brute force) */
+static void obj_center(const oracle_args* A, int k, int dim, float c[3]) {
+  c[0] = A->centers[k * dim];
+  c[1] = A->centers[k * dim + 1];
+  c[2] = dim == 3 ? A->centers[k * dim + 2] : 0.0f;
+}
+
+static float dotd(const float a[3], const float b[3], int dim) {
+  return dim == 2 ? a[0] * b[0] + a[1] * b[1] : dot3(a, b);
+}
+
+/* PackedCylinders.reflect / reflect_with_log_weight, PackedSpheres likewise.
+ * Returns d_perp of the hit (0 without hit). */
+static float packed_reflect(float r[3], const float st[3], const oracle_args* A, int dim, uint32_t* hits) {
+  const float L = A->cell, EPS = A->eps, NUDGE = A->nudge;
+  const float s[3] = {st[0], st[1], dim == 3 ? st[2] : 0.0f};
+  const float sl = sqrtf(dotd(s, s, dim));
+  float dh[3] = {0.0f, 0.0f, 0.0f};
+  if (sl > 0.0f) for (int c = 0; c < dim; ++c) dh[c] = s[c] / sl;
+  int imin = 0;
+  float tmin = INFINITY, disc_min = 0.0f;
+  for (int k = 0; k < A->n_obj; ++k) {          /* all objects, argmin first index */
+    float c[3], q[3] = {0, 0, 0};
+    obj_center(A, k, dim, c);
+    for (int d = 0; d < dim; ++d) q[d] = mi(r[d] - c[d], L);
+    const float Rk = A->radii[k];
+    const float dp = dotd(dh, q, dim);
+    const float disc = dp * dp - (dotd(q, q, dim) - Rk * Rk);
+    const float t = -dp - sqrtf(fmaxf(disc, 0.0f));
+    const int valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+    const float tv = valid ? t : INFINITY;
+    if (tv < tmin) { tmin = tv; imin = k; disc_min = disc; }
+  }
+  const int any_hit = isfinite(tmin);
+  const float ts = any_hit ? tmin : 0.0f;
+  float c[3], qc[3] = {0, 0, 0};
+  obj_center(A, imin, dim, c);
+  const float Rh = A->radii[imin];
+  for (int d = 0; d < dim; ++d) qc[d] = mi(r[d] - c[d], L);
+  float n[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) n[d] = (qc[d] + ts * dh[d]) / Rh;
+  const float dd = 2.0f * dotd(dh, n, dim);
+  float rf[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) rf[d] = dh[d] - dd * n[d];
+  const float rfn = sqrtf(dotd(rf, rf, dim));
+  for (int d = 0; d < dim; ++d) rf[d] = rfn > 0.0f ? rf[d] / fmaxf(rfn, 1e-30f) : 0.0f;
+  const float remaining = sl - ts;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float f[3];
+  for (int d = 0; d < dim; ++d) {
+    const float refl = ((r[d] + ts * dh[d]) + NUDGE * n[d]) + rf[d] * go;
+    f[d] = any_hit ? refl : r[d] + s[d];
+  }
+  /* safety clamp over all objects on the folded end point */
+  float fo[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) fo[d] = mi(f[d], L);
+  int kcl = 0;
+  float pmin = INFINITY, qcl[3] = {0, 0, 0};
+  for (int k = 0; k < A->n_obj; ++k) {
+    float ck[3], q[3] = {0, 0, 0};
+    obj_center(A, k, dim, ck);
+    for (int d = 0; d < dim; ++d) q[d] = mi(fo[d] - ck[d], L);
+    const float d2 = dotd(q, q, dim);
+    const float Rk = A->radii[k];
+    const float pen = d2 < Rk * Rk ? d2 / (Rk * Rk) : 1.0f;
+    if (pen < pmin) { pmin = pen; kcl = k; memcpy(qcl, q, sizeof q); }
+  }
+  const int inside_any = pmin < 1.0f;
+  const float dcl = sqrtf(dotd(qcl, qcl, dim));
+  const float scale = inside_any ? (A->radii[kcl] + NUDGE) / fmaxf(dcl, NUDGE) : 1.0f;
+  for (int d = 0; d < dim; ++d) f[d] = f[d] + qcl[d] * (scale - 1.0f);
+  r[0] = f[0]; r[1] = f[1];
+  r[2] = dim == 3 ? f[2] : r[2] + st[2];
+  if (any_hit) (*hits)++;
+  return any_hit ? remaining * (sqrtf(fmaxf(disc_min, 0.0f)) / Rh) : 0.0f;
+}
+
+static float packed_permeate(float r[3], const float st[3], float u, const oracle_args* A, int dim,
+                             uint32_t* hits, uint32_t* ncross) {
+  const float L = A->cell, EPS = A->eps, NUDGE = A->nudge;
+  const float s[3] = {st[0], st[1], dim == 3 ? st[2] : 0.0f};
+  const float sl = sqrtf(dotd(s, s, dim));
+  float dh[3] = {0.0f, 0.0f, 0.0f};
+  if (sl > 0.0f) for (int c = 0; c < dim; ++c) dh[c] = s[c] / sl;
+  int imin = 0, in_min = 0;
+  float tmin = INFINITY, disc_min = 0.0f;
+  for (int k = 0; k < A->n_obj; ++k) {
+    float c[3], q[3] = {0, 0, 0};
+    obj_center(A, k, dim, c);
+    for (int d = 0; d < dim; ++d) q[d] = mi(r[d] - c[d], L);
+    const float Rk = A->radii[k];
+    const float dist2 = dotd(q, q, dim);
+    const int inside = dist2 < Rk * Rk;
+    const float dp = dotd(dh, q, dim);
+    const float disc = dp * dp - (dist2 - Rk * Rk);
+    const float sq = sqrtf(fmaxf(disc, 0.0f));
+    const float t = inside ? (-dp + sq) : (-dp - sq);
+    const int valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+    const float tv = valid ? t : INFINITY;
+    if (tv < tmin) { tmin = tv; imin = k; disc_min = disc; in_min = inside; }
+  }
+  const int any_hit = isfinite(tmin);
+  if (!any_hit) {
+    r[0] = r[0] + st[0]; r[1] = r[1] + st[1]; r[2] = r[2] + st[2];
+    return 0.0f;
+  }
+  const float ts = tmin;
+  float c[3], qc[3] = {0, 0, 0};
+  obj_center(A, imin, dim, c);
+  const float Rh = A->radii[imin];
+  for (int d = 0; d < dim; ++d) qc[d] = mi(r[d] - c[d], L);
+  float hr[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) hr[d] = qc[d] + ts * dh[d];
+  const float hl = sqrtf(dotd(hr, hr, dim));
+  float n[3] = {0, 0, 0}, hs[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) { n[d] = hr[d] / fmaxf(hl, 1e-30f); hs[d] = Rh * n[d]; }
+  const float remaining = sl - ts;
+  const float dperp = remaining * (sqrtf(fmaxf(disc_min, 0.0f)) / Rh);
+  const float pt = fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+  const int transmit = u < pt;
+  (*hits)++;
+  if (transmit) {
+    (*ncross)++;
+    r[0] = r[0] + st[0]; r[1] = r[1] + st[1]; r[2] = r[2] + st[2];
+    return 0.0f;
+  }
+  const float dd = 2.0f * dotd(dh, n, dim);
+  float rf[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) rf[d] = dh[d] - dd * n[d];
+  const float rfn = sqrtf(dotd(rf, rf, dim));
+  for (int d = 0; d < dim; ++d) rf[d] = rfn > 0.0f ? rf[d] / fmaxf(rfn, 1e-30f) : 0.0f;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  float qr[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) {
+    const float nd = in_min ? -n[d] : n[d];
+    qr[d] = (hs[d] + NUDGE * nd) + rf[d] * go;
+  }
+  const float qn = sqrtf(dotd(qr, qr, dim));
+  const float qs = fmaxf(qn, 1e-30f);
+  const float target = in_min ? Rh - NUDGE : Rh + NUDGE;
+  const int wrong = in_min ? (qn >= Rh) : (qn <= Rh);
+  if (wrong) for (int d = 0; d < dim; ++d) qr[d] = qr[d] * target / qs;
+  for (int d = 0; d < dim; ++d) r[d] = r[d] + (qr[d] - qc[d]);
+  if (dim == 2) r[2] = r[2] + st[2];
+  return -2.0f * A->rho_over_D * dperp;
+}
+
+static int classify(const float r[3], const oracle_args* A) {
+  switch (A->kind) {
+    case K_CYL: return (r[0] * r[0] + r[1] * r[1]) < A->size * A->size ? 0 : 1;
+    case K_SPH: return dot3(r, r) < A->size * A->size ? 0 : 1;
+    case K_PCYL:
+    case K_PSPH: {
+      const int dim = A->kind == K_PCYL ? 2 : 3;
+      int id = 0;
+      for (int k = 0; k < A->n_obj; ++k) {
+        float c[3], q[3] = {0, 0, 0};
+        obj_center(A, k, dim, c);
+        for (int d = 0; d < dim; ++d) q[d] = mi(r[d] - c[d], A->cell);
+        if (dotd(q, q, dim) < A->radii[k] * A->radii[k] && k + 1 > id) id = k + 1;
+      }
+      return id;
+    }
+    default: return 0;
+  }
+}
+
+/* Start position sampler: rejection from Philox draws with tag 1. */
+static void sample_start(const oracle_args* A, uint32_t w, float r[3]) {
+  r[0] = r[1] = r[2] = 0.0f;
+  if (A->kind == K_FREE) return;
+  const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
+  for (uint32_t att = 0; att < 4096u; ++att) {
+    const uint32_t ctr[4] = {w, 0u, att, 1u};
+    uint32_t o[4];
+    philox10(ctr, key[0], key[1], o);
+    const float u0 = unit_u(o[0]), u1 = unit_u(o[1]), u2 = unit_u(o[2]);
+    if (A->kind == K_BOX) { r[0] = u0 * A->size; return; }
+    if (A->kind == K_CYL) {
+      r[0] = (2.0f * u0 - 1.0f) * A->size; r[1] = (2.0f * u1 - 1.0f) * A->size;
+      if (r[0] * r[0] + r[1] * r[1] < A->size * A->size) return;
+    } else if (A->kind == K_SPH) {
+      r[0] = (2.0f * u0 - 1.0f) * A->size; r[1] = (2.0f * u1 - 1.0f) * A->size;
+      r[2] = (2.0f * u2 - 1.0f) * A->size;
+      if (dot3(r, r) < A->size * A->size) return;
+    } else {
+      r[0] = (u0 - 0.5f) * A->cell; r[1] = (u1 - 0.5f) * A->cell;
+      r[2] = A->kind == K_PSPH ? (u2 - 0.5f) * A->cell : 0.0f;
+      if (classify(r, A) == 0) return;
+    }
+  }
+}
+
+/* One sub-step: direction, geometry, returns dlog (before chi gating). */
+static float substep(float r[3], const float dir[3], float u, const oracle_args* A,
+                     uint32_t* hits, uint32_t* ncross) {
+  const float st[3] = {dir[0] * A->step_len, dir[1] * A->step_len, dir[2] * A->step_len};
+  switch (A->kind) {
+    case K_FREE:
+      r[0] = r[0] + st[0]; r[1] = r[1] + st[1]; r[2] = r[2] + st[2];
+      return 0.0f;
+    case K_BOX: return box_step(r, st, A->size, A->rho_over_D, A->wall == W_SURFACE, hits);
+    case K_CYL:
+      if (A->wall == W_PERMEATE) return cyl_permeate(r, st, u, A, hits, ncross);
+      return -2.0f * A->rho_over_D * cyl_reflect(r, st, A, hits);
+    case K_SPH:
+      if (A->wall == W_PERMEATE) return sph_permeate(r, st, u, A, hits, ncross);
+      return -2.0f * A->rho_over_D * sph_reflect(r, st, A, hits);
+    default: {
+      const int dim = A->kind == K_PCYL ? 2 : 3;
+      if (A->wall == W_PERMEATE) return packed_permeate(r, st, u, A, dim, hits, ncross);
+      return -2.0f * A->rho_over_D * packed_reflect(r, st, A, dim, hits);
+    }
+  }
+}
+
+int oracle_walk(const oracle_args* A, oracle_out* O) {
+  const int64_t N = A->n_walkers;
+  const int64_t w0 = A->walker_begin, w1 = A->walker_end > 0 ? A->walker_end : N;
+  const int M = A->n_meas, T = A->n_t, NS = A->n_sub;
+  const int64_t S = (int64_t)T * NS;
+  const int rows = A->n_echo > 0 ? A->n_echo : 1;
+  const int packed = A->kind == K_PCYL || A->kind == K_PSPH;
+  const int dim = A->kind == K_PSPH ? 3 : 2;
+  const float L = A->cell;
+  const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
+  double* acc = (double*)calloc((size_t)rows * M * 2, sizeof(double));
+  uint64_t hits_total = 0;
+  if (!acc) return 1;
+#ifdef _OPENMP
+  if (A->threads > 0) omp_set_num_threads(A->threads);
+#endif
+#pragma omp parallel
+  {
+    double* loc = (double*)calloc((size_t)rows * M * 2, sizeof(double));
+    float* phi = (float*)malloc(sizeof(float) * (M > 0 ? M : 1));
+    double* phi64 = (double*)malloc(sizeof(double) * (M > 0 ? M : 1));
+    uint64_t hits_loc = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t w = w0; w < w1; ++w) {
+      float r[3];
+      int cell[3] = {0, 0, 0};
+      if (A->r0) { r[0] = A->r0[w * 3]; r[1] = A->r0[w * 3 + 1]; r[2] = A->r0[w * 3 + 2]; }
+      else sample_start(A, (uint32_t)w, r);
+      if (packed && A->cell_form) {
+        for (int d = 0; d < dim; ++d) {
+          const float k = floorf(r[d] / L + 0.5f);
+          if (k != 0.0f) { r[d] = r[d] - L * k; cell[d] += (int)k; }
+        }
+      }
+      int comp = classify(r, A);
+      if (O->comp_origin) O->comp_origin[w] = comp;
+      for (int m = 0; m < M; ++m) { phi[m] = 0.0f; phi64[m] = 0.0; }
+      float logw = 0.0f;
+      uint32_t hits = 0, ncross = 0;
+      uint32_t rnd[4] = {0, 0, 0, 0};
+      int e_next = 0;
+      for (int t = 0; t < T; ++t) {
+        const float chi = A->chi ? A->chi[t] : 1.0f;
+        for (int s = 0; s < NS; ++s) {
+          const int64_t gs = (int64_t)t * NS + s;
+          float dir[3], u = 0.0f;
+          if (A->inj_n) {
+            const float* nv = A->inj_n + ((size_t)w * S + gs) * 3;
+            const float nn = sqrtf((nv[0] * nv[0] + nv[1] * nv[1]) + nv[2] * nv[2]);
+            dir[0] = nv[0] / nn; dir[1] = nv[1] / nn; dir[2] = nv[2] / nn;
+            if (A->inj_u) u = A->inj_u[(size_t)w * S + gs];
+          } else {
+            if ((gs & 1) == 0 || gs == 0) {
+              const uint32_t ctr[4] = {(uint32_t)w, 0u, (uint32_t)(gs >> 1), 0u};
+              philox10(ctr, key[0], key[1], rnd);
+            }
+            oracle_direction((gs & 1) ? rnd[2] : rnd[0], (gs & 1) ? rnd[3] : rnd[1], dir);
+            if (A->wall == W_PERMEATE) {
+              const uint32_t ctr[4] = {(uint32_t)w, 0u, (uint32_t)gs, 2u};
+              uint32_t o[4];
+              philox10(ctr, key[0], key[1], o);
+              u = unit_u(o[0]);
+            }
+          }
+          float dlog = substep(r, dir, u, A, &hits, &ncross);
+          if (A->wall != W_REFLECT) dlog = dlog * chi;
+          if (A->has_t2) dlog = dlog - (A->dt_sub * A->inv_T2) * chi;
+          if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
+          logw = logw + dlog;
+          /* phase of every measurement, right-endpoint rule */
+          float ru[3] = {r[0], r[1], r[2]};
+          if (packed && A->cell_form)
+            for (int d = 0; d < 3; ++d) ru[d] = r[d] + L * (float)cell[d];
+          for (int m = 0; m < M; ++m) {
+            const float* g = A->G + ((size_t)m * T + t) * 3;
+            const float gr = (g[0] * ru[0] + g[1] * ru[1]) + g[2] * ru[2];
+            if (A->phase64) phi64[m] += (double)A->gamma_dt_sub * (double)gr;
+            else phi[m] = phi[m] + A->gamma_dt_sub * gr;
+          }
+        }
+        if (packed && A->cell_form) {
+          for (int d = 0; d < dim; ++d) {
+            const float k = floorf(r[d] / L + 0.5f);
+            if (k != 0.0f) { r[d] = r[d] - L * k; cell[d] += (int)k; }
+          }
+        }
+        if (O->comp_full || (O->comp_final && t == T - 1)) comp = classify(r, A);
+        if (O->comp_full) O->comp_full[(size_t)w * T + t] = comp;
+        if (e_next < A->n_echo && A->echo_idx[e_next] == t) {
+          const double wg = exp((double)logw);
+          for (int m = 0; m < M; ++m) {
+            const double ph = A->phase64 ? phi64[m] : (double)phi[m];
+            loc[((size_t)e_next * M + m) * 2] += wg * cos(ph);
+            loc[((size_t)e_next * M + m) * 2 + 1] += wg * sin(ph);
+          }
+          ++e_next;
+        }
+      }
+      if (A->n_echo == 0) {
+        const double wg = exp((double)logw);
+        for (int m = 0; m < M; ++m) {
+          const double ph = A->phase64 ? phi64[m] : (double)phi[m];
+          loc[m * 2] += wg * cos(ph);
+          loc[m * 2 + 1] += wg * sin(ph);
+        }
+      }
+      if (O->final_pos)
+        for (int d = 0; d < 3; ++d)
+          O->final_pos[w * 3 + d] = (packed && A->cell_form) ? r[d] + L * (float)cell[d] : r[d];
+      if (O->comp_final) O->comp_final[w] = comp;
+      if (O->n_cross) O->n_cross[w] = ncross;
+      if (O->log_w) O->log_w[w] = logw;
+      if (O->phi)
+        for (int m = 0; m < M; ++m) O->phi[w * M + m] = A->phase64 ? (float)phi64[m] : phi[m];
+      hits_loc += hits;
+    }
+#pragma omp critical
+    {
+      for (int i = 0; i < rows * M * 2; ++i) acc[i] += loc[i];
+      hits_total += hits_loc;
+    }
+    free(loc);
+    free(phi);
+    free(phi64);
+  }
+  const double inv = 1.0 / (double)(w1 - w0);
+  for (int i = 0; i < rows * M; ++i) {
+    O->signal[i] = acc[2 * i] * inv;
+    if (O->signal_im) O->signal_im[i] = acc[2 * i + 1] * inv;
+  }
+  O->n_hits = hits_total;
+  free(acc);
+  return 0;
+}

@@ -1,0 +1,274 @@
+"""Pins the CPU oracle (walk_oracle.c) before it is trusted as the GPU's checker.
+
+Every check here is independent of the oracle's own formulas: published
+known-answer vectors, textbook geometry (normal incidence, slab folding, ray/circle
+roots solved in float64), invariants (containment, weights only on reflection)
+and closed-form Monte-Carlo limits (exp(-bD), exp(-t/T2), Brownstein-Tarr).
+"""
+import numpy as np
+import pytest
+
+import dmipy_sim_amd as ds
+from substrates import D, packed_cyl, packed_sph
+
+GAMMA = ds.GAMMA
+
+
+# --------------------------------------------------------------- Philox KAT
+# Known-answer vectors of Philox4x32-10 published with the Random123 library
+# (Salmon et al., SC'11): (counter, key) -> output.
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff,) * 4, (0xffffffff,) * 2, (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(oracle, ctr, key, expect):
+    assert [int(v) for v in oracle.philox4x32_10(ctr, key)] == list(expect)
+
+
+def _wave(n_t=1, n_meas=1, dt=1e-4):
+    return ds.Waveform(G=np.zeros((n_meas, n_t, 3), np.float32), dt=dt, echo_idx=n_t - 1)
+
+
+def _one_step(oracle, geom, r0, normal, u=None, T2=None, wall_plan=None):
+    """Single walker, single sub-step probe of the oracle's step."""
+    wf = _wave()
+    plan = oracle.oracle_plan(geom, D, wf.dt, T2)
+    plan["n_sub"] = 1
+    plan["dt_sub"] = float(np.float32(wf.dt))
+    if wall_plan:
+        plan.update(wall_plan)
+    kw = {"inject_normals": np.asarray(normal, np.float32).reshape(1, 1, 3)}
+    if u is not None:
+        kw["inject_uniforms"] = np.asarray([[u]], np.float32)
+    res = oracle.run_oracle(geom, D, wf, 1, r0=np.asarray(r0, np.float32).reshape(1, 3),
+                            plan=plan, **kw)
+    return res, plan
+
+
+def test_direction_sampler_matches_exact_trig(oracle):
+    """Philox-driven directions: unit norm and equal (to 1e-6) to the float64
+    direction (sqrt(1-z^2) cos 2pi v, sqrt(1-z^2) sin 2pi v, z) of the same draws."""
+    n = 20000
+    wf = _wave()
+    geom = ds.FreeDiffusion()
+    plan = oracle.oracle_plan(geom, D, wf.dt)
+    plan["step_len"] = 1.0
+    res = oracle.run_oracle(geom, D, wf, n, seed=77, plan=plan, r0=np.zeros((n, 3)))
+    d = res["final_pos"].astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-7)
+    out = np.array([oracle.philox4x32_10((w, 0, 0, 0), (77, 0)) for w in range(200)])
+    u = (out[:, 0] >> 8) * 2.0 ** -24
+    v = (out[:, 1] >> 8) * 2.0 ** -24
+    z = 1 - 2 * u
+    s = np.sqrt(1 - z * z)
+    exact = np.stack([s * np.cos(2 * np.pi * v), s * np.sin(2 * np.pi * v), z], axis=1)
+    np.testing.assert_allclose(d[:200], exact, atol=1e-6)
+    # isotropy: mean 0, second moments 1/3 (3-sigma bands)
+    assert np.all(np.abs(d.mean(axis=0)) < 3 * np.sqrt(1 / 3 / n))
+    np.testing.assert_allclose((d ** 2).mean(axis=0), 1 / 3, atol=3 * np.sqrt(4 / 45 / n))
+
+
+def test_free_step_is_fixed_length_unit_direction(oracle):
+    """r_new = r + (n/|n|) * sqrt(6 D dt) (physics.py:299-301)."""
+    res, plan = _one_step(oracle, ds.FreeDiffusion(), [1e-6, 2e-6, -3e-6], [0.3, -1.2, 2.0])
+    n = np.array([0.3, -1.2, 2.0])
+    expect = np.array([1e-6, 2e-6, -3e-6]) + n / np.linalg.norm(n) * np.sqrt(6 * D * 1e-4)
+    np.testing.assert_allclose(res["final_pos"][0], expect, rtol=2e-6)
+
+
+def test_cylinder_no_hit_is_straight(oracle):
+    geom = ds.Cylinder(5e-6, (0, 0, 1))
+    res, plan = _one_step(oracle, geom, [0.0, 0.0, 0.0], [1.0, 0.5, 0.2])
+    n = np.array([1.0, 0.5, 0.2], np.float32)
+    step = (n / np.float32(np.sqrt(np.float32((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])))) \
+        * np.float32(plan["step_len"])
+    np.testing.assert_array_equal(res["final_pos"][0], step.astype(np.float32))
+
+
+@pytest.mark.parametrize("kind", ["cyl", "sph"])
+def test_normal_incidence_mirror_and_bt_weight(oracle, kind):
+    """Walker at distance delta inside the wall moving straight out by s > delta:
+    mirror image 2R - x0 - s, and Brownstein-Tarr dlog = -2 (rho/D) (s - delta)
+    with d_perp = (remaining - d) cos(alpha), cos(alpha) = 1 at normal incidence."""
+    R, rho = 5e-6, 1e-5
+    geom = (ds.Cylinder(R, (0, 0, 1), surface_relaxivity_t2=rho) if kind == "cyl"
+            else ds.Sphere(R, surface_relaxivity_t2=rho))
+    wf = _wave()
+    plan = oracle.oracle_plan(geom, D, wf.dt)
+    s = plan["step_len"]
+    x0 = R - 0.4 * s
+    res, plan = _one_step(oracle, geom, [x0, 0.0, 0.0], [1.0, 0.0, 0.0])
+    x = float(res["final_pos"][0, 0])
+    assert abs(x - (2 * R - x0 - s)) < 1e-4 * R * 1e-2 + 1e-12
+    assert abs(float(res["final_pos"][0, 1])) == 0.0
+    expect = -2.0 * (rho / D) * (s - (R - x0))
+    assert abs(float(res["log_w"][0]) - expect) < 2e-5 * abs(expect) + 1e-9
+
+
+def test_box1d_fold_and_overshoot(oracle):
+    L, rho = 6e-6, 2e-5
+    geom = ds.Box1D(L, surface_relaxivity_t2=rho)
+    wf = _wave()
+    s = oracle.oracle_plan(geom, D, wf.dt)["step_len"]
+    for x0, sgn in [(L - 0.3 * s, 1.0), (0.2 * s, -1.0), (0.5 * L, 1.0)]:
+        res, _ = _one_step(oracle, geom, [x0, 0.0, 0.0], [sgn, 0.0, 0.0])
+        xr = x0 + sgn * s
+        over = max(xr - L, 0.0) + max(-xr, 0.0)
+        folded = xr % (2 * L)
+        folded = 2 * L - folded if folded > L else folded
+        assert abs(float(res["final_pos"][0, 0]) - folded) < 1e-12
+        np.testing.assert_allclose(float(res["log_w"][0]), -2 * rho / D * over, rtol=1e-5, atol=1e-12)
+
+
+@pytest.mark.parametrize("u,transmit", [(0.0, True), (0.9999999, False)])
+def test_powles_decision(oracle, u, transmit):
+    """Transmit iff u < min(1, 2 kappa d_perp / D); no weight on transmission."""
+    R = 5e-6
+    geom = ds.Cylinder(R, (0, 0, 1), permeability=1e-5, surface_relaxivity_t2=1e-5)
+    wf = _wave()
+    s = oracle.oracle_plan(geom, D, wf.dt)["step_len"]
+    x0 = R - 0.5 * s
+    res, plan = _one_step(oracle, geom, [x0, 0.0, 0.0], [1.0, 0.0, 0.0], u=u)
+    if transmit:
+        assert res["n_cross"][0] == 1 and res["comp_final"][0] == 1
+        np.testing.assert_allclose(res["final_pos"][0, 0], x0 + s, rtol=1e-6)
+        assert res["log_w"][0] == 0.0
+    else:
+        assert res["n_cross"][0] == 0 and res["comp_final"][0] == 0
+        assert res["final_pos"][0, 0] < R
+        assert res["log_w"][0] < 0.0
+
+
+def _ray_circle_hits(p, d, centers, radii, L, tmax):
+    """float64 brute force over periodic images: sorted (t, k, disc) entry roots
+    along a ray segment of length tmax."""
+    hits = []
+    for k, (c, R) in enumerate(zip(centers, radii)):
+        for ix in (-1, 0, 1):
+            for iy in (-1, 0, 1):
+                q = p - (c + L * np.array([ix, iy]))
+                b = np.dot(d, q)
+                disc = b * b - (np.dot(q, q) - R * R)
+                if disc <= 0:
+                    continue
+                t = -b - np.sqrt(disc)
+                if 0 < t < tmax:
+                    hits.append((t, k, disc))
+    return sorted(hits)
+
+
+def test_packed_cylinders_hit_matches_float64_geometry(oracle):
+    """Reflection off a packing: the float32 oracle matches a float64 ray/circle
+    solution of the same step (hit point, specular direction, nudge) to 1e-10 m."""
+    geom = packed_cyl()
+    wf = _wave()
+    plan = oracle.oracle_plan(geom, D, wf.dt)
+    plan["step_len"] = 1.5e-6
+    rng = np.random.default_rng(0)
+    L = geom._L_float
+    checked = 0
+    for trial in range(400):
+        r0 = geom.init_positions(1, trial)[0].astype(np.float64)
+        nrm = rng.standard_normal(3)
+        u = nrm / np.linalg.norm(nrm)
+        dxy = u[:2] / np.linalg.norm(u[:2])
+        sxy = np.linalg.norm(u[:2]) * plan["step_len"]
+        hits = _ray_circle_hits(r0[:2], dxy, geom._centers_np, geom._radii_np, L, sxy)
+        if not hits:
+            continue
+        t, k, disc = hits[0]
+        # skip float32/float64 near-ties: grazing rays, ends of the segment, two walls
+        if disc < 1e-16 or sxy - t < 1e-9 or (len(hits) > 1 and hits[1][0] - t < 1e-9):
+            continue
+        hit = r0[:2] + t * dxy
+        c = geom._centers_np[k]
+        q = hit - c
+        q -= L * np.round(q / L)
+        n = q / np.linalg.norm(q)
+        refl = dxy - 2 * np.dot(dxy, n) * n
+        nudge = plan["nudge"]
+        expect = hit + nudge * n + refl * max(sxy - t - nudge, 0.0)
+        qe = expect - geom._centers_np
+        qe -= L * np.round(qe / L)
+        if np.any(np.linalg.norm(qe, axis=1) < geom._radii_np * (1 + 1e-6)):
+            continue   # the safety clamp would act: not a single-reflection case
+        res, _ = _one_step(oracle, geom, r0, nrm, wall_plan={"step_len": plan["step_len"]})
+        np.testing.assert_allclose(res["final_pos"][0, :2], expect, atol=1e-10)
+        checked += 1
+    assert checked > 20
+
+
+@pytest.mark.parametrize("factory", [lambda: ds.Cylinder(3e-6, (0, 0, 1)),
+                                     lambda: ds.Sphere(3e-6),
+                                     lambda: ds.Cylinder(3e-6, (0, 0, 1), surface_relaxivity_t2=2e-5)])
+def test_containment_after_walk(oracle, factory):
+    """max |r| < R (1 + 3e-4) after a long walk (tests/test_cylinder.py:113-151)."""
+    geom = factory()
+    wf = _wave(n_t=200, dt=2e-4)
+    res = oracle.run_oracle(geom, D, wf, 2000, seed=3)
+    r = res["final_pos"].astype(np.float64)
+    rad = np.linalg.norm(r[:, :2] if isinstance(geom, ds.Cylinder) else r, axis=1)
+    assert rad.max() < geom.radius * (1 + 3e-4)
+    assert rad.max() > 0.95 * geom.radius
+    assert np.all(res["comp_final"] == 0)
+
+
+@pytest.mark.parametrize("factory", [packed_cyl, packed_sph])
+def test_packed_exterior_stays_outside(oracle, factory):
+    geom = factory()
+    wf = _wave(n_t=100, dt=2e-4)
+    res = oracle.run_oracle(geom, D, wf, 1000, seed=4)
+    assert np.all(res["comp_final"] == 0)
+    L = geom._L_float
+    dim = geom._centers_np.shape[1]
+    p = res["final_pos"][:, :dim].astype(np.float64)
+    for c, R in zip(geom._centers_np, geom._radii_np):
+        q = p - c
+        q -= L * np.round(q / L)
+        assert np.min(np.linalg.norm(q, axis=1)) > R * (1 - 1e-4)
+
+
+def test_free_pgse_matches_stejskal_tanner(oracle):
+    """README quick-start workload: exp(-bD) within 3 Monte-Carlo SE."""
+    wf = ds.set_b(ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]] * 3, 300), [0, 1e9, 2e9])
+    N = 50000
+    res = oracle.run_oracle(ds.FreeDiffusion(), D, wf, N, seed=0, threads=8)
+    b = np.array([0, 1e9, 2e9])
+    E = np.exp(-b * D)
+    se = np.sqrt(np.maximum(((1 + np.exp(-4 * b * D)) / 2 - E ** 2) / N, 1e-12))
+    assert np.all(np.abs(res["signal"] - E) <= 3 * se + 1e-9), (res["signal"], E, se)
+
+
+def test_t2_decay_uses_n_t_samples(oracle):
+    """Simulated time is n_t*dt (all samples including G[0], tests/test_t2.py:47-49)."""
+    wf = _wave(n_t=500, dt=8e-5)
+    T2 = 0.08
+    res = oracle.run_oracle(ds.FreeDiffusion(), D, wf, 64, T2=T2)
+    np.testing.assert_allclose(res["signal"][0], np.exp(-500 * 8e-5 / T2), rtol=2e-5)
+
+
+def test_brownstein_tarr_fast_diffusion_sphere(oracle):
+    """rho R / D << 1: S(t) = exp(-rho (S/V) t), S/V = 3/R, within 5 % in the rate
+    (tests/physics/test_surface_relaxivity.py:127-230)."""
+    R, rho = 2e-6, 1e-5
+    geom = ds.Sphere(R, surface_relaxivity_t2=rho)
+    wf = _wave(n_t=100, dt=2e-4)
+    res = oracle.run_oracle(geom, D, wf, 4000, seed=8, threads=8)
+    t = 100 * 2e-4
+    rate = -np.log(res["signal"][0]) / t
+    assert abs(rate / (3 * rho / R) - 1) < 0.05
+
+
+def test_permeable_weights_only_on_reflection(oracle):
+    """With rho set and kappa very large almost every wall event transmits, and a
+    transmission carries no Brownstein-Tarr weight (geometries.py:419-422)."""
+    geom = ds.Sphere(3e-6, permeability=1e3, surface_relaxivity_t2=1e-5)
+    wf = _wave(n_t=20, dt=1e-4)
+    res = oracle.run_oracle(geom, D, wf, 500, seed=2)
+    assert res["n_cross"].sum() > 100
+    assert np.mean(res["log_w"] != 0.0) < 0.01
