@@ -1,0 +1,190 @@
+"""Throughput of the Monte-Carlo walk on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder|free|packed]
+
+A step is one complete walk of the workload (all walkers, all n_t x n_sub
+sub-steps, all measurements, float64 ensemble reduction) with the waveform and
+geometry tables already resident in HBM.  The default workload is the
+Cylinder PGSE configuration (R = 5 um, rho = 1e-6 m/s, 1e6 walkers, n_t = 1000,
+128 measurements).  Prints one JSON line on stdout; progress goes to stderr.
+
+Roofline: the walk keeps every walker in registers, so it is bound by float32
+vector (VALU) issue, not by HBM.  ``achieved`` counts algorithmic flops per
+walker-step (frozen model ``falg`` below: FMA = 2, other arithmetic = 1, RNG
+integer work excluded) times walker-steps per launch, divided by the walk
+kernel's average duration from HIP events on the kernel's stream.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "dmipy-sim_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "walker-steps/s (1 GPU); signals/s surrogate vs MC at RMSE≤5e-3 S/S0"
+PEAK_FP32_TFLOPS = 157.3
+
+# HBM bytes per walk launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each
+# scaled by the counter's reading on a copy kernel of known size (factors and
+# commands in profiles/walk_hbm_traffic.md).  None until measured.
+TRAFFIC_BYTES = {
+    "cylinder_pgse": None,
+    "free_pgse": None,
+    "packed_cylinders_free_waveform": None,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def falg(cfg, plan, hbar, kbar):
+    """Algorithmic float32 flops per walker-step (FMA = 2, add/mul/cmp/sqrt/div = 1):
+    direction 12 + move 3 + phase (6 n_meas / n_sub + 3 [n_sub > 1]) + wall + weight."""
+    from dmipy_sim_amd import geometries as geo
+    n_meas = cfg["waveform"].G.shape[0]
+    n_sub = plan.n_sub
+    f = 12 + 3 + 6.0 * n_meas / n_sub + (3 if n_sub > 1 else 0)
+    kind, wall = plan.kind, plan.wall
+    if kind == geo.KIND_BOX1D and wall == 1:
+        f += 12
+    elif kind == geo.KIND_CYLINDER:
+        f += 40 if wall == 2 else 30 + 47 * hbar
+    elif kind == geo.KIND_SPHERE:
+        f += 45 if wall == 2 else 40 + 60 * hbar
+    elif kind == geo.KIND_PACKED_CYL:
+        f += 22 * kbar + 45
+    elif kind == geo.KIND_PACKED_SPH:
+        f += 26 * kbar + 55
+    if plan.rho_over_D != 0.0 or plan.has_t2 or plan.has_t1:
+        f += 3
+    return f
+
+
+def build_workload(name):
+    from dmipy_sim_amd import configs
+    if name == "cylinder":
+        return configs.cylinder_pgse()
+    if name == "free":
+        return configs.free_pgse()
+    if name == "packed":
+        return configs.packed_cylinders_free_waveform()
+    raise SystemExit(f"unknown workload {name!r}")
+
+
+def slice_waveform(wf, n_t):
+    from dmipy_sim_amd import Waveform
+    return Waveform(G=wf.G[:, :n_t].copy(), dt=wf.dt, echo_idx=n_t - 1)
+
+
+def cpu_baseline(cfg, target_s, reps):
+    """The oracle (reference-faithful float32 restatement: fixed bounce loops,
+    all objects tested, per-sub-step phase of every measurement) on the host cores,
+    over a bounded walker sample of the same workload."""
+    from oracle import oracle as orc
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    geom, wf, D = cfg["geometry"], cfg["waveform"], cfg["diffusivity"]
+    plan = orc.oracle_plan(geom, D, wf.dt)
+    n_t = wf.G.shape[1]
+    probe = 64 * threads
+    t0 = time.perf_counter()
+    orc.run_oracle(geom, D, wf, probe, seed=1, threads=threads, plan=plan)
+    dt = time.perf_counter() - t0
+    per_walker = dt / probe
+    n = int(max(probe, min(cfg["n_walkers"], target_s / max(per_walker, 1e-9))))
+    n = (n // threads) * threads or threads
+    rates = []
+    for r in range(reps):
+        t0 = time.perf_counter()
+        orc.run_oracle(geom, D, wf, n, seed=2 + r, threads=threads, plan=plan)
+        el = time.perf_counter() - t0
+        rates.append(n * n_t * plan["n_sub"] / el)
+        log(f"cpu_baseline rep {r}: {n} walkers in {el:.2f} s -> {rates[-1]:.3e} WS/s")
+    return {"value": statistics.median(rates), "unit": "walker-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"{n} of {cfg['n_walkers']} walkers x all {n_t} samples x n_sub "
+                       f"{plan['n_sub']}, median of {reps} runs, OpenMP over walkers")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="cylinder", choices=["cylinder", "free", "packed"])
+    ap.add_argument("--slice-t", type=int, default=0,
+                    help="packed workload: walk only the first N waveform samples")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=6.0)
+    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--meas-block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(walkers shard across GPUs only in the design).", file=sys.stderr)
+        sys.exit(2)
+
+    import dmipy_sim_amd as ds
+    cfg = build_workload(args.workload)
+    wf = cfg["waveform"]
+    sliced = None
+    if args.workload == "packed":
+        n_slice = args.slice_t or 100
+        if n_slice < wf.G.shape[1]:
+            wf = slice_waveform(wf, n_slice)
+            sliced = n_slice
+    log(f"workload {cfg['name']}: N={cfg['n_walkers']} n_meas={wf.G.shape[0]} n_t={wf.G.shape[1]}")
+    pw = ds.PlannedWalk(cfg["geometry"], cfg["diffusivity"], wf, cfg["n_walkers"],
+                        meas_block=args.meas_block)
+    plan = pw.plan
+    log(f"plan: wall={plan.wall} n_sub={plan.n_sub} kahan={pw.kahan} "
+        f"walker-steps/launch={pw.walker_steps:.3e}")
+    probe = pw.run(seed=cfg["seed"], counters=True)
+    hbar = probe["n_hits"] / pw.walker_steps
+    kbar = probe["n_cand"] / pw.walker_steps
+    log(f"counters: hits/WS={hbar:.4f} candidates/WS={kbar:.3f}; signal[:4]={probe['signal'][:4]}")
+    for i in range(args.warmup):
+        pw.run(seed=cfg["seed"] + 1 + i)
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        kms.append(pw.run(seed=cfg["seed"] + 100 + i)["kernel_ms"])
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    value = pw.walker_steps * args.steps / wall
+    kmean = statistics.mean(kms)
+    f = falg(cfg, plan, hbar, kbar)
+    achieved = f * pw.walker_steps / (kmean * 1e-3) / 1e12
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, kernel {kmean:.3f} ms, "
+        f"{value:.4e} WS/s, F_alg={f:.1f} -> {achieved:.2f} TFLOP/s")
+    traffic = TRAFFIC_BYTES.get(cfg["name"])
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": traffic, "kernel": "walk_kernel", "kernel_ms": round(kmean, 4),
+                "flops_per_walker_step": round(f, 2), "hits_per_walker_step": round(hbar, 5),
+                "candidates_per_walker_step": round(kbar, 4)}
+    cpu = None
+    if not args.no_cpu:
+        ccfg = dict(cfg)
+        ccfg["waveform"] = wf
+        cpu = cpu_baseline(ccfg, args.cpu_seconds, args.cpu_reps)
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    config = {"workload": cfg["name"], "walkers": cfg["n_walkers"], "n_t": int(wf.G.shape[1]),
+              "n_meas": int(wf.G.shape[0]), "n_sub": plan.n_sub,
+              "geometry": type(cfg["geometry"]).__name__, "diffusivity": cfg["diffusivity"],
+              "phase": "kahan" if pw.kahan else "fp32", "ensemble_sum": "fp64"}
+    if sliced:
+        config["time_slice"] = f"first {sliced} of 2000 samples"
+    line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

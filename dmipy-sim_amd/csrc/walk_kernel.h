@@ -36,6 +36,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 #include "dmw_rng.h"
 
 #pragma clang fp contract(off)
@@ -592,6 +595,44 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
   return p;
 }
 
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [0, N), so array
+// indices stay constants and per-walker arrays stay in registers.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// sin and cos of a phase: quarter-turn reduction r = x - q*pi/2 with pi/2 split in
+// two float32 parts (Cody-Waite, fused multiply-adds), then the Taylor polynomials
+// of sin (to r^9) and cos (to r^10) on |r| <= pi/4.  Absolute error ~1e-7 for
+// |x| < 1e5 rad.
+__device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
+  const float q = rintf(x * 0.636619772f);
+  float r = fmaf(-q, 1.57079637f, x);
+  r = fmaf(-q, -4.37113883e-8f, r);
+  const float r2 = r * r;
+  float ps = fmaf(r2, 2.75573192e-6f, -1.98412698e-4f);
+  ps = fmaf(r2, ps, 8.33333333e-3f);
+  ps = fmaf(r2, ps, -1.66666667e-1f);
+  const float sr = fmaf(r * r2, ps, r);
+  float pc = fmaf(r2, -2.75573192e-7f, 2.48015873e-5f);
+  pc = fmaf(r2, pc, -1.38888889e-3f);
+  pc = fmaf(r2, pc, 4.16666667e-2f);
+  pc = fmaf(r2, pc, -0.5f);
+  const float cr = fmaf(r2, pc, 1.0f);
+  const int qi = ((int)q) & 3;
+  float cs = cr, sn = sr;
+  if (qi == 1) { cs = -sr; sn = cr; }
+  else if (qi == 2) { cs = -cr; sn = -sr; }
+  else if (qi == 3) { cs = sr; sn = -cr; }
+  *s = sn;
+  *c = cs;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -603,10 +644,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 template <int MB, bool KAHAN>
 __device__ __forceinline__ void emit_partials(const float (&phi)[MB], const float (&kc)[MB], float wgt,
                                               bool active, int lane, double* dst) {
-#pragma unroll
-  for (int m = 0; m < MB; ++m) {
+  static_for<MB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
     float sn, cs;
-    sincosf(phi[m], &sn, &cs);
+    phase_sincos(phi[m], &sn, &cs);
     if (KAHAN) {
       const float c0 = cs;
       cs = fmaf(kc[m], sn, cs);
@@ -615,7 +656,7 @@ __device__ __forceinline__ void emit_partials(const float (&phi)[MB], const floa
     const double re = wave_sum(active ? (double)(wgt * cs) : 0.0);
     const double im = wave_sum(active ? (double)(wgt * sn) : 0.0);
     if (lane == 0) { dst[2 * m] = re; dst[2 * m + 1] = im; }
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -661,8 +702,11 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
   if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
 
   float phi[MB], kc[MB];
-#pragma unroll
-  for (int m = 0; m < MB; ++m) { phi[m] = 0.0f; kc[m] = 0.0f; }
+  static_for<MB>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    phi[m] = 0.0f;
+    kc[m] = 0.0f;
+  });
   float logw = 0.0f;
   uint32_t hits = 0, ncross = 0, ncand = 0;
   const float L = a.cell;
@@ -751,8 +795,8 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
     // G(t) is wave-uniform: read it through the constant address space so the
     // loads are scalar (s_load) and the values arrive in SGPRs.
     const cfloat* g = (const cfloat*)(a.Gs + ((size_t)t * a.n_meas_pad + m0) * 3);
-#pragma unroll
-    for (int m = 0; m < MB; ++m) {
+    static_for<MB>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
       const float dphi = fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
       if (KAHAN) {
         const float yk = dphi - kc[m];
@@ -762,7 +806,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
       } else {
         phi[m] = phi[m] + dphi;
       }
-    }
+    });
     if (PACKED) fold<DIM>(p, a);
     if (a.comp_full) comp = classify<GEOM>(p, a, tb);
     if (primary && active) {
@@ -787,9 +831,10 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
                            a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
   if (active) {
     if (a.phi) {
-#pragma unroll
-      for (int m = 0; m < MB; ++m)
+      static_for<MB>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
         if (m0 + m < a.n_meas) a.phi[(size_t)(m0 + m) * N + w] = phi[m] - kc[m];
+      });
     }
     if (primary) {
       if (a.final_pos) {

@@ -12,7 +12,7 @@ from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders
                          PackedSpheres, pack_cylinders, pack_spheres, pack_dense)
 from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
                       collision_sub_steps, replay_sub_steps)
-from .core import simulate, simulate_cpmg, simulate_mixture, walk
+from .core import simulate, simulate_cpmg, simulate_mixture, walk, PlannedWalk
 from .gpu import check_gpu, gpu_available
 from .batch import BatchedWalk
 
@@ -23,5 +23,5 @@ __all__ = [
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",
-    "BatchedWalk",
+    "BatchedWalk", "PlannedWalk",
 ]

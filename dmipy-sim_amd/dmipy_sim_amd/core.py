@@ -106,10 +106,10 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
                         size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
                         rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
                         centers=nat.fptr(plan.centers), radii=nat.fptr(plan.radii))
-    wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(np.ascontiguousarray(G)),
+    G = np.ascontiguousarray(G, dtype=np.float32)   # referenced by wd until the call returns
+    wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
                         chi=nat.fptr(chi), n_echo=0 if echo is None else len(echo),
                         echo_idx=nat.iptr(echo))
-    G_keep = wd  # noqa: F841  (keeps G alive through the call)
     rd = nat.Run_t(n_walkers=N, seed=int(seed) & (2**64 - 1), n_sub=plan.n_sub,
                    step_len=plan.step_len, gamma_dt_sub=plan.gamma_dt_sub, dt_sub=plan.dt_sub,
                    inv_T2=plan.inv_T2, inv_T1=plan.inv_T1, has_t2=int(plan.has_t2),
@@ -352,3 +352,65 @@ def simulate_cpmg(n_walkers, diffusivity, waveform, geometry, *, T2=None, seed=1
     res = walk(geometry, diffusivity, waveform, n_walkers, seed=seed, T2=T2,
                echo_indices=uniq)
     return res["signal"][inv].astype(np.float32)
+
+
+class PlannedWalk:
+    """A walk whose inputs (waveform, geometry tables) stay resident on the GPU;
+    ``run(seed)`` launches the walk and the reduction only.  Used for throughput
+    measurement and for repeated walks of one configuration with new seeds."""
+
+    def __init__(self, geometry, diffusivity, waveform, n_walkers, T2=None, T1=None,
+                 kahan=None, meas_block=0, substep_rule="fused"):
+        import ctypes as C
+        G = np.asarray(waveform.G, dtype=np.float32)
+        n_meas, n_t, _ = G.shape
+        self.plan = make_walk_plan(geometry, diffusivity, waveform.dt, T2=T2, T1=T1,
+                                   substep_rule=substep_rule)
+        plan = self.plan
+        if plan.rot is not None:
+            G = np.einsum('ij,mtj->mti', plan.rot.astype(np.float64), G)
+        self._G = np.ascontiguousarray(G, dtype=np.float32)
+        chi = getattr(waveform, 'chi_perp', None)
+        self._chi = None if chi is None else np.ascontiguousarray(chi, dtype=np.float32).reshape(n_t)
+        if kahan is None:
+            kahan = _kahan_auto(n_t)
+        gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
+                            n_obj=0 if plan.radii is None else len(plan.radii),
+                            size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
+                            rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
+                            centers=nat.fptr(plan.centers), radii=nat.fptr(plan.radii))
+        wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(self._G), chi=nat.fptr(self._chi),
+                            n_echo=0, echo_idx=None)
+        rd = nat.Run_t(n_walkers=int(n_walkers), seed=0, n_sub=plan.n_sub,
+                       step_len=plan.step_len, gamma_dt_sub=plan.gamma_dt_sub,
+                       dt_sub=plan.dt_sub, inv_T2=plan.inv_T2, inv_T1=plan.inv_T1,
+                       has_t2=int(plan.has_t2), has_t1=int(plan.has_t1),
+                       kahan=int(bool(kahan)), meas_block=int(meas_block),
+                       r0=None, inj_normals=None, inj_uniforms=None)
+        h = C.c_void_p()
+        nat.check(nat.lib().dmw_plan_create(C.byref(h), gd, wd, rd))
+        self._h = h
+        self.n_meas, self.n_t, self.n_walkers = n_meas, n_t, int(n_walkers)
+        self.walker_steps = int(n_walkers) * n_t * plan.n_sub
+        self.kahan = bool(kahan)
+
+    def run(self, seed=123, counters=False):
+        sig = np.zeros(self.n_meas, np.float64)
+        out = nat.Result_t(signal=nat.dptr(sig), want_counters=int(bool(counters)))
+        nat.check(nat.lib().dmw_plan_execute(self._h, int(seed) & (2**64 - 1), out))
+        res = {"signal": sig, "kernel_ms": out.kernel_ms}
+        if counters:
+            res["n_hits"] = int(out.n_hits)
+            res["n_cand"] = int(out.n_cand)
+        return res
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().dmw_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

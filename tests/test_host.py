@@ -1,0 +1,194 @@
+"""Host-side logic: sub-step rules and float32 constants, packers, the C ABI
+surface, and argument handling of simulate() that runs before any GPU work."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import dmipy_sim_amd as ds
+from dmipy_sim_amd import _native
+from substrates import CASES, D
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.parametrize("name,factory,T2,T1", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("dt", [1e-4, 2.5e-5, 5.005e-5])
+def test_plan_constants_match_oracle_restatement(oracle, name, factory, T2, T1, dt):
+    """The product's plan and the oracle's independent restatement of
+    make_step_fn agree on n_sub and every float32 constant, bit for bit."""
+    g = factory()
+    p = ds.make_walk_plan(g, D, dt, T2=T2, T1=T1)
+    o = oracle.oracle_plan(g, D, dt, T2, T1)
+    assert p.n_sub == o["n_sub"] and p.wall == o["wall"] and p.kind == o["kind"]
+    for k in ("step_len", "gamma_dt_sub", "dt_sub", "rho_over_D", "kappa_over_D",
+              "inv_T2", "inv_T1", "eps", "nudge", "size", "cell"):
+        assert np.float32(getattr(p, k)) == np.float32(o[k]), k
+
+
+def test_substep_rules_closed_form():
+    """R/25 permeable (dt_max = R^2/(3750 D)), R/8 surface, 1 otherwise
+    (physics.py:35-48, 163-188); Cylinder PGSE workload gives n_sub = 2."""
+    dt = 0.05 / 999
+    assert ds.make_walk_plan(ds.Cylinder(5e-6, (0, 0, 1), surface_relaxivity_t2=1e-6), D, dt).n_sub == 2
+    assert ds.make_walk_plan(ds.Cylinder(5e-6, (0, 0, 1)), D, dt).n_sub == 1
+    R = 3e-6
+    assert ds.make_walk_plan(ds.Sphere(R, permeability=1e-5), D, dt).n_sub == \
+        int(np.ceil(dt / (R * R / (3750 * D))))
+    assert ds.make_walk_plan(ds.Box1D(5e-6, surface_relaxivity_t2=1e-5), D, dt).n_sub == 1
+    g = ds.Sphere(R, surface_relaxivity_t2=1e-5)
+    g.surface_substep_frac = 0
+    assert ds.make_walk_plan(g, D, dt).n_sub == 1
+    assert ds.make_walk_plan(ds.Sphere(R, surface_relaxivity_t2=1e-5), D, dt,
+                             substep_rule="replay").n_sub == int(np.ceil(dt / (R * R / (216 * D))))
+
+
+def test_packed_workload_substeps():
+    """64 Gamma(4, 0.75 um) radii, seed 0: smallest radius 0.58 um -> n_sub = 558."""
+    from dmipy_sim_amd import configs
+    cfg = configs.packed_cylinders_free_waveform()
+    g = cfg["geometry"]
+    assert len(g._radii_np) == 64 and abs(g.volume_fraction() - 0.6) < 1e-12
+    assert ds.make_walk_plan(g, D, cfg["waveform"].dt).n_sub == 558
+    assert g.min_gap > 0
+
+
+# ---------------------------------------------------------------- packers
+def _no_overlap(c, r, L):
+    for i in range(len(r)):
+        for j in range(i + 1, len(r)):
+            dq = c[i] - c[j]
+            dq -= L * np.round(dq / L)
+            assert np.linalg.norm(dq) >= r[i] + r[j] - 1e-10
+
+
+def test_pack_cylinders_rsa_invariants():
+    """tests/test_packed_cylinders.py:27-76."""
+    c, L, vf = ds.pack_cylinders(np.full(8, 2e-6), target_vf=0.20, seed=42)
+    assert c.shape == (8, 2)
+    _no_overlap(c, np.full(8, 2e-6), L)
+    radii = np.random.default_rng(0).uniform(1e-6, 4e-6, 6)
+    c, L, vf = ds.pack_cylinders(radii, target_vf=0.15, seed=7)
+    _no_overlap(c, radii, L)
+    c, L, vf = ds.pack_cylinders(np.full(5, 3e-6), target_vf=0.20, seed=1)
+    assert vf == pytest.approx(np.pi * 5 * 9e-12 / L ** 2, rel=1e-6)
+    assert vf == pytest.approx(0.20, rel=1e-3)
+
+
+def test_pack_errors():
+    with pytest.raises(ValueError, match="exactly one"):
+        ds.pack_cylinders([1e-6], target_vf=0.2, L=100e-6)
+    with pytest.raises(ValueError, match="exactly one"):
+        ds.pack_cylinders([1e-6])
+    with pytest.raises(ValueError, match="positive"):
+        ds.pack_cylinders([-1e-6], target_vf=0.1)
+    with pytest.raises(ValueError, match="target_vf"):
+        ds.pack_cylinders([1e-6], target_vf=1.5)
+    with pytest.raises(ValueError, match="exceeds"):
+        ds.pack_spheres([30e-6], L=40e-6)
+
+
+def test_pack_spheres_and_dense_packing():
+    r = np.full(10, 2e-6)
+    c, L, vf = ds.pack_spheres(r, target_vf=0.2, seed=3)
+    _no_overlap(c, r, L)
+    radii = np.random.default_rng(5).gamma(4.0, 0.75e-6, 64)
+    c, L, vf = ds.pack_dense(radii, 0.6, seed=0, dim=2, gap=0.1)
+    assert vf == pytest.approx(0.6, rel=1e-12)
+    _no_overlap(c, radii, L)
+    g = ds.PackedCylinders(radii, c, L)
+    assert g.min_gap >= 0.1 * radii.min() * 0.999
+
+
+def test_packed_init_positions_outside():
+    radii = np.full(6, 2e-6)
+    c, L, _ = ds.pack_spheres(radii, target_vf=0.2, seed=1)
+    g = ds.PackedSpheres(radii, c, L)
+    p = g.init_positions(2000, 3)
+    for ci, ri in zip(c, radii):
+        q = p - ci
+        q -= L * np.round(q / L)
+        assert np.all(np.linalg.norm(q, axis=1) > ri)
+
+
+def test_geometry_closed_forms():
+    """tests/test_geometry_properties.py: volumes, areas, volume fractions."""
+    s = ds.Sphere(2e-6)
+    assert s.volume() == pytest.approx(4 / 3 * np.pi * 8e-18)
+    assert s.surface_area() == pytest.approx(4 * np.pi * 4e-12)
+    cy = ds.Cylinder(2e-6, (1, 1, 0))
+    assert cy.volume() == pytest.approx(np.pi * 4e-12)
+    assert cy.surface_area(include_caps=True) == pytest.approx(2 * np.pi * 2e-6 + 2 * np.pi * 4e-12)
+    np.testing.assert_allclose(cy._R.astype(np.float64) @ cy.orientation, [0, 0, 1], atol=1e-6)
+
+
+# ---------------------------------------------------------------- C ABI
+def _header_symbols():
+    txt = open(os.path.join(ROOT, "include", "dmipy_walk.h")).read()
+    return set(re.findall(r"^\s*(?:int|const char\*)\s+(dmw_\w+)\s*\(", txt, re.M))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.lib()
+    decl = _header_symbols()
+    assert decl == set(_native.SIGNATURES), decl ^ set(_native.SIGNATURES)
+    for name in decl:
+        assert hasattr(lib, name), name
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every field of the ctypes mirrors sits at the offset gcc gives it when
+    compiling include/dmipy_walk.h, and every struct has gcc's size."""
+    import ctypes as C
+    import subprocess
+    structs = {"dmw_geometry": _native.Geometry_t, "dmw_waveform": _native.Waveform_t,
+               "dmw_run": _native.Run_t, "dmw_result": _native.Result_t,
+               "dmw_instance": _native.Instance_t, "dmw_batch_stats": _native.BatchStats_t}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dmipy_walk.h"',
+             'int main(void) {']
+    for cname, ct in structs.items():
+        lines.append(f'printf("{cname} __sizeof__ %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    seen = 0
+    for line in filter(None, out):
+        cname, field, val = line.split()
+        ct = structs[cname]
+        got = C.sizeof(ct) if field == "__sizeof__" else getattr(ct, field).offset
+        assert got == int(val), (cname, field, got, val)
+        seen += 1
+    assert seen == sum(len(ct._fields_) + 1 for ct in structs.values())
+
+
+def test_device_count_without_gpu_is_zero_or_more():
+    assert _native.device_count() >= 0
+
+
+def test_simulate_argument_errors_before_gpu():
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 50)
+    with pytest.raises(ValueError, match="return_compartments"):
+        ds.simulate(10, D, wf, ds.FreeDiffusion(), return_compartments="bogus")
+    with pytest.raises(ValueError, match="engine"):
+        ds.simulate(10, D, wf, ds.FreeDiffusion(), engine="jax")
+    with pytest.raises(NotImplementedError, match="replay"):
+        ds.simulate(10, D, wf, ds.Sphere(3e-6, permeability=1e-5), engine="replay")
+    with pytest.raises(ValueError, match="echo_indices"):
+        ds.simulate_cpmg(10, D, wf, ds.FreeDiffusion())
+
+
+def test_no_cpu_fallback_without_gpu():
+    """Without a device the walk raises; there is no CPU path to fall back to."""
+    if _native.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 50)
+    with pytest.raises(RuntimeError, match="no HIP device"):
+        ds.simulate(10, D, wf, ds.FreeDiffusion(), require_gpu=False)
+    with pytest.raises(RuntimeError):
+        ds.simulate(10, D, wf, ds.FreeDiffusion(), require_gpu=True)
