@@ -183,12 +183,12 @@ struct dmw_plan {
   unsigned grid_x = 0;
   size_t lds = 0;
   dmw::WalkArgs args{};
-  DevBuf Gs, chi, echo, objs, cstart, citems, r0, inj_n, inj_u;
+  DevBuf Gs, GsT, chi, echo, objs, cstart, citems, r0, inj_n, inj_u;
   DevBuf partials, echo_partials, sig;
   DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~dmw_plan() {
-    for (DevBuf* b : {&Gs, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &partials,
+    for (DevBuf* b : {&Gs, &GsT, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &partials,
                       &echo_partials, &sig, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
                       &o_logw, &o_phi, &o_counters})
       b->release();
@@ -212,10 +212,13 @@ int dmw_set_device(int device) {
 
 const char* dmw_last_error(void) { return g_err.c_str(); }
 
+// Measurements per kernel pass.  Blocks of 32 or more without compensation use
+// the matrix-core phase update; the compensated (Kahan) form keeps phase and
+// compensation in VGPRs and is limited to 64 measurements per pass.
 static int pick_meas_block(int n_meas, int requested, bool kahan) {
   if (requested > 0) {
     for (int mb : dmw::kMeasBlocks)
-      if (mb == requested) return mb;
+      if (mb == requested && !(kahan && mb > 64)) return mb;
     return -1;
   }
   if (n_meas <= 4) return 4;
@@ -267,7 +270,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   P->mb = pick_meas_block(wave->n_meas, run->meas_block, P->kahan);
   if (P->mb < 0) {
     delete P;
-    return fail(DMW_EINVAL, "meas_block must be 0, 4, 16, 32, 64 or 128");
+    return fail(DMW_EINVAL, "meas_block must be 0, 4, 16, 32, 64 or 128 (at most 64 with kahan)");
   }
   P->n_chunks = (wave->n_meas + P->mb - 1) / P->mb;
   P->n_meas_pad = P->n_chunks * P->mb;
@@ -293,6 +296,13 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
           gs[((size_t)t * P->n_meas_pad + m) * 3 + c] =
               (float)(gdt * (double)wave->G[((size_t)m * wave->n_t + t) * 3 + c]);
     TRY(upload(P->Gs, gs.data(), gs.size() * 4, st));
+    // Component-major copy for the matrix-core phase update: (n_t, 4, n_meas_pad).
+    std::vector<float> gt((size_t)wave->n_t * 4 * P->n_meas_pad, 0.0f);
+    for (int t = 0; t < wave->n_t; ++t)
+      for (int c = 0; c < 3; ++c)
+        for (int m = 0; m < P->n_meas_pad; ++m)
+          gt[((size_t)t * 4 + c) * P->n_meas_pad + m] = gs[((size_t)t * P->n_meas_pad + m) * 3 + c];
+    TRY(upload(P->GsT, gt.data(), gt.size() * 4, st));
   }
   if (wave->chi) TRY(upload(P->chi, wave->chi, (size_t)wave->n_t * 4, st));
   if (wave->n_echo > 0) TRY(upload(P->echo, wave->echo_idx, (size_t)wave->n_echo * 4, st));
@@ -339,6 +349,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.n_meas = wave->n_meas;
   a.n_meas_pad = P->n_meas_pad;
   a.Gs = P->Gs.as<float>();
+  a.GsT = P->GsT.as<float>();
   a.chi = wave->chi ? P->chi.as<float>() : nullptr;
   a.n_echo = wave->n_echo;
   a.echo_idx = wave->n_echo > 0 ? P->echo.as<int>() : nullptr;

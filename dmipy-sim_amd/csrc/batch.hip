@@ -35,6 +35,7 @@ struct BatchArgs {
   const int2* units;        // (instance, tile) in queue order
   int n_units;
   const float* G;           // (n_t, n_meas_pad, 3) T/m, not scaled
+  const float* GT;          // (n_t, 4, n_meas_pad) component-major, row 3 = 0
   const float* chi;
   int n_t, n_meas, n_meas_pad;
   const unsigned long long* seed;  // device, read at kernel start
@@ -54,16 +55,19 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.size = I.size; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
   a.instance = (uint32_t)inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
   Tables tb{nullptr, nullptr, nullptr};
+  a.GsT = b.GT;
+  a.Gs = b.G;
+  a.n_meas_pad = b.n_meas_pad;
   Pos p = sample_start<GEOM>(w, a, tb);
-  float phi[MB], kc[MB];
-#pragma unroll
-  for (int m = 0; m < MB; ++m) { phi[m] = 0.0f; kc[m] = 0.0f; }
+  PhaseAcc<MB, false> ph;
+  ph.init();
   float logw = 0.0f;
   uint32_t hits = 0, ncross = 0;
   const int n_sub = I.n_sub;
   const float fsub = (float)n_sub;
   u32x4 rnd{0u, 0u, 0u, 0u};
   for (int t = 0; t < b.n_t; ++t) {
+    ph.prefetch(a, t, 0, lane);
     const float chi = b.chi ? b.chi[t] : 1.0f;
     const float x0 = p.x, y0 = p.y, z0 = p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -107,14 +111,10 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
     } else {
       Px = p.x; Py = p.y; Pz = p.z;
     }
-    Px = Px * I.gdt; Py = Py * I.gdt; Pz = Pz * I.gdt;
-    const cfloat* g = (const cfloat*)(b.G + (size_t)t * b.n_meas_pad * 3);
-#pragma unroll
-    for (int m = 0; m < MB; ++m)
-      phi[m] = phi[m] + fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
+    ph.update(a, t, 0, lane, Px * I.gdt, Py * I.gdt, Pz * I.gdt);
   }
   const float wgt = expf(logw);
-  emit_partials<MB, false>(phi, kc, wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
+  ph.emit(wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
   (void)hits; (void)ncross;
 }
 
@@ -165,7 +165,8 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const InstDev* __rest
 struct dmw_batch {
   int n_inst = 0, n_units = 0, n_meas = 0, n_meas_pad = 0, mb = 64, grid = 0;
   uint64_t walker_steps = 0;
-  void *d_inst = nullptr, *d_units = nullptr, *d_uid = nullptr, *d_G = nullptr, *d_chi = nullptr;
+  void *d_inst = nullptr, *d_units = nullptr, *d_uid = nullptr, *d_G = nullptr, *d_GT = nullptr,
+       *d_chi = nullptr;
   void *d_seed = nullptr, *d_queue = nullptr, *d_part = nullptr, *d_out = nullptr;
   unsigned long long* h_seed = nullptr;
   hipStream_t st = nullptr;
@@ -177,7 +178,7 @@ struct dmw_batch {
   ~dmw_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void* p : {d_inst, d_units, d_uid, d_G, d_chi, d_seed, d_queue, d_part, d_out})
+    for (void* p : {d_inst, d_units, d_uid, d_G, d_GT, d_chi, d_seed, d_queue, d_part, d_out})
       if (p) (void)hipFree(p);
     if (h_seed) (void)hipHostFree(h_seed);
     for (hipEvent_t e : {ev0, ev1, ev2})
@@ -288,6 +289,13 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst,
   BT(hipMalloc(&B->d_units, q_units.size() * sizeof(int2)), "hipMalloc");
   BT(hipMalloc(&B->d_uid, order.size() * sizeof(int)), "hipMalloc");
   BT(hipMalloc(&B->d_G, G.size() * 4), "hipMalloc");
+  std::vector<float> GT((size_t)wave->n_t * 4 * B->n_meas_pad, 0.0f);
+  for (int t = 0; t < wave->n_t; ++t)
+    for (int c = 0; c < 3; ++c)
+      for (int m = 0; m < B->n_meas_pad; ++m)
+        GT[((size_t)t * 4 + c) * B->n_meas_pad + m] = G[((size_t)t * B->n_meas_pad + m) * 3 + c];
+  BT(hipMalloc(&B->d_GT, GT.size() * 4), "hipMalloc");
+  BT(hipMemcpy(B->d_GT, GT.data(), GT.size() * 4, hipMemcpyHostToDevice), "H2D");
   if (wave->chi) BT(hipMalloc(&B->d_chi, (size_t)wave->n_t * 4), "hipMalloc");
   BT(hipMalloc(&B->d_seed, 8), "hipMalloc");
   BT(hipMalloc(&B->d_queue, 4), "hipMalloc");
@@ -309,6 +317,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst,
   a.unit_id = (const int*)B->d_uid;
   a.n_units = B->n_units;
   a.G = (const float*)B->d_G;
+  a.GT = (const float*)B->d_GT;
   a.chi = wave->chi ? (const float*)B->d_chi : nullptr;
   a.n_t = wave->n_t;
   a.n_meas = wave->n_meas;

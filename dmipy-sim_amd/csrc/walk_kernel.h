@@ -71,6 +71,7 @@ struct WalkArgs {
   // waveform
   int n_t, n_meas, n_meas_pad;
   const float* Gs;           // device (n_t, n_meas_pad, 3), scaled by gamma*dt_sub
+  const float* GsT;          // device (n_t, 4, n_meas_pad): same, component-major, row 3 = 0
   const float* chi;          // device (n_t) or null
   int n_echo;
   const int* echo_idx;       // device (n_echo)
@@ -660,10 +661,161 @@ __device__ __forceinline__ void emit_partials(const float (&phi)[MB], const floa
 }
 
 // ---------------------------------------------------------------------------
+// Phase accumulators.  Per waveform sample the walk adds, for every measurement
+// m, dphi = Gs[t][m] . P with P the sum of the n_sub sub-step positions.
+// ---------------------------------------------------------------------------
+
+// VALU form: lane = walker, phi[m] (+ Kahan compensation) in VGPRs; G(t) read
+// through the constant address space (scalar loads).  Used for small
+// measurement blocks and for the compensated form.
+template <int MB, bool KAHAN>
+struct PhaseVALU {
+  float phi[MB], kc[MB];
+  __device__ __forceinline__ void init() {
+    static_for<MB>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      phi[m] = 0.0f;
+      kc[m] = 0.0f;
+    });
+  }
+  __device__ __forceinline__ void prefetch(const WalkArgs&, int, int, int) {}
+  __device__ __forceinline__ void update(const WalkArgs& a, int t, int m0, int, float Px, float Py, float Pz) {
+    const cfloat* g = (const cfloat*)(a.Gs + ((size_t)t * a.n_meas_pad + m0) * 3);
+    static_for<MB>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const float dphi = fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
+      if (KAHAN) {
+        const float yk = dphi - kc[m];
+        const float tk = phi[m] + yk;
+        kc[m] = (tk - phi[m]) - yk;
+        phi[m] = tk;
+      } else {
+        phi[m] = phi[m] + dphi;
+      }
+    });
+  }
+  __device__ __forceinline__ void emit(float wgt, bool active, int lane, double* dst) const {
+    emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane, dst);
+  }
+  __device__ __forceinline__ void store_phi(const WalkArgs& a, int m0, long long wave_base, bool active,
+                                            int lane) const {
+    if (!active) return;
+    const long long N = a.n_walkers;
+    static_for<MB>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if (m0 + m < a.n_meas) a.phi[(size_t)(m0 + m) * N + wave_base + lane] = phi[m] - kc[m];
+    });
+  }
+};
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Matrix-core form: per sample the wave computes the (64 walkers x 4) . (4 x MB)
+// product Phi += P . Gs(t) with v_mfma_f32_16x16x4f32 (K = x, y, z, 0).
+// Accumulator tile (wt, nt) holds walkers 16 wt + 4 (lane/16) + r (r = 0..3) and
+// measurement 16 nt + lane%16; A operands are gathered across lanes, B operands
+// are one coalesced load per lane per measurement tile (Gs^T layout).
+template <int MB>
+struct PhaseMFMA {
+  static constexpr int NT = MB / 16;
+  floatx4 acc[4][NT];
+  float bv[NT];
+  __device__ __forceinline__ void init() {
+    static_for<4>([&](auto wc) {
+      static_for<NT>([&](auto nc) {
+        acc[decltype(wc)::value][decltype(nc)::value] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+      });
+    });
+  }
+  // B operands of sample t: lane (k = lane/16, j = lane%16) reads Gs^T[t][k][m0 + 16 nt + j].
+  __device__ __forceinline__ void prefetch(const WalkArgs& a, int t, int m0, int lane) {
+    const float* gt = a.GsT + ((size_t)t * 4 + (lane >> 4)) * a.n_meas_pad + m0 + (lane & 15);
+    static_for<NT>([&](auto nc) {
+      constexpr int nt = decltype(nc)::value;
+      bv[nt] = gt[16 * nt];
+    });
+  }
+  __device__ __forceinline__ void update(const WalkArgs&, int, int, int lane, float Px, float Py, float Pz) {
+    const int kq = lane >> 4, jq = lane & 15;
+    static_for<4>([&](auto wc) {
+      constexpr int wt = decltype(wc)::value;
+      const int src = 16 * wt + jq;
+      const float xs = __shfl(Px, src, kWave), ys = __shfl(Py, src, kWave), zs = __shfl(Pz, src, kWave);
+      const float av = kq == 0 ? xs : (kq == 1 ? ys : (kq == 2 ? zs : 0.0f));
+      static_for<NT>([&](auto nc) {
+        constexpr int nt = decltype(nc)::value;
+        acc[wt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nt], acc[wt][nt], 0, 0, 0);
+      });
+    });
+  }
+  __device__ __forceinline__ void emit(float wgt, bool active, int lane, double* dst) const {
+    const int kq = lane >> 4, jq = lane & 15;
+    float wv[4][4];
+    static_for<4>([&](auto wc) {
+      constexpr int wt = decltype(wc)::value;
+      static_for<4>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const int src = 16 * wt + 4 * kq + r;
+        const float ws = __shfl(wgt, src, kWave);
+        const int as = __shfl(active ? 1 : 0, src, kWave);
+        wv[wt][r] = as ? ws : 0.0f;
+      });
+    });
+    static_for<NT>([&](auto nc) {
+      constexpr int nt = decltype(nc)::value;
+      double re = 0.0, im = 0.0;
+      static_for<4>([&](auto wc) {
+        constexpr int wt = decltype(wc)::value;
+        static_for<4>([&](auto rc) {
+          constexpr int r = decltype(rc)::value;
+          float sn, cs;
+          phase_sincos(acc[wt][nt][r], &sn, &cs);
+          re += (double)(wv[wt][r] * cs);
+          im += (double)(wv[wt][r] * sn);
+        });
+      });
+      re += __shfl_xor(re, 16, kWave);
+      im += __shfl_xor(im, 16, kWave);
+      re += __shfl_xor(re, 32, kWave);
+      im += __shfl_xor(im, 32, kWave);
+      if (lane < 16) {
+        dst[2 * (16 * nt + jq)] = re;
+        dst[2 * (16 * nt + jq) + 1] = im;
+      }
+    });
+  }
+  __device__ __forceinline__ void store_phi(const WalkArgs& a, int m0, long long wave_base, bool,
+                                            int lane) const {
+    const int kq = lane >> 4, jq = lane & 15;
+    const long long N = a.n_walkers;
+    static_for<4>([&](auto wc) {
+      constexpr int wt = decltype(wc)::value;
+      static_for<4>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        const long long w = wave_base + 16 * wt + 4 * kq + r;
+        static_for<NT>([&](auto nc) {
+          constexpr int nt = decltype(nc)::value;
+          const int m = m0 + 16 * nt + jq;
+          if (w < N && m < a.n_meas) a.phi[(size_t)m * N + w] = acc[wt][nt][r];
+        });
+      });
+    });
+  }
+};
+
+// Kahan compensation and blocks narrower than one 16-wide matrix tile run on the
+// VALU; every other block uses the matrix cores.
+template <int MB, bool KAHAN>
+using PhaseAcc = typename std::conditional<(KAHAN || MB < 32), PhaseVALU<MB, KAHAN>, PhaseMFMA<MB>>::type;
+
+template <int MB, bool KAHAN>
+constexpr bool uses_mfma() { return !(KAHAN || MB < 32); }
+
+// ---------------------------------------------------------------------------
 // The walk kernel.  Grid: x = walker tiles of 256, y = measurement blocks.
 // ---------------------------------------------------------------------------
 template <int GEOM, int WALL, int MB, bool KAHAN>
-__global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void walk_kernel(const WalkArgs a) {
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
   constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
   extern __shared__ float4 lds[];
@@ -701,12 +853,9 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
   int comp = classify<GEOM>(p, a, tb);
   if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
 
-  float phi[MB], kc[MB];
-  static_for<MB>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    phi[m] = 0.0f;
-    kc[m] = 0.0f;
-  });
+  PhaseAcc<MB, KAHAN> ph;
+  ph.init();
+  const long long wave_base = wave_id * kWave;
   float logw = 0.0f;
   uint32_t hits = 0, ncross = 0, ncand = 0;
   const float L = a.cell;
@@ -716,6 +865,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
   int next_echo = 0;
 
   for (int t = 0; t < a.n_t; ++t) {
+    ph.prefetch(a, t, m0, lane);
     const float chi = a.chi ? a.chi[t] : 1.0f;
     const float x0 = p.x, y0 = p.y, z0 = p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
@@ -792,21 +942,7 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
         Px = ex; Py = ey; Pz = ez;
       }
     }
-    // G(t) is wave-uniform: read it through the constant address space so the
-    // loads are scalar (s_load) and the values arrive in SGPRs.
-    const cfloat* g = (const cfloat*)(a.Gs + ((size_t)t * a.n_meas_pad + m0) * 3);
-    static_for<MB>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      const float dphi = fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
-      if (KAHAN) {
-        const float yk = dphi - kc[m];
-        const float tk = phi[m] + yk;
-        kc[m] = (tk - phi[m]) - yk;
-        phi[m] = tk;
-      } else {
-        phi[m] = phi[m] + dphi;
-      }
-    });
+    ph.update(a, t, m0, lane, Px, Py, Pz);
     if (PACKED) fold<DIM>(p, a);
     if (a.comp_full) comp = classify<GEOM>(p, a, tb);
     if (primary && active) {
@@ -819,23 +955,17 @@ __global__ __launch_bounds__(kBlock) void walk_kernel(const WalkArgs a) {
     }
     if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
       const float wgt = expf(logw);
-      emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane,
-                               a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
+      ph.emit(wgt, active, lane,
+              a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
       ++next_echo;
     }
   }
 
   if (a.comp_final && !a.comp_full) comp = classify<GEOM>(p, a, tb);
   const float wgt = expf(logw);
-  emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane,
-                           a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
+  ph.emit(wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
+  if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
   if (active) {
-    if (a.phi) {
-      static_for<MB>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
-        if (m0 + m < a.n_meas) a.phi[(size_t)(m0 + m) * N + w] = phi[m] - kc[m];
-      });
-    }
     if (primary) {
       if (a.final_pos) {
         a.final_pos[w] = PACKED ? p.x + L * (float)p.cx : p.x;

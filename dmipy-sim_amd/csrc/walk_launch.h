@@ -45,7 +45,7 @@ hipError_t launch_reduce_partials(const double* in, long long n_rows, int n_cols
       case 16: return k ? go_##NAME<W, 16, true>(g, lds, st, a) : go_##NAME<W, 16, false>(g, lds, st, a); \
       case 32: return k ? go_##NAME<W, 32, true>(g, lds, st, a) : go_##NAME<W, 32, false>(g, lds, st, a); \
       case 64: return k ? go_##NAME<W, 64, true>(g, lds, st, a) : go_##NAME<W, 64, false>(g, lds, st, a); \
-      case 128: return k ? go_##NAME<W, 128, true>(g, lds, st, a) : go_##NAME<W, 128, false>(g, lds, st, a); \
+      case 128: return k ? hipErrorInvalidValue : go_##NAME<W, 128, false>(g, lds, st, a);          \
       default: return hipErrorInvalidValue;                                                     \
     }                                                                                           \
   }                                                                                             \

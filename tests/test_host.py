@@ -192,3 +192,28 @@ def test_no_cpu_fallback_without_gpu():
         ds.simulate(10, D, wf, ds.FreeDiffusion(), require_gpu=False)
     with pytest.raises(RuntimeError):
         ds.simulate(10, D, wf, ds.FreeDiffusion(), require_gpu=True)
+
+
+def test_oom_backoff_halves_walker_batches(monkeypatch):
+    """simulate() retries at half the walker batch on "out of memory"
+    (core.py:366-394) and recombines chunks with seeds seed+1+b (core.py:765-818).
+    The walk is replaced by a host stub that fails above 250 walkers."""
+    from dmipy_sim_amd import core
+    calls = []
+
+    def fake_walk(geometry, diffusivity, waveform, n_walkers, seed=123, **kw):
+        calls.append((n_walkers, seed))
+        if n_walkers > 250:
+            raise RuntimeError("hipMalloc: out of memory")
+        return {"signal": np.full(waveform.G.shape[0], float(seed)), "kernel_ms": 0.0}
+
+    monkeypatch.setattr(core, "walk", fake_walk)
+    monkeypatch.setattr(core, "check_gpu", lambda *a, **k: True)
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 20)
+    with pytest.warns(RuntimeWarning, match="OOM"):
+        s = ds.simulate(1000, D, wf, ds.FreeDiffusion(), seed=10)
+    sizes = [c[0] for c in calls]
+    assert sizes[0] == 1000 and 250 in sizes
+    ok = [c for c in calls if c[0] <= 250]
+    assert [c[1] for c in ok] == [11, 12, 13, 14]
+    assert s[0] == pytest.approx(np.mean([11, 12, 13, 14]))
