@@ -28,11 +28,12 @@ sys.path.insert(0, ROOT)
 METRIC = "walker-steps/s (1 GPU); signals/s surrogate vs MC at RMSE≤5e-3 S/S0"
 PEAK_FP32_TFLOPS = 157.3
 
-# HBM bytes per walk launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, each
-# scaled by the counter's reading on a copy kernel of known size (factors and
-# commands in profiles/walk_hbm_traffic.md).  None until measured.
+# HBM bytes per walk launch from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
+# runs of this script, each scaled by the counter's reading on a 256 MiB
+# device-to-device copy (FETCH x 2.000, WRITE x 1.000; profiles/walk_kernel_cylinder_pgse.md).
+# None where not measured.
 TRAFFIC_BYTES = {
-    "cylinder_pgse": None,
+    "cylinder_pgse": 19_252_929 + 32_172_843,
     "free_pgse": None,
     "packed_cylinders_free_waveform": None,
 }
@@ -122,7 +123,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=6.0)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--meas-block", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="run only a device-to-device copy kernel of BYTES bytes (HBM "
+                         "counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy // 4
+        src_t = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst_t = torch.empty_like(src_t)
+        for _ in range(3):
+            dst_t.copy_(src_t)
+        torch.cuda.synchronize()
+        log(f"copied {args.calibrate_copy} bytes 3 times")
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(walkers shard across GPUs only in the design).", file=sys.stderr)
