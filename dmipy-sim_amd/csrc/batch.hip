@@ -1,14 +1,16 @@
 // batch.hip -- one persistent launch walking thousands of substrate instances.
 //
 // The reference runs one simulate() call per substrate (core.py:193), e.g. the
-// serial loop of simulate_mixture (core.py:821-858).  Here every instance
-// (Cylinder or Sphere, reflecting / relaxing / permeable wall, its own D, radius,
-// rho, kappa, walker count and sub-step count) is cut into 64-walker work units;
-// a persistent grid of wavefronts pops units from a queue ordered by decreasing
-// cost (walkers x n_t x n_sub), so the long permeable instances start first and
-// the short ones fill the tail.  Each unit writes float64 partial sums; a second
-// kernel reduces every instance's units in a fixed order.  The three launches
-// (queue reset, walk, reduce) are captured once in a hipGraph and replayed.
+// serial loop of simulate_mixture (core.py:821-858).  Here every instance (any
+// analytic geometry with its own wall, D, sizes, rho, kappa, walker count,
+// sub-step count and waveform set) is cut into 64-walker work units; a
+// persistent grid of wavefronts pops units from a queue ordered by decreasing
+// cost (n_sub), so the long instances start first and the short ones fill the
+// tail.  Packed instances keep their object tables and periodic neighbour grids
+// in one concatenated device array (read through the L1/L2 caches).  Each unit
+// writes float64 partial sums; a second kernel reduces every instance's units
+// in a fixed order.  The three launches (queue reset, walk, reduce) are
+// captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,14 +21,19 @@
 
 #include "../../include/dmipy_walk.h"
 #include "batch.h"
+#include "grid.h"
 #include "walk_kernel.h"
 
 namespace dmw {
 
 struct InstDev {
   float size, eps, nudge, rhoD, kappaD, step_len, dt_sub, inv_T2, gdt;
+  float cell, inv_cell, grid_inv_h;
   int kind, wall, n_sub, has_t2;
+  int grid_n, exact_minimage, n_obj;
+  int obj_off, start_off, items_off;
   long long n_walkers;
+  long long g_off, gt_off;   // offsets (floats) of the instance's waveform set
   int unit_begin, unit_end;
 };
 
@@ -34,10 +41,13 @@ struct BatchArgs {
   const InstDev* inst;
   const int2* units;        // (instance, tile) in queue order
   int n_units;
-  const float* G;           // (n_t, n_meas_pad, 3) T/m, not scaled
-  const float* GT;          // (n_t, 4, n_meas_pad) component-major, row 3 = 0
+  const float* G;           // sets x (n_t, n_meas_pad, 3) T/m, not scaled
+  const float* GT;          // sets x (n_t, 4, n_meas_pad) component-major, row 3 = 0
   const float* chi;
   int n_t, n_meas, n_meas_pad;
+  const float4* objs;       // concatenated object tables of packed instances
+  const int* starts;        // concatenated grid cell starts
+  const int* items;         // concatenated grid cell lists
   const unsigned long long* seed;  // device, read at kernel start
   unsigned int* queue;      // work counter
   double* partials;         // (n_units, n_meas_pad, 2) indexed by unit id
@@ -47,24 +57,30 @@ struct BatchArgs {
 template <int GEOM, int WALL, int MB>
 __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, int inst, int tile,
                                           int unit, uint32_t seed_lo, uint32_t seed_hi) {
+  constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
+  constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
   const int lane = threadIdx.x & (kWave - 1);
   const long long widx = (long long)tile * kWave + lane;
   const bool active = widx < I.n_walkers;
   const uint32_t w = (uint32_t)(active ? widx : I.n_walkers - 1);
   WalkArgs a{};
   a.size = I.size; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
+  a.cell = I.cell; a.inv_cell = I.inv_cell; a.grid_n = I.grid_n; a.grid_inv_h = I.grid_inv_h;
+  a.exact_minimage = I.exact_minimage; a.n_obj = I.n_obj;
   a.instance = (uint32_t)inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
-  Tables tb{nullptr, nullptr, nullptr};
-  a.GsT = b.GT;
-  a.Gs = b.G;
+  a.Gs = b.G + I.g_off;
+  a.GsT = b.GT + I.gt_off;
   a.n_meas_pad = b.n_meas_pad;
+  Tables tb{nullptr, nullptr, nullptr};
+  if (PACKED) tb = Tables{b.objs + I.obj_off, b.starts + I.start_off, b.items + I.items_off};
   Pos p = sample_start<GEOM>(w, a, tb);
   PhaseAcc<MB, false> ph;
   ph.init();
   float logw = 0.0f;
-  uint32_t hits = 0, ncross = 0;
+  uint32_t hits = 0, ncross = 0, ncand = 0;
   const int n_sub = I.n_sub;
   const float fsub = (float)n_sub;
+  const float L = I.cell;
   u32x4 rnd{0u, 0u, 0u, 0u};
   for (int t = 0; t < b.n_t; ++t) {
     ph.prefetch(a, t, 0, lane);
@@ -79,23 +95,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
       iso_direction((gs & 1u) ? rnd.z : rnd.x, (gs & 1u) ? rnd.w : rnd.y, &ux, &uy, &uz);
       const float sx = ux * I.step_len, sy = uy * I.step_len, sz = uz * I.step_len;
       const PowlesSrc ps{w, (uint32_t)inst, gs, seed_lo, seed_hi, 0.0f, false};
-      float dlog = 0.0f;
-      if (GEOM == G_CYL) {
-        if (WALL == W_PERMEATE) {
-          dlog = cyl_permeate(p.x, p.y, sx, sy, a, ps, hits, ncross);
-        } else {
-          const float ds = cyl_reflect(p.x, p.y, sx, sy, a, hits);
-          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
-        }
-        p.z = p.z + sz;
-      } else {
-        if (WALL == W_PERMEATE) {
-          dlog = sph_permeate(p, sx, sy, sz, a, ps, hits, ncross);
-        } else {
-          const float ds = sph_reflect(p, sx, sy, sz, a, hits);
-          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
-        }
-      }
+      float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       if (WALL != W_REFLECT) dlog = dlog * chi;
       if (I.has_t2) dlog = dlog - (I.dt_sub * I.inv_T2) * chi;
       logw = logw + dlog;
@@ -105,39 +105,64 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
         az = az + (p.z - z0);
       }
     }
+    float ox = x0, oy = y0, oz = z0, ex = p.x, ey = p.y, ez = p.z;
+    if (PACKED) {
+      ox = ox + L * (float)p.cx; oy = oy + L * (float)p.cy; oz = oz + L * (float)p.cz;
+      ex = ex + L * (float)p.cx; ey = ey + L * (float)p.cy; ez = ez + L * (float)p.cz;
+    }
     float Px, Py, Pz;
     if (n_sub > 1) {
-      Px = fmaf(fsub, x0, ax); Py = fmaf(fsub, y0, ay); Pz = fmaf(fsub, z0, az);
+      Px = fmaf(fsub, ox, ax); Py = fmaf(fsub, oy, ay); Pz = fmaf(fsub, oz, az);
     } else {
-      Px = p.x; Py = p.y; Pz = p.z;
+      Px = ex; Py = ey; Pz = ez;
     }
     ph.update(a, t, 0, lane, Px * I.gdt, Py * I.gdt, Pz * I.gdt);
+    if (PACKED) fold<DIM>(p, a);
   }
   const float wgt = expf(logw);
   ph.emit(wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
-  (void)hits; (void)ncross;
+  (void)hits; (void)ncross; (void)ncand;
 }
 
 template <int MB>
-__global__ __launch_bounds__(kBlock) void batch_kernel(const BatchArgs b) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
   const uint32_t seed_lo = (uint32_t)seed, seed_hi = (uint32_t)(seed >> 32);
+  const unsigned int inc = (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u;
   for (;;) {
-    int pos = 0;
-    if ((threadIdx.x & (kWave - 1)) == 0) pos = (int)atomicAdd(b.queue, 1u);
-    pos = __shfl(pos, 0, kWave);
-    if (pos >= b.n_units) break;  // wave-uniform: every wave reaches this exit
+    // Pop one unit per wave.  Every lane executes the atomic (lane 0 adds 1, the
+    // others 0) so the pop sits in uniform control flow, and the position is
+    // taken from lane 0 with readfirstlane: a scalar, so the exit test, the
+    // instance loads and the dispatch below are uniform branches.  (A pop under
+    // `if (lane == 0)` followed by a lane shuffle lets the optimiser thread the
+    // loop back edge through the lane test, and lanes 1..63 then shuffle a
+    // position lane 0 has not fetched.)
+    const unsigned int old = atomicAdd(b.queue, inc);
+    const int pos = (int)__builtin_amdgcn_readfirstlane(old);
+    if (pos >= b.n_units) break;  // uniform: every wave reaches this exit
     const int2 u = b.units[pos];
     const int unit = b.unit_id[pos];
     const InstDev I = b.inst[u.x];
-    const int key = I.kind * 3 + I.wall;
-    switch (key) {
-      case G_CYL * 3 + W_REFLECT: walk_unit<G_CYL, W_REFLECT, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
-      case G_CYL * 3 + W_SURFACE: walk_unit<G_CYL, W_SURFACE, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
-      case G_CYL * 3 + W_PERMEATE: walk_unit<G_CYL, W_PERMEATE, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
-      case G_SPH * 3 + W_REFLECT: walk_unit<G_SPH, W_REFLECT, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
-      case G_SPH * 3 + W_SURFACE: walk_unit<G_SPH, W_SURFACE, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
-      default: walk_unit<G_SPH, W_PERMEATE, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
+    switch (I.kind * 3 + I.wall) {
+#define DMW_UNIT(G, W) \
+      case G * 3 + W: walk_unit<G, W, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
+      DMW_UNIT(G_FREE, W_REFLECT)
+      DMW_UNIT(G_BOX1D, W_REFLECT)
+      DMW_UNIT(G_BOX1D, W_SURFACE)
+      DMW_UNIT(G_CYL, W_REFLECT)
+      DMW_UNIT(G_CYL, W_SURFACE)
+      DMW_UNIT(G_CYL, W_PERMEATE)
+      DMW_UNIT(G_SPH, W_REFLECT)
+      DMW_UNIT(G_SPH, W_SURFACE)
+      DMW_UNIT(G_SPH, W_PERMEATE)
+      DMW_UNIT(G_PCYL, W_REFLECT)
+      DMW_UNIT(G_PCYL, W_SURFACE)
+      DMW_UNIT(G_PCYL, W_PERMEATE)
+      DMW_UNIT(G_PSPH, W_REFLECT)
+      DMW_UNIT(G_PSPH, W_SURFACE)
+      DMW_UNIT(G_PSPH, W_PERMEATE)
+#undef DMW_UNIT
+      default: break;
     }
   }
 }
@@ -165,25 +190,32 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const InstDev* __rest
 struct dmw_batch {
   int n_inst = 0, n_units = 0, n_meas = 0, n_meas_pad = 0, mb = 64, grid = 0;
   uint64_t walker_steps = 0;
-  void *d_inst = nullptr, *d_units = nullptr, *d_uid = nullptr, *d_G = nullptr, *d_GT = nullptr,
-       *d_chi = nullptr;
-  void *d_seed = nullptr, *d_queue = nullptr, *d_part = nullptr, *d_out = nullptr;
+  std::vector<void*> allocs;
+  void *d_inst = nullptr, *d_seed = nullptr, *d_queue = nullptr, *d_part = nullptr, *d_out = nullptr;
   unsigned long long* h_seed = nullptr;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool use_graph = true;
   dmw::BatchArgs args{};
   ~dmw_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (void* p : {d_inst, d_units, d_uid, d_G, d_GT, d_chi, d_seed, d_queue, d_part, d_out})
-      if (p) (void)hipFree(p);
+    for (void* p : allocs) (void)hipFree(p);
     if (h_seed) (void)hipHostFree(h_seed);
-    for (hipEvent_t e : {ev0, ev1, ev2})
+    for (hipEvent_t e : {ev0, ev1})
       if (e) (void)hipEventDestroy(e);
     if (st) (void)hipStreamDestroy(st);
+  }
+  hipError_t upload(void** dst, const void* src, size_t bytes) {
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return e;
+    allocs.push_back(p);
+    *dst = p;
+    if (bytes && src) e = hipMemcpy(p, src, bytes, hipMemcpyHostToDevice);
+    return e;
   }
 };
 
@@ -209,52 +241,88 @@ hipError_t enqueue_batch(dmw_batch* B) {
 
 extern "C" {
 
-int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst,
-                     const dmw_waveform* wave, int32_t waves_per_simd, int32_t use_graph) {
-  if (!out || !inst || !wave || n_inst < 1) return dmw::batch_fail(DMW_EINVAL, "null or empty batch");
+int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, int32_t n_sets,
+                     const dmw_waveform* sets, int32_t waves_per_simd, int32_t use_graph) {
+  if (!out || !inst || !sets || n_inst < 1 || n_sets < 1) return dmw::batch_fail(DMW_EINVAL, "null or empty batch");
   *out = nullptr;
-  if (wave->n_meas < 1 || wave->n_meas > 64 || wave->n_t < 1 || !wave->G)
+  const int n_meas = sets[0].n_meas, n_t = sets[0].n_t;
+  if (n_meas < 1 || n_meas > 64 || n_t < 1)
     return dmw::batch_fail(DMW_EINVAL, "batched launch needs 1..64 measurements");
+  for (int s = 0; s < n_sets; ++s)
+    if (sets[s].n_meas != n_meas || sets[s].n_t != n_t || !sets[s].G)
+      return dmw::batch_fail(DMW_EINVAL, "every waveform set needs the same n_meas and n_t");
+  const bool have_chi = sets[0].chi != nullptr;
   for (int i = 0; i < n_inst; ++i) {
     const dmw_instance& I = inst[i];
-    if (I.geom.kind != DMW_CYLINDER && I.geom.kind != DMW_SPHERE)
-      return dmw::batch_fail(DMW_EUNSUPPORTED, "batched instances must be Cylinder or Sphere");
-    if (I.geom.wall < 0 || I.geom.wall > 2 || I.n_walkers < 1 || I.n_sub < 1)
+    if (I.geom.kind < DMW_FREE || I.geom.kind > DMW_PACKED_SPHERES) return dmw::batch_fail(DMW_EINVAL, "unknown geometry kind");
+    if (I.geom.wall < 0 || I.geom.wall > 2 || I.n_walkers < 1 || I.n_sub < 1 || I.wave_set < 0 || I.wave_set >= n_sets)
       return dmw::batch_fail(DMW_EINVAL, "bad instance");
+    if ((I.geom.kind == DMW_FREE && I.geom.wall != DMW_WALL_REFLECT) ||
+        (I.geom.kind == DMW_BOX1D && I.geom.wall == DMW_WALL_PERMEATE))
+      return dmw::batch_fail(DMW_EUNSUPPORTED, "wall mode not available for this geometry");
+    const bool packed = I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES;
+    if (packed && (I.geom.n_obj < 1 || !I.geom.centers || !I.geom.radii || !(I.geom.cell > 0.0f)))
+      return dmw::batch_fail(DMW_EINVAL, "packed instance needs objects and a positive cell side");
   }
   int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return dmw::batch_fail(DMW_ENODEV, "no HIP device");
+  if (hipGetDevice(&dev) != hipSuccess) return dmw::batch_fail(DMW_ENODEV, "no HIP device");
   dmw_batch* B = new dmw_batch();
   B->n_inst = n_inst;
-  B->n_meas = wave->n_meas;
-  B->mb = wave->n_meas <= 16 ? 16 : 64;
+  B->n_meas = n_meas;
+  B->mb = n_meas <= 16 ? 16 : 64;
   B->n_meas_pad = B->mb;
   B->use_graph = use_graph != 0;
+  const int Mp = B->n_meas_pad;
 
   std::vector<dmw::InstDev> hi(n_inst);
   std::vector<int2> units;
   std::vector<double> cost;
+  std::vector<float4> objs;
+  std::vector<int> starts, items;
   for (int i = 0; i < n_inst; ++i) {
     const dmw_instance& I = inst[i];
     dmw::InstDev& d = hi[i];
     d.size = I.geom.size; d.eps = I.geom.eps; d.nudge = I.geom.nudge;
     d.rhoD = I.geom.rho_over_D; d.kappaD = I.geom.kappa_over_D;
     d.step_len = I.step_len; d.dt_sub = I.dt_sub; d.inv_T2 = I.inv_T2; d.gdt = I.gamma_dt_sub;
-    d.kind = I.geom.kind == DMW_CYLINDER ? dmw::G_CYL : dmw::G_SPH;
+    d.kind = I.geom.kind;
     d.wall = I.geom.wall; d.n_sub = I.n_sub; d.has_t2 = I.has_t2;
     d.n_walkers = I.n_walkers;
+    d.cell = I.geom.cell;
+    d.inv_cell = I.geom.cell > 0.0f ? 1.0f / I.geom.cell : 0.0f;
+    d.g_off = (long long)I.wave_set * n_t * Mp * 3;
+    d.gt_off = (long long)I.wave_set * n_t * 4 * Mp;
+    d.grid_n = 1; d.grid_inv_h = 0.0f; d.exact_minimage = 0; d.n_obj = 0;
+    d.obj_off = d.start_off = d.items_off = 0;
+    if (I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES) {
+      const int dim = I.geom.kind == DMW_PACKED_CYLINDERS ? 2 : 3;
+      Grid G = build_grid(I.geom, I.step_len, dim);
+      d.obj_off = (int)objs.size();
+      d.start_off = (int)starts.size();
+      d.items_off = (int)items.size();
+      for (int k = 0; k < I.geom.n_obj; ++k) {
+        float4 o = make_float4(0.0f, 0.0f, 0.0f, I.geom.radii[k]);
+        o.x = I.geom.centers[k * dim];
+        o.y = I.geom.centers[k * dim + 1];
+        if (dim == 3) o.z = I.geom.centers[k * dim + 2];
+        objs.push_back(o);
+      }
+      starts.insert(starts.end(), G.start.begin(), G.start.end());
+      items.insert(items.end(), G.items.begin(), G.items.end());
+      d.grid_n = G.n;
+      d.grid_inv_h = 1.0f / G.h;
+      d.exact_minimage = G.exact_minimage ? 1 : 0;
+      d.n_obj = I.geom.n_obj;
+    }
     d.unit_begin = (int)units.size();
     const int tiles = (int)((I.n_walkers + dmw::kWave - 1) / dmw::kWave);
+    const double packed_cost = (I.geom.kind >= DMW_PACKED_CYLINDERS) ? 2.0 : 1.0;
     for (int j = 0; j < tiles; ++j) {
       units.push_back(make_int2(i, j));
-      const long long nw = std::min<long long>(dmw::kWave, I.n_walkers - (long long)j * dmw::kWave);
-      // Lanes of a partial tile still run, so the cost is the full tile.
-      (void)nw;
-      cost.push_back((double)I.n_sub * (1.0 + (I.geom.wall == DMW_WALL_PERMEATE ? 0.3 : 0.0)));
+      cost.push_back((double)I.n_sub * packed_cost * (I.geom.wall == DMW_WALL_PERMEATE ? 1.3 : 1.0));
     }
     d.unit_end = (int)units.size();
-    B->walker_steps += (uint64_t)I.n_walkers * (uint64_t)wave->n_t * (uint64_t)I.n_sub;
+    B->walker_steps += (uint64_t)I.n_walkers * (uint64_t)n_t * (uint64_t)I.n_sub;
   }
   B->n_units = (int)units.size();
   std::vector<int> order(B->n_units);
@@ -263,11 +331,15 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst,
   std::vector<int2> q_units(B->n_units);
   for (int k = 0; k < B->n_units; ++k) q_units[k] = units[order[k]];
 
-  std::vector<float> G((size_t)wave->n_t * B->n_meas_pad * 3, 0.0f);
-  for (int m = 0; m < wave->n_meas; ++m)
-    for (int t = 0; t < wave->n_t; ++t)
-      for (int c = 0; c < 3; ++c)
-        G[((size_t)t * B->n_meas_pad + m) * 3 + c] = wave->G[((size_t)m * wave->n_t + t) * 3 + c];
+  std::vector<float> G((size_t)n_sets * n_t * Mp * 3, 0.0f), GT((size_t)n_sets * n_t * 4 * Mp, 0.0f);
+  for (int s = 0; s < n_sets; ++s)
+    for (int m = 0; m < n_meas; ++m)
+      for (int t = 0; t < n_t; ++t)
+        for (int c = 0; c < 3; ++c) {
+          const float v = sets[s].G[((size_t)m * n_t + t) * 3 + c];
+          G[(((size_t)s * n_t + t) * Mp + m) * 3 + c] = v;
+          GT[(((size_t)s * n_t + t) * 4 + c) * Mp + m] = v;
+        }
 
   int n_cu = 256;
   hipDeviceProp_t prop;
@@ -279,49 +351,44 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst,
     delete B;
     return dmw::batch_hip_fail(err, what);
   };
-#define BT(x, what)                              \
-  do {                                           \
-    hipError_t err_ = (x);                       \
+  void *d_units, *d_uid, *d_G, *d_GT, *d_chi = nullptr, *d_objs, *d_starts, *d_items;
+#define BT(x, what)                                       \
+  do {                                                    \
+    hipError_t err_ = (x);                                \
     if (err_ != hipSuccess) return fail_free(err_, what); \
   } while (0)
   BT(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking), "hipStreamCreate");
-  BT(hipMalloc(&B->d_inst, hi.size() * sizeof(dmw::InstDev)), "hipMalloc");
-  BT(hipMalloc(&B->d_units, q_units.size() * sizeof(int2)), "hipMalloc");
-  BT(hipMalloc(&B->d_uid, order.size() * sizeof(int)), "hipMalloc");
-  BT(hipMalloc(&B->d_G, G.size() * 4), "hipMalloc");
-  std::vector<float> GT((size_t)wave->n_t * 4 * B->n_meas_pad, 0.0f);
-  for (int t = 0; t < wave->n_t; ++t)
-    for (int c = 0; c < 3; ++c)
-      for (int m = 0; m < B->n_meas_pad; ++m)
-        GT[((size_t)t * 4 + c) * B->n_meas_pad + m] = G[((size_t)t * B->n_meas_pad + m) * 3 + c];
-  BT(hipMalloc(&B->d_GT, GT.size() * 4), "hipMalloc");
-  BT(hipMemcpy(B->d_GT, GT.data(), GT.size() * 4, hipMemcpyHostToDevice), "H2D");
-  if (wave->chi) BT(hipMalloc(&B->d_chi, (size_t)wave->n_t * 4), "hipMalloc");
-  BT(hipMalloc(&B->d_seed, 8), "hipMalloc");
-  BT(hipMalloc(&B->d_queue, 4), "hipMalloc");
-  BT(hipMalloc(&B->d_part, (size_t)B->n_units * B->n_meas_pad * 2 * 8), "hipMalloc");
-  BT(hipMalloc(&B->d_out, (size_t)n_inst * B->n_meas_pad * 2 * 8), "hipMalloc");
+  BT(B->upload(&B->d_inst, hi.data(), hi.size() * sizeof(dmw::InstDev)), "batch upload");
+  BT(B->upload(&d_units, q_units.data(), q_units.size() * sizeof(int2)), "batch upload");
+  BT(B->upload(&d_uid, order.data(), order.size() * sizeof(int)), "batch upload");
+  BT(B->upload(&d_G, G.data(), G.size() * 4), "batch upload");
+  BT(B->upload(&d_GT, GT.data(), GT.size() * 4), "batch upload");
+  if (have_chi) BT(B->upload(&d_chi, sets[0].chi, (size_t)n_t * 4), "batch upload");
+  BT(B->upload(&d_objs, objs.data(), objs.size() * sizeof(float4)), "batch upload");
+  BT(B->upload(&d_starts, starts.data(), starts.size() * 4), "batch upload");
+  BT(B->upload(&d_items, items.data(), items.size() * 4), "batch upload");
+  BT(B->upload(&B->d_seed, nullptr, 8), "batch alloc");
+  BT(B->upload(&B->d_queue, nullptr, 4), "batch alloc");
+  BT(B->upload(&B->d_part, nullptr, (size_t)B->n_units * Mp * 2 * 8), "batch alloc");
+  BT(B->upload(&B->d_out, nullptr, (size_t)n_inst * Mp * 2 * 8), "batch alloc");
   BT(hipHostMalloc((void**)&B->h_seed, 8, hipHostMallocDefault), "hipHostMalloc");
-  BT(hipMemcpy(B->d_inst, hi.data(), hi.size() * sizeof(dmw::InstDev), hipMemcpyHostToDevice), "H2D");
-  BT(hipMemcpy(B->d_units, q_units.data(), q_units.size() * sizeof(int2), hipMemcpyHostToDevice), "H2D");
-  BT(hipMemcpy(B->d_uid, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice), "H2D");
-  BT(hipMemcpy(B->d_G, G.data(), G.size() * 4, hipMemcpyHostToDevice), "H2D");
-  if (wave->chi) BT(hipMemcpy(B->d_chi, wave->chi, (size_t)wave->n_t * 4, hipMemcpyHostToDevice), "H2D");
   BT(hipEventCreate(&B->ev0), "hipEventCreate");
   BT(hipEventCreate(&B->ev1), "hipEventCreate");
-  BT(hipEventCreate(&B->ev2), "hipEventCreate");
 
   dmw::BatchArgs& a = B->args;
   a.inst = (const dmw::InstDev*)B->d_inst;
-  a.units = (const int2*)B->d_units;
-  a.unit_id = (const int*)B->d_uid;
+  a.units = (const int2*)d_units;
+  a.unit_id = (const int*)d_uid;
   a.n_units = B->n_units;
-  a.G = (const float*)B->d_G;
-  a.GT = (const float*)B->d_GT;
-  a.chi = wave->chi ? (const float*)B->d_chi : nullptr;
-  a.n_t = wave->n_t;
-  a.n_meas = wave->n_meas;
-  a.n_meas_pad = B->n_meas_pad;
+  a.G = (const float*)d_G;
+  a.GT = (const float*)d_GT;
+  a.chi = have_chi ? (const float*)d_chi : nullptr;
+  a.n_t = n_t;
+  a.n_meas = n_meas;
+  a.n_meas_pad = Mp;
+  a.objs = (const float4*)d_objs;
+  a.starts = (const int*)d_starts;
+  a.items = (const int*)d_items;
   a.seed = (const unsigned long long*)B->d_seed;
   a.queue = (unsigned int*)B->d_queue;
   a.partials = (double*)B->d_part;

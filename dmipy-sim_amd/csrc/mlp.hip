@@ -1,0 +1,636 @@
+// mlp.hip -- the learned surrogate: a GELU MLP with BF16 weights and FP32
+// accumulation on the matrix cores (v_mfma_f32_16x16x32_bf16), trained by its
+// own forward, backward and Adam kernels.
+//
+// No counterpart in the reference; it maps substrate parameters plus a
+// waveform embedding to the walk's signal and is trained on signals the walk
+// produces (dmipy_sim_amd/surrogate.py).
+//
+// Layers: x (n_in, padded to a multiple of 32) -> [W_l, b_l, GELU] x depth ->
+// w_out . h + b_out.  Per layer l the forward pass stores Z_l = X_l W_l + b_l
+// (fp32) and Y_l = GELU(Z_l) (bf16, the next layer's input).  Backward:
+//   dZ_out-layer    = dy * w_out * GELU'(Z_last)                 (bf16)
+//   dY_{l-1}        = dZ_l . W_l^T, dZ_{l-1} = dY_{l-1} GELU'(Z_{l-1})
+//   dW_l            = X_l^T . dZ_l (fp32),  db_l = sum_rows dZ_l
+// Master weights, gradients and Adam moments are fp32; the MFMA operands are
+// bf16 copies refreshed by the Adam kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dmipy_walk.h"
+#include "batch.h"
+
+namespace dmw {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
+__device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
+
+// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi).
+__device__ __forceinline__ float gelu(float x) {
+  const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
+  return 0.5f * x * (1.0f + tanhf(u));
+}
+__device__ __forceinline__ float gelu_grad(float x) {
+  const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
+  const float th = tanhf(u);
+  return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * 0.7978845608f * (1.0f + 0.134145f * x * x);
+}
+
+enum Epi { EPI_FWD = 0, EPI_DX = 1, EPI_DW = 2 };
+
+constexpr int TM = 64, TN = 64, TK = 32, LDSK = 40;  // tile and padded LDS row (bf16)
+
+// C[M x N] = A[M x K] . B[K x N] with bf16 operands and fp32 accumulation.
+//   TA: A stored transposed (element (m,k) at A[k*lda + m]); else A[m*lda + k].
+//   TB: B stored transposed (element (k,n) at B[n*ldb + k]); else B[k*ldb + n].
+// M, N multiples of 64, K a multiple of 32 (checked on the host).  256 threads:
+// 4 wavefronts in a 2x2 grid, each owning a 32x32 block = 2x2 MFMA 16x16 tiles.
+template <int EPI, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                                 const __bf16* __restrict__ B, int ldb,
+                                                 const float* __restrict__ bias, float* __restrict__ Zout,
+                                                 __bf16* __restrict__ Yout, const float* __restrict__ Zprev,
+                                                 float* __restrict__ Cout, int ldc) {
+  __shared__ __bf16 As[TM][LDSK];
+  __shared__ __bf16 Bs[TN][LDSK];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    // A tile 64 x 32: each thread moves 8 elements.
+    if (!TA) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + r) * lda + k0 + c);
+      *reinterpret_cast<bf16x8*>(&As[r][c]) = v;
+    } else {
+      const int kk = tid >> 3, mm = (tid & 7) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(A + (size_t)(k0 + kk) * lda + m0 + mm);
+      for (int e = 0; e < 8; ++e) As[mm + e][kk] = v[e];
+    }
+    // B tile 32 x 64 stored as Bs[n][k].
+    if (TB) {
+      const int r = tid >> 2, c = (tid & 3) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + r) * ldb + k0 + c);
+      *reinterpret_cast<bf16x8*>(&Bs[r][c]) = v;
+    } else {
+      const int kk = tid >> 3, nn = (tid & 7) * 8;
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(B + (size_t)(k0 + kk) * ldb + n0 + nn);
+      for (int e = 0; e < 8; ++e) Bs[nn + e][kk] = v[e];
+    }
+    __syncthreads();
+    const int kq = (lane >> 4) * 8, rq = lane & 15;
+    bf16x8 af[2], bfr[2];
+    for (int i = 0; i < 2; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + rq][kq]);
+    for (int j = 0; j < 2; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + rq][kq]);
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+  // D layout: lane holds column (lane % 16), rows 4 (lane / 16) + r of each tile.
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        const int n = n0 + wn + 16 * j + (lane & 15);
+        const float v = acc[i][j][r];
+        if (EPI == EPI_FWD) {
+          const float z = v + bias[n];
+          Zout[(size_t)m * ldc + n] = z;
+          Yout[(size_t)m * ldc + n] = f2bf(gelu(z));
+        } else if (EPI == EPI_DX) {
+          Yout[(size_t)m * ldc + n] = f2bf(v * gelu_grad(Zprev[(size_t)m * ldc + n]));
+        } else {
+          Cout[(size_t)m * ldc + n] = v;
+        }
+      }
+}
+
+// db[n] = sum_m dZ[m][n] (fp32 sum of bf16 values), one thread per column.
+__global__ void col_sum_kernel(int M, int N, const __bf16* __restrict__ dZ, float* __restrict__ db) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += bf2f(dZ[(size_t)m * N + n]);
+  db[n] = s;
+}
+
+// y[b] = Y[b] . w + b0; with targets also the squared error and dy = 2 (y - t) / B.
+__global__ void out_fwd_kernel(int B, int H, const __bf16* __restrict__ Y, const float* __restrict__ w,
+                               const float* __restrict__ b0, const float* __restrict__ t,
+                               float* __restrict__ y, float* __restrict__ dy, float* __restrict__ sq) {
+  const int b = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= B) return;
+  float s = 0.0f;
+  for (int i = lane; i < H; i += 64) s += bf2f(Y[(size_t)b * H + i]) * w[i];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) {
+    const float v = s + b0[0];
+    y[b] = v;
+    if (t) {
+      const float e = v - t[b];
+      dy[b] = 2.0f * e / (float)B;
+      sq[b] = e * e;
+    }
+  }
+}
+
+// Output layer backward: dZ_last[b][i] = dy[b] w[i] GELU'(Z_last[b][i]).
+__global__ void out_bwd_dz_kernel(int B, int H, const float* __restrict__ dy, const float* __restrict__ w,
+                                  const float* __restrict__ Z, __bf16* __restrict__ dZ) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)B * H) return;
+  const int i = (int)(idx % H);
+  const int b = (int)(idx / H);
+  dZ[idx] = f2bf(dy[b] * w[i] * gelu_grad(Z[idx]));
+}
+
+// dw[i] = sum_b Y[b][i] dy[b]; db0 = sum_b dy[b] (thread H computes the bias).
+__global__ void out_bwd_w_kernel(int B, int H, const __bf16* __restrict__ Y, const float* __restrict__ dy,
+                                 float* __restrict__ dw, float* __restrict__ db0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > H) return;
+  float s = 0.0f;
+  if (i < H) {
+    for (int b = 0; b < B; ++b) s += bf2f(Y[(size_t)b * H + i]) * dy[b];
+    dw[i] = s;
+  } else {
+    for (int b = 0; b < B; ++b) s += dy[b];
+    db0[0] = s;
+  }
+}
+
+// Adam with bias correction; refreshes the bf16 weight copy of matrix entries.
+__global__ void adam_kernel(long long n, float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m1, float* __restrict__ m2, float lr, float b1, float b2,
+                            float eps, float c1, float c2, const int* __restrict__ bf_index,
+                            __bf16* __restrict__ pbf) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.0f - b1) * gi;
+  const float v = b2 * m2[i] + (1.0f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = v;
+  const float upd = lr * (a / c1) / (sqrtf(v / c2) + eps);
+  const float np = p[i] - upd;
+  p[i] = np;
+  const int j = bf_index[i];
+  if (j >= 0) pbf[j] = f2bf(np);
+}
+
+__global__ void to_bf16_kernel(long long n, const float* __restrict__ p, const int* __restrict__ bf_index,
+                               __bf16* __restrict__ pbf) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = bf_index[i];
+  if (j >= 0) pbf[j] = f2bf(p[i]);
+}
+
+// Gather a training batch: rows idx[b] of the resident dataset.
+__global__ void gather_kernel(int B, int K, const int* __restrict__ idx, const __bf16* __restrict__ X,
+                              const float* __restrict__ T, __bf16* __restrict__ Xb, float* __restrict__ Tb) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (size_t)B * K) return;
+  const int b = (int)(e / K), k = (int)(e % K);
+  const int r = idx[b];
+  Xb[e] = X[(size_t)r * K + k];
+  if (k == 0) Tb[b] = T[r];
+}
+
+__global__ void f32_to_bf16_kernel(long long n, const float* __restrict__ a, __bf16* __restrict__ b) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+}  // namespace dmw
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+struct dmw_mlp {
+  int n_in = 0, k0 = 0, H = 0, depth = 0, max_batch = 0;
+  long long n_params = 0;          // internal (padded) parameter count
+  std::vector<long long> w_off, b_off;  // offsets of W_l, b_l in the flat fp32 vector
+  long long wout_off = 0, bout_off = 0;
+  std::vector<long long> wbf_off;  // offsets of W_l in the bf16 copy
+  long long n_bf = 0;
+  hipStream_t st = nullptr;
+  float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  int* bf_index = nullptr;
+  __bf16* pbf = nullptr;
+  std::vector<__bf16*> Y;   // Y[0] = input batch (B x k0), Y[l+1] = GELU(Z_l) (B x H)
+  std::vector<float*> Z;    // Z[l] (B x H)
+  __bf16 *dZa = nullptr, *dZb = nullptr;
+  float *out = nullptr, *dy = nullptr, *sq = nullptr, *tb = nullptr;
+  // resident dataset
+  __bf16* dX = nullptr;
+  float* dT = nullptr;
+  long long n_data = 0;
+  int* dIdx = nullptr;
+  long long idx_cap = 0;
+  int adam_t = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  std::vector<void*> allocs;
+  ~dmw_mlp() {
+    for (void* a : allocs) (void)hipFree(a);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+  }
+  template <class T>
+  hipError_t alloc(T** ptr, size_t count) {
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, count * sizeof(T));
+    if (e == hipSuccess) {
+      allocs.push_back(q);
+      *ptr = reinterpret_cast<T*>(q);
+    }
+    return e;
+  }
+};
+
+namespace {
+
+#define MLP_TRY(x, what)                                   \
+  do {                                                     \
+    hipError_t e_ = (x);                                   \
+    if (e_ != hipSuccess) return dmw::batch_hip_fail(e_, what); \
+  } while (0)
+
+template <int EPI, bool TA, bool TB>
+hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
+                const float* bias, float* Z, __bf16* Y, const float* Zprev, float* C, int ldc) {
+  hipLaunchKernelGGL((dmw::gemm_bf16<EPI, TA, TB>), dim3(N / dmw::TN, M / dmw::TM), dim3(256), 0, st, M, N,
+                     K, A, lda, B, ldb, bias, Z, Y, Zprev, C, ldc);
+  return hipGetLastError();
+}
+
+unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
+
+// Forward pass of a batch already in m->Y[0]; logits in m->out; with targets
+// also dy and squared errors.
+int forward(dmw_mlp* m, int B, const float* targets_dev) {
+  hipStream_t st = m->st;
+  for (int l = 0; l < m->depth; ++l) {
+    const int K = l == 0 ? m->k0 : m->H;
+    MLP_TRY((gemm<dmw::EPI_FWD, false, false>(st, B, m->H, K, m->Y[l], K, m->pbf + m->wbf_off[l], m->H,
+                                              m->p + m->b_off[l], m->Z[l], m->Y[l + 1], nullptr, nullptr,
+                                              m->H)),
+            "mlp forward gemm");
+  }
+  hipLaunchKernelGGL(dmw::out_fwd_kernel, dim3(blocks(B, 4)), dim3(256), 0, st, B, m->H, m->Y[m->depth],
+                     m->p + m->wout_off, m->p + m->bout_off, targets_dev, m->out, m->dy, m->sq);
+  MLP_TRY(hipGetLastError(), "mlp output");
+  return DMW_OK;
+}
+
+int backward(dmw_mlp* m, int B) {
+  hipStream_t st = m->st;
+  const int H = m->H, D = m->depth;
+  hipLaunchKernelGGL(dmw::out_bwd_w_kernel, dim3(blocks(H + 1, 256)), dim3(256), 0, st, B, H, m->Y[D], m->dy,
+                     m->g + m->wout_off, m->g + m->bout_off);
+  hipLaunchKernelGGL(dmw::out_bwd_dz_kernel, dim3(blocks((long long)B * H, 256)), dim3(256), 0, st, B, H,
+                     m->dy, m->p + m->wout_off, m->Z[D - 1], m->dZa);
+  MLP_TRY(hipGetLastError(), "mlp output backward");
+  __bf16* dz = m->dZa;
+  __bf16* dz_next = m->dZb;
+  for (int l = D - 1; l >= 0; --l) {
+    const int K = l == 0 ? m->k0 : H;
+    // dW_l = Y_l^T dZ_l  (K x H), db_l = colsum dZ_l
+    MLP_TRY((gemm<dmw::EPI_DW, true, false>(st, K, H, B, m->Y[l], K, dz, H, nullptr, nullptr, nullptr, nullptr,
+                                            m->g + m->w_off[l], H)),
+            "mlp dW gemm");
+    hipLaunchKernelGGL(dmw::col_sum_kernel, dim3(blocks(H, 256)), dim3(256), 0, st, B, H, dz,
+                       m->g + m->b_off[l]);
+    MLP_TRY(hipGetLastError(), "mlp bias grad");
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l^T) * GELU'(Z_{l-1})
+      MLP_TRY((gemm<dmw::EPI_DX, false, true>(st, B, H, H, dz, H, m->pbf + m->wbf_off[l], H, nullptr, nullptr,
+                                              dz_next, m->Z[l - 1], nullptr, H)),
+              "mlp dX gemm");
+      std::swap(dz, dz_next);
+    }
+  }
+  return DMW_OK;
+}
+
+int upload_batch(dmw_mlp* m, const float* x, int B) {
+  // host fp32 (B, n_in) -> device bf16 (B, k0) with zero padding
+  std::vector<float> pad((size_t)B * m->k0, 0.0f);
+  for (int b = 0; b < B; ++b) std::memcpy(&pad[(size_t)b * m->k0], x + (size_t)b * m->n_in, m->n_in * 4);
+  float* tmp = reinterpret_cast<float*>(m->Z[0]);  // scratch large enough (B x H >= B x k0)
+  MLP_TRY(hipStreamSynchronize(m->st), "mlp sync");
+  MLP_TRY(hipMemcpy(tmp, pad.data(), pad.size() * 4, hipMemcpyHostToDevice), "H2D");
+  hipLaunchKernelGGL(dmw::f32_to_bf16_kernel, dim3(blocks((long long)pad.size(), 256)), dim3(256), 0, m->st,
+                     (long long)pad.size(), tmp, m->Y[0]);
+  MLP_TRY(hipGetLastError(), "mlp input convert");
+  return DMW_OK;
+}
+
+int check_batch(dmw_mlp* m, int B) {
+  if (B < 64 || B % 64 != 0 || B > m->max_batch)
+    return dmw::batch_fail(DMW_EINVAL, "batch must be a multiple of 64 and at most max_batch");
+  return DMW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmw_mlp_create(dmw_mlp** out, int32_t n_in, int32_t width, int32_t depth, int32_t max_batch) {
+  if (!out || n_in < 1 || width < 64 || width % 64 != 0 || depth < 1 || max_batch < 64 || max_batch % 64 != 0)
+    return dmw::batch_fail(DMW_EINVAL, "mlp: n_in >= 1, width a multiple of 64, depth >= 1, max_batch a multiple of 64");
+  *out = nullptr;
+  dmw_mlp* m = new dmw_mlp();
+  m->n_in = n_in;
+  m->k0 = ((n_in + 63) / 64) * 64;
+  m->H = width;
+  m->depth = depth;
+  m->max_batch = max_batch;
+  long long off = 0, boff = 0;
+  for (int l = 0; l < depth; ++l) {
+    const int K = l == 0 ? m->k0 : width;
+    m->w_off.push_back(off);
+    m->wbf_off.push_back(boff);
+    off += (long long)K * width;
+    boff += (long long)K * width;
+    m->b_off.push_back(off);
+    off += width;
+  }
+  m->wout_off = off;
+  off += width;
+  m->bout_off = off;
+  off += 1;
+  m->n_params = off;
+  m->n_bf = boff;
+  auto fail = [&](hipError_t e, const char* what) {
+    delete m;
+    return dmw::batch_hip_fail(e, what);
+  };
+#define MC(x, w)                       \
+  do {                                 \
+    hipError_t e_ = (x);               \
+    if (e_ != hipSuccess) return fail(e_, w); \
+  } while (0)
+  MC(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking), "hipStreamCreate");
+  MC(m->alloc(&m->p, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->g, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->m1, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->m2, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->bf_index, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->pbf, m->n_bf), "hipMalloc");
+  m->Y.resize(depth + 1);
+  m->Z.resize(depth);
+  MC(m->alloc(&m->Y[0], (size_t)max_batch * m->k0), "hipMalloc");
+  for (int l = 0; l < depth; ++l) {
+    MC(m->alloc(&m->Y[l + 1], (size_t)max_batch * width), "hipMalloc");
+    MC(m->alloc(&m->Z[l], (size_t)max_batch * std::max(width, m->k0)), "hipMalloc");
+  }
+  MC(m->alloc(&m->dZa, (size_t)max_batch * width), "hipMalloc");
+  MC(m->alloc(&m->dZb, (size_t)max_batch * width), "hipMalloc");
+  MC(m->alloc(&m->out, max_batch), "hipMalloc");
+  MC(m->alloc(&m->dy, max_batch), "hipMalloc");
+  MC(m->alloc(&m->sq, max_batch), "hipMalloc");
+  MC(m->alloc(&m->tb, max_batch), "hipMalloc");
+  MC(hipEventCreate(&m->e0), "hipEventCreate");
+  MC(hipEventCreate(&m->e1), "hipEventCreate");
+  // bf16 index map: weight matrices map to their bf16 copy, biases to -1
+  std::vector<int> idx(m->n_params, -1);
+  for (int l = 0; l < depth; ++l) {
+    const int K = l == 0 ? m->k0 : width;
+    for (long long i = 0; i < (long long)K * width; ++i) idx[m->w_off[l] + i] = (int)(m->wbf_off[l] + i);
+  }
+  MC(hipMemcpy(m->bf_index, idx.data(), idx.size() * 4, hipMemcpyHostToDevice), "H2D");
+  std::vector<float> zeros(m->n_params, 0.0f);
+  MC(hipMemcpy(m->p, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
+  MC(hipMemcpy(m->m1, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
+  MC(hipMemcpy(m->m2, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
+#undef MC
+  *out = m;
+  return DMW_OK;
+}
+
+int dmw_mlp_destroy(dmw_mlp* m) {
+  delete m;
+  return DMW_OK;
+}
+
+int64_t dmw_mlp_num_params(dmw_mlp* m) {
+  // public (unpadded) layout: W_0 is (n_in x H)
+  return m ? m->n_params - (long long)(m->k0 - m->n_in) * m->H : 0;
+}
+
+// Public flat layout: for each layer W_l (K_l x H, row-major; K_0 = n_in), b_l (H);
+// then w_out (H), b_out (1).
+static void map_public(dmw_mlp* m, std::vector<long long>& pos) {
+  pos.clear();
+  for (int l = 0; l < m->depth; ++l) {
+    const int K = l == 0 ? m->n_in : m->H;
+    for (long long i = 0; i < (long long)K * m->H; ++i) pos.push_back(m->w_off[l] + i);
+    for (int j = 0; j < m->H; ++j) pos.push_back(m->b_off[l] + j);
+  }
+  for (int j = 0; j < m->H; ++j) pos.push_back(m->wout_off + j);
+  pos.push_back(m->bout_off);
+}
+
+int dmw_mlp_set_params(dmw_mlp* m, const float* params, int64_t n) {
+  if (!m || !params || n != dmw_mlp_num_params(m)) return dmw::batch_fail(DMW_EINVAL, "mlp: parameter count mismatch");
+  std::vector<long long> pos;
+  map_public(m, pos);
+  std::vector<float> full(m->n_params, 0.0f);
+  for (size_t i = 0; i < pos.size(); ++i) full[pos[i]] = params[i];
+  MLP_TRY(hipMemcpy(m->p, full.data(), full.size() * 4, hipMemcpyHostToDevice), "H2D");
+  std::vector<float> zeros(m->n_params, 0.0f);
+  MLP_TRY(hipMemcpy(m->m1, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
+  MLP_TRY(hipMemcpy(m->m2, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
+  m->adam_t = 0;
+  hipLaunchKernelGGL(dmw::to_bf16_kernel, dim3(blocks(m->n_params, 256)), dim3(256), 0, m->st, m->n_params, m->p,
+                     m->bf_index, m->pbf);
+  MLP_TRY(hipGetLastError(), "to_bf16");
+  MLP_TRY(hipStreamSynchronize(m->st), "mlp set params");
+  return DMW_OK;
+}
+
+static int copy_public(dmw_mlp* m, const float* dev, float* out) {
+  std::vector<long long> pos;
+  map_public(m, pos);
+  std::vector<float> full(m->n_params);
+  MLP_TRY(hipStreamSynchronize(m->st), "mlp sync");
+  MLP_TRY(hipMemcpy(full.data(), dev, full.size() * 4, hipMemcpyDeviceToHost), "D2H");
+  for (size_t i = 0; i < pos.size(); ++i) out[i] = full[pos[i]];
+  return DMW_OK;
+}
+
+int dmw_mlp_get_params(dmw_mlp* m, float* params, int64_t n) {
+  if (!m || !params || n != dmw_mlp_num_params(m)) return dmw::batch_fail(DMW_EINVAL, "mlp: parameter count mismatch");
+  return copy_public(m, m->p, params);
+}
+
+int dmw_mlp_forward(dmw_mlp* m, const float* x, int32_t batch, float* y) {
+  if (!m || !x || !y) return dmw::batch_fail(DMW_EINVAL, "null argument");
+  int rc = check_batch(m, batch);
+  if (rc) return rc;
+  if ((rc = upload_batch(m, x, batch))) return rc;
+  if ((rc = forward(m, batch, nullptr))) return rc;
+  MLP_TRY(hipMemcpyAsync(y, m->out, (size_t)batch * 4, hipMemcpyDeviceToHost, m->st), "D2H");
+  MLP_TRY(hipStreamSynchronize(m->st), "mlp forward");
+  return DMW_OK;
+}
+
+int dmw_mlp_gradients(dmw_mlp* m, const float* x, const float* t, int32_t batch, float* grads, float* loss) {
+  if (!m || !x || !t || !grads) return dmw::batch_fail(DMW_EINVAL, "null argument");
+  int rc = check_batch(m, batch);
+  if (rc) return rc;
+  if ((rc = upload_batch(m, x, batch))) return rc;
+  MLP_TRY(hipMemcpyAsync(m->tb, t, (size_t)batch * 4, hipMemcpyHostToDevice, m->st), "H2D");
+  if ((rc = forward(m, batch, m->tb))) return rc;
+  if ((rc = backward(m, batch))) return rc;
+  if (loss) {
+    std::vector<float> sq(batch);
+    MLP_TRY(hipMemcpyAsync(sq.data(), m->sq, (size_t)batch * 4, hipMemcpyDeviceToHost, m->st), "D2H");
+    MLP_TRY(hipStreamSynchronize(m->st), "mlp gradients");
+    double s = 0;
+    for (float v : sq) s += v;
+    *loss = (float)(s / batch);
+  }
+  return copy_public(m, m->g, grads);
+}
+
+int dmw_mlp_set_data(dmw_mlp* m, const float* x, const float* t, int64_t n) {
+  if (!m || !x || !t || n < 1) return dmw::batch_fail(DMW_EINVAL, "null argument");
+  if (m->dX) {
+    (void)hipFree(m->dX);
+    (void)hipFree(m->dT);
+    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dX), m->allocs.end());
+    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dT), m->allocs.end());
+    m->dX = nullptr;
+    m->dT = nullptr;
+  }
+  MLP_TRY(m->alloc(&m->dX, (size_t)n * m->k0), "hipMalloc dataset");
+  MLP_TRY(m->alloc(&m->dT, (size_t)n), "hipMalloc dataset");
+  const long long chunk = 65536;
+  std::vector<float> pad((size_t)chunk * m->k0);
+  float* tmp = nullptr;
+  MLP_TRY(hipMalloc(&tmp, pad.size() * 4), "hipMalloc");
+  for (long long r0 = 0; r0 < n; r0 += chunk) {
+    const long long rows = std::min(chunk, n - r0);
+    std::fill(pad.begin(), pad.end(), 0.0f);
+    for (long long r = 0; r < rows; ++r)
+      std::memcpy(&pad[(size_t)r * m->k0], x + (size_t)(r0 + r) * m->n_in, m->n_in * 4);
+    hipError_t e = hipMemcpy(tmp, pad.data(), (size_t)rows * m->k0 * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(dmw::f32_to_bf16_kernel, dim3(blocks(rows * m->k0, 256)), dim3(256), 0, m->st,
+                         rows * m->k0, tmp, m->dX + (size_t)r0 * m->k0);
+      e = hipStreamSynchronize(m->st);
+    }
+    if (e != hipSuccess) {
+      (void)hipFree(tmp);
+      return dmw::batch_hip_fail(e, "dataset upload");
+    }
+  }
+  (void)hipFree(tmp);
+  MLP_TRY(hipMemcpy(m->dT, t, (size_t)n * 4, hipMemcpyHostToDevice), "H2D");
+  m->n_data = n;
+  return DMW_OK;
+}
+
+int dmw_mlp_train(dmw_mlp* m, int32_t batch, int32_t n_steps, const int32_t* indices, const float* lr,
+                  float beta1, float beta2, float eps, float* losses, double* train_ms) {
+  if (!m || !indices || !lr || !m->dX) return dmw::batch_fail(DMW_EINVAL, "mlp: dataset not set or null argument");
+  int rc = check_batch(m, batch);
+  if (rc) return rc;
+  const long long need = (long long)batch * n_steps;
+  for (long long i = 0; i < need; ++i)
+    if (indices[i] < 0 || indices[i] >= m->n_data) return dmw::batch_fail(DMW_EINVAL, "mlp: index out of range");
+  if (need > m->idx_cap) {
+    if (m->dIdx) {
+      (void)hipFree(m->dIdx);
+      m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dIdx), m->allocs.end());
+      m->dIdx = nullptr;
+    }
+    MLP_TRY(m->alloc(&m->dIdx, (size_t)need), "hipMalloc indices");
+    m->idx_cap = need;
+  }
+  MLP_TRY(hipMemcpyAsync(m->dIdx, indices, (size_t)need * 4, hipMemcpyHostToDevice, m->st), "H2D");
+  std::vector<float> sq_host(batch);
+  MLP_TRY(hipEventRecord(m->e0, m->st), "event");
+  for (int s = 0; s < n_steps; ++s) {
+    hipLaunchKernelGGL(dmw::gather_kernel, dim3(blocks((long long)batch * m->k0, 256)), dim3(256), 0, m->st, batch,
+                       m->k0, m->dIdx + (size_t)s * batch, m->dX, m->dT, m->Y[0], m->tb);
+    MLP_TRY(hipGetLastError(), "gather");
+    if ((rc = forward(m, batch, m->tb))) return rc;
+    if ((rc = backward(m, batch))) return rc;
+    m->adam_t += 1;
+    const float c1 = 1.0f - std::pow(beta1, (float)m->adam_t), c2 = 1.0f - std::pow(beta2, (float)m->adam_t);
+    hipLaunchKernelGGL(dmw::adam_kernel, dim3(blocks(m->n_params, 256)), dim3(256), 0, m->st, m->n_params, m->p,
+                       m->g, m->m1, m->m2, lr[s], beta1, beta2, eps, c1, c2, m->bf_index, m->pbf);
+    MLP_TRY(hipGetLastError(), "adam");
+    if (losses) {
+      MLP_TRY(hipMemcpyAsync(sq_host.data(), m->sq, (size_t)batch * 4, hipMemcpyDeviceToHost, m->st), "D2H");
+      MLP_TRY(hipStreamSynchronize(m->st), "train step");
+      double acc = 0;
+      for (float v : sq_host) acc += v;
+      losses[s] = (float)(acc / batch);
+    }
+  }
+  MLP_TRY(hipEventRecord(m->e1, m->st), "event");
+  MLP_TRY(hipStreamSynchronize(m->st), "train");
+  if (train_ms) {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, m->e0, m->e1);
+    *train_ms = ms;
+  }
+  return DMW_OK;
+}
+
+int dmw_mlp_predict(dmw_mlp* m, int64_t n, float* y, double* kernel_ms) {
+  if (!m || !y || !m->dX || n < 1 || n > m->n_data) return dmw::batch_fail(DMW_EINVAL, "mlp: dataset not set");
+  const int B = m->max_batch;
+  std::vector<int> idx(B);
+  if ((long long)B > m->idx_cap) {
+    if (m->dIdx) {
+      (void)hipFree(m->dIdx);
+      m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dIdx), m->allocs.end());
+      m->dIdx = nullptr;
+    }
+    MLP_TRY(m->alloc(&m->dIdx, (size_t)B), "hipMalloc indices");
+    m->idx_cap = B;
+  }
+  double total = 0;
+  for (long long r0 = 0; r0 < n; r0 += B) {
+    const long long rows = std::min<long long>(B, n - r0);
+    const int Bp = (int)(((rows + 63) / 64) * 64);
+    for (int b = 0; b < Bp; ++b) idx[b] = (int)std::min<long long>(r0 + b, n - 1);
+    MLP_TRY(hipMemcpyAsync(m->dIdx, idx.data(), (size_t)Bp * 4, hipMemcpyHostToDevice, m->st), "H2D");
+    MLP_TRY(hipEventRecord(m->e0, m->st), "event");
+    hipLaunchKernelGGL(dmw::gather_kernel, dim3(blocks((long long)Bp * m->k0, 256)), dim3(256), 0, m->st, Bp,
+                       m->k0, m->dIdx, m->dX, m->dT, m->Y[0], m->tb);
+    MLP_TRY(hipGetLastError(), "gather");
+    int rc = forward(m, Bp, nullptr);
+    if (rc) return rc;
+    MLP_TRY(hipEventRecord(m->e1, m->st), "event");
+    MLP_TRY(hipMemcpyAsync(y + r0, m->out, (size_t)rows * 4, hipMemcpyDeviceToHost, m->st), "D2H");
+    MLP_TRY(hipStreamSynchronize(m->st), "predict");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, m->e0, m->e1);
+    total += ms;
+  }
+  if (kernel_ms) *kernel_ms = total;
+  return DMW_OK;
+}
+
+}  // extern "C"

@@ -563,6 +563,44 @@ __device__ __forceinline__ void fold(Pos& p, const WalkArgs& a) {
   }
 }
 
+// One sub-step of wall physics for any geometry; returns the log-weight
+// increment before the chi gating (physics.py:303-311).
+template <int GEOM, int WALL>
+__device__ __forceinline__ float wall_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                           const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
+                                           uint32_t& ncross, uint32_t& ncand) {
+  constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
+  float dlog = 0.0f;
+  if (GEOM == G_FREE) {
+    p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz;
+  } else if (GEOM == G_BOX1D) {
+    dlog = box_step<WALL>(p, sx, sy, sz, a, hits);
+  } else if (GEOM == G_CYL) {
+    if (WALL == W_PERMEATE) {
+      dlog = cyl_permeate(p.x, p.y, sx, sy, a, ps, hits, ncross);
+    } else {
+      const float ds = cyl_reflect(p.x, p.y, sx, sy, a, hits);
+      if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
+    }
+    p.z = p.z + sz;
+  } else if (GEOM == G_SPH) {
+    if (WALL == W_PERMEATE) {
+      dlog = sph_permeate(p, sx, sy, sz, a, ps, hits, ncross);
+    } else {
+      const float ds = sph_reflect(p, sx, sy, sz, a, hits);
+      if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
+    }
+  } else {
+    if (WALL == W_PERMEATE) {
+      dlog = packed_permeate<DIM>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+    } else {
+      const float dp = packed_reflect<DIM>(p, sx, sy, sz, a, tb, hits, ncand);
+      if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * dp;
+    }
+  }
+  return dlog;
+}
+
 // ---------------------------------------------------------------------------
 // Start positions sampled in-kernel (reference: host rejection sampling,
 // geometries.py:87-91, 186-195, 477-491, 1717-1744, 2421-2443).
@@ -888,34 +926,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       }
       const float sx = ux * a.step_len, sy = uy * a.step_len, sz = uz * a.step_len;
       const PowlesSrc ps{w, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
-      float dlog = 0.0f;
-      if (GEOM == G_FREE) {
-        p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz;
-      } else if (GEOM == G_BOX1D) {
-        dlog = box_step<WALL>(p, sx, sy, sz, a, hits);
-      } else if (GEOM == G_CYL) {
-        if (WALL == W_PERMEATE) {
-          dlog = cyl_permeate(p.x, p.y, sx, sy, a, ps, hits, ncross);
-        } else {
-          const float ds = cyl_reflect(p.x, p.y, sx, sy, a, hits);
-          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
-        }
-        p.z = p.z + sz;
-      } else if (GEOM == G_SPH) {
-        if (WALL == W_PERMEATE) {
-          dlog = sph_permeate(p, sx, sy, sz, a, ps, hits, ncross);
-        } else {
-          const float ds = sph_reflect(p, sx, sy, sz, a, hits);
-          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
-        }
-      } else {
-        if (WALL == W_PERMEATE) {
-          dlog = packed_permeate<DIM>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
-        } else {
-          const float dp = packed_reflect<DIM>(p, sx, sy, sz, a, tb, hits, ncand);
-          if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * dp;
-        }
-      }
+      float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
       if (WALL != W_REFLECT) dlog = dlog * chi;
       if (a.has_t2) dlog = dlog - (a.dt_sub * a.inv_T2) * chi;

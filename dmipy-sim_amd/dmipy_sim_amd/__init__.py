@@ -15,6 +15,8 @@ from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
 from .core import simulate, simulate_cpmg, simulate_mixture, walk, PlannedWalk
 from .gpu import check_gpu, gpu_available
 from .batch import BatchedWalk
+from . import surrogate
+from .surrogate import MLP, SurrogateEnsemble
 
 __all__ = [
     "GAMMA", "DEFAULT_SLEW_RATE", "Waveform", "pgse", "ogse", "cpmg", "calc_b", "set_b",
@@ -23,5 +25,5 @@ __all__ = [
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",
-    "BatchedWalk", "PlannedWalk",
+    "BatchedWalk", "PlannedWalk", "surrogate", "MLP", "SurrogateEnsemble",
 ]

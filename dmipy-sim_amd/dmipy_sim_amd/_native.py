@@ -50,7 +50,7 @@ class Result_t(C.Structure):
 class Instance_t(C.Structure):
     _fields_ = [("geom", Geometry_t), ("n_walkers", C.c_int64), ("n_sub", C.c_int32),
                 ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
-                ("inv_T2", C.c_float), ("has_t2", C.c_int32)]
+                ("inv_T2", C.c_float), ("has_t2", C.c_int32), ("wave_set", C.c_int32)]
 
 
 class BatchStats_t(C.Structure):
@@ -70,9 +70,21 @@ SIGNATURES = {
     "dmw_plan_execute": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Result_t)]),
     "dmw_plan_destroy": (C.c_int, [C.c_void_p]),
     "dmw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Instance_t),
-                                   C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
+                                   C.c_int32, C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
     "dmw_batch_execute": (C.c_int, [C.c_void_p, C.c_uint64, _dp, C.POINTER(BatchStats_t)]),
     "dmw_batch_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32,
+                                 C.c_int32]),
+    "dmw_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_mlp_num_params": (C.c_int64, [C.c_void_p]),
+    "dmw_mlp_set_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
+    "dmw_mlp_get_params": (C.c_int, [C.c_void_p, _fp, C.c_int64]),
+    "dmw_mlp_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "dmw_mlp_gradients": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, _fp, _fp]),
+    "dmw_mlp_set_data": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int64]),
+    "dmw_mlp_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _ip, _fp, C.c_float,
+                                C.c_float, C.c_float, _fp, _dp]),
+    "dmw_mlp_predict": (C.c_int, [C.c_void_p, C.c_int64, _fp, _dp]),
 }
 
 

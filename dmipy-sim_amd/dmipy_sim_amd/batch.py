@@ -1,17 +1,20 @@
 """Many substrate instances in one persistent, graph-captured launch.
 
-Each instance is a ``Cylinder`` or ``Sphere`` (reflecting, relaxing or permeable
-wall) with its own diffusivity, walker count and sub-step count; all share one
-waveform.  The result for instance i equals, in distribution, a separate
-``simulate(n_walkers_i, D_i, waveform, geometry_i)`` call (dmipy_sim/core.py:193),
-i.e. the serial per-substrate loop the reference would run.
+Each instance is any geometry the walk supports (free, Box1D, Cylinder, Sphere,
+PackedCylinders, PackedSpheres; reflecting, relaxing or permeable walls) with
+its own diffusivity, T2, walker count and sub-step count, and names one of a
+list of waveforms (all with the same number of measurements, <= 64, and the
+same n_t).  The result for instance i equals, in distribution, a separate
+``simulate(n_walkers_i, D_i, waveforms[k_i], geometry_i)`` call
+(dmipy_sim/core.py:193) -- the serial per-substrate loop the reference runs,
+e.g. simulate_mixture (core.py:821-858).  Instance i uses the Philox instance
+word i, so instance 0 walks exactly as ``walk(..., seed)`` does.
 """
 import ctypes as C
 
 import numpy as np
 
 from . import _native as nat
-from . import geometries as geo
 from .physics import make_walk_plan
 
 
@@ -19,45 +22,87 @@ class BatchedWalk:
     """Plan once (upload, cost-sorted work queue, hipGraph), execute per seed.
 
     ``instances`` is a list of dicts with keys 'geometry', 'diffusivity',
-    'n_walkers' and optionally 'T2'.
+    'n_walkers' and optionally 'T2' and 'waveform' (index into ``waveforms``,
+    default 0).  ``waveforms`` is one Waveform or a list of them.
     """
 
-    def __init__(self, instances, waveform, waves_per_simd=0, use_graph=True):
-        G = np.ascontiguousarray(np.asarray(waveform.G, dtype=np.float32))
-        n_meas, n_t, _ = G.shape
+    def __init__(self, instances, waveforms, waves_per_simd=0, use_graph=True):
+        if not isinstance(waveforms, (list, tuple)):
+            waveforms = [waveforms]
+        if len(instances) < 1 or len(waveforms) < 1:
+            raise ValueError("need at least one instance and one waveform")
+        Gs = [np.asarray(w.G, dtype=np.float32) for w in waveforms]
+        n_meas, n_t, _ = Gs[0].shape
+        for G in Gs:
+            if G.shape != (n_meas, n_t, 3):
+                raise ValueError("every waveform needs the same (n_measurements, n_t)")
+        if n_meas > 64:
+            raise ValueError("batched walks take at most 64 measurements per waveform")
+        chis = [getattr(w, 'chi_perp', None) for w in waveforms]
+        chi0 = None if chis[0] is None else np.asarray(chis[0], np.float32).reshape(n_t)
+        for c in chis[1:]:
+            same = (c is None and chi0 is None) or (
+                c is not None and chi0 is not None and np.array_equal(
+                    np.asarray(c, np.float32).reshape(n_t), chi0))
+            if not same:
+                raise ValueError("every waveform of a batch needs the same chi_perp")
+        dts = {float(w.dt) for w in waveforms}
+
         arr = (nat.Instance_t * len(instances))()
         self.n_sub = np.zeros(len(instances), np.int64)
         self.n_walkers = np.zeros(len(instances), np.int64)
+        self._half = np.ones(len(instances), np.float64)
+        set_key = {}
+        sets_G = []
+        self._keep = []          # host arrays referenced by the descriptors
         for i, spec in enumerate(instances):
-            g = spec['geometry']
-            if g.kind not in (geo.KIND_CYLINDER, geo.KIND_SPHERE):
-                raise NotImplementedError("batched instances must be Cylinder or Sphere")
-            if getattr(g, '_R', None) is not None and not g._is_identity_rotation:
-                raise NotImplementedError("batched cylinders must be along z")
-            plan = make_walk_plan(g, spec['diffusivity'], waveform.dt, T2=spec.get('T2'))
-            gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall, n_obj=0, size=plan.size,
-                                cell=0.0, eps=plan.eps, nudge=plan.nudge,
+            k = int(spec.get('waveform', 0))
+            if not 0 <= k < len(waveforms):
+                raise ValueError(f"instance {i}: waveform index {k} out of range")
+            wf = waveforms[k]
+            if spec.get('T1') is not None:
+                raise NotImplementedError("batched walks do not model T1")
+            plan = make_walk_plan(spec['geometry'], spec['diffusivity'], wf.dt, T2=spec.get('T2'))
+            rot = plan.rot
+            key = (k, None if rot is None else np.asarray(rot, np.float64).tobytes())
+            if key not in set_key:
+                G = Gs[k]
+                if rot is not None:
+                    G = np.einsum('ij,mtj->mti', np.asarray(rot, np.float64), G)
+                set_key[key] = len(sets_G)
+                sets_G.append(np.ascontiguousarray(G, dtype=np.float32))
+            centers = None if plan.centers is None else np.ascontiguousarray(plan.centers, np.float32)
+            radii = None if plan.radii is None else np.ascontiguousarray(plan.radii, np.float32)
+            self._keep += [centers, radii]
+            gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
+                                n_obj=0 if radii is None else len(radii), size=plan.size,
+                                cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
                                 rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
-                                centers=None, radii=None)
+                                centers=nat.fptr(centers), radii=nat.fptr(radii))
             arr[i] = nat.Instance_t(geom=gd, n_walkers=int(spec['n_walkers']), n_sub=plan.n_sub,
                                     step_len=plan.step_len, gamma_dt_sub=plan.gamma_dt_sub,
                                     dt_sub=plan.dt_sub, inv_T2=plan.inv_T2,
-                                    has_t2=int(plan.has_t2))
+                                    has_t2=int(plan.has_t2), wave_set=set_key[key])
             self.n_sub[i] = plan.n_sub
             self.n_walkers[i] = int(spec['n_walkers'])
-        chi = getattr(waveform, 'chi_perp', None)
-        self._chi = None if chi is None else np.ascontiguousarray(chi, dtype=np.float32).reshape(n_t)
-        self._G = G
-        wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G), chi=nat.fptr(self._chi),
-                            n_echo=0, echo_idx=None)
+            if getattr(wf, 'stimulated_echo', False):
+                self._half[i] = 0.5
+        self._chi = None if chi0 is None else np.ascontiguousarray(chi0)
+        self._sets = sets_G
+        wds = (nat.Waveform_t * len(sets_G))()
+        for s, G in enumerate(sets_G):
+            wds[s] = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
+                                    chi=nat.fptr(self._chi), n_echo=0, echo_idx=None)
         h = C.c_void_p()
-        nat.check(nat.lib().dmw_batch_create(C.byref(h), len(instances), arr, wd,
+        nat.check(nat.lib().dmw_batch_create(C.byref(h), len(instances), arr, len(sets_G), wds,
                                              int(waves_per_simd), int(bool(use_graph))))
         self._h = h
+        self._keep = None
         self.n_inst = len(instances)
+        self.n_sets = len(sets_G)
         self.n_meas = n_meas
         self.n_t = n_t
-        self.stimulated_echo = bool(getattr(waveform, 'stimulated_echo', False))
+        self.dt = dts.pop() if len(dts) == 1 else None
 
     def run(self, seed=123):
         """Signals (n_instances, n_measurements) float64 and timing stats."""
@@ -65,8 +110,7 @@ class BatchedWalk:
         st = nat.BatchStats_t()
         nat.check(nat.lib().dmw_batch_execute(self._h, int(seed) & (2**64 - 1),
                                               nat.dptr(out), C.byref(st)))
-        if self.stimulated_echo:
-            out *= 0.5
+        out *= self._half[:, None]
         return out, {"kernel_ms": st.kernel_ms, "total_ms": st.total_ms,
                      "walker_steps": int(st.walker_steps), "n_units": st.n_units,
                      "grid": st.grid}

@@ -1,0 +1,389 @@
+"""Learned surrogate of the walk: a BF16 GELU MLP trained on batched-MC signals.
+
+No counterpart in the reference.  The network maps six substrate parameters of
+a PackedCylinders substrate (mean radius, Gamma shape, volume fraction, log
+permeability, log surface relaxivity, diffusivity) plus a 64-value embedding of
+one measurement's gradient waveform to that measurement's signal S/S0.  It is
+trained on the GPU by the library's own MFMA forward / backward / Adam kernels
+(csrc/mlp.hip, entry points dmw_mlp_* of include/dmipy_walk.h) on signals the
+batched persistent walk produces.  A 4-member ensemble gives a spread per
+prediction; predictions whose spread exceeds the accuracy bound are recomputed
+by the Monte-Carlo walk, which is also the judge of the surrogate's error.
+
+Waveform embedding.  With q(t) = gamma * int_0^t G (accumulated in float64) and
+Q_k its discrete Fourier coefficients, the b-value splits over frequencies,
+b = sum_k b_k with b_k = 2 |Q_k|^2 dt / n_t (Parseval; k = 0 and Nyquist once).
+In the Gaussian-phase picture ln S = -sum_k b_k D(omega_k) with D(omega) the
+diffusion spectrum of the substrate, so the per-frequency b_k of the transverse
+(x, y) and axial (z) components are the natural inputs.  The embedding is
+31 transverse bins + 31 axial bins (the first 24 frequencies one by one, the
+rest in 7 geometric groups) + the transverse and axial b-values, in units of
+1e9 s/m^2.
+"""
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _native as nat
+from . import geometries as geo
+from . import waveforms as wfm
+from .constants import GAMMA
+
+N_SUBSTRATE = 6
+N_WAVE = 64
+N_IN = N_SUBSTRATE + N_WAVE
+
+# Prior over substrate instances of the training generator.
+PRIOR = dict(mean_r=(1.5e-6, 3.5e-6), shape=(2.0, 8.0), vf=(0.3, 0.65),
+             kappa=(1e-6, 1e-4), rho=(1e-7, 1e-5), D=(1.0e-9, 2.5e-9))
+R_MIN = 1.2e-6      # radii are clipped here (bounds the permeable sub-step count)
+N_CYL = 32
+
+
+# ---------------------------------------------------------------------------
+# The network.
+# ---------------------------------------------------------------------------
+class MLP:
+    """GELU MLP on the GPU (bf16 weights, fp32 accumulation and master weights).
+
+    Parameters live on the device; ``get_params``/``set_params`` use the flat
+    public layout: per hidden layer W_l (K_l x width, row-major), b_l; then
+    w_out (width), b_out.
+    """
+
+    def __init__(self, n_in=N_IN, width=512, depth=8, max_batch=65536, seed=0):
+        self.n_in, self.width, self.depth, self.max_batch = int(n_in), int(width), int(depth), int(max_batch)
+        h = C.c_void_p()
+        nat.check(nat.lib().dmw_mlp_create(C.byref(h), self.n_in, self.width, self.depth,
+                                           self.max_batch))
+        self._h = h
+        self.n_params = int(nat.lib().dmw_mlp_num_params(h))
+        self.n_data = 0
+        self.set_params(self.init_params(seed))
+
+    def init_params(self, seed):
+        """He-normal weights (std sqrt(2/K)), zero biases, output std 1/sqrt(width)."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for l in range(self.depth):
+            K = self.n_in if l == 0 else self.width
+            parts.append(rng.standard_normal(K * self.width) * np.sqrt(2.0 / K))
+            parts.append(np.zeros(self.width))
+        parts.append(rng.standard_normal(self.width) / np.sqrt(self.width))
+        parts.append(np.zeros(1))
+        return np.concatenate(parts).astype(np.float32)
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        nat.check(nat.lib().dmw_mlp_set_params(self._h, nat.fptr(p), p.size))
+
+    def get_params(self):
+        p = np.empty(self.n_params, np.float32)
+        nat.check(nat.lib().dmw_mlp_get_params(self._h, nat.fptr(p), p.size))
+        return p
+
+    def _pad(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.n_in)
+        n = x.shape[0]
+        B = max(64, -(-n // 64) * 64)
+        if B != n:
+            x = np.concatenate([x, np.repeat(x[-1:], B - n, axis=0)])
+        return x, n, B
+
+    def forward(self, x):
+        """Network output for rows of x (n, n_in); any n up to max_batch."""
+        x, n, B = self._pad(x)
+        if B > self.max_batch:
+            raise ValueError("batch larger than max_batch; use set_data + predict")
+        y = np.empty(B, np.float32)
+        nat.check(nat.lib().dmw_mlp_forward(self._h, nat.fptr(x), B, nat.fptr(y)))
+        return y[:n]
+
+    def gradients(self, x, t):
+        """(gradient of the mean squared error in the public layout, loss); the
+        batch must be a multiple of 64."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.n_in)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        g = np.empty(self.n_params, np.float32)
+        loss = C.c_float()
+        nat.check(nat.lib().dmw_mlp_gradients(self._h, nat.fptr(x), nat.fptr(t), x.shape[0],
+                                              nat.fptr(g), C.byref(loss)))
+        return g, float(loss.value)
+
+    def set_data(self, x, t):
+        """Keep a dataset resident in HBM for ``train`` and ``predict``."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, self.n_in)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        nat.check(nat.lib().dmw_mlp_set_data(self._h, nat.fptr(x), nat.fptr(t), x.shape[0]))
+        self.n_data = x.shape[0]
+
+    def train(self, batch, indices, lr, beta1=0.9, beta2=0.999, eps=1e-8, want_losses=False):
+        """Adam steps over ``indices`` (n_steps * batch resident-row indices);
+        ``lr`` is a scalar or one value per step.  Returns (losses or None, ms)."""
+        idx = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1)
+        if idx.size % batch:
+            raise ValueError("indices must hold whole batches")
+        n_steps = idx.size // batch
+        lr = np.ascontiguousarray(np.broadcast_to(np.asarray(lr, np.float32), (n_steps,)))
+        losses = np.empty(n_steps, np.float32) if want_losses else None
+        ms = C.c_double()
+        nat.check(nat.lib().dmw_mlp_train(self._h, int(batch), n_steps, nat.iptr(idx), nat.fptr(lr),
+                                          float(beta1), float(beta2), float(eps),
+                                          nat.fptr(losses), C.byref(ms)))
+        return losses, float(ms.value)
+
+    def predict(self, n=None):
+        """Outputs for the first n resident rows (default all) and kernel ms."""
+        n = self.n_data if n is None else int(n)
+        y = np.empty(n, np.float32)
+        ms = C.c_double()
+        nat.check(nat.lib().dmw_mlp_predict(self._h, n, nat.fptr(y), C.byref(ms)))
+        return y, float(ms.value)
+
+    def flops_per_sample(self):
+        """Forward multiply-adds x 2 of the padded network the kernels run."""
+        k0 = -(-self.n_in // 64) * 64
+        return 2.0 * (k0 * self.width + (self.depth - 1) * self.width ** 2 + self.width)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.lib().dmw_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Inputs: substrate parameters and the waveform embedding.
+# ---------------------------------------------------------------------------
+def _bins(n_freq):
+    """Index groups of the embedding: 24 single frequencies, then 7 geometric groups."""
+    edges = [np.array([k]) for k in range(min(24, n_freq))]
+    if n_freq > 24:
+        cuts = np.unique(np.round(np.geomspace(24, n_freq, 8)).astype(int))
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            edges.append(np.arange(a, b))
+    while len(edges) < 31:
+        edges.append(np.array([], dtype=int))
+    return edges[:31]
+
+
+def waveform_features(waveform):
+    """(n_meas, 64) embedding of every measurement of a Waveform."""
+    G = np.asarray(waveform.G, dtype=np.float64)
+    dt = float(waveform.dt)
+    n_t = G.shape[1]
+    q = GAMMA * np.cumsum(G, axis=1) * dt                      # float64 q(t)
+    Q = np.fft.rfft(q, axis=1)
+    w = np.full(Q.shape[1], 2.0)
+    w[0] = 1.0
+    if n_t % 2 == 0:
+        w[-1] = 1.0
+    bk = (np.abs(Q) ** 2) * (w * dt / n_t)[None, :, None]     # (n_meas, n_freq, 3), sums to b
+    perp = bk[:, :, 0] + bk[:, :, 1]
+    axial = bk[:, :, 2]
+    groups = _bins(Q.shape[1])
+    out = np.zeros((G.shape[0], N_WAVE))
+    for j, idx in enumerate(groups):
+        if idx.size:
+            out[:, j] = perp[:, idx].sum(axis=1)
+            out[:, 31 + j] = axial[:, idx].sum(axis=1)
+    out[:, 62] = perp.sum(axis=1)
+    out[:, 63] = axial.sum(axis=1)
+    return out * 1e-9
+
+
+def substrate_features(p):
+    """Six normalised inputs from a parameter dict (keys of PRIOR)."""
+    return np.array([
+        p["mean_r"] / 1e-6 - 2.5,
+        (p["shape"] - 5.0) / 3.0,
+        (p["vf"] - 0.475) / 0.175,
+        (np.log10(p["kappa"]) + 5.0),
+        (np.log10(p["rho"]) + 6.0),
+        p["D"] / 1e-9 - 1.75,
+    ])
+
+
+def features(params, waveform):
+    """(n_meas, 70) network inputs for one substrate and a waveform."""
+    wf = waveform_features(waveform)
+    s = np.broadcast_to(substrate_features(params), (wf.shape[0], N_SUBSTRATE))
+    return np.concatenate([s, wf], axis=1).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# The training generator: PackedCylinders instances and free waveforms.
+# ---------------------------------------------------------------------------
+def sample_params(rng):
+    lo, hi = PRIOR["kappa"]
+    rlo, rhi = PRIOR["rho"]
+    return dict(mean_r=rng.uniform(*PRIOR["mean_r"]), shape=rng.uniform(*PRIOR["shape"]),
+                vf=rng.uniform(*PRIOR["vf"]), kappa=float(np.exp(rng.uniform(np.log(lo), np.log(hi)))),
+                rho=float(np.exp(rng.uniform(np.log(rlo), np.log(rhi)))), D=rng.uniform(*PRIOR["D"]))
+
+
+def build_substrate(p, seed, n_cyl=N_CYL):
+    """PackedCylinders with radii ~ Gamma(shape, mean_r / shape) clipped at
+    R_MIN, densely packed at the volume fraction (config-3 generator)."""
+    rng = np.random.default_rng(seed)
+    radii = np.maximum(rng.gamma(p["shape"], p["mean_r"] / p["shape"], n_cyl), R_MIN)
+    centers, L, _ = geo.pack_dense(radii, p["vf"], seed=seed, dim=2, gap=0.1)
+    return geo.PackedCylinders(radii, centers, L, surface_relaxivity_t2=p["rho"],
+                               permeability=p["kappa"])
+
+
+def random_waveforms(rng, n_meas=64, n_t=250, T=0.025, gmax=(0.02, 0.3)):
+    """Free waveforms as in the config-3 generator: per axis 4 sines with
+    f ~ U(10, 500) Hz, phase ~ U(0, 2 pi), amplitude ~ U(0, 1); zero time mean;
+    max |G| ~ U(gmax); the first row is b = 0."""
+    dt = T / (n_t - 1)
+    t = np.arange(n_t) * dt
+    f = rng.uniform(10, 500, (n_meas, 3, 4))
+    ph = rng.uniform(0, 2 * np.pi, (n_meas, 3, 4))
+    A = rng.uniform(0, 1, (n_meas, 3, 4))
+    G = np.sum(A[..., None] * np.sin(2 * np.pi * f[..., None] * t + ph[..., None]), axis=2)
+    G = np.transpose(G, (0, 2, 1))                                 # (n_meas, n_t, 3)
+    G -= G.mean(axis=1, keepdims=True)
+    G *= (rng.uniform(*gmax, n_meas) / np.max(np.linalg.norm(G, axis=2), axis=1))[:, None, None]
+    G[0] = 0.0
+    return wfm.Waveform(G=G.astype(np.float32), dt=float(dt), echo_idx=n_t - 1)
+
+
+def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=512,
+             walk_seed=None, log=None):
+    """Batched-MC dataset: (X (n*n_meas, 70) float32, S (n*n_meas) float64, info).
+
+    Every instance gets its own substrate (parameters from PRIOR) and its own
+    n_meas random waveforms; instances run ``chunk`` at a time in one batched
+    persistent launch each.
+    """
+    from .batch import BatchedWalk
+    rng = np.random.default_rng(seed)
+    X, S = [], []
+    gpu_ms = 0.0
+    walker_steps = 0
+    params = []
+    for c0 in range(0, n_instances, chunk):
+        specs, waves = [], []
+        for i in range(c0, min(n_instances, c0 + chunk)):
+            p = sample_params(rng)
+            geom = build_substrate(p, seed=int(rng.integers(2**31)))
+            wf = random_waveforms(rng, n_meas=n_meas, n_t=n_t, T=T)
+            specs.append(dict(geometry=geom, diffusivity=p["D"], n_walkers=int(n_walkers),
+                              waveform=len(waves)))
+            waves.append(wf)
+            params.append(p)
+            X.append(features(p, wf))
+        bw = BatchedWalk(specs, waves)
+        sig, st = bw.run(seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
+        bw.close()
+        S.append(sig.reshape(-1))
+        gpu_ms += st["total_ms"]
+        walker_steps += st["walker_steps"]
+        if log:
+            log(f"generate: {min(n_instances, c0 + chunk)}/{n_instances} instances, "
+                f"{st['total_ms']:.0f} ms")
+    info = dict(gpu_ms=gpu_ms, walker_steps=walker_steps, params=params,
+                signals=n_instances * n_meas, n_walkers=int(n_walkers))
+    return np.concatenate(X), np.concatenate(S), info
+
+
+# ---------------------------------------------------------------------------
+# Ensemble with Monte-Carlo fallback.
+# ---------------------------------------------------------------------------
+class SurrogateEnsemble:
+    """``n_members`` MLPs trained from different initialisations and batch
+    orders; the prediction is their mean and the spread their standard
+    deviation."""
+
+    def __init__(self, n_members=4, width=512, depth=8, max_batch=65536, seed=0):
+        self.members = [MLP(N_IN, width, depth, max_batch, seed=seed + 101 * k)
+                        for k in range(n_members)]
+        self.seed = seed
+
+    def fit(self, X, S, epochs=40, batch=1024, lr=1e-3, lr_final=2e-5, log=None):
+        """Adam with a cosine learning-rate decay; returns per-member final
+        losses and total training ms."""
+        n = X.shape[0]
+        steps_per_epoch = n // batch
+        total = steps_per_epoch * epochs
+        s = np.arange(total)
+        lrs = lr_final + 0.5 * (lr - lr_final) * (1 + np.cos(np.pi * s / max(1, total - 1)))
+        ms_total = 0.0
+        finals = []
+        for k, m in enumerate(self.members):
+            m.set_data(X, S)
+            rng = np.random.default_rng(self.seed + 7 * k + 1)
+            idx = np.concatenate([rng.permutation(n)[:steps_per_epoch * batch] for _ in range(epochs)])
+            per = 2000          # steps per call: bounds the index upload
+            last = None
+            for a in range(0, total, per):
+                b = min(total, a + per)
+                want = b == total
+                losses, ms = m.train(batch, idx[a * batch:b * batch], lrs[a:b], want_losses=want)
+                ms_total += ms
+                if want:
+                    last = float(np.mean(losses[-min(50, len(losses)):]))
+            finals.append(last)
+            if log:
+                log(f"member {k}: final loss {last:.3e}")
+        return finals, ms_total
+
+    def predict(self, X):
+        """(mean, spread, kernel ms) over the members for rows of X."""
+        outs, ms = [], 0.0
+        for m in self.members:
+            m.set_data(X, np.zeros(X.shape[0], np.float32))
+            y, t = m.predict()
+            outs.append(y.astype(np.float64))
+            ms += t
+        outs = np.stack(outs)
+        return outs.mean(axis=0), outs.std(axis=0), ms
+
+    def close(self):
+        for m in self.members:
+            m.close()
+
+
+def predict_with_fallback(ensemble, params_list, waveforms, threshold=5e-3, n_walkers=20_000,
+                          seed=0, n_meas=None):
+    """Signals for every (instance, measurement): the ensemble mean where its
+    spread is at most ``threshold``, else the batched Monte-Carlo walk of that
+    instance (all its measurements are walked; the flagged ones are replaced).
+
+    Returns (signals (n_inst, n_meas), fallback mask, timing dict)."""
+    from .batch import BatchedWalk
+    X = np.concatenate([features(p, w) for p, w in zip(params_list, waveforms)])
+    n_meas = waveforms[0].G.shape[0]
+    t0 = time.perf_counter()
+    mean, spread, ms = ensemble.predict(X)
+    t_pred = time.perf_counter() - t0
+    mean = mean.reshape(len(params_list), n_meas)
+    flag = spread.reshape(len(params_list), n_meas) > threshold
+    out = mean.copy()
+    rows = np.nonzero(flag.any(axis=1))[0]
+    mc_ms = 0.0
+    if rows.size:
+        rng = np.random.default_rng(seed)
+        specs, waves = [], []
+        for i in rows:
+            geom = build_substrate(params_list[i], seed=int(rng.integers(2**31)))
+            specs.append(dict(geometry=geom, diffusivity=params_list[i]["D"],
+                              n_walkers=int(n_walkers), waveform=len(waves)))
+            waves.append(waveforms[i])
+        bw = BatchedWalk(specs, waves)
+        sig, st = bw.run(seed)
+        bw.close()
+        mc_ms = st["total_ms"]
+        for j, i in enumerate(rows):
+            out[i, flag[i]] = sig[j, flag[i]]
+    return out, flag, dict(predict_s=t_pred, predict_kernel_ms=ms, mc_ms=mc_ms,
+                           n_fallback_instances=int(rows.size))

@@ -149,13 +149,16 @@ int dmw_plan_create(dmw_plan** plan, const dmw_geometry* geom,
 int dmw_plan_execute(dmw_plan* plan, uint64_t seed, dmw_result* out);
 int dmw_plan_destroy(dmw_plan* plan);
 
-/* ---- Batched persistent launch (many substrate instances, one waveform) ----
+/* ---- Batched persistent launch (many substrate instances, one launch) ----
  * Has no single counterpart in the reference, where each instance is a separate
  * simulate() call (core.py:193) -- e.g. the serial loop of simulate_mixture
- * (core.py:821-858).  Instances may be Cylinder or Sphere (walls reflect, relax or
- * permeate); every instance shares the waveform. */
+ * (core.py:821-858).  Instances may be any geometry kind with any wall mode the
+ * geometry supports; each names one of n_sets waveform sets, which share n_meas
+ * (<= 64) and n_t.  Walker streams use the instance index as Philox instance
+ * word, so instance i of a batch walks exactly as dmw_walk would with the same
+ * seed and that instance word. */
 typedef struct dmw_instance {
-  dmw_geometry geom;         /* kind DMW_CYLINDER or DMW_SPHERE */
+  dmw_geometry geom;         /* any kind; packed kinds bring centers/radii */
   int64_t n_walkers;
   int32_t n_sub;
   float step_len;
@@ -163,6 +166,7 @@ typedef struct dmw_instance {
   float dt_sub;
   float inv_T2;
   int32_t has_t2;
+  int32_t wave_set;          /* index into the waveform sets of dmw_batch_create */
 } dmw_instance;
 
 typedef struct dmw_batch dmw_batch;
@@ -175,13 +179,44 @@ typedef struct dmw_batch_stats {
   int32_t grid;              /* persistent workgroups */
 } dmw_batch_stats;
 
+/* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored).
+ * waves_per_simd: persistent wavefronts per SIMD (0 = 2). */
 int dmw_batch_create(dmw_batch** batch, int32_t n_inst, const dmw_instance* inst,
-                     const dmw_waveform* shared, int32_t waves_per_simd,
+                     int32_t n_sets, const dmw_waveform* sets, int32_t waves_per_simd,
                      int32_t use_graph);
 /* signals: (n_inst, n_meas) Re <w e^{i phi}> per instance. */
 int dmw_batch_execute(dmw_batch* batch, uint64_t seed, double* signals,
                       dmw_batch_stats* stats);
 int dmw_batch_destroy(dmw_batch* batch);
+
+/* ---- Learned surrogate (no counterpart in the reference) ----
+ * A GELU MLP (depth hidden layers of `width` units) with BF16 weights and FP32
+ * accumulation on the matrix cores, trained by its own forward/backward/Adam
+ * kernels on signals the walk produces; it maps substrate parameters plus a
+ * waveform embedding to the signal.  Public parameter layout (float32, flat):
+ * for each hidden layer W_l (K_l x width, row-major, K_0 = n_in) then b_l
+ * (width); then w_out (width) and b_out (1).  Batches are multiples of 64. */
+typedef struct dmw_mlp dmw_mlp;
+
+int dmw_mlp_create(dmw_mlp** mlp, int32_t n_in, int32_t width, int32_t depth, int32_t max_batch);
+int dmw_mlp_destroy(dmw_mlp* mlp);
+int64_t dmw_mlp_num_params(dmw_mlp* mlp);
+/* Set master weights (resets the Adam moments); get them back. */
+int dmw_mlp_set_params(dmw_mlp* mlp, const float* params, int64_t n);
+int dmw_mlp_get_params(dmw_mlp* mlp, float* params, int64_t n);
+/* y (batch) = network(x (batch, n_in)). */
+int dmw_mlp_forward(dmw_mlp* mlp, const float* x, int32_t batch, float* y);
+/* Gradients of the mean squared error against targets t, in the public layout. */
+int dmw_mlp_gradients(dmw_mlp* mlp, const float* x, const float* t, int32_t batch, float* grads,
+                      float* loss);
+/* Keep a dataset resident in HBM (x: (n, n_in), t: (n)). */
+int dmw_mlp_set_data(dmw_mlp* mlp, const float* x, const float* t, int64_t n);
+/* n_steps Adam steps on batches of resident rows indices[s*batch ... +batch). */
+int dmw_mlp_train(dmw_mlp* mlp, int32_t batch, int32_t n_steps, const int32_t* indices,
+                  const float* lr, float beta1, float beta2, float eps, float* losses,
+                  double* train_ms);
+/* Predictions for the first n resident rows, in batches of max_batch. */
+int dmw_mlp_predict(dmw_mlp* mlp, int64_t n, float* y, double* kernel_ms);
 
 #ifdef __cplusplus
 }

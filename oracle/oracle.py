@@ -47,7 +47,8 @@ class OracleArgs(C.Structure):
                 ("n_walkers", C.c_int64), ("seed", C.c_uint64), ("r0", _fp),
                 ("inj_n", _fp), ("inj_u", _fp), ("cell_form", C.c_int32),
                 ("phase64", C.c_int32), ("threads", C.c_int32),
-                ("walker_begin", C.c_int64), ("walker_end", C.c_int64)]
+                ("walker_begin", C.c_int64), ("walker_end", C.c_int64),
+                ("instance", C.c_uint32)]
 
 
 class OracleOut(C.Structure):
@@ -153,7 +154,8 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
 
 def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                inject_normals=None, inject_uniforms=None, cell_form=1, phase64=False,
-               threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None):
+               threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None,
+               instance=0):
     """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame)."""
     p = plan if plan is not None else oracle_plan(geom, D, waveform.dt, T2, T1)
     G = np.ascontiguousarray(waveform.G if G_native is None else G_native, dtype=np.float32)
@@ -179,7 +181,8 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                    echo_idx=None if echo is None else echo.ctypes.data_as(_ip),
                    n_walkers=N, seed=int(seed) & (2**64 - 1), r0=fp(r0), inj_n=fp(inj_n),
                    inj_u=fp(inj_u), cell_form=int(cell_form), phase64=int(bool(phase64)),
-                   threads=int(threads), walker_begin=int(w0), walker_end=int(w1))
+                   threads=int(threads), walker_begin=int(w0), walker_end=int(w1),
+                   instance=int(instance))
     rows = 1 if echo is None else len(echo)
     res = {"signal": np.zeros(rows * n_meas), "signal_im": np.zeros(rows * n_meas),
            "final_pos": np.zeros((N, 3), np.float32), "comp_origin": np.zeros(N, np.int32),

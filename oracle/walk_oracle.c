@@ -69,6 +69,7 @@ typedef struct {
   int32_t threads;
   int64_t walker_begin;    /* sub-range of walkers to run (cpu timing slices) */
   int64_t walker_end;      /* 0 = all */
+  uint32_t instance;       /* second Philox counter word (substrate instance) */
 } oracle_args;
 
 typedef struct {
@@ -499,7 +500,7 @@ static void sample_start(const oracle_args* A, uint32_t w, float r[3]) {
   if (A->kind == K_FREE) return;
   const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
   for (uint32_t att = 0; att < 4096u; ++att) {
-    const uint32_t ctr[4] = {w, 0u, att, 1u};
+    const uint32_t ctr[4] = {w, A->instance, att, 1u};
     uint32_t o[4];
     philox10(ctr, key[0], key[1], o);
     const float u0 = unit_u(o[0]), u1 = unit_u(o[1]), u2 = unit_u(o[2]);
@@ -595,12 +596,12 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             if (A->inj_u) u = A->inj_u[(size_t)w * S + gs];
           } else {
             if ((gs & 1) == 0 || gs == 0) {
-              const uint32_t ctr[4] = {(uint32_t)w, 0u, (uint32_t)(gs >> 1), 0u};
+              const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)(gs >> 1), 0u};
               philox10(ctr, key[0], key[1], rnd);
             }
             oracle_direction((gs & 1) ? rnd[2] : rnd[0], (gs & 1) ? rnd[3] : rnd[1], dir);
             if (A->wall == W_PERMEATE) {
-              const uint32_t ctr[4] = {(uint32_t)w, 0u, (uint32_t)gs, 2u};
+              const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)gs, 2u};
               uint32_t o[4];
               philox10(ctr, key[0], key[1], o);
               u = unit_u(o[0]);

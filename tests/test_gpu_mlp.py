@@ -1,0 +1,116 @@
+"""Surrogate MLP kernels (csrc/mlp.hip) against the CPU oracle (oracle/mlp_oracle.py).
+
+The oracle places the bfloat16 roundings where the kernels do, so the remaining
+differences are float32 accumulation order and the rare bfloat16 rounding flip
+they cause; the bars below allow for those and nothing more (a wrong transpose,
+a missing GELU' or a wrong bias gradient is off by O(1) relative).
+"""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+N_IN, WIDTH, DEPTH = 70, 128, 3
+
+
+def _net(gpu, seed=0, max_batch=256, n_in=N_IN, width=WIDTH, depth=DEPTH):
+    return gpu.MLP(n_in, width, depth, max_batch=max_batch, seed=seed)
+
+
+def _data(n, seed, n_in=N_IN):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((n, n_in)).astype(np.float32)
+    t = np.tanh(x[:, 0] - 0.5 * x[:, 1] * x[:, 2]).astype(np.float32)
+    return x, t
+
+
+def test_param_roundtrip_and_count(gpu):
+    m = _net(gpu)
+    assert m.n_params == mo.num_params(N_IN, WIDTH, DEPTH)
+    p = np.random.default_rng(1).standard_normal(m.n_params).astype(np.float32)
+    m.set_params(p)
+    np.testing.assert_array_equal(m.get_params(), p)
+    with pytest.raises(ValueError):
+        m.set_params(p[:-1])
+    m.close()
+
+
+def test_forward_matches_oracle(gpu):
+    print("mlp forward 70->128x3, batch 192", flush=True)
+    m = _net(gpu)
+    p = m.get_params()
+    x, _ = _data(192, 2)
+    y = m.forward(x)
+    ref = mo.forward(p, x, N_IN, WIDTH, DEPTH, bf16=True)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    assert np.max(np.abs(y - ref)) <= 2e-3 * scale
+    assert np.sqrt(np.mean((y - ref) ** 2)) <= 2e-4 * scale
+    # ragged row count is padded on the host; result rows unchanged
+    np.testing.assert_array_equal(m.forward(x[:70]), y[:70])
+    m.close()
+
+
+def test_gradients_match_oracle(gpu):
+    print("mlp gradients 70->128x3, batch 128", flush=True)
+    m = _net(gpu, seed=3)
+    p = m.get_params()
+    x, t = _data(128, 4)
+    g, loss = m.gradients(x, t)
+    gr, lr = mo.gradients(p, x, t, N_IN, WIDTH, DEPTH, bf16=True)
+    assert loss == pytest.approx(lr, rel=1e-3)
+    Ws, bs, wo, bo = mo.unpack(g, N_IN, WIDTH, DEPTH)
+    Wr, br, wor, bor = mo.unpack(gr, N_IN, WIDTH, DEPTH)
+    blocks = [(f"W{l}", Ws[l], Wr[l]) for l in range(DEPTH)] + \
+             [(f"b{l}", bs[l], br[l]) for l in range(DEPTH)] + [("w_out", wo, wor)]
+    for name, a, b in blocks:
+        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+        assert rel <= 1e-2, f"{name}: relative error {rel:.3g}"
+    assert bo == pytest.approx(bor, rel=1e-4, abs=1e-7)
+    # and the oracle's rounded gradient is close to the exact one (pinned by finite differences)
+    ge, _ = mo.gradients(p, x, t, N_IN, WIDTH, DEPTH, bf16=False)
+    assert np.linalg.norm(gr - ge) / np.linalg.norm(ge) <= 5e-2
+    m.close()
+
+
+def test_adam_step_matches_oracle(gpu):
+    print("mlp one Adam step", flush=True)
+    m = _net(gpu, seed=5)
+    p0 = m.get_params().astype(np.float64)
+    x, t = _data(128, 6)
+    g, _ = m.gradients(x, t)
+    m.set_data(x, t)
+    lr = 1e-3
+    m.train(128, np.arange(128), lr)
+    p1 = m.get_params().astype(np.float64)
+    pr, _, _ = mo.adam_step(p0, g.astype(np.float64), 0.0, 0.0, 1, lr)
+    np.testing.assert_allclose(p1, pr, rtol=0, atol=2e-6 * lr / 1e-3 + 1e-7)
+    m.close()
+
+
+def test_training_reduces_loss_and_predict_equals_forward(gpu):
+    print("mlp train 400 steps", flush=True)
+    m = _net(gpu, seed=7, max_batch=256)
+    x, t = _data(4096, 8)
+    m.set_data(x, t)
+    rng = np.random.default_rng(9)
+    idx = np.concatenate([rng.permutation(4096) for _ in range(25)])[:400 * 256]
+    losses, ms = m.train(256, idx, 1e-3, want_losses=True)
+    assert ms > 0
+    assert np.mean(losses[-20:]) < 0.1 * np.mean(losses[:5])
+    y, _ = m.predict(1000)
+    np.testing.assert_array_equal(y, m.forward(x[:1000]))
+    m.close()
+
+
+def test_batch_validation(gpu):
+    m = _net(gpu, max_batch=128)
+    x, t = _data(100, 1)
+    with pytest.raises(ValueError):
+        m.gradients(x, t)                      # not a multiple of 64
+    with pytest.raises(ValueError):
+        m.forward(np.zeros((200, N_IN), np.float32))   # above max_batch
+    with pytest.raises(ValueError):
+        m.train(64, np.arange(64), 1e-3)       # no resident data
+    m.close()

@@ -126,7 +126,7 @@ def test_geometry_closed_forms():
 # ---------------------------------------------------------------- C ABI
 def _header_symbols():
     txt = open(os.path.join(ROOT, "include", "dmipy_walk.h")).read()
-    return set(re.findall(r"^\s*(?:int|const char\*)\s+(dmw_\w+)\s*\(", txt, re.M))
+    return set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(dmw_\w+)\s*\(", txt, re.M))
 
 
 def test_library_exports_every_declared_symbol():
@@ -217,3 +217,43 @@ def test_oom_backoff_halves_walker_batches(monkeypatch):
     ok = [c for c in calls if c[0] <= 250]
     assert [c[1] for c in ok] == [11, 12, 13, 14]
     assert s[0] == pytest.approx(np.mean([11, 12, 13, 14]))
+
+
+def test_batch_argument_errors_before_gpu():
+    """The batched launch validates its descriptors before touching a device."""
+    import ctypes as C
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 20)
+    with pytest.raises(ValueError, match="64"):
+        ds.BatchedWalk([dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=64)],
+                       ds.tile_waveform(wf, 65))
+    with pytest.raises(ValueError, match="waveform index"):
+        ds.BatchedWalk([dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=64,
+                             waveform=1)], wf)
+    with pytest.raises(NotImplementedError, match="T1"):
+        ds.BatchedWalk([dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=64,
+                             T1=1.0)], wf)
+    G = np.ascontiguousarray(wf.G, np.float32)
+    wd = (_native.Waveform_t * 1)()
+    wd[0] = _native.Waveform_t(n_meas=1, n_t=G.shape[1], G=_native.fptr(G), chi=None,
+                               n_echo=0, echo_idx=None)
+    inst = (_native.Instance_t * 1)()
+    gd = _native.Geometry_t(kind=3, wall=0, n_obj=0, size=3e-6, cell=0.0, eps=3e-13,
+                            nudge=3e-10, rho_over_D=0.0, kappa_over_D=0.0,
+                            centers=None, radii=None)
+    h = C.c_void_p()
+    lib = _native.lib()
+    for bad in (dict(wave_set=1), dict(n_walkers=0), dict(n_sub=0)):
+        kw = dict(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7, gamma_dt_sub=1.0,
+                  dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
+        kw.update(bad)
+        inst[0] = _native.Instance_t(**kw)
+        assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EINVAL
+    gd.kind = 4       # packed without objects
+    inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7,
+                                 gamma_dt_sub=1.0, dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
+    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EINVAL
+    gd.kind = 0
+    gd.wall = 1       # free diffusion has no wall to relax on
+    inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7,
+                                 gamma_dt_sub=1.0, dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
+    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EUNSUPPORTED

@@ -1,0 +1,129 @@
+"""Pins of the surrogate MLP's CPU restatement (oracle/mlp_oracle.py).
+
+- bfloat16 rounding against torch's own float32 -> bfloat16 conversion and
+  hand-worked ties;
+- the exact (unrounded) backward pass against central finite differences of
+  the forward pass, so a dropped GELU' factor, a transposed weight or a wrong
+  loss scale fails;
+- the Adam step against torch.optim.Adam;
+- the waveform embedding against Parseval's identity (the bins sum to the
+  rectangle-rule b-value) and against calc_b.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_oracle as mo
+
+import dmipy_sim_amd as ds
+from dmipy_sim_amd import surrogate as sg
+
+
+def test_bf16_round_matches_torch_and_ties():
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(10000) * 10 ** rng.uniform(-8, 8, 10000),
+                        [0.0, -0.0, 1.0, -2.5, 3.0e38, 1e-40]]).astype(np.float32)
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy().astype(np.float64)
+    np.testing.assert_array_equal(mo.bf16_round(x), ref)
+    # 1 + 2^-8 is halfway between 1 and 1 + 2^-7: ties go to the even mantissa.
+    assert mo.bf16_round(np.float32(1 + 2.0 ** -8)) == 1.0
+    assert mo.bf16_round(np.float32(1 + 3 * 2.0 ** -8)) == 1 + 2.0 ** -6
+
+
+def test_gelu_grad_is_derivative():
+    x = np.linspace(-6, 6, 1001)
+    h = 1e-6
+    fd = (mo.gelu(x + h) - mo.gelu(x - h)) / (2 * h)
+    np.testing.assert_allclose(mo.gelu_grad(x), fd, atol=1e-8)
+    # tanh-form GELU against the erf form: within 1e-3 everywhere
+    from scipy.special import erf
+    np.testing.assert_allclose(mo.gelu(x), 0.5 * x * (1 + erf(x / np.sqrt(2))), atol=1e-3)
+
+
+@pytest.mark.parametrize("n_in,width,depth", [(5, 8, 1), (7, 6, 3)])
+def test_exact_gradient_matches_finite_differences(n_in, width, depth):
+    rng = np.random.default_rng(n_in)
+    n = mo.num_params(n_in, width, depth)
+    p = rng.standard_normal(n) * 0.5
+    x = rng.standard_normal((9, n_in))
+    t = rng.standard_normal(9)
+    g, loss = mo.gradients(p, x, t, n_in, width, depth, bf16=False)
+    y = mo.forward(p, x, n_in, width, depth, bf16=False)
+    assert loss == pytest.approx(np.mean((y - t) ** 2))
+    h = 1e-6
+    fd = np.empty(n)
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = h
+        lp = np.mean((mo.forward(p + e, x, n_in, width, depth, bf16=False) - t) ** 2)
+        lm = np.mean((mo.forward(p - e, x, n_in, width, depth, bf16=False) - t) ** 2)
+        fd[i] = (lp - lm) / (2 * h)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-7)
+
+
+def test_layout_matches_torch_module():
+    """Public layout and forward pass against a torch.nn stack (float64, no rounding)."""
+    n_in, width, depth = 6, 8, 3
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(mo.num_params(n_in, width, depth))
+    Ws, bs, w_out, b_out = mo.unpack(p, n_in, width, depth)
+    layers = []
+    for l in range(depth):
+        lin = torch.nn.Linear(Ws[l].shape[0], width).double()
+        lin.weight.data = torch.from_numpy(Ws[l].T.copy())
+        lin.bias.data = torch.from_numpy(bs[l].copy())
+        layers += [lin, torch.nn.GELU(approximate="tanh")]
+    out = torch.nn.Linear(width, 1).double()
+    out.weight.data = torch.from_numpy(w_out[None, :].copy())
+    out.bias.data = torch.tensor([b_out], dtype=torch.float64)
+    net = torch.nn.Sequential(*layers, out)
+    x = rng.standard_normal((11, n_in))
+    ref = net(torch.from_numpy(x)).detach().numpy()[:, 0]
+    np.testing.assert_allclose(mo.forward(p, x, n_in, width, depth, bf16=False), ref, rtol=1e-12)
+    np.testing.assert_allclose(mo.pack(Ws, bs, w_out, b_out), p)
+
+
+def test_adam_matches_torch():
+    rng = np.random.default_rng(2)
+    p0 = rng.standard_normal(50)
+    grads = [rng.standard_normal(50) for _ in range(5)]
+    tp = torch.nn.Parameter(torch.from_numpy(p0.copy()))
+    opt = torch.optim.Adam([tp], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    p, m, v = p0.copy(), np.zeros(50), np.zeros(50)
+    for k, g in enumerate(grads, start=1):
+        tp.grad = torch.from_numpy(g.copy())
+        opt.step()
+        p, m, v = mo.adam_step(p, g, m, v, k, 1e-2)
+    np.testing.assert_allclose(p, tp.detach().numpy(), rtol=1e-12, atol=1e-14)
+
+
+def test_waveform_embedding_parseval_and_calc_b():
+    rng = np.random.default_rng(1)
+    wf = sg.random_waveforms(rng, n_meas=8, n_t=250)
+    f = sg.waveform_features(wf)
+    assert f.shape == (8, 64)
+    q = ds.GAMMA * np.cumsum(np.asarray(wf.G, np.float64), axis=1) * wf.dt
+    b_rect = np.sum(q ** 2, axis=(1, 2)) * wf.dt * 1e-9
+    np.testing.assert_allclose(f[:, :62].sum(axis=1), b_rect, rtol=1e-10)
+    np.testing.assert_allclose(f[:, 62] + f[:, 63], b_rect, rtol=1e-10)
+    np.testing.assert_allclose(f[:, :31].sum(axis=1), f[:, 62], rtol=1e-10)
+    assert f[0].max() == 0.0                      # the b = 0 row
+    b = np.asarray(ds.calc_b(wf), np.float64) * 1e-9
+    np.testing.assert_allclose(f[1:, 62] + f[1:, 63], b[1:], rtol=2e-2)
+    # a z-only waveform has no transverse part
+    G = np.zeros_like(wf.G)
+    G[:, :, 2] = wf.G[:, :, 0]
+    fz = sg.waveform_features(ds.Waveform(G=G, dt=wf.dt, echo_idx=wf.echo_idx))
+    assert np.all(fz[:, :31] == 0) and np.all(fz[:, 62] == 0)
+
+
+def test_features_shape_and_prior():
+    rng = np.random.default_rng(3)
+    p = sg.sample_params(rng)
+    for k, (lo, hi) in sg.PRIOR.items():
+        assert lo <= p[k] <= hi
+    x = sg.features(p, sg.random_waveforms(rng, n_meas=5))
+    assert x.shape == (5, sg.N_IN) and x.dtype == np.float32
+    assert np.all(np.abs(x[:, :6]) <= 1.5)
+    geom = sg.build_substrate(p, seed=4)
+    assert np.min(geom._radii_np) >= sg.R_MIN * (1 - 1e-6)
