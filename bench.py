@@ -1,12 +1,16 @@
 """Throughput of the Monte-Carlo walk on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cylinder|free|packed]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload cylinder|free|packed|batch|surrogate]
 
 A step is one complete walk of the workload (all walkers, all n_t x n_sub
 sub-steps, all measurements, float64 ensemble reduction) with the waveform and
 geometry tables already resident in HBM.  The default workload is the
 Cylinder PGSE configuration (R = 5 um, rho = 1e-6 m/s, 1e6 walkers, n_t = 1000,
-128 measurements).  Prints one JSON line on stdout; progress goes to stderr.
+128 measurements).  ``batch`` is the 4096-instance Sphere/Cylinder batch in one
+persistent graph-captured launch (a step = the whole batch); ``surrogate``
+trains and measures the learned surrogate against the batched walk.  Prints one
+JSON line on stdout; progress goes to stderr.
 
 Roofline: the walk keeps every walker in registers, so it is bound by float32
 vector (VALU) issue, not by HBM.  ``achieved`` counts algorithmic flops per
@@ -66,6 +70,168 @@ def falg(cfg, plan, hbar, kbar):
     return f
 
 
+def batch_flops(plans, walker_steps, n_meas):
+    """Sum over instances of falg x walker-steps (hits not counted: the batched
+    launch has no per-instance counters, so the wall term is its hit-free part)."""
+    total = 0.0
+    for plan, ws in zip(plans, walker_steps):
+        total += falg({"waveform": _Shape(n_meas)}, plan, 0.0, 0.0) * ws
+    return total
+
+
+class _Shape:
+    def __init__(self, n_meas):
+        import numpy as np
+        self.G = np.zeros((n_meas, 1, 3), np.float32)
+
+
+def run_batch(args):
+    """4096 Sphere/Cylinder instances (permeable, relaxing walls; uneven walker
+    counts) x a shared 64-measurement PGSE/OGSE protocol, one persistent launch."""
+    import dmipy_sim_amd as ds
+    from dmipy_sim_amd import configs
+    wf = configs.batch_protocol()
+    inst = configs.batch_instances(args.n_inst)
+    if args.walker_scale != 1.0:
+        for s in inst:
+            s["n_walkers"] = max(64, int(s["n_walkers"] * args.walker_scale))
+    t0 = time.perf_counter()
+    bw = ds.BatchedWalk(inst, wf, waves_per_simd=args.waves_per_simd)
+    log(f"batch: {len(inst)} instances, {int(bw.n_walkers.sum())} walkers, n_t={bw.n_t}, "
+        f"n_meas={bw.n_meas}, n_sub median {int(sorted(bw.n_sub)[len(inst)//2])}, "
+        f"plan {time.perf_counter() - t0:.1f} s")
+    plans = [ds.make_walk_plan(s["geometry"], s["diffusivity"], wf.dt) for s in inst]
+    ws = [int(s["n_walkers"]) * bw.n_t * p.n_sub for s, p in zip(inst, plans)]
+    sig, st = bw.run(seed=1)
+    log(f"first launch: {st['total_ms']:.1f} ms, units {st['n_units']}, grid {st['grid']}")
+    for i in range(args.warmup):
+        bw.run(seed=2 + i)
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        kms.append(bw.run(seed=100 + i)[1]["total_ms"])
+    wall = time.perf_counter() - t0
+    total_ws = st["walker_steps"]
+    value = total_ws * args.steps / wall
+    kmean = statistics.mean(kms)
+    flops = batch_flops(plans, ws, bw.n_meas)
+    achieved = flops / (kmean * 1e-3) / 1e12
+    log(f"timed {args.steps}: {1e3 * wall / args.steps:.1f} ms/step, graph {kmean:.1f} ms, "
+        f"{value:.4e} WS/s, {achieved:.2f} TFLOP/s")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(inst, wf, plans, args.cpu_seconds)
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": None, "kernel": "batch_kernel", "kernel_ms": round(kmean, 3),
+                "flops_per_walker_step": round(flops / total_ws, 2)}
+    config = {"workload": "batch_4096_sphere_cylinder", "instances": len(inst),
+              "walkers": int(bw.n_walkers.sum()), "n_t": bw.n_t, "n_meas": bw.n_meas,
+              "walker_steps_per_launch": total_ws, "graph": True,
+              "phase": "fp32", "ensemble_sum": "fp64"}
+    line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_batch(inst, wf, plans, target_s):
+    """The oracle on a sample of the batch: every 256th instance, walkers cut so
+    the sample takes about target_s seconds on the host cores."""
+    from oracle import oracle as orc
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    pick = list(range(0, len(inst), 256))
+    ws_total = sum(wf.G.shape[1] * plans[i].n_sub for i in pick)   # per walker, all picked
+    t0 = time.perf_counter()
+    for i in pick:
+        s = inst[i]
+        orc.run_oracle(s["geometry"], s["diffusivity"], wf, threads, seed=1, threads=threads)
+    probe = time.perf_counter() - t0
+    n = max(threads, int(target_s / max(probe / threads, 1e-9)) // threads * threads)
+    t0 = time.perf_counter()
+    for i in pick:
+        s = inst[i]
+        orc.run_oracle(s["geometry"], s["diffusivity"], wf, n, seed=2, threads=threads)
+    el = time.perf_counter() - t0
+    rate = n * ws_total / el
+    log(f"cpu_baseline: {len(pick)} instances x {n} walkers in {el:.2f} s -> {rate:.3e} WS/s")
+    return {"value": rate, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{len(pick)} of {len(inst)} instances (every 256th) x {n} walkers, "
+                      "all samples and sub-steps, OpenMP over walkers"}
+
+
+PEAK_BF16_TFLOPS = 2500.0
+
+
+def run_surrogate(args):
+    """Surrogate signals/s (4-member 8 x 512 GELU ensemble, batch 65,536, inputs
+    resident) against the batched walk at the walker count that reaches the
+    same RMSE bound, both on this GPU; accuracy against held-out walks with
+    2e5 walkers per substrate."""
+    from dmipy_sim_amd import surrogate as sg
+    t0 = time.perf_counter()
+    r = sg.benchmark(n_train_inst=args.surr_instances, n_walkers=args.surr_walkers,
+                     epochs=args.surr_epochs, log=log)
+    wall = time.perf_counter() - t0
+    sur = r["surrogate"]
+    roofline = {"bound": "mfma", "achieved": round(sur["tflops"], 2), "peak": PEAK_BF16_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(sur["tflops"] / PEAK_BF16_TFLOPS, 4),
+                "traffic": None, "kernel": "gemm_bf16 (forward chain)",
+                "kernel_ms": round(sur["ms"], 4)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_surrogate(r["mc"]["walkers"], args.cpu_seconds)
+        cpu["gpu_surrogate_over_cpu"] = round(sur["signals_per_s"] / cpu["value"], 1)
+    config = {"workload": "surrogate_packed_cylinders", "network": "8x512 GELU, bf16 weights, fp32 accumulate",
+              "members": sur["members"], "inference_batch": sur["rows"],
+              "training_signals": r["train"]["signals"], "training_walkers": r["train"]["walkers"],
+              "epochs": r["fit"]["epochs"], "held_out_signals": r["held_out"]["signals"],
+              "held_out_walkers": r["held_out"]["walkers"], "n_t": 250, "n_meas": 64}
+    line = {"metric": METRIC, "value": sur["signals_per_s"], "unit": "signals/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": sur["ms"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+            "config": config, "roofline": roofline, "cpu_baseline": cpu,
+            "accuracy": {"rmse_surrogate": r["rmse_surrogate"], "rmse_hybrid": r["rmse_hybrid"],
+                         "fallback_fraction": r["fallback_fraction"], "bound": 5e-3},
+            "mc": {"walkers": r["mc"]["walkers"], "rmse": r["mc"]["rmse"],
+                   "signals_per_s": r["mc"]["signals_per_s"]},
+            "gain_over_mc": r["gain"], "gain_hybrid": r["gain_hybrid"],
+            "train_ms_total": r["fit"]["train_ms"], "wall_s": round(wall, 1)}
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_surrogate(n_walkers, target_s):
+    """The oracle walking generator substrates on the host cores: signals/s at
+    the walker count the GPU walk needs for the accuracy bound."""
+    import numpy as np
+    from oracle import oracle as orc
+    from dmipy_sim_amd import surrogate as sg
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rng = np.random.default_rng(11)
+    p = sg.sample_params(rng)
+    geom = sg.build_substrate(p, seed=5)
+    wf = sg.random_waveforms(rng)
+    plan = orc.oracle_plan(geom, p["D"], wf.dt)
+    n_t, n_meas = wf.G.shape[1], wf.G.shape[0]
+    t0 = time.perf_counter()
+    orc.run_oracle(geom, p["D"], wf, threads, seed=1, threads=threads, plan=plan)
+    per = (time.perf_counter() - t0) / threads
+    n = max(threads, int(target_s / max(per, 1e-9)) // threads * threads)
+    t0 = time.perf_counter()
+    orc.run_oracle(geom, p["D"], wf, n, seed=2, threads=threads, plan=plan)
+    el = time.perf_counter() - t0
+    ws_rate = n * n_t * plan["n_sub"] / el
+    sps = ws_rate / (n_walkers * n_t * plan["n_sub"]) * n_meas
+    log(f"cpu_baseline: {n} walkers x {n_t} x n_sub {plan['n_sub']} in {el:.2f} s -> "
+        f"{ws_rate:.3e} WS/s = {sps:.3g} signals/s at {n_walkers} walkers")
+    return {"value": sps, "unit": "signals/s", "cores": threads, "kind": "port",
+            "walker_steps_per_s": ws_rate,
+            "sample": f"one generator substrate (n_sub {plan['n_sub']}), {n} walkers x "
+                      f"{n_t} samples x 64 measurements; signals/s at {n_walkers} walkers"}
+
+
 def build_workload(name):
     from dmipy_sim_amd import configs
     if name == "cylinder":
@@ -116,7 +282,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cylinder", choices=["cylinder", "free", "packed"])
+    ap.add_argument("--workload", default="cylinder",
+                    choices=["cylinder", "free", "packed", "batch", "surrogate"])
+    ap.add_argument("--n-inst", type=int, default=4096, help="batch workload: instances")
+    ap.add_argument("--walker-scale", type=float, default=1.0,
+                    help="batch workload: scale every instance's walker count")
+    ap.add_argument("--waves-per-simd", type=int, default=0, help="batch workload: persistent waves")
+    ap.add_argument("--surr-instances", type=int, default=3125,
+                    help="surrogate workload: training substrates (x 64 signals)")
+    ap.add_argument("--surr-walkers", type=int, default=10_000,
+                    help="surrogate workload: walkers per training substrate")
+    ap.add_argument("--surr-epochs", type=int, default=60)
     ap.add_argument("--slice-t", type=int, default=0,
                     help="packed workload: walk only the first N waveform samples")
     ap.add_argument("--no-cpu", action="store_true")
@@ -141,6 +317,11 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(walkers shard across GPUs only in the design).", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        return run_batch(args)
+    if args.workload == "surrogate":
+        return run_surrogate(args)
 
     import dmipy_sim_amd as ds
     cfg = build_workload(args.workload)

@@ -53,17 +53,25 @@ constexpr int TM = 64, TN = 64, TK = 32, LDSK = 40;  // tile and padded LDS row 
 //   TB: B stored transposed (element (k,n) at B[n*ldb + k]); else B[k*ldb + n].
 // M, N multiples of 64, K a multiple of 32 (checked on the host).  256 threads:
 // 4 wavefronts in a 2x2 grid, each owning a 32x32 block = 2x2 MFMA 16x16 tiles.
+//
+// EPI_DW with colsum != nullptr: the blocks of the first row of tiles also sum
+// the B operand over K (colsum[n] = sum_k B[k][n], the bias gradient when B is
+// dZ) from the tiles already staged in LDS: thread (n = tid % 64, q = tid / 64)
+// adds B[k][n] for k in the q-th quarter of each K tile.
 template <int EPI, bool TA, bool TB>
 __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                                  const __bf16* __restrict__ B, int ldb,
                                                  const float* __restrict__ bias, float* __restrict__ Zout,
                                                  __bf16* __restrict__ Yout, const float* __restrict__ Zprev,
-                                                 float* __restrict__ Cout, int ldc) {
+                                                 float* __restrict__ Cout, int ldc, float* __restrict__ colsum) {
   __shared__ __bf16 As[TM][LDSK];
   __shared__ __bf16 Bs[TN][LDSK];
+  __shared__ float red[4][TN];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
   const int wm = (wv >> 1) * 32, wn = (wv & 1) * 32;
+  const bool do_colsum = (EPI == EPI_DW) && colsum != nullptr && blockIdx.y == 0;
+  float csum = 0.0f;
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -97,7 +105,16 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const __bf
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (do_colsum) {
+      const bf16x8 v = *reinterpret_cast<const bf16x8*>(&Bs[lane][wv * 8]);
+      for (int e = 0; e < 8; ++e) csum += bf2f(v[e]);
+    }
     __syncthreads();
+  }
+  if (do_colsum) {
+    red[wv][lane] = csum;
+    __syncthreads();
+    if (wv == 0) colsum[n0 + lane] = (red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]);
   }
   // D layout: lane holds column (lane % 16), rows 4 (lane / 16) + r of each tile.
   for (int i = 0; i < 2; ++i)
@@ -116,15 +133,6 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const __bf
           Cout[(size_t)m * ldc + n] = v;
         }
       }
-}
-
-// db[n] = sum_m dZ[m][n] (fp32 sum of bf16 values), one thread per column.
-__global__ void col_sum_kernel(int M, int N, const __bf16* __restrict__ dZ, float* __restrict__ db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  float s = 0.0f;
-  for (int m = 0; m < M; ++m) s += bf2f(dZ[(size_t)m * N + n]);
-  db[n] = s;
 }
 
 // y[b] = Y[b] . w + b0; with targets also the squared error and dy = 2 (y - t) / B.
@@ -158,14 +166,35 @@ __global__ void out_bwd_dz_kernel(int B, int H, const float* __restrict__ dy, co
   dZ[idx] = f2bf(dy[b] * w[i] * gelu_grad(Z[idx]));
 }
 
-// dw[i] = sum_b Y[b][i] dy[b]; db0 = sum_b dy[b] (thread H computes the bias).
-__global__ void out_bwd_w_kernel(int B, int H, const __bf16* __restrict__ Y, const float* __restrict__ dy,
-                                 float* __restrict__ dw, float* __restrict__ db0) {
+// Output-layer weight gradient dw[i] = sum_b Y[b][i] dy[b] in two deterministic
+// stages.  Stage 1: block (column tile x, row chunk y of 64 rows), thread
+// (i = tid % 64, q = tid / 64) sums rows q, q+4, ... of the chunk; the four
+// partial sums are combined in LDS into part[y][i].  Stage 2 adds the chunks in
+// order and also forms db0 = sum_b dy[b] (thread H).
+__global__ __launch_bounds__(256) void out_bwd_w_part_kernel(int B, int H, const __bf16* __restrict__ Y,
+                                                             const float* __restrict__ dy,
+                                                             float* __restrict__ part) {
+  __shared__ float red[4][64];
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63), q = threadIdx.x >> 6;
+  const int r0 = blockIdx.y * 64;
+  float s = 0.0f;
+  for (int r = r0 + q; r < r0 + 64 && r < B; r += 4) s += bf2f(Y[(size_t)r * H + i]) * dy[r];
+  red[q][threadIdx.x & 63] = s;
+  __syncthreads();
+  if (q == 0) {
+    const int c = threadIdx.x & 63;
+    part[(size_t)blockIdx.y * H + i] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
+  }
+}
+
+__global__ void out_bwd_w_sum_kernel(int B, int H, int n_chunks, const float* __restrict__ part,
+                                     const float* __restrict__ dy, float* __restrict__ dw,
+                                     float* __restrict__ db0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > H) return;
   float s = 0.0f;
   if (i < H) {
-    for (int b = 0; b < B; ++b) s += bf2f(Y[(size_t)b * H + i]) * dy[b];
+    for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * H + i];
     dw[i] = s;
   } else {
     for (int b = 0; b < B; ++b) s += dy[b];
@@ -236,6 +265,7 @@ struct dmw_mlp {
   std::vector<float*> Z;    // Z[l] (B x H)
   __bf16 *dZa = nullptr, *dZb = nullptr;
   float *out = nullptr, *dy = nullptr, *sq = nullptr, *tb = nullptr;
+  float* part = nullptr;           // (max_batch / 64, H) output-layer partial sums
   // resident dataset
   __bf16* dX = nullptr;
   float* dT = nullptr;
@@ -273,9 +303,10 @@ namespace {
 
 template <int EPI, bool TA, bool TB>
 hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb,
-                const float* bias, float* Z, __bf16* Y, const float* Zprev, float* C, int ldc) {
+                const float* bias, float* Z, __bf16* Y, const float* Zprev, float* C, int ldc,
+                float* colsum = nullptr) {
   hipLaunchKernelGGL((dmw::gemm_bf16<EPI, TA, TB>), dim3(N / dmw::TN, M / dmw::TM), dim3(256), 0, st, M, N,
-                     K, A, lda, B, ldb, bias, Z, Y, Zprev, C, ldc);
+                     K, A, lda, B, ldb, bias, Z, Y, Zprev, C, ldc, colsum);
   return hipGetLastError();
 }
 
@@ -301,8 +332,11 @@ int forward(dmw_mlp* m, int B, const float* targets_dev) {
 int backward(dmw_mlp* m, int B) {
   hipStream_t st = m->st;
   const int H = m->H, D = m->depth;
-  hipLaunchKernelGGL(dmw::out_bwd_w_kernel, dim3(blocks(H + 1, 256)), dim3(256), 0, st, B, H, m->Y[D], m->dy,
-                     m->g + m->wout_off, m->g + m->bout_off);
+  const int n_chunks = (B + 63) / 64;
+  hipLaunchKernelGGL(dmw::out_bwd_w_part_kernel, dim3(H / 64, n_chunks), dim3(256), 0, st, B, H, m->Y[D], m->dy,
+                     m->part);
+  hipLaunchKernelGGL(dmw::out_bwd_w_sum_kernel, dim3(blocks(H + 1, 256)), dim3(256), 0, st, B, H, n_chunks,
+                     m->part, m->dy, m->g + m->wout_off, m->g + m->bout_off);
   hipLaunchKernelGGL(dmw::out_bwd_dz_kernel, dim3(blocks((long long)B * H, 256)), dim3(256), 0, st, B, H,
                      m->dy, m->p + m->wout_off, m->Z[D - 1], m->dZa);
   MLP_TRY(hipGetLastError(), "mlp output backward");
@@ -310,13 +344,10 @@ int backward(dmw_mlp* m, int B) {
   __bf16* dz_next = m->dZb;
   for (int l = D - 1; l >= 0; --l) {
     const int K = l == 0 ? m->k0 : H;
-    // dW_l = Y_l^T dZ_l  (K x H), db_l = colsum dZ_l
+    // dW_l = Y_l^T dZ_l  (K x H); db_l = colsum dZ_l from the same LDS tiles
     MLP_TRY((gemm<dmw::EPI_DW, true, false>(st, K, H, B, m->Y[l], K, dz, H, nullptr, nullptr, nullptr, nullptr,
-                                            m->g + m->w_off[l], H)),
+                                            m->g + m->w_off[l], H, m->g + m->b_off[l])),
             "mlp dW gemm");
-    hipLaunchKernelGGL(dmw::col_sum_kernel, dim3(blocks(H, 256)), dim3(256), 0, st, B, H, dz,
-                       m->g + m->b_off[l]);
-    MLP_TRY(hipGetLastError(), "mlp bias grad");
     if (l > 0) {
       // dZ_{l-1} = (dZ_l W_l^T) * GELU'(Z_{l-1})
       MLP_TRY((gemm<dmw::EPI_DX, false, true>(st, B, H, H, dz, H, m->pbf + m->wbf_off[l], H, nullptr, nullptr,
@@ -406,6 +437,7 @@ int dmw_mlp_create(dmw_mlp** out, int32_t n_in, int32_t width, int32_t depth, in
   MC(m->alloc(&m->dy, max_batch), "hipMalloc");
   MC(m->alloc(&m->sq, max_batch), "hipMalloc");
   MC(m->alloc(&m->tb, max_batch), "hipMalloc");
+  MC(m->alloc(&m->part, (size_t)(max_batch / 64) * width), "hipMalloc");
   MC(hipEventCreate(&m->e0), "hipEventCreate");
   MC(hipEventCreate(&m->e1), "hipEventCreate");
   // bf16 index map: weight matrices map to their bf16 copy, biases to -1

@@ -94,12 +94,14 @@ class MLP:
         return x, n, B
 
     def forward(self, x):
-        """Network output for rows of x (n, n_in); any n up to max_batch."""
+        """Network output for rows of x (n, n_in), in chunks of max_batch rows."""
         x, n, B = self._pad(x)
-        if B > self.max_batch:
-            raise ValueError("batch larger than max_batch; use set_data + predict")
         y = np.empty(B, np.float32)
-        nat.check(nat.lib().dmw_mlp_forward(self._h, nat.fptr(x), B, nat.fptr(y)))
+        for r0 in range(0, B, self.max_batch):
+            xc = np.ascontiguousarray(x[r0:r0 + self.max_batch])
+            yc = np.empty(xc.shape[0], np.float32)
+            nat.check(nat.lib().dmw_mlp_forward(self._h, nat.fptr(xc), xc.shape[0], nat.fptr(yc)))
+            y[r0:r0 + xc.shape[0]] = yc
         return y[:n]
 
     def gradients(self, x, t):
@@ -201,15 +203,29 @@ def waveform_features(waveform):
 
 
 def substrate_features(p):
-    """Six normalised inputs from a parameter dict (keys of PRIOR)."""
+    """Six normalised inputs from a parameter dict: mean radius, Gamma shape
+    (log), volume fraction, log10 permeability, log10 relaxivity, diffusivity."""
     return np.array([
         p["mean_r"] / 1e-6 - 2.5,
-        (p["shape"] - 5.0) / 3.0,
+        np.clip((np.log(p["shape"]) - np.log(4.5)) / 0.7, -3.0, 3.0),
         (p["vf"] - 0.475) / 0.175,
         (np.log10(p["kappa"]) + 5.0),
         (np.log10(p["rho"]) + 6.0),
         p["D"] / 1e-9 - 1.75,
     ])
+
+
+def realized_params(p, geometry):
+    """The parameters a drawn substrate actually has: the sample mean radius,
+    the moment estimate of the Gamma shape (mean^2 / variance) and the volume
+    fraction pi sum r^2 / L^2 of its cell; permeability, relaxivity and D as
+    drawn.  A finite cell's radii deviate from the prior's nominal values by
+    several percent, which the signal resolves."""
+    r = np.asarray(geometry._radii_np, np.float64)
+    L = float(geometry._L_float)
+    mean = float(r.mean())
+    return dict(p, mean_r=mean, shape=mean * mean / max(float(r.var()), 1e-30),
+                vf=float(np.pi * np.sum(r * r) / (L * L)))
 
 
 def features(params, waveform):
@@ -271,16 +287,19 @@ def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=51
     gpu_ms = 0.0
     walker_steps = 0
     params = []
+    instances = []
     for c0 in range(0, n_instances, chunk):
         specs, waves = [], []
         for i in range(c0, min(n_instances, c0 + chunk)):
             p = sample_params(rng)
             geom = build_substrate(p, seed=int(rng.integers(2**31)))
+            p = realized_params(p, geom)
             wf = random_waveforms(rng, n_meas=n_meas, n_t=n_t, T=T)
             specs.append(dict(geometry=geom, diffusivity=p["D"], n_walkers=int(n_walkers),
                               waveform=len(waves)))
             waves.append(wf)
             params.append(p)
+            instances.append((dict(geometry=geom, diffusivity=p["D"]), wf))
             X.append(features(p, wf))
         bw = BatchedWalk(specs, waves)
         sig, st = bw.run(seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
@@ -291,7 +310,7 @@ def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=51
         if log:
             log(f"generate: {min(n_instances, c0 + chunk)}/{n_instances} instances, "
                 f"{st['total_ms']:.0f} ms")
-    info = dict(gpu_ms=gpu_ms, walker_steps=walker_steps, params=params,
+    info = dict(gpu_ms=gpu_ms, walker_steps=walker_steps, params=params, instances=instances,
                 signals=n_instances * n_meas, n_walkers=int(n_walkers))
     return np.concatenate(X), np.concatenate(S), info
 
@@ -302,17 +321,30 @@ def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=51
 class SurrogateEnsemble:
     """``n_members`` MLPs trained from different initialisations and batch
     orders; the prediction is their mean and the spread their standard
-    deviation."""
+    deviation (both of S).  With ``target="log"`` the networks regress -ln S,
+    which the Gaussian-phase form makes close to linear in the waveform bins."""
 
-    def __init__(self, n_members=4, width=512, depth=8, max_batch=65536, seed=0):
+    def __init__(self, n_members=4, width=512, depth=8, max_batch=65536, seed=0, target="log"):
+        if target not in ("log", "signal"):
+            raise ValueError("target must be 'log' (the network predicts -ln S) or 'signal'")
         self.members = [MLP(N_IN, width, depth, max_batch, seed=seed + 101 * k)
                         for k in range(n_members)]
         self.seed = seed
+        self.target = target
+
+    def _encode(self, S):
+        S = np.asarray(S, np.float64)
+        return (-np.log(np.clip(S, 1e-3, None)) if self.target == "log" else S).astype(np.float32)
+
+    def _decode(self, y):
+        y = np.asarray(y, np.float64)
+        return np.exp(-y) if self.target == "log" else y
 
     def fit(self, X, S, epochs=40, batch=1024, lr=1e-3, lr_final=2e-5, log=None):
         """Adam with a cosine learning-rate decay; returns per-member final
         losses and total training ms."""
         n = X.shape[0]
+        S = self._encode(S)
         steps_per_epoch = n // batch
         total = steps_per_epoch * epochs
         s = np.arange(total)
@@ -343,7 +375,7 @@ class SurrogateEnsemble:
         for m in self.members:
             m.set_data(X, np.zeros(X.shape[0], np.float32))
             y, t = m.predict()
-            outs.append(y.astype(np.float64))
+            outs.append(self._decode(y))
             ms += t
         outs = np.stack(outs)
         return outs.mean(axis=0), outs.std(axis=0), ms
@@ -353,37 +385,131 @@ class SurrogateEnsemble:
             m.close()
 
 
-def predict_with_fallback(ensemble, params_list, waveforms, threshold=5e-3, n_walkers=20_000,
-                          seed=0, n_meas=None):
-    """Signals for every (instance, measurement): the ensemble mean where its
-    spread is at most ``threshold``, else the batched Monte-Carlo walk of that
-    instance (all its measurements are walked; the flagged ones are replaced).
+def instance_params(geometry, diffusivity):
+    """Network parameters of a PackedCylinders instance (realized statistics)."""
+    if geometry.permeability is None or geometry.surface_relaxivity_t2 is None:
+        raise ValueError("the surrogate covers PackedCylinders with permeability and "
+                         "surface relaxivity set")
+    p = dict(kappa=float(geometry.permeability), rho=float(geometry.surface_relaxivity_t2),
+             D=float(diffusivity))
+    return realized_params(p, geometry)
 
-    Returns (signals (n_inst, n_meas), fallback mask, timing dict)."""
+
+def predict_with_fallback(ensemble, instances, waveforms, threshold=5e-3, n_walkers=10_000,
+                          seed=0):
+    """Signals for every (instance, measurement): the ensemble mean where its
+    spread is at most ``threshold``, else the batched Monte-Carlo walk.
+
+    ``instances`` are dicts with 'geometry' (PackedCylinders with permeability
+    and surface relaxivity) and 'diffusivity'; ``waveforms[i]`` is instance i's
+    Waveform (all with the same number of measurements and n_t).  An instance
+    with any flagged measurement is walked with ``n_walkers`` walkers and its
+    flagged measurements are replaced.  Returns (signals (n_inst, n_meas),
+    fallback mask, timing dict)."""
     from .batch import BatchedWalk
-    X = np.concatenate([features(p, w) for p, w in zip(params_list, waveforms)])
+    X = np.concatenate([features(instance_params(s["geometry"], s["diffusivity"]), w)
+                        for s, w in zip(instances, waveforms)])
     n_meas = waveforms[0].G.shape[0]
-    t0 = time.perf_counter()
     mean, spread, ms = ensemble.predict(X)
-    t_pred = time.perf_counter() - t0
-    mean = mean.reshape(len(params_list), n_meas)
-    flag = spread.reshape(len(params_list), n_meas) > threshold
+    mean = mean.reshape(len(instances), n_meas)
+    flag = spread.reshape(len(instances), n_meas) > threshold
     out = mean.copy()
     rows = np.nonzero(flag.any(axis=1))[0]
     mc_ms = 0.0
     if rows.size:
-        rng = np.random.default_rng(seed)
-        specs, waves = [], []
-        for i in rows:
-            geom = build_substrate(params_list[i], seed=int(rng.integers(2**31)))
-            specs.append(dict(geometry=geom, diffusivity=params_list[i]["D"],
-                              n_walkers=int(n_walkers), waveform=len(waves)))
-            waves.append(waveforms[i])
-        bw = BatchedWalk(specs, waves)
+        specs = [dict(geometry=instances[i]["geometry"], diffusivity=instances[i]["diffusivity"],
+                      n_walkers=int(n_walkers), waveform=j) for j, i in enumerate(rows)]
+        bw = BatchedWalk(specs, [waveforms[i] for i in rows])
         sig, st = bw.run(seed)
         bw.close()
         mc_ms = st["total_ms"]
         for j, i in enumerate(rows):
             out[i, flag[i]] = sig[j, flag[i]]
-    return out, flag, dict(predict_s=t_pred, predict_kernel_ms=ms, mc_ms=mc_ms,
-                           n_fallback_instances=int(rows.size))
+    return out, flag, dict(predict_kernel_ms=ms, mc_ms=mc_ms, n_fallback_instances=int(rows.size))
+
+
+# ---------------------------------------------------------------------------
+# Accuracy and throughput against the Monte-Carlo walk.
+# ---------------------------------------------------------------------------
+def mc_ladder(n_inst, seed, ref_signals, walkers=(256, 512, 1024, 2048, 4096, 8192, 16384),
+              target=5e-3, log=None):
+    """The method without the surrogate: the batched walk at increasing walker
+    counts on the held-out instances; returns the first count whose RMSE
+    against the reference signals is at most ``target`` (or the last) with its
+    RMSE."""
+    rows = []
+    for nw in walkers:
+        _, S, info = generate(n_inst, nw, seed=seed, chunk=n_inst, walk_seed=5000 + nw)
+        rmse = float(np.sqrt(np.mean((S - ref_signals) ** 2)))
+        rows.append((nw, rmse))
+        if log:
+            log(f"mc ladder: {nw} walkers -> rmse {rmse:.4f}")
+        if rmse <= target:
+            break
+    return rows[-1][0], rows[-1][1], rows
+
+
+def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_000, epochs=60,
+              batch=1024, members=4, width=512, depth=8, infer_rows=65536, threshold=5e-3,
+              seed=1, target="log", lr=1e-3, log=None):
+    """Train the ensemble on batched-MC signals and measure it against the walk.
+
+    Returns a dict with the held-out RMSE of the ensemble mean, of the hybrid
+    (MC where the spread exceeds ``threshold``), the surrogate's inference
+    signals/s at ``infer_rows`` rows per batch (device time of all members,
+    inputs resident), the walk's signals/s at the walker count that reaches the
+    same RMSE bound, and their ratio."""
+    log = log or (lambda s: None)
+    res = {}
+    X, S, info = generate(n_train_inst, n_walkers, seed=seed, log=log)
+    res["train"] = dict(signals=int(S.size), walkers=n_walkers, gpu_ms=info["gpu_ms"],
+                        walker_steps=int(info["walker_steps"]))
+    log(f"training set: {S.size} signals, {info['gpu_ms']:.0f} ms of walk")
+    Xh, Sh, ih = generate(n_held, held_walkers, seed=seed + 1, chunk=n_held)
+    res["held_out"] = dict(signals=int(Sh.size), walkers=held_walkers,
+                           se_max=float(np.sqrt(0.5 / held_walkers)))
+    ens = SurrogateEnsemble(members, width, depth, max_batch=infer_rows, seed=seed, target=target)
+    finals, train_ms = ens.fit(X, S, epochs=epochs, batch=batch, lr=lr, log=log)
+    res["fit"] = dict(final_losses=finals, train_ms=train_ms, epochs=epochs, batch=batch,
+                      steps_per_member=(S.size // batch) * epochs)
+    mean, spread, _ = ens.predict(Xh)
+    err = mean - Sh
+    res["rmse_surrogate"] = float(np.sqrt(np.mean(err ** 2)))
+    res["spread_mean"] = float(spread.mean())
+    insts = [a for a, _ in ih["instances"]]
+    waves = [w for _, w in ih["instances"]]
+    hybrid, flag, _ = predict_with_fallback(ens, insts, waves, threshold=threshold,
+                                            n_walkers=n_walkers, seed=777)
+    res["fallback_fraction"] = float(flag.mean())
+    res["rmse_hybrid"] = float(np.sqrt(np.mean((hybrid.reshape(-1) - Sh) ** 2)))
+    log(f"held-out rmse: surrogate {res['rmse_surrogate']:.4f}, hybrid {res['rmse_hybrid']:.4f}, "
+        f"fallback {res['fallback_fraction']:.3%}")
+    # inference throughput: infer_rows resident rows through every member
+    Xi = X[np.arange(infer_rows) % X.shape[0]]
+    ms_total = 0.0
+    for m in ens.members:
+        m.set_data(Xi, np.zeros(infer_rows, np.float32))
+        m.predict()                        # warm-up
+        reps = [m.predict()[1] for _ in range(5)]
+        ms_total += float(np.median(reps))
+    sur_sps = infer_rows / (ms_total * 1e-3)
+    flops = ens.members[0].flops_per_sample() * infer_rows * members
+    res["surrogate"] = dict(rows=infer_rows, members=members, ms=ms_total,
+                            signals_per_s=sur_sps, tflops=flops / (ms_total * 1e-3) / 1e12)
+    log(f"surrogate inference: {infer_rows} rows x {members} members in {ms_total:.3f} ms "
+        f"-> {sur_sps:.3e} signals/s, {res['surrogate']['tflops']:.1f} TFLOP/s")
+    # the walk at the accuracy bound
+    nw, rmse_mc, ladder = mc_ladder(n_held, seed + 1, Sh, target=threshold, log=log)
+    n_rate = max(256, min(n_train_inst, 1024))
+    _, Sr, ir = generate(n_rate, nw, seed=seed + 2)
+    mc_sps = Sr.size / (ir["gpu_ms"] * 1e-3)
+    res["mc"] = dict(walkers=nw, rmse=rmse_mc, ladder=ladder, signals_per_s=mc_sps,
+                     instances_timed=n_rate)
+    eff = infer_rows / (ms_total * 1e-3 + res["fallback_fraction"] * infer_rows / mc_sps)
+    res["hybrid_signals_per_s"] = eff
+    res["gain"] = sur_sps / mc_sps
+    res["gain_hybrid"] = eff / mc_sps
+    log(f"walk at rmse<={threshold}: {nw} walkers, {mc_sps:.3e} signals/s; gain {res['gain']:.0f}x "
+        f"(hybrid {res['gain_hybrid']:.0f}x)")
+    ens.close()
+    return res

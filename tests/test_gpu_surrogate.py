@@ -1,0 +1,50 @@
+"""The learned surrogate against the method without it (the batched walk).
+
+A reduced version of the benchmark pipeline: the ensemble is trained on the
+GPU by the library's own kernels on walk signals of PackedCylinders substrates,
+then judged against held-out walks with many walkers.  The test asserts what
+the surrogate is for: at least 100x the walk's signals/s at the walker count
+the walk needs to reach the same RMSE bound (5e-3 in S/S0), with the hybrid
+(walk wherever the ensemble spread exceeds the bound) inside that bound.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_surrogate_gain_over_walk(gpu):
+    from dmipy_sim_amd import surrogate as sg
+    print("surrogate: 1024 substrates x 5000 walkers, 4 members 8x512, 60 epochs", flush=True)
+    r = sg.benchmark(n_train_inst=1024, n_walkers=5000, n_held=16, held_walkers=100_000,
+                     epochs=60, members=4, infer_rows=65536, seed=3,
+                     log=lambda s: print(s, flush=True))
+    print({k: r[k] for k in ("rmse_surrogate", "rmse_hybrid", "fallback_fraction", "gain",
+                             "gain_hybrid")}, flush=True)
+    assert r["rmse_hybrid"] <= 5e-3
+    assert r["mc"]["rmse"] <= 5e-3 or r["mc"]["walkers"] == 16384
+    assert r["gain"] >= 100.0
+    assert r["gain_hybrid"] >= 100.0
+    # the ensemble spread is informative: flagged rows are a small minority
+    assert r["fallback_fraction"] <= 0.05
+
+
+def test_fallback_replaces_flagged_measurements(gpu):
+    """With a zero threshold every measurement is recomputed by the walk; with
+    an infinite one none is."""
+    from dmipy_sim_amd import surrogate as sg
+    X, S, info = sg.generate(4, 4000, seed=9, chunk=4)
+    ens = sg.SurrogateEnsemble(2, 128, 2, max_batch=4096, seed=1)
+    ens.fit(X, S, epochs=2, batch=128)
+    insts = [a for a, _ in info["instances"]]
+    waves = [w for _, w in info["instances"]]
+    out0, flag0, t0 = sg.predict_with_fallback(ens, insts, waves, threshold=0.0, n_walkers=4000,
+                                               seed=5)
+    assert flag0.all() and t0["n_fallback_instances"] == 4
+    # the fallback is the walk itself: same substrates, same walker count, another seed
+    assert np.sqrt(np.mean((out0.reshape(-1) - S) ** 2)) < 0.02
+    outi, flagi, ti = sg.predict_with_fallback(ens, insts, waves, threshold=np.inf)
+    assert not flagi.any() and ti["mc_ms"] == 0.0
+    mean, _, _ = ens.predict(X)
+    np.testing.assert_allclose(outi.reshape(-1), mean, rtol=0, atol=0)
+    ens.close()
