@@ -171,6 +171,8 @@ __global__ void out_bwd_dz_kernel(int B, int H, const float* __restrict__ dy, co
 // (i = tid % 64, q = tid / 64) sums rows q, q+4, ... of the chunk; the four
 // partial sums are combined in LDS into part[y][i].  Stage 2 adds the chunks in
 // order and also forms db0 = sum_b dy[b] (thread H).
+// Blocks of the first column tile also write the chunk's sum of dy to
+// part[n_chunks * H + chunk] for the bias.
 __global__ __launch_bounds__(256) void out_bwd_w_part_kernel(int B, int H, const __bf16* __restrict__ Y,
                                                              const float* __restrict__ dy,
                                                              float* __restrict__ part) {
@@ -185,6 +187,12 @@ __global__ __launch_bounds__(256) void out_bwd_w_part_kernel(int B, int H, const
     const int c = threadIdx.x & 63;
     part[(size_t)blockIdx.y * H + i] = (red[0][c] + red[1][c]) + (red[2][c] + red[3][c]);
   }
+  if (blockIdx.x == 0 && threadIdx.x < 64) {
+    const int r = r0 + threadIdx.x;
+    float v = r < B ? dy[r] : 0.0f;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (threadIdx.x == 0) part[(size_t)gridDim.y * H + blockIdx.y] = v;
+  }
 }
 
 __global__ void out_bwd_w_sum_kernel(int B, int H, int n_chunks, const float* __restrict__ part,
@@ -197,7 +205,7 @@ __global__ void out_bwd_w_sum_kernel(int B, int H, int n_chunks, const float* __
     for (int c = 0; c < n_chunks; ++c) s += part[(size_t)c * H + i];
     dw[i] = s;
   } else {
-    for (int b = 0; b < B; ++b) s += dy[b];
+    for (int c = 0; c < n_chunks; ++c) s += part[(size_t)n_chunks * H + c];
     db0[0] = s;
   }
 }
@@ -265,7 +273,7 @@ struct dmw_mlp {
   std::vector<float*> Z;    // Z[l] (B x H)
   __bf16 *dZa = nullptr, *dZb = nullptr;
   float *out = nullptr, *dy = nullptr, *sq = nullptr, *tb = nullptr;
-  float* part = nullptr;           // (max_batch / 64, H) output-layer partial sums
+  float* part = nullptr;           // (max_batch / 64) x (H + 1) output-layer partial sums
   // resident dataset
   __bf16* dX = nullptr;
   float* dT = nullptr;
@@ -437,7 +445,7 @@ int dmw_mlp_create(dmw_mlp** out, int32_t n_in, int32_t width, int32_t depth, in
   MC(m->alloc(&m->dy, max_batch), "hipMalloc");
   MC(m->alloc(&m->sq, max_batch), "hipMalloc");
   MC(m->alloc(&m->tb, max_batch), "hipMalloc");
-  MC(m->alloc(&m->part, (size_t)(max_batch / 64) * width), "hipMalloc");
+  MC(m->alloc(&m->part, (size_t)(max_batch / 64) * (width + 1)), "hipMalloc");
   MC(hipEventCreate(&m->e0), "hipEventCreate");
   MC(hipEventCreate(&m->e1), "hipEventCreate");
   // bf16 index map: weight matrices map to their bf16 copy, biases to -1
