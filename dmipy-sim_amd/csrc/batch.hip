@@ -35,6 +35,9 @@ struct InstDev {
   long long n_walkers;
   long long g_off, gt_off;   // offsets (floats) of the instance's waveform set
   int unit_begin, unit_end;
+  unsigned long long seed_offset;
+  unsigned int stream;       // Philox instance word
+  int pad_;
 };
 
 struct BatchArgs {
@@ -55,8 +58,11 @@ struct BatchArgs {
 };
 
 template <int GEOM, int WALL, int MB>
-__device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, int inst, int tile,
-                                          int unit, uint32_t seed_lo, uint32_t seed_hi) {
+__device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, int tile, int unit,
+                                          unsigned long long base_seed) {
+  const unsigned long long seed = base_seed + I.seed_offset;
+  const uint32_t seed_lo = (uint32_t)seed, seed_hi = (uint32_t)(seed >> 32);
+  const uint32_t inst = I.stream;
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
   constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
   const int lane = threadIdx.x & (kWave - 1);
@@ -67,7 +73,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.size = I.size; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
   a.cell = I.cell; a.inv_cell = I.inv_cell; a.grid_n = I.grid_n; a.grid_inv_h = I.grid_inv_h;
   a.exact_minimage = I.exact_minimage; a.n_obj = I.n_obj;
-  a.instance = (uint32_t)inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
+  a.instance = inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
   a.Gs = b.G + I.g_off;
   a.GsT = b.GT + I.gt_off;
   a.n_meas_pad = b.n_meas_pad;
@@ -90,11 +96,11 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
     for (int s = 0; s < n_sub; ++s) {
       const uint32_t gs = (uint32_t)t * (uint32_t)n_sub + (uint32_t)s;
       if (!(gs & 1u) || (t == 0 && s == 0))
-        rnd = philox4x32_10({w, (uint32_t)inst, gs >> 1, TAG_STEP}, seed_lo, seed_hi);
+        rnd = philox4x32_10({w, inst, gs >> 1, TAG_STEP}, seed_lo, seed_hi);
       float ux, uy, uz;
       iso_direction((gs & 1u) ? rnd.z : rnd.x, (gs & 1u) ? rnd.w : rnd.y, &ux, &uy, &uz);
       const float sx = ux * I.step_len, sy = uy * I.step_len, sz = uz * I.step_len;
-      const PowlesSrc ps{w, (uint32_t)inst, gs, seed_lo, seed_hi, 0.0f, false};
+      const PowlesSrc ps{w, inst, gs, seed_lo, seed_hi, 0.0f, false};
       float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       if (WALL != W_REFLECT) dlog = dlog * chi;
       if (I.has_t2) dlog = dlog - (I.dt_sub * I.inv_T2) * chi;
@@ -127,7 +133,6 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
 template <int MB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
-  const uint32_t seed_lo = (uint32_t)seed, seed_hi = (uint32_t)(seed >> 32);
   const unsigned int inc = (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u;
   for (;;) {
     // Pop one unit per wave.  Every lane executes the atomic (lane 0 adds 1, the
@@ -145,7 +150,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const InstDev I = b.inst[u.x];
     switch (I.kind * 3 + I.wall) {
 #define DMW_UNIT(G, W) \
-      case G * 3 + W: walk_unit<G, W, MB>(b, I, u.x, u.y, unit, seed_lo, seed_hi); break;
+      case G * 3 + W: walk_unit<G, W, MB>(b, I, u.y, unit, seed); break;
       DMW_UNIT(G_FREE, W_REFLECT)
       DMW_UNIT(G_BOX1D, W_REFLECT)
       DMW_UNIT(G_BOX1D, W_SURFACE)
@@ -288,6 +293,9 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     d.kind = I.geom.kind;
     d.wall = I.geom.wall; d.n_sub = I.n_sub; d.has_t2 = I.has_t2;
     d.n_walkers = I.n_walkers;
+    d.seed_offset = I.seed_offset;
+    d.stream = I.stream < 0 ? (unsigned int)i : (unsigned int)I.stream;
+    d.pad_ = 0;
     d.cell = I.geom.cell;
     d.inv_cell = I.geom.cell > 0.0f ? 1.0f / I.geom.cell : 0.0f;
     d.g_off = (long long)I.wave_set * n_t * Mp * 3;

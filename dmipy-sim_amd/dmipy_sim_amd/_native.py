@@ -50,7 +50,8 @@ class Result_t(C.Structure):
 class Instance_t(C.Structure):
     _fields_ = [("geom", Geometry_t), ("n_walkers", C.c_int64), ("n_sub", C.c_int32),
                 ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
-                ("inv_T2", C.c_float), ("has_t2", C.c_int32), ("wave_set", C.c_int32)]
+                ("inv_T2", C.c_float), ("has_t2", C.c_int32), ("wave_set", C.c_int32),
+                ("stream", C.c_int32), ("seed_offset", C.c_uint64)]
 
 
 class BatchStats_t(C.Structure):

@@ -7,8 +7,9 @@ list of waveforms (all with the same number of measurements, <= 64, and the
 same n_t).  The result for instance i equals, in distribution, a separate
 ``simulate(n_walkers_i, D_i, waveforms[k_i], geometry_i)`` call
 (dmipy_sim/core.py:193) -- the serial per-substrate loop the reference runs,
-e.g. simulate_mixture (core.py:821-858).  Instance i uses the Philox instance
-word i, so instance 0 walks exactly as ``walk(..., seed)`` does.
+e.g. simulate_mixture (core.py:821-858).  By default instance i uses the Philox
+instance word i, so instance 0 walks exactly as ``walk(..., seed)`` does; with
+``stream=0, seed_offset=k`` an instance walks as ``walk(..., seed + k)``.
 """
 import ctypes as C
 
@@ -22,8 +23,10 @@ class BatchedWalk:
     """Plan once (upload, cost-sorted work queue, hipGraph), execute per seed.
 
     ``instances`` is a list of dicts with keys 'geometry', 'diffusivity',
-    'n_walkers' and optionally 'T2' and 'waveform' (index into ``waveforms``,
-    default 0).  ``waveforms`` is one Waveform or a list of them.
+    'n_walkers' and optionally 'T2', 'waveform' (index into ``waveforms``,
+    default 0), 'stream' (Philox instance word, default the instance index) and
+    'seed_offset' (added to the run seed, default 0).  ``waveforms`` is one
+    Waveform or a list of them.
     """
 
     def __init__(self, instances, waveforms, waves_per_simd=0, use_graph=True):
@@ -82,7 +85,9 @@ class BatchedWalk:
             arr[i] = nat.Instance_t(geom=gd, n_walkers=int(spec['n_walkers']), n_sub=plan.n_sub,
                                     step_len=plan.step_len, gamma_dt_sub=plan.gamma_dt_sub,
                                     dt_sub=plan.dt_sub, inv_T2=plan.inv_T2,
-                                    has_t2=int(plan.has_t2), wave_set=set_key[key])
+                                    has_t2=int(plan.has_t2), wave_set=set_key[key],
+                                    stream=int(spec.get('stream', -1)),
+                                    seed_offset=int(spec.get('seed_offset', 0)) & (2**64 - 1))
             self.n_sub[i] = plan.n_sub
             self.n_walkers[i] = int(spec['n_walkers'])
             if getattr(wf, 'stimulated_echo', False):

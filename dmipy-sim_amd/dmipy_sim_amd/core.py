@@ -19,6 +19,7 @@ import numpy as np
 from . import _native as nat
 from . import geometries as geo
 from .gpu import check_gpu
+from . import autotune
 from .physics import make_walk_plan, WALL_PERMEATE
 
 _REPLAY_AUTO_GEOM_NAMES = frozenset({
@@ -102,6 +103,8 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
             inject_uniforms = None
     if kahan is None:
         kahan = _kahan_auto(n_t, n_meas)
+    if not meas_block:
+        meas_block = autotune.cached_block(plan, n_meas, kahan)
 
     gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
                         n_obj=0 if plan.radii is None else len(plan.radii),
@@ -310,11 +313,33 @@ def _simulate_in_walker_batches(n_walkers, walker_batch_size, *, seed, diffusivi
 
 
 def simulate_mixture(compartments, waveform, seed=123):
-    """Volume-fraction-weighted sum of independent compartments, seeds seed+i
-    (core.py:821-858)."""
+    """Volume-fraction-weighted sum of independent compartments, compartment i
+    walked with seed + i (core.py:821-858).
+
+    All compartments run in one batched persistent launch (each as instance
+    word 0 with seed offset i, i.e. the walk ``simulate(..., seed=seed + i)``
+    performs) when the waveform has at most 64 measurements and needs no
+    compensated phase; otherwise one ``simulate`` call per compartment."""
     total = sum(c['fraction'] for c in compartments)
     if abs(total - 1.0) > 1e-6:
         raise ValueError(f"Compartment fractions must sum to 1, got {total}")
+    n_meas, n_t = np.asarray(waveform.G).shape[:2]
+    if compartments and n_meas <= 64 and not _kahan_auto(n_t, n_meas):
+        from .batch import BatchedWalk
+        check_gpu(None)
+        specs = [dict(geometry=c['geometry'], diffusivity=c['diffusivity'],
+                      n_walkers=int(c['n_walkers']), stream=0, seed_offset=i)
+                 for i, c in enumerate(compartments)]
+        bw = BatchedWalk(specs, waveform)
+        try:
+            sig, _ = bw.run(seed)
+        finally:
+            bw.close()
+        signal = None
+        for i, comp in enumerate(compartments):
+            w = comp['fraction'] * sig[i].astype(np.float32)
+            signal = w if signal is None else signal + w
+        return signal
     signal = None
     for i, comp in enumerate(compartments):
         s = simulate(n_walkers=comp['n_walkers'], diffusivity=comp['diffusivity'],
@@ -359,7 +384,9 @@ def simulate_cpmg(n_walkers, diffusivity, waveform, geometry, *, T2=None, seed=1
 class PlannedWalk:
     """A walk whose inputs (waveform, geometry tables) stay resident on the GPU;
     ``run(seed)`` launches the walk and the reduction only.  Used for throughput
-    measurement and for repeated walks of one configuration with new seeds."""
+    measurement and for repeated walks of one configuration with new seeds.
+    ``meas_block``: measurements per kernel pass (0: the tuned value if one is
+    cached, else the built-in choice; "tune": time the candidates now)."""
 
     def __init__(self, geometry, diffusivity, waveform, n_walkers, T2=None, T1=None,
                  kahan=None, meas_block=0, substep_rule="fused"):
@@ -376,6 +403,12 @@ class PlannedWalk:
         self._chi = None if chi is None else np.ascontiguousarray(chi, dtype=np.float32).reshape(n_t)
         if kahan is None:
             kahan = _kahan_auto(n_t, n_meas)
+        if meas_block == "tune":
+            meas_block, _ = autotune.tune_walk(geometry, diffusivity, waveform, n_walkers, T2=T2,
+                                               T1=T1, kahan=kahan)
+        elif not meas_block:
+            meas_block = autotune.cached_block(plan, n_meas, kahan)
+        self.meas_block = int(meas_block)
         gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
                             n_obj=0 if plan.radii is None else len(plan.radii),
                             size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,

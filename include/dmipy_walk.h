@@ -154,9 +154,9 @@ int dmw_plan_destroy(dmw_plan* plan);
  * simulate() call (core.py:193) -- e.g. the serial loop of simulate_mixture
  * (core.py:821-858).  Instances may be any geometry kind with any wall mode the
  * geometry supports; each names one of n_sets waveform sets, which share n_meas
- * (<= 64) and n_t.  Walker streams use the instance index as Philox instance
- * word, so instance i of a batch walks exactly as dmw_walk would with the same
- * seed and that instance word. */
+ * (<= 64) and n_t.  Instance i walks the Philox stream (key seed + seed_offset,
+ * instance word `stream`, or i when stream < 0): with stream 0 and seed_offset k
+ * it walks the trajectories dmw_walk walks with seed + k. */
 typedef struct dmw_instance {
   dmw_geometry geom;         /* any kind; packed kinds bring centers/radii */
   int64_t n_walkers;
@@ -167,6 +167,8 @@ typedef struct dmw_instance {
   float inv_T2;
   int32_t has_t2;
   int32_t wave_set;          /* index into the waveform sets of dmw_batch_create */
+  int32_t stream;            /* Philox instance word; < 0: the instance's index */
+  uint64_t seed_offset;      /* added to the execute seed for this instance */
 } dmw_instance;
 
 typedef struct dmw_batch dmw_batch;

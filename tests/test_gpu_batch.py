@@ -105,3 +105,19 @@ def test_batch_chi_schedule(gpu, oracle):
     for i, s in enumerate(specs):
         o = oracle.run_oracle(s["geometry"], D, wf, s["n_walkers"], seed=21, T2=0.05, instance=i)
         assert np.max(np.abs(sig[i] - o["signal"])) <= 1e-5
+
+
+def test_simulate_mixture_equals_serial_calls(gpu):
+    """simulate_mixture runs every compartment in one batched launch; compartment
+    i walks the trajectories of simulate(..., seed=seed+i) (core.py:821-858)."""
+    wf = small_pgse()
+    comps = [dict(fraction=0.4, n_walkers=500, diffusivity=D, geometry=ds.Cylinder(3e-6, (0, 0, 1))),
+             dict(fraction=0.3, n_walkers=300, diffusivity=1.5e-9, geometry=ds.Sphere(4e-6)),
+             dict(fraction=0.2, n_walkers=257, diffusivity=3e-9, geometry=ds.FreeDiffusion()),
+             dict(fraction=0.1, n_walkers=200, diffusivity=D, geometry=CASES[12][1]())]
+    mix = gpu.simulate_mixture(comps, wf, seed=31)
+    ref = sum(c["fraction"] * gpu.simulate(c["n_walkers"], c["diffusivity"], wf, c["geometry"],
+                                           seed=31 + i)
+              for i, c in enumerate(comps))
+    assert mix.dtype == np.float32 and mix.shape == (wf.G.shape[0],)
+    assert np.max(np.abs(mix - ref)) <= 1e-5

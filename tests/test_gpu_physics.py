@@ -248,3 +248,19 @@ def test_independent_streams_agree_with_oracle(gpu, oracle):
                           inject_uniforms=rng.random((n, S)).astype(np.float32))["signal"]
     se = np.sqrt(2.0 / n)
     assert np.all(np.abs(g - o) <= 3 * se), np.max(np.abs(g - o)) / se
+
+
+def test_autotune_picks_a_timed_block_and_walk_uses_it(gpu):
+    from dmipy_sim_amd import autotune
+    autotune.clear()
+    wf = small_pgse(n_t=40, n_dirs=8)               # 32 measurements
+    geom = gpu.Cylinder(3e-6, (0, 0, 1), surface_relaxivity_t2=1e-5)
+    best, ms = autotune.tune_walk(geom, D, wf, 20000, probe_walkers=8192, probe_samples=16)
+    assert set(ms) == {4, 16, 32} and best in ms and all(v > 0 for v in ms.values())
+    pw = gpu.PlannedWalk(geom, D, wf, 4096)
+    assert pw.meas_block == best                    # the cached choice is used
+    a = pw.run(seed=5)["signal"]
+    b = gpu.walk(geom, D, wf, 4096, seed=5, meas_block=4)["signal"]
+    assert np.max(np.abs(a - b)) <= 1e-5            # any block walks the same trajectories
+    pw.close()
+    autotune.clear()

@@ -257,3 +257,25 @@ def test_batch_argument_errors_before_gpu():
     inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7,
                                  gamma_dt_sub=1.0, dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
     assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EUNSUPPORTED
+
+
+def test_autotune_candidates_and_cache(tmp_path):
+    from dmipy_sim_amd import autotune
+    assert autotune.candidate_blocks(3, False) == [4]
+    assert autotune.candidate_blocks(16, False) == [4, 16]
+    assert autotune.candidate_blocks(90, False) == [4, 16, 32, 64, 128]
+    assert autotune.candidate_blocks(128, True) == [4, 16, 32, 64]
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 50)
+    plan = ds.make_walk_plan(ds.Cylinder(5e-6, (0, 0, 1)), D, wf.dt)
+    autotune.clear()
+    assert autotune.cached_block(plan, 128, False) == 0
+    key = autotune.plan_key(plan, 128, False)
+    autotune._cache[key] = {"meas_block": 64, "ms": {64: 1.0, 128: 2.0}}
+    assert autotune.cached_block(plan, 128, False) == 64
+    assert autotune.cached_block(plan, 128, True) == 0      # compensation is part of the key
+    path = tmp_path / "tune.json"
+    autotune.save(str(path))
+    autotune.clear()
+    autotune.load(str(path))
+    assert autotune.cached_block(plan, 128, False) == 64
+    autotune.clear()
