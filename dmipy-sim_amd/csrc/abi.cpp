@@ -19,6 +19,7 @@
 #include "walk_launch.h"
 #include "batch.h"
 #include "grid.h"
+#include "traj.h"
 
 namespace {
 
@@ -467,6 +468,68 @@ int dmw_walk(const dmw_geometry* geom, const dmw_waveform* wave, const dmw_run* 
   rc = dmw_plan_execute(P, run->seed, out);
   dmw_plan_destroy(P);
   return rc;
+}
+
+
+// Walk-once producer: the walk kernel with a one-measurement zero waveform and
+// the float16 trajectory channels as its outputs (dmipy_sim/core.py:972-1529).
+int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run, int32_t n_t,
+                    int32_t channels, const float* lab_from_native) {
+  if (!out || !geom || !run || n_t < 1) return fail(DMW_EINVAL, "null argument or empty trajectory");
+  *out = nullptr;
+  const long long N = run->n_walkers;
+  if (N < 1) return fail(DMW_EINVAL, "n_walkers must be >= 1");
+  if ((channels & 2) && !((geom->kind == DMW_CYLINDER || geom->kind == DMW_SPHERE) &&
+                          geom->wall == DMW_WALL_PERMEATE))
+    channels &= ~2;  // single-compartment geometries: occupancy is identically 0 (filled by the caller)
+  std::vector<float> G((size_t)n_t * 3, 0.0f);
+  dmw_waveform wave{1, n_t, G.data(), nullptr, 0, nullptr};
+  dmw_run r = *run;
+  r.kahan = 0;
+  r.meas_block = 4;
+  r.has_t2 = 0;
+  r.has_t1 = 0;
+  dmw_plan* P = nullptr;
+  int rc = dmw_plan_create(&P, geom, &wave, &r);
+  if (rc) return rc;
+  hipStream_t st;
+  rc = ensure_stream(&st);
+  if (rc) {
+    dmw_plan_destroy(P);
+    return rc;
+  }
+  dmw_traj* T = new dmw_traj();
+  T->N = N;
+  T->n_t = n_t;
+  auto bail = [&](hipError_t e, const char* what) {
+    delete T;
+    dmw_plan_destroy(P);
+    return hip_fail(e, what);
+  };
+  hipError_t e = hipMalloc((void**)&T->traj, (size_t)3 * n_t * N * 2);
+  if (e == hipSuccess && (channels & 1)) e = hipMalloc((void**)&T->dlog, (size_t)n_t * N * 2);
+  if (e == hipSuccess && (channels & 2)) e = hipMalloc((void**)&T->comp, (size_t)n_t * N * 2);
+  if (e != hipSuccess) return bail(e, "hipMalloc trajectories");
+  dmw::WalkArgs a = P->args;
+  a.seed_lo = (uint32_t)(run->seed & 0xFFFFFFFFu);
+  a.seed_hi = (uint32_t)(run->seed >> 32);
+  a.traj_h = T->traj;
+  a.dlog_h = T->dlog;
+  a.comp_h = T->comp;
+  a.traj_rotate = lab_from_native ? 1 : 0;
+  for (int i = 0; i < 9; ++i) a.traj_R[i] = lab_from_native ? lab_from_native[i] : 0.0f;
+  const dim3 grid(P->grid_x, P->n_chunks);
+  e = hipEventRecord(P->ev0, st);
+  if (e == hipSuccess) e = launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a);
+  if (e == hipSuccess) e = hipEventRecord(P->ev1, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, P->ev0, P->ev1);
+  if (e != hipSuccess) return bail(e, "trajectory walk");
+  T->walk_ms = ms;
+  dmw_plan_destroy(P);
+  *out = T;
+  return DMW_OK;
 }
 
 }  // extern "C"

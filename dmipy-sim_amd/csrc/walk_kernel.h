@@ -93,6 +93,12 @@ struct WalkArgs {
   float* log_w;              // (N)
   float* phi;                // (n_meas_pad, N)
   unsigned long long* counters;  // [2]: wall events, candidates tested
+  // trajectory channels of the walk-once producer (float16 bits, SoA)
+  uint16_t* traj_h;          // (n_t, 3, N): unfolded position after every sample
+  uint16_t* dlog_h;          // (n_t, N): wall log-weight summed over the sample's sub-steps
+  uint16_t* comp_h;          // (n_t, N): fraction of the sample's sub-steps outside (permeable Cylinder/Sphere)
+  int traj_rotate;           // 1: store traj_R . position (lab frame of a rotated geometry)
+  float traj_R[9];           // row-major lab-from-native rotation
 };
 
 // ---------------------------------------------------------------------------
@@ -634,6 +640,18 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
   return p;
 }
 
+// float32 -> float16 bits, round to nearest even (numpy's astype(float16)).
+__device__ __forceinline__ uint16_t f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
+
+// Compartment id of the walk-once producer for permeable Cylinder / Sphere:
+// outside iff the float32 norm reaches R (dmipy_sim/core.py:1231-1245).
+template <int GEOM>
+__device__ __forceinline__ float outside_norm(const Pos& p, const WalkArgs& a) {
+  if (GEOM == G_CYL) return sqrtf(p.x * p.x + p.y * p.y) < a.size ? 0.0f : 1.0f;
+  if (GEOM == G_SPH) return sqrtf((p.x * p.x + p.y * p.y) + p.z * p.z) < a.size ? 0.0f : 1.0f;
+  return 0.0f;
+}
+
 // Compile-time loop: f(std::integral_constant<int, i>) for i in [0, N), so array
 // indices stay constants and per-walker arrays stay in registers.
 template <class F, int... I>
@@ -907,6 +925,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     const float chi = a.chi ? a.chi[t] : 1.0f;
     const float x0 = p.x, y0 = p.y, z0 = p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    float lw_sample = 0.0f, occ = 0.0f;
     for (int s = 0; s < n_sub; ++s) {
       const uint32_t gs = (uint32_t)t * (uint32_t)n_sub + (uint32_t)s;
       float ux, uy, uz, uinj = 0.0f;
@@ -932,6 +951,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       if (a.has_t2) dlog = dlog - (a.dt_sub * a.inv_T2) * chi;
       if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       logw = logw + dlog;
+      if (a.dlog_h) lw_sample = lw_sample + dlog;
+      if (a.comp_h) occ = occ + outside_norm<GEOM>(p, a);
       if (n_sub > 1) {
         ax = ax + (p.x - x0);
         ay = ay + (p.y - y0);
@@ -963,6 +984,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         a.pos_full[((size_t)t * 3 + 1) * N + w] = PACKED ? p.y + L * (float)p.cy : p.y;
         a.pos_full[((size_t)t * 3 + 2) * N + w] = PACKED ? p.z + L * (float)p.cz : p.z;
       }
+      if (a.traj_h) {
+        float qx = PACKED ? p.x + L * (float)p.cx : p.x;
+        float qy = PACKED ? p.y + L * (float)p.cy : p.y;
+        float qz = PACKED ? p.z + L * (float)p.cz : p.z;
+        if (a.traj_rotate) {
+          const float* R = a.traj_R;
+          const float lx = (R[0] * qx + R[1] * qy) + R[2] * qz;
+          const float ly = (R[3] * qx + R[4] * qy) + R[5] * qz;
+          const float lz = (R[6] * qx + R[7] * qy) + R[8] * qz;
+          qx = lx; qy = ly; qz = lz;
+        }
+        a.traj_h[((size_t)t * 3 + 0) * N + w] = f16_bits(qx);
+        a.traj_h[((size_t)t * 3 + 1) * N + w] = f16_bits(qy);
+        a.traj_h[((size_t)t * 3 + 2) * N + w] = f16_bits(qz);
+      }
+      if (a.dlog_h) a.dlog_h[(size_t)t * N + w] = f16_bits(lw_sample);
+      if (a.comp_h) a.comp_h[(size_t)t * N + w] = f16_bits(occ / fsub);
     }
     if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
       const float wgt = expf(logw);

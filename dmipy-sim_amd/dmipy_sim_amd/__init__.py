@@ -15,6 +15,7 @@ from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
 from .core import simulate, simulate_cpmg, simulate_mixture, walk, PlannedWalk
 from .gpu import check_gpu, gpu_available
 from .batch import BatchedWalk
+from .trajectories import simulate_trajectories, replay, TrajectorySet
 from . import surrogate
 from . import autotune
 from .surrogate import MLP, SurrogateEnsemble
@@ -26,5 +27,6 @@ __all__ = [
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",
-    "BatchedWalk", "PlannedWalk", "surrogate", "MLP", "SurrogateEnsemble", "autotune",
+    "BatchedWalk", "PlannedWalk", "surrogate", "MLP", "SurrogateEnsemble", "autotune", "simulate_trajectories", "replay",
+    "TrajectorySet",
 ]

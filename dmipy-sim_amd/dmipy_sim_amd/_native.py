@@ -54,6 +54,12 @@ class Instance_t(C.Structure):
                 ("stream", C.c_int32), ("seed_offset", C.c_uint64)]
 
 
+class Replay_t(C.Structure):
+    _fields_ = [("n_meas", C.c_int32), ("n_t", C.c_int32), ("G", _dp), ("scale", C.c_double),
+                ("chi", _dp), ("chi_rows", C.c_int32), ("log_w0", _dp),
+                ("rho_over_D", C.c_double)]
+
+
 class BatchStats_t(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("total_ms", C.c_double),
                 ("walker_steps", C.c_uint64), ("n_units", C.c_int32), ("grid", C.c_int32)]
@@ -74,6 +80,16 @@ SIGNATURES = {
                                    C.c_int32, C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
     "dmw_batch_execute": (C.c_int, [C.c_void_p, C.c_uint64, _dp, C.POINTER(BatchStats_t)]),
     "dmw_batch_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_traj_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Geometry_t), C.POINTER(Run_t),
+                                  C.c_int32, C.c_int32, _fp]),
+    "dmw_traj_from_host": (C.c_int, [C.POINTER(C.c_void_p), C.c_int64, C.c_int32,
+                                     C.POINTER(C.c_uint16), C.POINTER(C.c_uint16)]),
+    "dmw_traj_download": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16),
+                                    C.POINTER(C.c_uint16)]),
+    "dmw_traj_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32),
+                                C.POINTER(C.c_int32), _dp]),
+    "dmw_traj_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_replay_run": (C.c_int, [C.c_void_p, C.POINTER(Replay_t), _dp, _dp, _dp, _dp]),
     "dmw_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32,
                                  C.c_int32]),
     "dmw_mlp_destroy": (C.c_int, [C.c_void_p]),

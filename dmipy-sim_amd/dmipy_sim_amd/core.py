@@ -200,7 +200,6 @@ def simulate(
         if gap is not None:
             raise NotImplementedError(f"engine='replay' cannot serve this run: {gap}. "
                                       "Use engine='fused' (or 'auto').")
-        substep_rule = "replay"
 
     check_gpu(n_walkers, require_gpu, what="simulate")
 
@@ -236,6 +235,9 @@ def simulate(
 
     if hasattr(waveform, 'waveform'):
         waveform = waveform.waveform
+    if engine == "replay":
+        return _simulate_via_replay(n_walkers, diffusivity, waveform, geometry, seed=seed,
+                                    T2=T2, T1=T1, r0=r0)
     track = return_compartments is not False
     outs = []
     if return_positions == 'full':
@@ -264,6 +266,33 @@ def simulate(
         result.append(res['log_w'])
         result.append(res['phi'])
     return result[0] if len(result) == 1 else tuple(result)
+
+
+def _simulate_via_replay(n_walkers, diffusivity, waveform, geometry, *, seed, T2, T1, r0):
+    """Signal via the walk-once path (dmipy_sim/core.py:115-190): the producer
+    walk saves float16 positions (and, with surface relaxivity, the boundary
+    log-weight at rho/D = 1); the waveform, T2/T1 and rho are then applied by
+    the float64 replay contraction.  Trajectories never leave the GPU."""
+    from .trajectories import TrajectorySet
+    G = np.asarray(waveform.G, dtype=np.float32)
+    dt = float(waveform.dt)
+    n_t = G.shape[1]
+    chi_perp = getattr(waveform, 'chi_perp', None)
+    chi = (np.asarray(chi_perp, dtype=np.float64).reshape(n_t)
+           if chi_perp is not None else np.ones(n_t, dtype=np.float64))
+    rho = getattr(geometry, 'surface_relaxivity_t2', None)
+    has_surf = (rho is not None and float(rho) > 0.0
+                and not isinstance(geometry, geo.FreeDiffusion))
+    ts = TrajectorySet.simulate(n_walkers, diffusivity, geometry, dt * (n_t - 1), dt, seed=seed,
+                                save_relaxation_data=has_surf, r0=r0)
+    try:
+        sig = ts.replay(G, dt, chi_perp=chi, T2=T2, T1=T1,
+                        surface_relaxivity=float(rho) if has_surf else None,
+                        D=float(diffusivity),
+                        stimulated_echo=bool(getattr(waveform, 'stimulated_echo', False)))
+    finally:
+        ts.close()
+    return np.asarray(sig, dtype=np.float32)
 
 
 def _simulate_in_walker_batches(n_walkers, walker_batch_size, *, seed, diffusivity, waveform,

@@ -191,6 +191,48 @@ int dmw_batch_execute(dmw_batch* batch, uint64_t seed, double* signals,
                       dmw_batch_stats* stats);
 int dmw_batch_destroy(dmw_batch* batch);
 
+/* ---- Walk once, replay many waveforms ----
+ * The reference's replay backend: simulate_trajectories (dmipy_sim/core.py:972-1529)
+ * walks the spins once with no waveform and saves float16 positions after every
+ * sample (plus, on request, the boundary log-weight at rho/D = 1 and the
+ * compartment occupancy of permeable Cylinder/Sphere walls); replay
+ * (dmipy_sim/trajectories.py:360-641) applies any waveform afterwards as the
+ * float64 contraction phi = GAMMA dt G_flat . traj^T and S = mean exp(log_w) cos(phi).
+ * Here the trajectories stay in device memory between the two halves. */
+typedef struct dmw_traj dmw_traj;
+
+/* Producer walk.  geom/run as for dmw_walk (run->n_sub, step_len from the
+ * producer's sub-step rule; gamma_dt_sub, T2, T1, kahan, meas_block ignored);
+ * channels: bit 0 = boundary log-weight, bit 1 = compartment occupancy;
+ * lab_from_native: 3x3 row-major rotation applied to the stored positions of a
+ * rotated geometry (NULL = store native-frame positions). */
+int dmw_traj_create(dmw_traj** traj, const dmw_geometry* geom, const dmw_run* run,
+                    int32_t n_t, int32_t channels, const float* lab_from_native);
+/* Trajectories given by the caller: traj (n_walkers, n_t, 3) and dlog
+ * (n_walkers, n_t) or NULL, float16 bit patterns. */
+int dmw_traj_from_host(dmw_traj** traj, int64_t n_walkers, int32_t n_t, const uint16_t* traj_h,
+                       const uint16_t* dlog_h);
+/* Copy channels back as (n_walkers, n_t, 3) / (n_walkers, n_t) float16 (NULL = skip). */
+int dmw_traj_download(dmw_traj* traj, uint16_t* traj_h, uint16_t* dlog_h, uint16_t* comp_h);
+int dmw_traj_info(dmw_traj* traj, int64_t* n_walkers, int32_t* n_t, int32_t* channels,
+                  double* walk_ms);
+int dmw_traj_destroy(dmw_traj* traj);
+
+typedef struct dmw_replay {
+  int32_t n_meas;
+  int32_t n_t;               /* must equal the trajectory set's n_t */
+  const double* G;           /* (n_meas, n_t, 3) T/m on the trajectory time grid */
+  double scale;              /* GAMMA * dt_traj */
+  const double* chi;         /* (chi_rows, n_t) transverse gate on the trajectory grid */
+  int32_t chi_rows;          /* 1 or n_meas */
+  const double* log_w0;      /* (n_meas) walker-independent log-weight (T2, T1 terms) */
+  double rho_over_D;         /* surface relaxivity / D; 0 = no surface term */
+} dmw_replay;
+
+/* signals (n_meas); optional phi and log_w (n_meas, n_walkers) float64. */
+int dmw_replay_run(dmw_traj* traj, const dmw_replay* replay, double* signals, double* phi,
+                   double* log_w, double* kernel_ms);
+
 /* ---- Learned surrogate (no counterpart in the reference) ----
  * A GELU MLP (depth hidden layers of `width` units) with BF16 weights and FP32
  * accumulation on the matrix cores, trained by its own forward/backward/Adam

@@ -54,7 +54,8 @@ class OracleArgs(C.Structure):
 class OracleOut(C.Structure):
     _fields_ = [("signal", _dp), ("signal_im", _dp), ("final_pos", _fp),
                 ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
-                ("n_cross", _up), ("log_w", _fp), ("phi", _fp), ("n_hits", C.c_uint64)]
+                ("n_cross", _up), ("log_w", _fp), ("phi", _fp), ("n_hits", C.c_uint64),
+                ("pos_full", _fp), ("dlog_full", _fp)]
 
 
 _lib = None
@@ -155,7 +156,7 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
 def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                inject_normals=None, inject_uniforms=None, cell_form=1, phase64=False,
                threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None,
-               instance=0):
+               instance=0, per_sample=False):
     """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame)."""
     p = plan if plan is not None else oracle_plan(geom, D, waveform.dt, T2, T1)
     G = np.ascontiguousarray(waveform.G if G_native is None else G_native, dtype=np.float32)
@@ -195,6 +196,11 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                   comp_final=res["comp_final"].ctypes.data_as(_ip), comp_full=None,
                   n_cross=res["n_cross"].ctypes.data_as(_up), log_w=fp(res["log_w"]),
                   phi=fp(res["phi"]))
+    if per_sample:
+        res["pos_full"] = np.zeros((N, n_t, 3), np.float32)
+        res["dlog_full"] = np.zeros((N, n_t), np.float32)
+        O.pos_full = fp(res["pos_full"])
+        O.dlog_full = fp(res["dlog_full"])
     rc = lib().oracle_walk(C.byref(A), C.byref(O))
     if rc != 0:
         raise RuntimeError("oracle_walk failed")

@@ -83,6 +83,8 @@ typedef struct {
   float* log_w;
   float* phi;              /* (N, n_meas) */
   uint64_t n_hits;
+  float* pos_full;         /* (N, n_t, 3) unfolded position after every sample, or NULL */
+  float* dlog_full;        /* (N, n_t) log-weight summed over each sample's sub-steps, or NULL */
 } oracle_out;
 
 enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5 };
@@ -586,6 +588,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       int e_next = 0;
       for (int t = 0; t < T; ++t) {
         const float chi = A->chi ? A->chi[t] : 1.0f;
+        float dlog_sample = 0.0f;
         for (int s = 0; s < NS; ++s) {
           const int64_t gs = (int64_t)t * NS + s;
           float dir[3], u = 0.0f;
@@ -612,6 +615,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
           if (A->has_t2) dlog = dlog - (A->dt_sub * A->inv_T2) * chi;
           if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
           logw = logw + dlog;
+          dlog_sample = dlog_sample + dlog;
           /* phase of every measurement, right-endpoint rule */
           float ru[3] = {r[0], r[1], r[2]};
           if (packed && A->cell_form)
@@ -629,6 +633,10 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             if (k != 0.0f) { r[d] = r[d] - L * k; cell[d] += (int)k; }
           }
         }
+        if (O->pos_full)
+          for (int d = 0; d < 3; ++d)
+            O->pos_full[((size_t)w * T + t) * 3 + d] = (packed && A->cell_form) ? r[d] + L * (float)cell[d] : r[d];
+        if (O->dlog_full) O->dlog_full[(size_t)w * T + t] = dlog_sample;
         if (O->comp_full || (O->comp_final && t == T - 1)) comp = classify(r, A);
         if (O->comp_full) O->comp_full[(size_t)w * T + t] = comp;
         if (e_next < A->n_echo && A->echo_idx[e_next] == t) {

@@ -144,7 +144,8 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     structs = {"dmw_geometry": _native.Geometry_t, "dmw_waveform": _native.Waveform_t,
                "dmw_run": _native.Run_t, "dmw_result": _native.Result_t,
-               "dmw_instance": _native.Instance_t, "dmw_batch_stats": _native.BatchStats_t}
+               "dmw_instance": _native.Instance_t, "dmw_batch_stats": _native.BatchStats_t,
+               "dmw_replay": _native.Replay_t}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "dmipy_walk.h"',
              'int main(void) {']
     for cname, ct in structs.items():
