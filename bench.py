@@ -163,6 +163,60 @@ def cpu_baseline_batch(inst, wf, plans, target_s):
 
 
 PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def run_replay(args):
+    """The walk-once path on the Cylinder PGSE workload: the producer walk
+    (float16 positions and the rho/D = 1 boundary channel, producer sub-step
+    rule) and the float64 replay contraction, trajectories resident in HBM.
+    A step is one producer walk plus one replay of the 128-measurement waveform."""
+    import numpy as np
+    import dmipy_sim_amd as ds
+    from dmipy_sim_amd import configs
+    cfg = configs.cylinder_pgse()
+    wf, geom, D, N = cfg["waveform"], cfg["geometry"], cfg["diffusivity"], cfg["n_walkers"]
+    n_t = wf.G.shape[1]
+    n_meas = wf.G.shape[0]
+    rho = float(geom.surface_relaxivity_t2)
+    def one(seed):
+        ts = ds.TrajectorySet.simulate(N, D, geom, wf.dt * (n_t - 1), wf.dt, seed=seed,
+                                       save_relaxation_data=True)
+        S = ts.replay(wf.G, wf.dt, surface_relaxivity=rho, D=D)
+        out = (ts.walk_ms, ts.last_replay_ms, ts.sub_steps, S)
+        ts.close()
+        return out
+    w_ms, r_ms, sub, S = one(cfg["seed"])
+    log(f"replay workload: N={N} n_t={n_t} n_meas={n_meas} sub_steps={sub}; producer {w_ms:.2f} ms, "
+        f"replay {r_ms:.2f} ms; S[:4]={S[:4]}")
+    for i in range(args.warmup):
+        one(cfg["seed"] + 1 + i)
+    walks, reps = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        w, r, _, _ = one(cfg["seed"] + 100 + i)
+        walks.append(w)
+        reps.append(r)
+    wall = time.perf_counter() - t0
+    ws = N * n_t * sub
+    value = ws * args.steps / wall
+    rmean = statistics.mean(reps)
+    flops = 2.0 * n_meas * 3 * n_t * N
+    achieved = flops / (rmean * 1e-3) / 1e12
+    log(f"timed {args.steps}: {1e3 * wall / args.steps:.1f} ms/step (producer {statistics.mean(walks):.2f} ms, "
+        f"replay {rmean:.2f} ms -> {achieved:.2f} TFLOP/s fp64)")
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "kernel": "replay_kernel", "kernel_ms": round(rmean, 4),
+                "producer_ms": round(statistics.mean(walks), 3),
+                "flops_per_replay": flops}
+    config = {"workload": "cylinder_pgse_replay", "walkers": N, "n_t": n_t, "n_meas": n_meas,
+              "sub_steps": sub, "positions": "float16", "contraction": "fp64"}
+    line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f16/f64",
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
 
 
 def run_surrogate(args):
@@ -283,7 +337,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="cylinder",
-                    choices=["cylinder", "free", "packed", "batch", "surrogate"])
+                    choices=["cylinder", "free", "packed", "batch", "surrogate", "replay"])
     ap.add_argument("--n-inst", type=int, default=4096, help="batch workload: instances")
     ap.add_argument("--walker-scale", type=float, default=1.0,
                     help="batch workload: scale every instance's walker count")
@@ -322,6 +376,8 @@ def main():
         return run_batch(args)
     if args.workload == "surrogate":
         return run_surrogate(args)
+    if args.workload == "replay":
+        return run_replay(args)
 
     import dmipy_sim_amd as ds
     cfg = build_workload(args.workload)
