@@ -182,7 +182,7 @@ def run_replay(args):
     def one(seed):
         ts = ds.TrajectorySet.simulate(N, D, geom, wf.dt * (n_t - 1), wf.dt, seed=seed,
                                        save_relaxation_data=True)
-        S = ts.replay(wf.G, wf.dt, surface_relaxivity=rho, D=D)
+        S = ts.replay(wf.G, wf.dt, surface_relaxivity=rho, D=D, meas_tiles=args.meas_tiles)
         out = (ts.walk_ms, ts.last_replay_ms, ts.sub_steps, S)
         ts.close()
         return out
@@ -347,6 +347,8 @@ def main():
     ap.add_argument("--surr-walkers", type=int, default=10_000,
                     help="surrogate workload: walkers per training substrate")
     ap.add_argument("--surr-epochs", type=int, default=60)
+    ap.add_argument("--meas-tiles", type=int, default=0,
+                    help="replay workload: 16-measurement tiles per pass (0 = automatic)")
     ap.add_argument("--slice-t", type=int, default=0,
                     help="packed workload: walk only the first N waveform samples")
     ap.add_argument("--no-cpu", action="store_true")

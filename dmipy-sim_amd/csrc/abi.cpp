@@ -498,6 +498,10 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
     dmw_plan_destroy(P);
     return rc;
   }
+  if (P->mb != 4 || P->kahan) {
+    dmw_plan_destroy(P);
+    return fail(DMW_EHIP, "producer walk needs the 4-measurement kernel variant");
+  }
   dmw_traj* T = new dmw_traj();
   T->N = N;
   T->n_t = n_t;

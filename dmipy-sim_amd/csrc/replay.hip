@@ -46,7 +46,9 @@ struct ReplayArgs {
   double rhoD;            // surface relaxivity / D
   long long N;
   int n_t, K, nk, nk2, MTT, M_pad, n_meas;
-  double* partials;       // (gridDim.x, M_pad)
+  double* partials;       // (signal workgroups, M_pad)
+  double* phi_buf;        // (M_pad, N) phases
+  double* lw_buf;         // (N) walker log-weight part (one chi row) or (M_pad, N) (per-measurement chi)
   double* phi_out;        // (n_meas, N) or null
   double* lw_out;         // (n_meas, N) or null
 };
@@ -55,7 +57,6 @@ struct ReplayArgs {
 // 2 per-measurement chi (a second matrix-core contraction over n_t).
 template <int MT, int SURF>
 __global__ __launch_bounds__(256) void replay_kernel(const ReplayArgs a) {
-  __shared__ double red[4][MT * 16];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int kq = lane >> 4, jq = lane & 15;
   const long long w = (long long)blockIdx.x * 64 + wv * 16 + jq;
@@ -66,58 +67,87 @@ __global__ __launch_bounds__(256) void replay_kernel(const ReplayArgs a) {
   doublex4 acc[MT];
 #pragma unroll
   for (int i = 0; i < MT; ++i) acc[i] = doublex4{0.0, 0.0, 0.0, 0.0};
+  // Branch-free operand loads (the padded k rows read row K-1 and are zeroed by
+  // the mask), so the loads of consecutive k-steps are not serialised behind a
+  // branch and a wait.
+  const uint16_t* tcol = a.traj + wc;
   for (int ks = 0; ks < a.nk; ++ks) {
     const int k = 4 * ks + kq;
-    const double b = k < a.K ? h2d(a.traj[(size_t)k * a.N + wc]) : 0.0;
+    const int kc = k < a.K ? k : a.K - 1;
+    const double keep = k < a.K ? 1.0 : 0.0;
+    const double b = keep * h2d(tcol[(size_t)kc * a.N]);
     const double* gt = a.Gt + ((size_t)ks * a.MTT + mt0) * 64 + lane;
 #pragma unroll
     for (int i = 0; i < MT; ++i) acc[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(gt[i * 64], b, acc[i], 0, 0, 0);
   }
 
-  double s_w = 0.0;
-  doublex4 acc2[SURF == 2 ? MT : 1];
-  if (SURF == 1) {
+  // Phase rows of this pass; with a boundary channel also the log-weight part
+  // that depends on the walker.
+  // Row offsets (16 i + 4 r) N are wavefront-uniform; one per-lane base pointer.
+  double* prow = a.phi_buf + (size_t)(16 * mt0 + kq) * a.N + wc;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (wok) prow[(size_t)(16 * i + 4 * r) * a.N] = a.scale * acc[i][r];
+  if (SURF == 1 && blockIdx.y == 0) {
+    double s_w = 0.0;
     for (int t = kq; t < a.n_t; t += 4) s_w += a.chi[t] * h2d(a.dlog[(size_t)t * a.N + wc]);
     s_w += __shfl_xor(s_w, 16, 64);
     s_w += __shfl_xor(s_w, 32, 64);
+    if (wok && kq == 0) a.lw_buf[w] = a.rhoD * s_w;
   } else if (SURF == 2) {
+    doublex4 acc2[MT];
 #pragma unroll
     for (int i = 0; i < MT; ++i) acc2[i] = doublex4{0.0, 0.0, 0.0, 0.0};
     for (int ks = 0; ks < a.nk2; ++ks) {
       const int t = 4 * ks + kq;
-      const double b = t < a.n_t ? h2d(a.dlog[(size_t)t * a.N + wc]) : 0.0;
+      const int tc = t < a.n_t ? t : a.n_t - 1;
+      const double b = (t < a.n_t ? 1.0 : 0.0) * h2d(a.dlog[(size_t)tc * a.N + wc]);
       const double* ct = a.Ct + ((size_t)ks * a.MTT + mt0) * 64 + lane;
 #pragma unroll
       for (int i = 0; i < MT; ++i) acc2[i] = __builtin_amdgcn_mfma_f64_16x16x4f64(ct[i * 64], b, acc2[i], 0, 0, 0);
     }
+    double* lrow = a.lw_buf + (size_t)(16 * mt0 + kq) * a.N + wc;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (wok) lrow[(size_t)(16 * i + 4 * r) * a.N] = a.rhoD * acc2[i][r];
   }
+}
 
-#pragma unroll
-  for (int i = 0; i < MT; ++i) {
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int ml = 16 * i + kq + 4 * r;        // row within this pass
-      const int m = 16 * mt0 + ml;
-      const bool ok = wok && m < a.n_meas;
-      const double phi = a.scale * acc[i][r];
-      double lw = a.lw0[m < a.M_pad ? m : 0];
-      if (SURF == 1) lw = lw + a.rhoD * s_w;
-      if (SURF == 2) lw = lw + a.rhoD * acc2[i][r];
-      double v = ok ? exp(lw) * cos(phi) : 0.0;
-      if (ok && a.phi_out) a.phi_out[(size_t)m * a.N + w] = phi;
-      if (ok && a.lw_out) a.lw_out[(size_t)m * a.N + w] = lw;
-      v += __shfl_xor(v, 1, 64);
-      v += __shfl_xor(v, 2, 64);
-      v += __shfl_xor(v, 4, 64);
-      v += __shfl_xor(v, 8, 64);
-      if (jq == 0) red[wv][ml] = v;
+// Signal terms: one thread per walker, 16 measurements per workgroup row
+// (grid.y); v = exp(log_w0[m] + walker part) cos(phi) in float64, summed over the
+// workgroup's 256 walkers in a fixed order into partials[blockIdx.x][m].
+template <int SURF>
+__global__ __launch_bounds__(256) void replay_signal_kernel(const ReplayArgs a) {
+  __shared__ double red[4][16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const long long w = (long long)blockIdx.x * 256 + threadIdx.x;
+  const bool wok = w < a.N;
+  const int m0 = blockIdx.y * 16;
+  const double lw_w = (SURF == 1 && wok) ? a.lw_buf[w] : 0.0;
+  for (int j = 0; j < 16; ++j) {
+    const int m = m0 + j;
+    double v = 0.0;
+    if (wok && m < a.n_meas) {
+      const double phi = a.phi_buf[(size_t)m * a.N + w];
+      double lw = a.lw0[m];
+      if (SURF == 1) lw = lw + lw_w;
+      if (SURF == 2) lw = lw + a.lw_buf[(size_t)m * a.N + w];
+      v = exp(lw) * cos(phi);
+      if (a.phi_out) a.phi_out[(size_t)m * a.N + w] = phi;
+      if (a.lw_out) a.lw_out[(size_t)m * a.N + w] = lw;
     }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wv][j] = v;
   }
   __syncthreads();
-  if (threadIdx.x < MT * 16) {
-    const int ml = threadIdx.x;
-    const double s = (red[0][ml] + red[1][ml]) + (red[2][ml] + red[3][ml]);
-    a.partials[(size_t)blockIdx.x * a.M_pad + 16 * mt0 + ml] = s;
+  if (threadIdx.x < 16) {
+    const int j = threadIdx.x;
+    a.partials[(size_t)blockIdx.x * a.M_pad + m0 + j] = (red[0][j] + red[1][j]) + (red[2][j] + red[3][j]);
   }
 }
 
@@ -154,6 +184,15 @@ hipError_t launch_transpose16(const uint16_t* in, uint16_t* out, long long rows,
 template <int MT, int SURF>
 static hipError_t launch_mt(dim3 grid, hipStream_t st, const ReplayArgs& a) {
   hipLaunchKernelGGL((replay_kernel<MT, SURF>), grid, dim3(256), 0, st, a);
+  return hipGetLastError();
+}
+
+static hipError_t launch_signal(int surf, dim3 grid, hipStream_t st, const ReplayArgs& a) {
+  switch (surf) {
+    case 0: hipLaunchKernelGGL((replay_signal_kernel<0>), grid, dim3(256), 0, st, a); break;
+    case 1: hipLaunchKernelGGL((replay_signal_kernel<1>), grid, dim3(256), 0, st, a); break;
+    default: hipLaunchKernelGGL((replay_signal_kernel<2>), grid, dim3(256), 0, st, a); break;
+  }
   return hipGetLastError();
 }
 
@@ -284,7 +323,10 @@ int dmw_replay_run(dmw_traj* T, const dmw_replay* rp, double* signals, double* p
     return dmw::batch_fail(DMW_EINVAL,
                            "surface relaxivity needs the boundary channel and chi of 1 or n_meas rows");
   const int n_meas = rp->n_meas, n_t = T->n_t;
-  const int MT = n_meas <= 16 ? 1 : n_meas <= 32 ? 2 : n_meas <= 64 ? 4 : 8;
+  if (rp->meas_tiles != 0 && rp->meas_tiles != 1 && rp->meas_tiles != 2 && rp->meas_tiles != 4 &&
+      rp->meas_tiles != 8)
+    return dmw::batch_fail(DMW_EINVAL, "meas_tiles must be 0, 1, 2, 4 or 8");
+  const int MT = rp->meas_tiles > 0 ? rp->meas_tiles : (n_meas <= 16 ? 1 : n_meas <= 32 ? 2 : n_meas <= 64 ? 4 : 8);
   const int MTT = ((n_meas + 16 * MT - 1) / (16 * MT)) * MT;
   const int M_pad = 16 * MTT;
   const int K = 3 * n_t, nk = (K + 3) / 4, nk2 = (n_t + 3) / 4;
@@ -313,7 +355,12 @@ int dmw_replay_run(dmw_traj* T, const dmw_replay* rp, double* signals, double* p
     RP_TRY(s.alloc(&dchi, (size_t)n_t * 8), "hipMalloc");
     RP_TRY(hipMemcpy(dchi, rp->chi, (size_t)n_t * 8, hipMemcpyHostToDevice), "H2D");
   }
-  RP_TRY(s.alloc(&dpart, (size_t)gx * M_pad * 8), "hipMalloc");
+  const unsigned gs = (unsigned)((T->N + 255) / 256);
+  void *dphib, *dlwb = nullptr;
+  RP_TRY(s.alloc(&dphib, (size_t)M_pad * T->N * 8), "hipMalloc phases");
+  if (surf_mode == 1) RP_TRY(s.alloc(&dlwb, (size_t)T->N * 8), "hipMalloc");
+  if (surf_mode == 2) RP_TRY(s.alloc(&dlwb, (size_t)M_pad * T->N * 8), "hipMalloc");
+  RP_TRY(s.alloc(&dpart, (size_t)gs * M_pad * 8), "hipMalloc");
   RP_TRY(s.alloc(&dsig, (size_t)M_pad * 8), "hipMalloc");
   if (phi) RP_TRY(s.alloc(&dphi, (size_t)n_meas * T->N * 8), "hipMalloc");
   if (log_w) RP_TRY(s.alloc(&dlw, (size_t)n_meas * T->N * 8), "hipMalloc");
@@ -336,6 +383,8 @@ int dmw_replay_run(dmw_traj* T, const dmw_replay* rp, double* signals, double* p
   a.M_pad = M_pad;
   a.n_meas = n_meas;
   a.partials = (double*)dpart;
+  a.phi_buf = (double*)dphib;
+  a.lw_buf = (double*)dlwb;
   a.phi_out = (double*)dphi;
   a.lw_out = (double*)dlw;
 
@@ -352,8 +401,9 @@ int dmw_replay_run(dmw_traj* T, const dmw_replay* rp, double* signals, double* p
     }
   }
   if (e == hipSuccess) e = hipEventRecord(e1, st);
+  if (e == hipSuccess) e = dmw::launch_signal(surf_mode, dim3(gs, M_pad / 16), st, a);
   if (e == hipSuccess)
-    e = dmw::launch_reduce_partials((const double*)dpart, gx, M_pad, 1.0 / (double)T->N, (double*)dsig, st);
+    e = dmw::launch_reduce_partials((const double*)dpart, gs, M_pad, 1.0 / (double)T->N, (double*)dsig, st);
   std::vector<double> hs(M_pad);
   if (e == hipSuccess) e = hipMemcpy(hs.data(), dsig, (size_t)M_pad * 8, hipMemcpyDeviceToHost);
   float ms = 0.0f;

@@ -874,6 +874,10 @@ template <int GEOM, int WALL, int MB, bool KAHAN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void walk_kernel(const WalkArgs a) {
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
   constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
+  // The float16 trajectory channels of the walk-once producer are compiled into
+  // the 4-measurement variant only (the producer launches with that block), so
+  // the other variants carry no per-sub-step channel code.
+  constexpr bool CHANNELS = (MB == 4 && !KAHAN);
   extern __shared__ float4 lds[];
 
   Tables tb{nullptr, nullptr, nullptr};
@@ -951,8 +955,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       if (a.has_t2) dlog = dlog - (a.dt_sub * a.inv_T2) * chi;
       if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       logw = logw + dlog;
-      if (a.dlog_h) lw_sample = lw_sample + dlog;
-      if (a.comp_h) occ = occ + outside_norm<GEOM>(p, a);
+      if (CHANNELS && a.dlog_h) lw_sample = lw_sample + dlog;
+      if (CHANNELS && a.comp_h) occ = occ + outside_norm<GEOM>(p, a);
       if (n_sub > 1) {
         ax = ax + (p.x - x0);
         ay = ay + (p.y - y0);
@@ -984,7 +988,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         a.pos_full[((size_t)t * 3 + 1) * N + w] = PACKED ? p.y + L * (float)p.cy : p.y;
         a.pos_full[((size_t)t * 3 + 2) * N + w] = PACKED ? p.z + L * (float)p.cz : p.z;
       }
-      if (a.traj_h) {
+      if (CHANNELS && a.traj_h) {
         float qx = PACKED ? p.x + L * (float)p.cx : p.x;
         float qy = PACKED ? p.y + L * (float)p.cy : p.y;
         float qz = PACKED ? p.z + L * (float)p.cz : p.z;
@@ -999,8 +1003,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         a.traj_h[((size_t)t * 3 + 1) * N + w] = f16_bits(qy);
         a.traj_h[((size_t)t * 3 + 2) * N + w] = f16_bits(qz);
       }
-      if (a.dlog_h) a.dlog_h[(size_t)t * N + w] = f16_bits(lw_sample);
-      if (a.comp_h) a.comp_h[(size_t)t * N + w] = f16_bits(occ / fsub);
+      if (CHANNELS && a.dlog_h) a.dlog_h[(size_t)t * N + w] = f16_bits(lw_sample);
+      if (CHANNELS && a.comp_h) a.comp_h[(size_t)t * N + w] = f16_bits(occ / fsub);
     }
     if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
       const float wgt = expf(logw);

@@ -57,7 +57,7 @@ class Instance_t(C.Structure):
 class Replay_t(C.Structure):
     _fields_ = [("n_meas", C.c_int32), ("n_t", C.c_int32), ("G", _dp), ("scale", C.c_double),
                 ("chi", _dp), ("chi_rows", C.c_int32), ("log_w0", _dp),
-                ("rho_over_D", C.c_double)]
+                ("rho_over_D", C.c_double), ("meas_tiles", C.c_int32)]
 
 
 class BatchStats_t(C.Structure):

@@ -187,7 +187,7 @@ class TrajectorySet:
         return tr.view(np.float16), (None if dl is None else dl.view(np.float16)), comp
 
     def replay(self, G, dt_wf, *, chi_perp=None, T2=None, T1=None, surface_relaxivity=None,
-               D=None, stimulated_echo=False, return_walker_signals=False):
+               D=None, stimulated_echo=False, return_walker_signals=False, meas_tiles=0):
         """Apply a waveform (dmipy_sim/trajectories.py:360-641) to these
         trajectories; returns signals (n_meas,) float64, or (phi, log_w_total,
         signals) with ``return_walker_signals``."""
@@ -209,7 +209,7 @@ class TrajectorySet:
         rp = nat.Replay_t(n_meas=n_meas, n_t=self.n_t, G=nat.dptr(Gd),
                           scale=GAMMA * self.dt_traj, chi=nat.dptr(chi),
                           chi_rows=n_meas if per_meas else 1, log_w0=nat.dptr(lw0),
-                          rho_over_D=rhoD)
+                          rho_over_D=rhoD, meas_tiles=int(meas_tiles))
         sig = np.zeros(n_meas, np.float64)
         phi = lw = None
         if return_walker_signals:

@@ -227,6 +227,7 @@ typedef struct dmw_replay {
   int32_t chi_rows;          /* 1 or n_meas */
   const double* log_w0;      /* (n_meas) walker-independent log-weight (T2, T1 terms) */
   double rho_over_D;         /* surface relaxivity / D; 0 = no surface term */
+  int32_t meas_tiles;        /* 16-measurement tiles per pass: 1, 2, 4, 8 or 0 = automatic */
 } dmw_replay;
 
 /* signals (n_meas); optional phi and log_w (n_meas, n_walkers) float64. */
