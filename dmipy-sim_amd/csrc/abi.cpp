@@ -165,12 +165,18 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (wave->n_meas < 1 || wave->n_t < 1 || !wave->G) return fail(DMW_EINVAL, "empty waveform");
   if (run->n_sub < 1) return fail(DMW_EINVAL, "n_sub must be >= 1");
   if ((long long)wave->n_t * run->n_sub >= (1LL << 32)) return fail(DMW_EINVAL, "n_t * n_sub too large");
-  if (geom->kind < DMW_FREE || geom->kind > DMW_PACKED_SPHERES) return fail(DMW_EINVAL, "unknown geometry kind");
+  if (geom->kind < DMW_FREE || geom->kind > DMW_PERMEABLE_SHELL_CYL) return fail(DMW_EINVAL, "unknown geometry kind");
   if (geom->wall < DMW_WALL_REFLECT || geom->wall > DMW_WALL_PERMEATE) return fail(DMW_EINVAL, "unknown wall mode");
   if (geom->kind == DMW_FREE && geom->wall != DMW_WALL_REFLECT)
     return fail(DMW_EUNSUPPORTED, "FreeDiffusion has no wall");
   if (geom->kind == DMW_BOX1D && geom->wall == DMW_WALL_PERMEATE)
     return fail(DMW_EUNSUPPORTED, "Box1D has no permeable wall");
+  if ((geom->kind == DMW_PERMEABLE_SHELL || geom->kind == DMW_PERMEABLE_SHELL_CYL) &&
+      geom->wall == DMW_WALL_SURFACE)
+    return fail(DMW_EUNSUPPORTED, "PermeableShell relaxes only on its permeable membrane");
+  if ((geom->kind == DMW_PERMEABLE_SHELL || geom->kind == DMW_PERMEABLE_SHELL_CYL) &&
+      !(geom->aux > geom->size && geom->size > 0.0f))
+    return fail(DMW_EINVAL, "PermeableShell needs 0 < r_inner < r_outer");
   const bool packed = geom->kind == DMW_PACKED_CYLINDERS || geom->kind == DMW_PACKED_SPHERES;
   if (packed && (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f)))
     return fail(DMW_EINVAL, "packed substrate needs objects and a positive cell side");
@@ -244,6 +250,10 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.nudge = geom->nudge;
   a.rhoD = geom->rho_over_D;
   a.kappaD = geom->kappa_over_D;
+  a.ax = geom->axes[0];
+  a.ay = geom->axes[1];
+  a.az = geom->axes[2];
+  a.aux = geom->aux;
   a.n_obj = 0;
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
@@ -339,6 +349,10 @@ static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid,
     case DMW_SPHERE: return dmw::launch_walk_sph(wall, mb, kahan, grid, lds, st, a);
     case DMW_PACKED_CYLINDERS: return dmw::launch_walk_pcyl(wall, mb, kahan, grid, lds, st, a);
     case DMW_PACKED_SPHERES: return dmw::launch_walk_psph(wall, mb, kahan, grid, lds, st, a);
+    case DMW_ELLIPSOID: return dmw::launch_walk_ell(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PERMEABLE_SLAB: return dmw::launch_walk_slab(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PERMEABLE_SHELL: return dmw::launch_walk_shell(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PERMEABLE_SHELL_CYL: return dmw::launch_walk_shellc(wall, mb, kahan, grid, lds, st, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -479,9 +493,11 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
   *out = nullptr;
   const long long N = run->n_walkers;
   if (N < 1) return fail(DMW_EINVAL, "n_walkers must be >= 1");
-  if ((channels & 2) && !((geom->kind == DMW_CYLINDER || geom->kind == DMW_SPHERE) &&
-                          geom->wall == DMW_WALL_PERMEATE))
-    channels &= ~2;  // single-compartment geometries: occupancy is identically 0 (filled by the caller)
+  const bool has_radius = geom->kind == DMW_CYLINDER || geom->kind == DMW_SPHERE ||
+                          geom->kind == DMW_PERMEABLE_SLAB || geom->kind == DMW_PERMEABLE_SHELL ||
+                          geom->kind == DMW_PERMEABLE_SHELL_CYL;
+  if ((channels & 2) && !(has_radius && geom->wall == DMW_WALL_PERMEATE))
+    channels &= ~2;  // no occupancy rule (no radius, or impermeable): identically 0 (filled by the caller)
   std::vector<float> G((size_t)n_t * 3, 0.0f);
   dmw_waveform wave{1, n_t, G.data(), nullptr, 0, nullptr};
   dmw_run r = *run;

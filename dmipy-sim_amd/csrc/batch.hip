@@ -29,6 +29,7 @@ namespace dmw {
 struct InstDev {
   float size, eps, nudge, rhoD, kappaD, step_len, dt_sub, inv_T2, gdt;
   float cell, inv_cell, grid_inv_h;
+  float ax, ay, az, aux;
   int kind, wall, n_sub, has_t2;
   int grid_n, exact_minimage, n_obj;
   int obj_off, start_off, items_off;
@@ -73,6 +74,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.size = I.size; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
   a.cell = I.cell; a.inv_cell = I.inv_cell; a.grid_n = I.grid_n; a.grid_inv_h = I.grid_inv_h;
   a.exact_minimage = I.exact_minimage; a.n_obj = I.n_obj;
+  a.ax = I.ax; a.ay = I.ay; a.az = I.az; a.aux = I.aux;
   a.instance = inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
   a.Gs = b.G + I.g_off;
   a.GsT = b.GT + I.gt_off;
@@ -166,6 +168,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       DMW_UNIT(G_PSPH, W_REFLECT)
       DMW_UNIT(G_PSPH, W_SURFACE)
       DMW_UNIT(G_PSPH, W_PERMEATE)
+      DMW_UNIT(G_ELL, W_REFLECT)
+      DMW_UNIT(G_ELL, W_SURFACE)
+      DMW_UNIT(G_ELL, W_PERMEATE)
+      DMW_UNIT(G_SLAB, W_REFLECT)
+      DMW_UNIT(G_SLAB, W_SURFACE)
+      DMW_UNIT(G_SLAB, W_PERMEATE)
+      DMW_UNIT(G_SHELL, W_REFLECT)
+      DMW_UNIT(G_SHELL, W_PERMEATE)
+      DMW_UNIT(G_SHELLC, W_REFLECT)
+      DMW_UNIT(G_SHELLC, W_PERMEATE)
 #undef DMW_UNIT
       default: break;
     }
@@ -259,11 +271,13 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   const bool have_chi = sets[0].chi != nullptr;
   for (int i = 0; i < n_inst; ++i) {
     const dmw_instance& I = inst[i];
-    if (I.geom.kind < DMW_FREE || I.geom.kind > DMW_PACKED_SPHERES) return dmw::batch_fail(DMW_EINVAL, "unknown geometry kind");
+    if (I.geom.kind < DMW_FREE || I.geom.kind > DMW_PERMEABLE_SHELL_CYL) return dmw::batch_fail(DMW_EINVAL, "unknown geometry kind");
     if (I.geom.wall < 0 || I.geom.wall > 2 || I.n_walkers < 1 || I.n_sub < 1 || I.wave_set < 0 || I.wave_set >= n_sets)
       return dmw::batch_fail(DMW_EINVAL, "bad instance");
     if ((I.geom.kind == DMW_FREE && I.geom.wall != DMW_WALL_REFLECT) ||
-        (I.geom.kind == DMW_BOX1D && I.geom.wall == DMW_WALL_PERMEATE))
+        (I.geom.kind == DMW_BOX1D && I.geom.wall == DMW_WALL_PERMEATE) ||
+        ((I.geom.kind == DMW_PERMEABLE_SHELL || I.geom.kind == DMW_PERMEABLE_SHELL_CYL) &&
+         I.geom.wall == DMW_WALL_SURFACE))
       return dmw::batch_fail(DMW_EUNSUPPORTED, "wall mode not available for this geometry");
     const bool packed = I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES;
     if (packed && (I.geom.n_obj < 1 || !I.geom.centers || !I.geom.radii || !(I.geom.cell > 0.0f)))
@@ -291,6 +305,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     d.rhoD = I.geom.rho_over_D; d.kappaD = I.geom.kappa_over_D;
     d.step_len = I.step_len; d.dt_sub = I.dt_sub; d.inv_T2 = I.inv_T2; d.gdt = I.gamma_dt_sub;
     d.kind = I.geom.kind;
+    d.ax = I.geom.axes[0]; d.ay = I.geom.axes[1]; d.az = I.geom.axes[2]; d.aux = I.geom.aux;
     d.wall = I.geom.wall; d.n_sub = I.n_sub; d.has_t2 = I.has_t2;
     d.n_walkers = I.n_walkers;
     d.seed_offset = I.seed_offset;

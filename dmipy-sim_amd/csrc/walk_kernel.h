@@ -45,7 +45,13 @@
 
 namespace dmw {
 
-enum GeomKind { G_FREE = 0, G_BOX1D = 1, G_CYL = 2, G_SPH = 3, G_PCYL = 4, G_PSPH = 5 };
+enum GeomKind {
+  G_FREE = 0, G_BOX1D = 1, G_CYL = 2, G_SPH = 3, G_PCYL = 4, G_PSPH = 5,
+  G_ELL = 6,    // Ellipsoid: axes = 1/a^2, 1/b^2, 1/c^2; aux = 1 - NUDGE/min(a,b,c)
+  G_SLAB = 7,   // PermeableSlab1D: size = L, membrane at L/2
+  G_SHELL = 8,  // PermeableShell, spherical: size = R_in, aux = R_out
+  G_SHELLC = 9  // PermeableShell, cylindrical (axis z in the native frame)
+};
 enum WallKind { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
 typedef __attribute__((address_space(4))) const float cfloat;
@@ -78,6 +84,7 @@ struct WalkArgs {
   // walkers
   long long n_walkers;
   uint32_t seed_lo, seed_hi, instance;
+  float ax, ay, az, aux;     // Ellipsoid 1/semi-axis^2 and clamp factor; shell R_out
   const float* r0;           // device (3, N) or null
   const float* inj_n;        // device (S, 3, N) or null, S = n_t * n_sub
   const float* inj_u;        // device (S, N) or null
@@ -337,6 +344,218 @@ __device__ __forceinline__ float sph_permeate(Pos& p, float sx, float sy, float 
 }
 
 // ---------------------------------------------------------------------------
+// Ellipsoid with semi-axes (a, b, c) along x, y, z (geometries.py:1137-1425).
+// The quadratic A t^2 + 2 B t + C of the ray against x.Dx = 1, D = diag(1/a^2 ...).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float ell_form(float ux, float uy, float uz, float vx, float vy, float vz,
+                                          const WalkArgs& a) {
+  return ((ux * a.ax) * vx + (uy * a.ay) * vy) + (uz * a.az) * vz;
+}
+
+// Reflection with the 10-iteration bounce scan and the normalised-coordinate
+// clamp; returns sum d_perp = (remaining - d) cos(alpha) (geometries.py:1187-1300).
+__device__ __forceinline__ float ell_reflect(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                             uint32_t& hits) {
+  const float EPS = a.eps, NUDGE = a.nudge;
+  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  float x = p.x, y = p.y, z = p.z, rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {
+    const float A = ell_form(dx, dy, dz, dx, dy, dz, a);
+    const float B = ell_form(x, y, z, dx, dy, dz, a);
+    const float C = ell_form(x, y, z, x, y, z, a) - 1.0f;
+    const float disc = fmaxf(B * B - A * C, 0.0f);
+    const float d = (-B + sqrtf(disc)) / A;
+    if (!((d > EPS) && (d < rem))) break;
+    const float hx = x + d * dx, hy = y + d * dy, hz = z + d * dz;
+    const float mx = hx * a.ax, my = hy * a.ay, mz = hz * a.az;
+    const float mn = sqrtf((mx * mx + my * my) + mz * mz);
+    const float nx = mx / mn, ny = my / mn, nz = mz / mn;
+    const float cosa = (dx * nx + dy * ny) + dz * nz;
+    const float tdd = 2.0f * cosa;
+    float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
+    const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
+    rx = rx / rn; ry = ry / rn; rz = rz / rn;
+    dsum = dsum + (rem - d) * cosa;
+    x = hx - NUDGE * nx; y = hy - NUDGE * ny; z = hz - NUDGE * nz;
+    dx = rx; dy = ry; dz = rz;
+    rem = (rem - d) - NUDGE;
+    hits++;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float fx = x + dx * rm, fy = y + dy * rm, fz = z + dz * rm;
+  const float qx = fx * sqrtf(a.ax), qy = fy * sqrtf(a.ay), qz = fz * sqrtf(a.az);
+  const float q = sqrtf((qx * qx + qy * qy) + qz * qz);
+  if (q >= 1.0f) { fx = (fx * a.aux) / q; fy = (fy * a.aux) / q; fz = (fz * a.aux) / q; }
+  p.x = fx; p.y = fy; p.z = fz;
+  return dsum;
+}
+
+// Powles permeation, single event per step (geometries.py:1302-1410).
+__device__ __forceinline__ float ell_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                              const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float EPS = a.eps, NUDGE = a.nudge;
+  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float x = p.x, y = p.y, z = p.z;
+  const float A = ell_form(dx, dy, dz, dx, dy, dz, a);
+  const float B = ell_form(x, y, z, dx, dy, dz, a);
+  const float C = ell_form(x, y, z, x, y, z, a) - 1.0f;
+  const float disc_raw = B * B - A * C;
+  const float sq = sqrtf(fmaxf(disc_raw, 0.0f));
+  const bool inside = C < 0.0f;
+  const float t_hit = inside ? (-B + sq) / A : (-B - sq) / A;
+  const bool any_hit = (disc_raw > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
+  if (!any_hit) { p.x = x + sx; p.y = y + sy; p.z = z + sz; return 0.0f; }
+  hits++;
+  const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
+  const float mx = hx * a.ax, my = hy * a.ay, mz = hz * a.az;
+  const float mn = sqrtf((mx * mx + my * my) + mz * mz);
+  const float nx = mx / mn, ny = my / mn, nz = mz / mn;
+  const float remaining = sl - t_hit;
+  const float dn = (dx * nx + dy * ny) + dz * nz;
+  const float dperp = remaining * fabsf(dn);
+  const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+  if (ps.get() < pr) {
+    p.x = x + sx; p.y = y + sy; p.z = z + sz; ncross++;
+    return 0.0f;
+  }
+  const float tdd = 2.0f * dn;
+  float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
+  const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
+  rx = rx / rn; ry = ry / rn; rz = rz / rn;
+  const float ux = inside ? -nx : nx, uy = inside ? -ny : ny, uz = inside ? -nz : nz;
+  const float qx = hx + NUDGE * ux, qy = hy + NUDGE * uy, qz = hz + NUDGE * uz;
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  p.x = qx + rx * go; p.y = qy + ry * go; p.z = qz + rz * go;
+  return (-2.0f * a.rhoD) * dperp;
+}
+
+// ---------------------------------------------------------------------------
+// PermeableSlab1D: membrane at x = L/2, reflecting walls at 0 and L folded by
+// the modular mirror (geometries.py:3221-3300).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float slab_fold(float x, float L) {
+  const float L2 = 2.0f * L;
+  float m = fmodf(x, L2);
+  if (m != 0.0f && ((m < 0.0f) != (L2 < 0.0f))) m = m + L2;
+  return m > L ? L2 - m : m;
+}
+
+template <int WALL>
+__device__ __forceinline__ float slab_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                           const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float L = a.size, xm = 0.5f * a.size;
+  const float x = p.x, xn = x + sx;
+  const bool crossed = (x - xm) * (xn - xm) < 0.0f;
+  float dlog = 0.0f, x1 = xn;
+  if (crossed) {
+    hits++;
+    const float dperp = fabsf(xn - xm);
+    bool transmit = false;
+    if (WALL == W_PERMEATE) {
+      const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+      transmit = ps.get() < pr;
+    }
+    if (transmit) {
+      ncross++;
+    } else {
+      x1 = 2.0f * xm - xn;
+      if (WALL != W_REFLECT) dlog = (-2.0f * a.rhoD) * dperp;
+    }
+  }
+  p.x = slab_fold(x1, L);
+  p.y = p.y + sy;
+  p.z = p.z + sz;
+  return dlog;
+}
+
+// ---------------------------------------------------------------------------
+// PermeableShell: permeable membrane at R_in inside a reflecting wall at R_out;
+// spherical (radial = r) or cylindrical (radial = (x, y, 0) in the native frame)
+// (geometries.py:3303-3431).  The reflecting form is the same routine with
+// kappa = rho = 0 (geometries.py:3428-3431).
+// ---------------------------------------------------------------------------
+template <bool CYLK>
+__device__ __forceinline__ float shell_norm(float x, float y, float z) {
+  return CYLK ? sqrtf((x * x + y * y) + 0.0f * 0.0f) : sqrtf((x * x + y * y) + z * z);
+}
+
+template <bool CYLK>
+__device__ __forceinline__ float shell_first_t(float b, float r2, float dd2, float Rk, float EPS, float sl) {
+  const float c = r2 - Rk * Rk;
+  const float disc = b * b - dd2 * c;
+  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  float t1 = (-b - sq) / dd2, t2 = (-b + sq) / dd2;
+  t1 = (disc > 0.0f && t1 > EPS && t1 < sl) ? t1 : 1e30f;
+  t2 = (disc > 0.0f && t2 > EPS && t2 < sl) ? t2 : 1e30f;
+  return fminf(t1, t2);
+}
+
+template <bool CYLK, int WALL>
+__device__ __forceinline__ float shell_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                            const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float Rin = a.size, Rout = a.aux, EPS = a.eps, NUDGE = a.nudge;
+  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float den = fmaxf(sl, EPS);
+  const float dx = sx / den, dy = sy / den, dz = sz / den;
+  const float x = p.x, y = p.y, z = p.z;
+  const float rz = CYLK ? 0.0f : z, dzr = CYLK ? 0.0f : dz;       // radial components
+  const float b = (dx * x + dy * y) + dzr * rz;
+  const float r2 = (x * x + y * y) + rz * rz;
+  const float dd2 = fmaxf((dx * dx + dy * dy) + dzr * dzr, EPS);
+  const float t_in = shell_first_t<CYLK>(b, r2, dd2, Rin, EPS, sl);
+  const float t_out = shell_first_t<CYLK>(b, r2, dd2, Rout, EPS, sl);
+  const bool hit_in = t_in < t_out;
+  const float t_hit = fminf(t_in, t_out);
+  const bool any_hit = t_hit < 1e29f;
+  float ox = x + sx, oy = y + sy, oz = z + sz;
+  float dlog = 0.0f;
+  if (any_hit) {
+    hits++;
+    const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
+    const float hzr = CYLK ? 0.0f : hz;
+    const float hn = fmaxf(sqrtf((hx * hx + hy * hy) + hzr * hzr), EPS);
+    const float nx = hx / hn, ny = hy / hn, nz = hzr / hn;
+    const float remaining = sl - t_hit;
+    const float dn = (dx * nx + dy * ny) + dz * nz;
+    const float dperp = remaining * fabsf(dn);
+    bool transmit = false;
+    if (WALL == W_PERMEATE && hit_in) {
+      const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
+      transmit = ps.get() < pr;
+    }
+    if (transmit) {
+      ncross++;
+    } else {
+      const float tdd = 2.0f * dn;
+      const float qx = dx - tdd * nx, qy = dy - tdd * ny, qz = dz - tdd * nz;
+      const float Rk = hit_in ? Rin : Rout;
+      const float rmag = shell_norm<CYLK>(x, y, z);
+      const float side = rmag < Rk ? -1.0f : 1.0f;
+      const float sn = side * NUDGE;
+      const float go = fmaxf(remaining - NUDGE, 0.0f);
+      ox = (hx + sn * nx) + qx * go;
+      oy = (hy + sn * ny) + qy * go;
+      oz = (hz + sn * nz) + qz * go;
+      if (WALL == W_PERMEATE && hit_in) dlog = (-2.0f * a.rhoD) * dperp;
+    }
+  }
+  // never outside the reflecting wall (mirror the overshoot back)
+  const float ozr = CYLK ? 0.0f : oz;
+  const float rm = sqrtf((ox * ox + oy * oy) + ozr * ozr);
+  const float over = rm - Rout;
+  if (over > 0.0f) {
+    const float rs = fmaxf(rm, EPS), t2 = 2.0f * over;
+    ox = ox - t2 * (ox / rs);
+    oy = oy - t2 * (oy / rs);
+    oz = oz - t2 * (ozr / rs);
+  }
+  p.x = ox; p.y = oy; p.z = oz;
+  return dlog;
+}
+
+// ---------------------------------------------------------------------------
 // Packed cylinders (2-D cell) and packed spheres (3-D cell).
 // ---------------------------------------------------------------------------
 struct Tables {
@@ -539,6 +758,12 @@ __device__ __forceinline__ int classify(const Pos& p, const WalkArgs& a, const T
   } else if (GEOM == G_SPH) {
     const float RR = a.size * a.size;
     return ((p.x * p.x + p.y * p.y) + p.z * p.z) < RR ? 0 : 1;
+  } else if (GEOM == G_ELL) {
+    return ell_form(p.x, p.y, p.z, p.x, p.y, p.z, a) < 1.0f ? 0 : 1;
+  } else if (GEOM == G_SLAB) {
+    return p.x < 0.5f * a.size ? 0 : 1;
+  } else if (GEOM == G_SHELL || GEOM == G_SHELLC) {
+    return shell_norm<GEOM == G_SHELLC>(p.x, p.y, p.z) < a.size ? 0 : 1;
   } else if (GEOM == G_PCYL || GEOM == G_PSPH) {
     constexpr int DIM = (GEOM == G_PCYL) ? 2 : 3;
     const int gc = cell_of<DIM>(p, a);
@@ -596,6 +821,17 @@ __device__ __forceinline__ float wall_step(Pos& p, float sx, float sy, float sz,
       const float ds = sph_reflect(p, sx, sy, sz, a, hits);
       if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
     }
+  } else if (GEOM == G_ELL) {
+    if (WALL == W_PERMEATE) {
+      dlog = ell_permeate(p, sx, sy, sz, a, ps, hits, ncross);
+    } else {
+      const float ds = ell_reflect(p, sx, sy, sz, a, hits);
+      if (WALL == W_SURFACE) dlog = (-2.0f * a.rhoD) * ds;
+    }
+  } else if (GEOM == G_SLAB) {
+    dlog = slab_step<WALL>(p, sx, sy, sz, a, ps, hits, ncross);
+  } else if (GEOM == G_SHELL || GEOM == G_SHELLC) {
+    dlog = shell_step<GEOM == G_SHELLC, WALL>(p, sx, sy, sz, a, ps, hits, ncross);
   } else {
     if (WALL == W_PERMEATE) {
       dlog = packed_permeate<DIM>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
@@ -625,11 +861,26 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
       p.x = (2.0f * u0 - 1.0f) * a.size;
       p.y = (2.0f * u1 - 1.0f) * a.size;
       if (p.x * p.x + p.y * p.y < a.size * a.size) return p;
-    } else if (GEOM == G_SPH) {
+    } else if (GEOM == G_SPH || GEOM == G_SHELL) {
       p.x = (2.0f * u0 - 1.0f) * a.size;
       p.y = (2.0f * u1 - 1.0f) * a.size;
       p.z = (2.0f * u2 - 1.0f) * a.size;
       if ((p.x * p.x + p.y * p.y) + p.z * p.z < a.size * a.size) return p;
+    } else if (GEOM == G_SHELLC) {
+      p.x = (2.0f * u0 - 1.0f) * a.size;
+      p.y = (2.0f * u1 - 1.0f) * a.size;
+      p.z = u2 * a.size;
+      if (p.x * p.x + p.y * p.y < a.size * a.size) return p;
+    } else if (GEOM == G_SLAB) {
+      p.x = u0 * (0.5f * a.size);
+      return p;
+    } else if (GEOM == G_ELL) {
+      // unit ball scaled by the semi-axes (uniform: constant Jacobian)
+      const float bx = 2.0f * u0 - 1.0f, by = 2.0f * u1 - 1.0f, bz = 2.0f * u2 - 1.0f;
+      p.x = bx / sqrtf(a.ax);
+      p.y = by / sqrtf(a.ay);
+      p.z = bz / sqrtf(a.az);
+      if ((bx * bx + by * by) + bz * bz < 1.0f && classify<GEOM>(p, a, tb) == 0) return p;
     } else {
       p.x = (u0 - 0.5f) * a.cell;
       p.y = (u1 - 0.5f) * a.cell;
@@ -643,12 +894,18 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
 // float32 -> float16 bits, round to nearest even (numpy's astype(float16)).
 __device__ __forceinline__ uint16_t f16_bits(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 
-// Compartment id of the walk-once producer for permeable Cylinder / Sphere:
-// outside iff the float32 norm reaches R (dmipy_sim/core.py:1231-1245).
+// Compartment id of the walk-once producer for permeable walls
+// (dmipy_sim/core.py:1166-1185, 1231-1245): the rule keys on the geometry's
+// `radius` attribute, so a Cylinder is outside iff its axial-plane norm reaches
+// R, and every other geometry with a radius (Sphere; PermeableSlab1D, radius
+// L/2; PermeableShell of either kind, radius R_in) iff the 3-D norm reaches it.
+// An Ellipsoid has no radius: always 0.
 template <int GEOM>
 __device__ __forceinline__ float outside_norm(const Pos& p, const WalkArgs& a) {
   if (GEOM == G_CYL) return sqrtf(p.x * p.x + p.y * p.y) < a.size ? 0.0f : 1.0f;
-  if (GEOM == G_SPH) return sqrtf((p.x * p.x + p.y * p.y) + p.z * p.z) < a.size ? 0.0f : 1.0f;
+  const float r3 = sqrtf((p.x * p.x + p.y * p.y) + p.z * p.z);
+  if (GEOM == G_SPH || GEOM == G_SHELL || GEOM == G_SHELLC) return r3 < a.size ? 0.0f : 1.0f;
+  if (GEOM == G_SLAB) return r3 < 0.5f * a.size ? 0.0f : 1.0f;
   return 0.0f;
 }
 

@@ -8,7 +8,8 @@ hand-written HIP kernels for gfx950 through the C ABI in include/dmipy_walk.h.
 from .constants import GAMMA, DEFAULT_SLEW_RATE
 from .waveforms import (Waveform, pgse, ogse, cpmg, calc_b, set_b, rotate_waveform,
                         tile_waveform, fibonacci_directions)
-from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders,
+from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders, Ellipsoid,
+                         PermeableSlab1D, PermeableShell,
                          PackedSpheres, pack_cylinders, pack_spheres, pack_dense)
 from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
                       collision_sub_steps, replay_sub_steps)
@@ -23,7 +24,8 @@ from .surrogate import MLP, SurrogateEnsemble
 __all__ = [
     "GAMMA", "DEFAULT_SLEW_RATE", "Waveform", "pgse", "ogse", "cpmg", "calc_b", "set_b",
     "rotate_waveform", "tile_waveform", "fibonacci_directions", "FreeDiffusion", "Box1D",
-    "Sphere", "Cylinder", "PackedCylinders", "PackedSpheres", "pack_cylinders",
+    "Sphere", "Cylinder", "PackedCylinders", "PackedSpheres", "Ellipsoid", "PermeableSlab1D",
+    "PermeableShell", "pack_cylinders",
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",

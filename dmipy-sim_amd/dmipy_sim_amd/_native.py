@@ -23,7 +23,7 @@ class Geometry_t(C.Structure):
     _fields_ = [("kind", C.c_int32), ("wall", C.c_int32), ("n_obj", C.c_int32),
                 ("size", C.c_float), ("cell", C.c_float), ("eps", C.c_float),
                 ("nudge", C.c_float), ("rho_over_D", C.c_float), ("kappa_over_D", C.c_float),
-                ("centers", _fp), ("radii", _fp)]
+                ("centers", _fp), ("radii", _fp), ("axes", C.c_float * 3), ("aux", C.c_float)]
 
 
 class Waveform_t(C.Structure):

@@ -74,14 +74,8 @@ class BatchedWalk:
                     G = np.einsum('ij,mtj->mti', np.asarray(rot, np.float64), G)
                 set_key[key] = len(sets_G)
                 sets_G.append(np.ascontiguousarray(G, dtype=np.float32))
-            centers = None if plan.centers is None else np.ascontiguousarray(plan.centers, np.float32)
-            radii = None if plan.radii is None else np.ascontiguousarray(plan.radii, np.float32)
-            self._keep += [centers, radii]
-            gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
-                                n_obj=0 if radii is None else len(radii), size=plan.size,
-                                cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
-                                rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
-                                centers=nat.fptr(centers), radii=nat.fptr(radii))
+            gd = plan.geometry_desc()
+            self._keep.append(plan)      # owns the arrays the descriptor points into
             arr[i] = nat.Instance_t(geom=gd, n_walkers=int(spec['n_walkers']), n_sub=plan.n_sub,
                                     step_len=plan.step_len, gamma_dt_sub=plan.gamma_dt_sub,
                                     dt_sub=plan.dt_sub, inv_T2=plan.inv_T2,

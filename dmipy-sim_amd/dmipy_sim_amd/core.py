@@ -106,11 +106,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     if not meas_block:
         meas_block = autotune.cached_block(plan, n_meas, kahan)
 
-    gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
-                        n_obj=0 if plan.radii is None else len(plan.radii),
-                        size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
-                        rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
-                        centers=nat.fptr(plan.centers), radii=nat.fptr(plan.radii))
+    gd = plan.geometry_desc()
     G = np.ascontiguousarray(G, dtype=np.float32)   # referenced by wd until the call returns
     wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
                         chi=nat.fptr(chi), n_echo=0 if echo is None else len(echo),
@@ -438,11 +434,7 @@ class PlannedWalk:
         elif not meas_block:
             meas_block = autotune.cached_block(plan, n_meas, kahan)
         self.meas_block = int(meas_block)
-        gd = nat.Geometry_t(kind=plan.kind, wall=plan.wall,
-                            n_obj=0 if plan.radii is None else len(plan.radii),
-                            size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
-                            rho_over_D=plan.rho_over_D, kappa_over_D=plan.kappa_over_D,
-                            centers=nat.fptr(plan.centers), radii=nat.fptr(plan.radii))
+        gd = plan.geometry_desc()
         wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(self._G), chi=nat.fptr(self._chi),
                             n_echo=0, echo_idx=None)
         rd = nat.Run_t(n_walkers=int(n_walkers), seed=0, n_sub=plan.n_sub,

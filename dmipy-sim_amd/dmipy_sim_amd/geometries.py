@@ -21,6 +21,7 @@ import numpy as np
 
 # Geometry kind codes shared with include/dmipy_walk.h (enum dmw_geom_kind).
 KIND_FREE, KIND_BOX1D, KIND_CYLINDER, KIND_SPHERE, KIND_PACKED_CYL, KIND_PACKED_SPH = range(6)
+KIND_ELLIPSOID, KIND_SLAB, KIND_SHELL, KIND_SHELL_CYL = range(6, 10)
 
 
 def _rng_from_key(key):
@@ -73,7 +74,7 @@ def _opt_float(x):
 
 class FreeDiffusion(Geometry):
     """Unbounded free diffusion; walkers start at the origin, compartment 0."""
-    kind = KIND_FREE
+    _walk_kind = KIND_FREE
 
     def init_positions(self, n_walkers, key=0):
         return np.zeros((n_walkers, 3), dtype=np.float32)
@@ -85,7 +86,7 @@ class Box1D(Geometry):
     ``surface_relaxivity_t2`` (m/s) applies the Brownstein-Tarr weight
     exp(-2 rho d_perp / D) with d_perp the overshoot past the wall.
     """
-    kind = KIND_BOX1D
+    _walk_kind = KIND_BOX1D
 
     def __init__(self, length, surface_relaxivity_t2=None):
         self.length = float(length)
@@ -106,7 +107,7 @@ class Box1D(Geometry):
 class Sphere(Geometry):
     """Sphere of ``radius`` at the origin, reflecting (optionally relaxing via
     ``surface_relaxivity_t2``) or permeable (``permeability``, Powles crossing)."""
-    kind = KIND_SPHERE
+    _walk_kind = KIND_SPHERE
 
     def __init__(self, radius, surface_relaxivity_t2=None, permeability=None):
         self.radius = float(radius)
@@ -136,7 +137,7 @@ class Cylinder(Geometry):
     The walk runs in the cylinder's native frame (axis = z); ``_R`` maps lab
     vectors into it (dmipy_sim/geometries.py:458-468).
     """
-    kind = KIND_CYLINDER
+    _walk_kind = KIND_CYLINDER
 
     def __init__(self, radius, orientation, surface_relaxivity_t2=None, permeability=None):
         self.radius = float(radius)
@@ -311,7 +312,7 @@ class PackedCylinders(Geometry):
     while wall tests use the minimum image.  Compartments: 0 extra-axonal, k inside
     cylinder k (1-based).
     """
-    kind = KIND_PACKED_CYL
+    _walk_kind = KIND_PACKED_CYL
 
     def __init__(self, radii, centers, L, orientation=(0., 0., 1.),
                  surface_relaxivity_t2=None, permeability=None):
@@ -362,7 +363,7 @@ class PackedCylinders(Geometry):
 class PackedSpheres(Geometry):
     """Spheres in a periodic cubic cell of side ``L`` (3-D analogue of
     ``PackedCylinders``).  Compartments: 0 outside, k inside sphere k (1-based)."""
-    kind = KIND_PACKED_SPH
+    _walk_kind = KIND_PACKED_SPH
 
     def __init__(self, radii, centers, L, surface_relaxivity_t2=None, permeability=None):
         radii = np.asarray(radii, dtype=np.float64).ravel()
@@ -399,3 +400,117 @@ class PackedSpheres(Geometry):
 
     def volume_fraction(self):
         return float((4.0 / 3.0) * np.pi * np.sum(self._radii_np ** 3) / self._L_float ** 3)
+
+
+class Ellipsoid(Geometry):
+    """Axis-aligned ellipsoid x^2/a^2 + y^2/b^2 + z^2/c^2 = 1, reflecting,
+    relaxing or permeable (dmipy_sim/geometries.py:1137-1425)."""
+    _walk_kind = KIND_ELLIPSOID
+
+    def __init__(self, semiaxes, surface_relaxivity_t2=None, permeability=None):
+        self.semiaxes = np.asarray(semiaxes, dtype=np.float64)
+        if self.semiaxes.shape != (3,) or np.any(self.semiaxes <= 0):
+            raise ValueError("semiaxes must be three positive lengths")
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.permeability = _opt_float(permeability)
+
+    def init_positions(self, n_walkers, key=0):
+        """Uniform in the ellipsoid: unit-ball rejection scaled by the semi-axes."""
+        rng = _rng_from_key(key)
+        out, have = [], 0
+        while have < n_walkers:
+            pts = rng.uniform(-1.0, 1.0, (n_walkers * 4, 3))
+            pts = pts[np.linalg.norm(pts, axis=1) < 1.0]
+            out.append(pts)
+            have += len(pts)
+        return (np.concatenate(out)[:n_walkers] * self.semiaxes).astype(np.float32)
+
+    def volume(self):
+        a, b, c = self.semiaxes
+        return (4.0 / 3.0) * np.pi * a * b * c
+
+    def surface_area(self):
+        """Thomsen's approximation, p = 1.6075 (relative error < 1.061 %)."""
+        p = 1.6075
+        ap, bp, cp = self.semiaxes ** p
+        return 4.0 * np.pi * ((ap * bp + ap * cp + bp * cp) / 3.0) ** (1.0 / p)
+
+
+class PermeableSlab1D(Geometry):
+    """Closed 1-D two-compartment slab: permeable membrane at x = L/2, reflecting
+    walls at 0 and L, y and z free (dmipy_sim/geometries.py:3221-3300).  Walkers
+    start in compartment A = {x < L/2}; f_A(t) = 1/2 + 1/2 exp(-4 kappa t / L)."""
+    _walk_kind = KIND_SLAB
+
+    def __init__(self, length, permeability, surface_relaxivity_t2=None):
+        self.length = float(length)
+        self.permeability = float(permeability)
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.radius = float(length) / 2.0        # sub-step length scale
+
+    def volume(self):
+        return self.length / 2.0
+
+    def surface_area(self):
+        return 1.0
+
+    def init_positions(self, n_walkers, key=0):
+        rng = _rng_from_key(key)
+        x = rng.uniform(0.0, self.length / 2.0, n_walkers)
+        z = np.zeros(n_walkers)
+        return np.stack([x, z, z], axis=1).astype(np.float32)
+
+
+class PermeableShell(Geometry):
+    """Closed radial two-compartment shell: permeable membrane at r = R_in inside a
+    reflecting wall at r = R_out; ``kind='sphere'`` (r = |x|) or ``'cylinder'``
+    (r = distance from the axis along ``orientation``)
+    (dmipy_sim/geometries.py:3303-3431).  The cylindrical shell walks in its
+    native frame (axis = z)."""
+
+    def __init__(self, r_inner, r_outer, permeability, kind='sphere',
+                 orientation=(0.0, 0.0, 1.0), surface_relaxivity_t2=None):
+        if kind not in ('sphere', 'cylinder'):
+            raise ValueError("kind must be 'sphere' or 'cylinder'")
+        self.r_inner = float(r_inner)
+        self.r_outer = float(r_outer)
+        self.permeability = float(permeability)
+        self.kind = kind
+        self.surface_relaxivity_t2 = _opt_float(surface_relaxivity_t2)
+        self.radius = float(r_inner)              # sub-step length scale
+        o = np.asarray(orientation, dtype=np.float64)
+        self._o = o / np.linalg.norm(o)
+        self._walk_kind = KIND_SHELL if kind == 'sphere' else KIND_SHELL_CYL
+        if kind == 'cylinder':
+            R = _rotation_to_z(self._o.astype(np.float32))
+            self._R = R.astype(np.float32)
+            self._R_inv = R.T.astype(np.float32)
+            self._is_identity_rotation = bool(np.allclose(R, np.eye(3)))
+
+    def volume(self):
+        if self.kind == 'sphere':
+            return (4.0 / 3.0) * np.pi * self.r_inner ** 3
+        return np.pi * self.r_inner ** 2
+
+    def surface_area(self):
+        if self.kind == 'sphere':
+            return 4.0 * np.pi * self.r_inner ** 2
+        return 2.0 * np.pi * self.r_inner
+
+    def init_positions(self, n_walkers, key=0):
+        """Uniform in compartment A (r < R_in); the cylindrical shell also spreads
+        walkers over [0, R_in) along its axis."""
+        rng = _rng_from_key(key)
+        if self.kind == 'sphere':
+            return Sphere(self.r_inner).init_positions(n_walkers, key)
+        out, have = [], 0
+        while have < n_walkers:
+            xy = rng.uniform(-self.r_inner, self.r_inner, (n_walkers * 2, 2))
+            out.append(xy[np.linalg.norm(xy, axis=1) < self.r_inner])
+            have += len(out[-1])
+        xy = np.concatenate(out)[:n_walkers]
+        zc = rng.uniform(0.0, self.r_inner, n_walkers)
+        native = np.column_stack([xy, zc])
+        if getattr(self, '_is_identity_rotation', True):
+            return native.astype(np.float32)
+        return (native @ self._R.astype(np.float64)).astype(np.float32)

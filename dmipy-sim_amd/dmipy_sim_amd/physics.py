@@ -132,6 +132,26 @@ class WalkPlan:
     radii: np.ndarray = None     # float32
     rot: np.ndarray = None       # lab -> native rotation (float32 3x3) or None
     extra: dict = field(default_factory=dict)
+    axes: tuple = (0.0, 0.0, 0.0)  # Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2
+    aux: float = 0.0               # Ellipsoid clamp factor; shell outer radius
+
+    def geometry_desc(self, wall=None, rho_over_D=None, kappa_over_D=None):
+        """The dmw_geometry descriptor of this plan (optionally with another wall
+        mode or wall constants, as the walk-once producer uses)."""
+        from . import _native as nat
+        c = None if self.centers is None else np.ascontiguousarray(self.centers, np.float32)
+        r = None if self.radii is None else np.ascontiguousarray(self.radii, np.float32)
+        self.extra["_desc_keep"] = (c, r)          # keep the arrays alive with the plan
+        g = nat.Geometry_t(kind=self.kind, wall=self.wall if wall is None else wall,
+                           n_obj=0 if r is None else len(r), size=self.size, cell=self.cell,
+                           eps=self.eps, nudge=self.nudge,
+                           rho_over_D=self.rho_over_D if rho_over_D is None else rho_over_D,
+                           kappa_over_D=self.kappa_over_D if kappa_over_D is None else kappa_over_D,
+                           centers=nat.fptr(c), radii=nat.fptr(r))
+        for i in range(3):
+            g.axes[i] = self.axes[i]
+        g.aux = self.aux
+        return g
 
 
 def _rotation_of(geometry):
@@ -148,7 +168,7 @@ def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fu
     with surface relaxivity, collision rule otherwise); ``"replay"`` follows the
     walk-once producer (R/6, or R/25 when permeable).
     """
-    kind = getattr(geometry, 'kind', None)
+    kind = getattr(geometry, '_walk_kind', None)
     if kind is None:
         raise NotImplementedError(
             f"geometry {type(geometry).__name__} is not supported by the HIP walk")
@@ -192,10 +212,30 @@ def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fu
     inv_T1 = float(f32(1.0 / T1)) if has_t1 else 0.0
 
     size, cell = 0.0, 0.0
+    axes, aux = (0.0, 0.0, 0.0), 0.0
     centers = radii = None
     if kind == geo.KIND_BOX1D:
         size = geometry.length
         eps = nudge = 0.0
+    elif kind == geo.KIND_ELLIPSOID:
+        # geometries.py:1208-1213, 1238-1240: inv_semi_sq = 1/(semi*semi) in float32,
+        # EPS/NUDGE from the smallest semi-axis, clamp factor 1 - NUDGE/min_semi.
+        semi = np.asarray(geometry.semiaxes, dtype=np.float64)
+        s32 = semi.astype(np.float32)
+        inv = f32(1.0) / (s32 * s32)
+        mn = float(np.min(semi))
+        eps = float(f32(1e-7 * mn))
+        nudge = float(f32(1e-4 * mn))
+        axes = tuple(float(v) for v in inv)
+        aux = float(f32(1.0) - f32(nudge) / f32(mn))
+    elif kind == geo.KIND_SLAB:
+        size = geometry.length
+        eps = nudge = 0.0
+    elif kind in (geo.KIND_SHELL, geo.KIND_SHELL_CYL):
+        size = geometry.r_inner
+        aux = float(f32(geometry.r_outer))
+        eps = float(f32(1e-7 * geometry.r_inner))
+        nudge = float(f32(1e-4 * geometry.r_inner))
     elif kind in (geo.KIND_CYLINDER, geo.KIND_SPHERE):
         size = geometry.radius
         eps = float(f32(1e-7 * geometry.radius))
@@ -216,4 +256,5 @@ def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fu
                     step_len=step_len, gamma_dt_sub=gamma_dt_sub, dt_sub=float(dt_sub),
                     inv_T2=inv_T2, inv_T1=inv_T1, has_t2=has_t2, has_t1=has_t1,
                     has_weight=bool(has_surf or has_perm or has_t2 or has_t1),
-                    centers=centers, radii=radii, rot=_rotation_of(geometry))
+                    centers=centers, radii=radii, rot=_rotation_of(geometry),
+                    axes=axes, aux=aux)

@@ -3,7 +3,7 @@ dmipy_sim/trajectories.py:360-641).
 
 ``simulate_trajectories`` walks the spins with no gradient and keeps float16
 positions after every saved sample (plus, on request, the boundary log-weight
-at rho/D = 1 and the compartment occupancy of permeable Cylinder/Sphere walls).
+at rho/D = 1 and the compartment occupancy of permeable walls).
 ``replay`` applies a waveform and relaxation to saved trajectories:
 
     phi[m,w]   = GAMMA dt_traj sum_t G[m,t,:] . r[w,t,:]                (float64)
@@ -85,6 +85,12 @@ def producer_plan(geometry, diffusivity, T_max, dt_save, save_relaxation_data=Fa
                 plan=plan, permeable=has_perm)
 
 
+# Geometries whose producer records compartment occupancy when permeable: the
+# reference's rule keys on a `radius` attribute (dmipy_sim/core.py:1166-1185).
+_OCCUPANCY_KINDS = (geo.KIND_CYLINDER, geo.KIND_SPHERE, geo.KIND_SLAB, geo.KIND_SHELL,
+                    geo.KIND_SHELL_CYL)
+
+
 def _half(a):
     return np.ascontiguousarray(a, dtype=np.float16).view(np.uint16)
 
@@ -104,7 +110,7 @@ class TrajectorySet:
         self.dt_traj = float(dt_traj)
         self.sub_steps = int(sub_steps)
         self.dt_sim = float(dt_sim) if dt_sim is not None else self.dt_traj
-        self.comp_kind = comp_kind          # None, 'occupancy' or 'zeros'
+        self.comp_kind = comp_kind          # None, 'occupancy', 'zeros' or 'zeros16'
 
     @classmethod
     def simulate(cls, n_walkers, diffusivity, geometry, T_max, dt_save, seed=42,
@@ -122,11 +128,8 @@ class TrajectorySet:
             if rot is not None:
                 r0n = (rot.astype(np.float64) @ r0n.T).T
             r0n = np.ascontiguousarray(r0n, dtype=np.float32)
-        gd = nat.Geometry_t(kind=plan.kind, wall=pp["wall"],
-                            n_obj=0 if plan.radii is None else len(plan.radii),
-                            size=plan.size, cell=plan.cell, eps=plan.eps, nudge=plan.nudge,
-                            rho_over_D=pp["rho_over_D"], kappa_over_D=pp["kappa_over_D"],
-                            centers=nat.fptr(plan.centers), radii=nat.fptr(plan.radii))
+        gd = plan.geometry_desc(wall=pp["wall"], rho_over_D=pp["rho_over_D"],
+                                  kappa_over_D=pp["kappa_over_D"])
         rd = nat.Run_t(n_walkers=N, seed=int(seed) & (2**64 - 1), n_sub=pp["sub_steps"],
                        step_len=pp["step_len"], gamma_dt_sub=0.0, dt_sub=float(f32(pp["dt_sim"])),
                        inv_T2=0.0, inv_T1=0.0, has_t2=0, has_t1=0, kahan=0, meas_block=0,
@@ -135,11 +138,12 @@ class TrajectorySet:
         channels = 0
         if save_relaxation_data:
             channels |= 1
-            if pp["permeable"] and plan.kind in (geo.KIND_CYLINDER, geo.KIND_SPHERE):
+            if pp["permeable"] and plan.kind in _OCCUPANCY_KINDS:
                 channels |= 2
                 comp_kind = "occupancy"
             else:
-                comp_kind = "zeros"
+                # permeable without a radius (Ellipsoid): float16 zeros; else int8
+                comp_kind = "zeros16" if pp["permeable"] else "zeros"
         lab = None
         if rot is not None:
             lab = np.ascontiguousarray(np.asarray(rot, np.float64).T, dtype=np.float32)
@@ -182,8 +186,9 @@ class TrajectorySet:
         comp = None
         if cp is not None:
             comp = cp.view(np.float16)
-        elif self.comp_kind == "zeros":
-            comp = np.zeros((self.n_walkers, self.n_t), np.int8)
+        elif self.comp_kind in ("zeros", "zeros16"):
+            comp = np.zeros((self.n_walkers, self.n_t),
+                            np.float16 if self.comp_kind == "zeros16" else np.int8)
         return tr.view(np.float16), (None if dl is None else dl.view(np.float16)), comp
 
     def replay(self, G, dt_wf, *, chi_perp=None, T2=None, T1=None, surface_relaxivity=None,

@@ -43,7 +43,11 @@ enum dmw_geom_kind {
   DMW_CYLINDER = 2,          /* Cylinder           geometries.py:427-844   */
   DMW_SPHERE = 3,            /* Sphere             geometries.py:143-424   */
   DMW_PACKED_CYLINDERS = 4,  /* PackedCylinders    geometries.py:1604-2242 */
-  DMW_PACKED_SPHERES = 5     /* PackedSpheres      geometries.py:2343-2786 */
+  DMW_PACKED_SPHERES = 5,    /* PackedSpheres      geometries.py:2343-2786 */
+  DMW_ELLIPSOID = 6,         /* Ellipsoid          geometries.py:1137-1425 */
+  DMW_PERMEABLE_SLAB = 7,    /* PermeableSlab1D    geometries.py:3221-3300 */
+  DMW_PERMEABLE_SHELL = 8,   /* PermeableShell, kind='sphere'   geometries.py:3303-3431 */
+  DMW_PERMEABLE_SHELL_CYL = 9 /* PermeableShell, kind='cylinder' (axis z in the native frame) */
 };
 
 enum dmw_wall {
@@ -76,6 +80,9 @@ typedef struct dmw_geometry {
   float kappa_over_D;        /* permeability / D */
   const float* centers;      /* (n_obj, 2) cylinders or (n_obj, 3) spheres */
   const float* radii;        /* (n_obj) */
+  float axes[3];             /* Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2 */
+  float aux;                 /* Ellipsoid: 1 - NUDGE/min(a,b,c) (clamp); shells: R_out.
+                                size = R_in for shells, the slab length for the slab */
 } dmw_geometry;
 
 /* Gradient waveform, Waveform.G of dmipy_sim/waveforms.py:39-90. */
@@ -195,7 +202,8 @@ int dmw_batch_destroy(dmw_batch* batch);
  * The reference's replay backend: simulate_trajectories (dmipy_sim/core.py:972-1529)
  * walks the spins once with no waveform and saves float16 positions after every
  * sample (plus, on request, the boundary log-weight at rho/D = 1 and the
- * compartment occupancy of permeable Cylinder/Sphere walls); replay
+ * compartment occupancy of permeable walls with a radius: Cylinder, Sphere,
+ * PermeableSlab1D, PermeableShell); replay
  * (dmipy_sim/trajectories.py:360-641) applies any waveform afterwards as the
  * float64 contraction phi = GAMMA dt G_flat . traj^T and S = mean exp(log_w) cos(phi).
  * Here the trajectories stay in device memory between the two halves. */

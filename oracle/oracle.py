@@ -26,7 +26,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle_walk.so")
 
 KIND = {"FreeDiffusion": 0, "Box1D": 1, "Cylinder": 2, "Sphere": 3,
-        "PackedCylinders": 4, "PackedSpheres": 5}
+        "PackedCylinders": 4, "PackedSpheres": 5, "Ellipsoid": 6, "PermeableSlab1D": 7,
+        "PermeableShell": 8}
 W_REFLECT, W_SURFACE, W_PERMEATE = 0, 1, 2
 
 _fp = C.POINTER(C.c_float)
@@ -48,14 +49,14 @@ class OracleArgs(C.Structure):
                 ("inj_n", _fp), ("inj_u", _fp), ("cell_form", C.c_int32),
                 ("phase64", C.c_int32), ("threads", C.c_int32),
                 ("walker_begin", C.c_int64), ("walker_end", C.c_int64),
-                ("instance", C.c_uint32)]
+                ("instance", C.c_uint32), ("axes", C.c_float * 3), ("aux", C.c_float)]
 
 
 class OracleOut(C.Structure):
     _fields_ = [("signal", _dp), ("signal_im", _dp), ("final_pos", _fp),
                 ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
                 ("n_cross", _up), ("log_w", _fp), ("phi", _fp), ("n_hits", C.c_uint64),
-                ("pos_full", _fp), ("dlog_full", _fp)]
+                ("pos_full", _fp), ("dlog_full", _fp), ("occ_full", _fp)]
 
 
 _lib = None
@@ -107,6 +108,8 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
     """Step constants of one walk, restated from the reference (see module doc)."""
     name = type(geom).__name__
     kind = KIND[name]
+    if name == "PermeableShell" and geom.kind == "cylinder":
+        kind = 9
     has_perm = getattr(geom, "permeability", None) is not None
     has_surf = getattr(geom, "surface_relaxivity_t2", None) is not None
     R = _radius(geom)
@@ -136,13 +139,30 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
              inv_T2=float(f32(1.0 / T2)) if T2 is not None else 0.0,
              inv_T1=float(f32(1.0 / T1)) if T1 is not None else 0.0,
              has_t2=T2 is not None, has_t1=T1 is not None, size=0.0, cell=0.0,
-             eps=0.0, nudge=0.0, centers=None, radii=None)
+             eps=0.0, nudge=0.0, centers=None, radii=None, axes=[0.0, 0.0, 0.0], aux=0.0)
     if kind == 1:
         p["size"] = float(f32(geom.length))
     elif kind in (2, 3):
         p["size"] = float(f32(geom.radius))
         p["eps"] = float(f32(1e-7 * geom.radius))
         p["nudge"] = float(f32(1e-4 * geom.radius))
+    elif kind == 6:
+        # Ellipsoid (geometries.py:1208-1213, 1238-1240)
+        semi = np.asarray(geom.semiaxes, np.float64)
+        s32 = semi.astype(np.float32)
+        inv = np.float32(1.0) / (s32 * s32)
+        mn = float(semi.min())
+        p["eps"] = float(f32(1e-7 * mn))
+        p["nudge"] = float(f32(1e-4 * mn))
+        p["axes"] = [float(v) for v in inv]
+        p["aux"] = float(np.float32(1.0) - np.float32(p["nudge"]) / np.float32(mn))
+    elif kind == 7:
+        p["size"] = float(f32(geom.length))
+    elif kind in (8, 9):
+        p["size"] = float(f32(geom.r_inner))
+        p["aux"] = float(f32(geom.r_outer))
+        p["eps"] = float(f32(1e-7 * geom.r_inner))
+        p["nudge"] = float(f32(1e-4 * geom.r_inner))
     elif kind in (4, 5):
         rmin = float(np.min(geom._radii_np))
         p["eps"] = float(f32(1e-7 * rmin))
@@ -183,7 +203,9 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                    n_walkers=N, seed=int(seed) & (2**64 - 1), r0=fp(r0), inj_n=fp(inj_n),
                    inj_u=fp(inj_u), cell_form=int(cell_form), phase64=int(bool(phase64)),
                    threads=int(threads), walker_begin=int(w0), walker_end=int(w1),
-                   instance=int(instance))
+                   instance=int(instance), aux=float(p.get("aux", 0.0)))
+    for k in range(3):
+        A.axes[k] = float(p.get("axes", [0.0, 0.0, 0.0])[k])
     rows = 1 if echo is None else len(echo)
     res = {"signal": np.zeros(rows * n_meas), "signal_im": np.zeros(rows * n_meas),
            "final_pos": np.zeros((N, 3), np.float32), "comp_origin": np.zeros(N, np.int32),
@@ -199,8 +221,12 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
     if per_sample:
         res["pos_full"] = np.zeros((N, n_t, 3), np.float32)
         res["dlog_full"] = np.zeros((N, n_t), np.float32)
+        res["occ_full"] = np.zeros((N, n_t), np.float32)
         O.pos_full = fp(res["pos_full"])
         O.dlog_full = fp(res["dlog_full"])
+        O.occ_full = fp(res["occ_full"])
+        res["comp_full"] = np.zeros((N, n_t), np.int32)
+        O.comp_full = res["comp_full"].ctypes.data_as(_ip)
     rc = lib().oracle_walk(C.byref(A), C.byref(O))
     if rc != 0:
         raise RuntimeError("oracle_walk failed")

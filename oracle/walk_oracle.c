@@ -70,6 +70,8 @@ typedef struct {
   int64_t walker_begin;    /* sub-range of walkers to run (cpu timing slices) */
   int64_t walker_end;      /* 0 = all */
   uint32_t instance;       /* second Philox counter word (substrate instance) */
+  float axes[3];           /* Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2 */
+  float aux;               /* Ellipsoid: 1 - NUDGE/min semi-axis; shell: R_out */
 } oracle_args;
 
 typedef struct {
@@ -85,9 +87,11 @@ typedef struct {
   uint64_t n_hits;
   float* pos_full;         /* (N, n_t, 3) unfolded position after every sample, or NULL */
   float* dlog_full;        /* (N, n_t) log-weight summed over each sample's sub-steps, or NULL */
+  float* occ_full;         /* (N, n_t) fraction of each sample's sub-steps outside, or NULL */
 } oracle_out;
 
-enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5 };
+enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5,
+       K_ELL = 6, K_SLAB = 7, K_SHELL = 8, K_SHELLC = 9 };
 enum { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
 /* ---------------------------------------------------------------- Philox */
@@ -476,10 +480,210 @@ static float packed_permeate(float r[3], const float st[3], float u, const oracl
   return -2.0f * A->rho_over_D * dperp;
 }
 
+/* ---------------------------------------------- Ellipsoid (geometries.py:1137-1425) */
+/* u . diag(1/a^2, 1/b^2, 1/c^2) . v, as jnp.dot(u * inv_semi_sq, v) */
+static float ell_q(const float u[3], const float v[3], const float inv[3]) {
+  const float w[3] = {u[0] * inv[0], u[1] * inv[1], u[2] * inv[2]};
+  return dot3(w, v);
+}
+
+/* reflect / reflect_with_log_weight: 10-iteration scan, then the clamp in
+   normalised coordinates (geometries.py:1187-1300); returns sum d_perp. */
+static float ell_reflect(float r[3], const float st[3], const oracle_args* A, uint32_t* hits) {
+  const float* inv = A->axes;
+  const float EPS = A->eps, NUDGE = A->nudge;
+  const float sl = norm3(st);
+  float d[3] = {st[0] / sl, st[1] / sl, st[2] / sl};
+  float p[3] = {r[0], r[1], r[2]};
+  float rem = sl, dsum = 0.0f;
+  for (int it = 0; it < 10; ++it) {
+    const float Aq = ell_q(d, d, inv), Bq = ell_q(p, d, inv), Cq = ell_q(p, p, inv) - 1.0f;
+    const float disc = fmaxf(Bq * Bq - Aq * Cq, 0.0f);
+    const float t = (-Bq + sqrtf(disc)) / Aq;
+    if (!(t > EPS && t < rem)) continue;   /* the scan's no-op iteration */
+    float h[3], n[3], rf[3];
+    for (int k = 0; k < 3; ++k) h[k] = p[k] + t * d[k];
+    const float m[3] = {h[0] * inv[0], h[1] * inv[1], h[2] * inv[2]};
+    const float mn = norm3(m);
+    for (int k = 0; k < 3; ++k) n[k] = m[k] / mn;
+    const float ca = dot3(d, n);
+    for (int k = 0; k < 3; ++k) rf[k] = d[k] - (2.0f * ca) * n[k];
+    const float rn = norm3(rf);
+    for (int k = 0; k < 3; ++k) rf[k] = rf[k] / rn;
+    dsum = dsum + (rem - t) * ca;
+    for (int k = 0; k < 3; ++k) { p[k] = h[k] - NUDGE * n[k]; d[k] = rf[k]; }
+    rem = (rem - t) - NUDGE;
+    (*hits)++;
+  }
+  const float rm = fmaxf(rem, 0.0f);
+  float f[3];
+  for (int k = 0; k < 3; ++k) f[k] = p[k] + d[k] * rm;
+  const float qv[3] = {f[0] * sqrtf(inv[0]), f[1] * sqrtf(inv[1]), f[2] * sqrtf(inv[2])};
+  const float q = norm3(qv);
+  if (q >= 1.0f)
+    for (int k = 0; k < 3; ++k) f[k] = (f[k] * A->aux) / q;
+  r[0] = f[0]; r[1] = f[1]; r[2] = f[2];
+  return dsum;
+}
+
+/* permeate: single Powles event per step (geometries.py:1302-1410). */
+static float ell_permeate(float r[3], const float st[3], float u, const oracle_args* A,
+                          uint32_t* hits, uint32_t* ncross) {
+  const float* inv = A->axes;
+  const float EPS = A->eps, NUDGE = A->nudge;
+  const float sl = norm3(st);
+  const float d[3] = {st[0] / sl, st[1] / sl, st[2] / sl};
+  const float Aq = ell_q(d, d, inv), Bq = ell_q(r, d, inv), Cq = ell_q(r, r, inv) - 1.0f;
+  const float disc_raw = Bq * Bq - Aq * Cq;
+  const float sq = sqrtf(fmaxf(disc_raw, 0.0f));
+  const int inside = Cq < 0.0f;
+  const float t_exit = (-Bq + sq) / Aq, t_entry = (-Bq - sq) / Aq;
+  const float th = inside ? t_exit : t_entry;
+  const int any = disc_raw > 0.0f && th > EPS && th < sl && sl > 0.0f;
+  if (!any) { for (int k = 0; k < 3; ++k) r[k] = r[k] + st[k]; return 0.0f; }
+  (*hits)++;
+  float h[3], n[3], rf[3];
+  for (int k = 0; k < 3; ++k) h[k] = r[k] + th * d[k];
+  const float m[3] = {h[0] * inv[0], h[1] * inv[1], h[2] * inv[2]};
+  const float mn = norm3(m);
+  for (int k = 0; k < 3; ++k) n[k] = m[k] / mn;
+  const float remaining = sl - th;
+  const float dn = dot3(d, n);
+  const float dperp = remaining * fabsf(dn);
+  const float p = fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+  if (u < p) { for (int k = 0; k < 3; ++k) r[k] = r[k] + st[k]; (*ncross)++; return 0.0f; }
+  for (int k = 0; k < 3; ++k) rf[k] = d[k] - (2.0f * dn) * n[k];
+  const float rn = norm3(rf);
+  const float go = fmaxf(remaining - NUDGE, 0.0f);
+  for (int k = 0; k < 3; ++k) {
+    const float nd = inside ? -n[k] : n[k];
+    r[k] = (h[k] + NUDGE * nd) + (rf[k] / rn) * go;
+  }
+  return -2.0f * A->rho_over_D * dperp;
+}
+
+/* ---------------------------------------------- PermeableSlab1D (geometries.py:3221-3300) */
+static float slab_step(float r[3], const float st[3], float u, const oracle_args* A,
+                       uint32_t* hits, uint32_t* ncross) {
+  const float L = A->size, xm = 0.5f * A->size;
+  const float x = r[0], xn = x + st[0];
+  const int crossed = (x - xm) * (xn - xm) < 0.0f;
+  const float dperp = crossed ? fabsf(xn - xm) : 0.0f;
+  int transmit = 0;
+  if (crossed && A->wall == W_PERMEATE) transmit = u < fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+  float x1 = xn, dlog = 0.0f;
+  if (crossed) {
+    (*hits)++;
+    if (transmit) (*ncross)++;
+    else {
+      x1 = 2.0f * xm - xn;
+      if (A->wall != W_REFLECT) dlog = -2.0f * A->rho_over_D * dperp;
+    }
+  }
+  const float L2 = 2.0f * L;
+  float mm = fmodf(x1, L2);
+  if (mm != 0.0f && ((mm < 0.0f) != (L2 < 0.0f))) mm = mm + L2;
+  r[0] = mm > L ? L2 - mm : mm;
+  r[1] = r[1] + st[1];
+  r[2] = r[2] + st[2];
+  return dlog;
+}
+
+/* ---------------------------------------------- PermeableShell (geometries.py:3303-3431) */
+static void shell_radial(const float v[3], int cylk, float out[3]) {
+  out[0] = v[0]; out[1] = v[1]; out[2] = cylk ? 0.0f : v[2];
+}
+
+static float shell_first_t(float b, float r2, float dd2, float Rk, float EPS, float sl) {
+  const float c = r2 - Rk * Rk;
+  const float disc = b * b - dd2 * c;
+  const float s = sqrtf(fmaxf(disc, 0.0f));
+  float t1 = (-b - s) / dd2, t2 = (-b + s) / dd2;
+  if (!(disc > 0.0f && t1 > EPS && t1 < sl)) t1 = 1e30f;
+  if (!(disc > 0.0f && t2 > EPS && t2 < sl)) t2 = 1e30f;
+  return fminf(t1, t2);
+}
+
+static float shell_step(float r[3], const float st[3], float u, const oracle_args* A,
+                        uint32_t* hits, uint32_t* ncross) {
+  const int cylk = A->kind == K_SHELLC;
+  const float Rin = A->size, Rout = A->aux, EPS = A->eps, NUDGE = A->nudge;
+  const float sl = norm3(st);
+  const float den = fmaxf(sl, EPS);
+  const float d[3] = {st[0] / den, st[1] / den, st[2] / den};
+  float rr[3], dd[3];
+  shell_radial(r, cylk, rr);
+  shell_radial(d, cylk, dd);
+  const float b = dot3(dd, rr), r2 = dot3(rr, rr), dd2 = fmaxf(dot3(dd, dd), EPS);
+  const float t_in = shell_first_t(b, r2, dd2, Rin, EPS, sl);
+  const float t_out = shell_first_t(b, r2, dd2, Rout, EPS, sl);
+  const int hit_in = t_in < t_out;
+  const float th = fminf(t_in, t_out);
+  const int any = th < 1e29f;
+  float o[3] = {r[0] + st[0], r[1] + st[1], r[2] + st[2]};
+  float dlog = 0.0f;
+  if (any) {
+    (*hits)++;
+    float h[3], hr[3], n[3];
+    for (int k = 0; k < 3; ++k) h[k] = r[k] + th * d[k];
+    shell_radial(h, cylk, hr);
+    const float hn = fmaxf(norm3(hr), EPS);
+    for (int k = 0; k < 3; ++k) n[k] = hr[k] / hn;
+    const float remaining = sl - th;
+    const float dn = dot3(d, n);
+    const float dperp = remaining * fabsf(dn);
+    int transmit = 0;
+    if (A->wall == W_PERMEATE && hit_in) transmit = u < fminf(1.0f, 2.0f * A->kappa_over_D * dperp);
+    if (transmit) (*ncross)++;
+    else {
+      const float Rk = hit_in ? Rin : Rout;
+      const float side = norm3(rr) < Rk ? -1.0f : 1.0f;
+      const float go = fmaxf(remaining - NUDGE, 0.0f);
+      for (int k = 0; k < 3; ++k) {
+        const float q = d[k] - (2.0f * dn) * n[k];
+        o[k] = (h[k] + (side * NUDGE) * n[k]) + q * go;
+      }
+      if (A->wall == W_PERMEATE && hit_in) dlog = -2.0f * A->rho_over_D * dperp;
+    }
+  }
+  float orad[3];
+  shell_radial(o, cylk, orad);
+  const float rm = norm3(orad);
+  const float over = rm - Rout;
+  if (over > 0.0f) {
+    const float rs = fmaxf(rm, EPS);
+    for (int k = 0; k < 3; ++k) o[k] = o[k] - (2.0f * over) * (orad[k] / rs);
+  }
+  r[0] = o[0]; r[1] = o[1]; r[2] = o[2];
+  return dlog;
+}
+
+/* Producer compartment occupancy rule (dmipy_sim/core.py:1166-1185): keyed on the
+   geometry's `radius`; Cylinder by the axial-plane norm, Sphere / slab (radius L/2) /
+   shells (radius R_in) by the 3-D norm; no radius (Ellipsoid, others): 0. */
+static float occupancy_outside(const float r[3], const oracle_args* A) {
+  const float r2 = sqrtf(r[0] * r[0] + r[1] * r[1]);
+  const float r3 = sqrtf((r[0] * r[0] + r[1] * r[1]) + r[2] * r[2]);
+  switch (A->kind) {
+    case K_CYL: return r2 < A->size ? 0.0f : 1.0f;
+    case K_SPH: case K_SHELL: case K_SHELLC: return r3 < A->size ? 0.0f : 1.0f;
+    case K_SLAB: return r3 < 0.5f * A->size ? 0.0f : 1.0f;
+    default: return 0.0f;
+  }
+}
+
 static int classify(const float r[3], const oracle_args* A) {
   switch (A->kind) {
     case K_CYL: return (r[0] * r[0] + r[1] * r[1]) < A->size * A->size ? 0 : 1;
     case K_SPH: return dot3(r, r) < A->size * A->size ? 0 : 1;
+    case K_ELL: return ell_q(r, r, A->axes) < 1.0f ? 0 : 1;
+    case K_SLAB: return r[0] < 0.5f * A->size ? 0 : 1;
+    case K_SHELL:
+    case K_SHELLC: {
+      float rr[3];
+      shell_radial(r, A->kind == K_SHELLC, rr);
+      return norm3(rr) < A->size ? 0 : 1;
+    }
     case K_PCYL:
     case K_PSPH: {
       const int dim = A->kind == K_PCYL ? 2 : 3;
@@ -510,10 +714,21 @@ static void sample_start(const oracle_args* A, uint32_t w, float r[3]) {
     if (A->kind == K_CYL) {
       r[0] = (2.0f * u0 - 1.0f) * A->size; r[1] = (2.0f * u1 - 1.0f) * A->size;
       if (r[0] * r[0] + r[1] * r[1] < A->size * A->size) return;
-    } else if (A->kind == K_SPH) {
+    } else if (A->kind == K_SPH || A->kind == K_SHELL) {
       r[0] = (2.0f * u0 - 1.0f) * A->size; r[1] = (2.0f * u1 - 1.0f) * A->size;
       r[2] = (2.0f * u2 - 1.0f) * A->size;
       if (dot3(r, r) < A->size * A->size) return;
+    } else if (A->kind == K_SHELLC) {
+      r[0] = (2.0f * u0 - 1.0f) * A->size; r[1] = (2.0f * u1 - 1.0f) * A->size;
+      r[2] = u2 * A->size;
+      if (r[0] * r[0] + r[1] * r[1] < A->size * A->size) return;
+    } else if (A->kind == K_SLAB) {
+      r[0] = u0 * (0.5f * A->size); r[1] = 0.0f; r[2] = 0.0f;
+      return;
+    } else if (A->kind == K_ELL) {
+      const float b[3] = {2.0f * u0 - 1.0f, 2.0f * u1 - 1.0f, 2.0f * u2 - 1.0f};
+      for (int k = 0; k < 3; ++k) r[k] = b[k] / sqrtf(A->axes[k]);
+      if (dot3(b, b) < 1.0f && classify(r, A) == 0) return;
     } else {
       r[0] = (u0 - 0.5f) * A->cell; r[1] = (u1 - 0.5f) * A->cell;
       r[2] = A->kind == K_PSPH ? (u2 - 0.5f) * A->cell : 0.0f;
@@ -537,6 +752,13 @@ static float substep(float r[3], const float dir[3], float u, const oracle_args*
     case K_SPH:
       if (A->wall == W_PERMEATE) return sph_permeate(r, st, u, A, hits, ncross);
       return -2.0f * A->rho_over_D * sph_reflect(r, st, A, hits);
+    case K_ELL:
+      if (A->wall == W_PERMEATE) return ell_permeate(r, st, u, A, hits, ncross);
+      return A->wall == W_SURFACE ? -2.0f * A->rho_over_D * ell_reflect(r, st, A, hits)
+                                  : (ell_reflect(r, st, A, hits), 0.0f);
+    case K_SLAB: return slab_step(r, st, u, A, hits, ncross);
+    case K_SHELL:
+    case K_SHELLC: return shell_step(r, st, u, A, hits, ncross);
     default: {
       const int dim = A->kind == K_PCYL ? 2 : 3;
       if (A->wall == W_PERMEATE) return packed_permeate(r, st, u, A, dim, hits, ncross);
@@ -588,7 +810,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       int e_next = 0;
       for (int t = 0; t < T; ++t) {
         const float chi = A->chi ? A->chi[t] : 1.0f;
-        float dlog_sample = 0.0f;
+        float dlog_sample = 0.0f, occ = 0.0f;
         for (int s = 0; s < NS; ++s) {
           const int64_t gs = (int64_t)t * NS + s;
           float dir[3], u = 0.0f;
@@ -616,6 +838,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
           if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
           logw = logw + dlog;
           dlog_sample = dlog_sample + dlog;
+          if (O->occ_full) occ = occ + occupancy_outside(r, A);
           /* phase of every measurement, right-endpoint rule */
           float ru[3] = {r[0], r[1], r[2]};
           if (packed && A->cell_form)
@@ -637,6 +860,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
           for (int d = 0; d < 3; ++d)
             O->pos_full[((size_t)w * T + t) * 3 + d] = (packed && A->cell_form) ? r[d] + L * (float)cell[d] : r[d];
         if (O->dlog_full) O->dlog_full[(size_t)w * T + t] = dlog_sample;
+        if (O->occ_full) O->occ_full[(size_t)w * T + t] = occ / (float)NS;
         if (O->comp_full || (O->comp_final && t == T - 1)) comp = classify(r, A);
         if (O->comp_full) O->comp_full[(size_t)w * T + t] = comp;
         if (e_next < A->n_echo && A->echo_idx[e_next] == t) {

@@ -53,6 +53,16 @@ CASES = [
     ("psph", lambda: packed_sph(), None, None),
     ("psph_rho", lambda: packed_sph(rho=2e-5), None, None),
     ("psph_perm", lambda: packed_sph(kappa=5e-5), None, None),
+    ("ell", lambda: ds.Ellipsoid((3e-6, 2e-6, 4e-6)), None, None),
+    ("ell_rho", lambda: ds.Ellipsoid((3e-6, 2e-6, 4e-6), surface_relaxivity_t2=2e-5), 0.07, None),
+    ("ell_perm", lambda: ds.Ellipsoid((3e-6, 2e-6, 4e-6), permeability=5e-5,
+                                      surface_relaxivity_t2=1e-5), None, None),
+    ("slab", lambda: ds.PermeableSlab1D(8e-6, permeability=3e-5), None, None),
+    ("slab_rho", lambda: ds.PermeableSlab1D(8e-6, permeability=3e-5,
+                                            surface_relaxivity_t2=2e-5), None, None),
+    ("shell", lambda: ds.PermeableShell(2e-6, 4e-6, 5e-5), None, None),
+    ("shell_cyl", lambda: ds.PermeableShell(2e-6, 4e-6, 5e-5, kind="cylinder",
+                                            surface_relaxivity_t2=1e-5), None, None),
 ]
 
 

@@ -264,3 +264,99 @@ def test_autotune_picks_a_timed_block_and_walk_uses_it(gpu):
     assert np.max(np.abs(a - b)) <= 1e-5            # any block walks the same trajectories
     pw.close()
     autotune.clear()
+
+
+# ---------------------------------------------------------------------------
+# Closed two-compartment cells: Monte-Carlo exchange time against the exact
+# finite-diffusion eigenvalue, and the equilibrium partition (detailed balance).
+# Slab: lambda tan(lambda L/2) = 2 kappa / D.  Shells: the lowest root of the
+# 3x3 determinant of {reflecting R_out, flux continuity at R_in, permeable jump
+# at R_in} for the radial (Bessel / spherical Bessel) modes; tau = 1/(D lambda^2).
+# Same cells, kappa, walker count and 2 % tolerances as the reference's ladder
+# (tests/validation/test_permeability_ladder.py:26-49).
+# ---------------------------------------------------------------------------
+def _lowest_root(f, hi, n=4000):
+    from scipy.optimize import brentq
+    xs = np.linspace(hi * 1e-4, hi, n)
+    vs = [f(x) for x in xs]
+    for a, b, fa, fb in zip(xs[:-1], xs[1:], vs[:-1], vs[1:]):
+        if np.sign(fa) != np.sign(fb):
+            return brentq(f, a, b)
+    raise RuntimeError("no root")
+
+
+def _tau_slab(L, kappa):
+    lam = _lowest_root(lambda l: l * np.tan(l * L / 2) - 2 * kappa / D, np.pi / L * 0.999)
+    return 1.0 / (D * lam ** 2)
+
+
+def _tau_shell(Rin, Rout, kappa, kind):
+    from scipy.special import jv, yv, spherical_jn, spherical_yn
+
+    def det(l):
+        a, b = l * Rin, l * Rout
+        if kind == "sphere":
+            F0, F0p = spherical_jn(0, a), spherical_jn(0, a, True)
+            G0, G0p = spherical_yn(0, a), spherical_yn(0, a, True)
+            Fbp, Gbp = spherical_jn(0, b, True), spherical_yn(0, b, True)
+        else:
+            F0, F0p, G0, G0p = jv(0, a), -jv(1, a), yv(0, a), -yv(1, a)
+            Fbp, Gbp = -jv(1, b), -yv(1, b)
+        M = np.array([[0.0, Fbp, Gbp],
+                      [F0p, -F0p, -G0p],
+                      [-D * l * F0p - kappa * F0, kappa * F0, kappa * G0]])
+        return np.linalg.det(M)
+
+    return 1.0 / (D * _lowest_root(det, 1.2e5) ** 2)
+
+
+def _f_inside(gpu, geom, t, dt, n):
+    n_t = max(2, int(round(t / dt)))
+    wf = gpu.Waveform(G=np.zeros((1, n_t, 3), np.float32), dt=float(dt), echo_idx=n_t - 1)
+    _, _, cf = gpu.simulate(n, D, wf, geom, seed=3, return_compartments="final")
+    return float((np.asarray(cf) == 0).mean()), n_t * dt
+
+
+def _exchange(gpu, geom, tau_ex, feq, R, n):
+    dt = (R / 50) ** 2 / (6 * D)
+    ts, fs = [], []
+    for m in (0.6, 0.9, 1.2, 1.5):
+        f, ta = _f_inside(gpu, geom, m * tau_ex, dt, n)
+        ts.append(ta)
+        fs.append(f)
+    y = (np.array(fs) - feq) / (1 - feq)
+    tau = -1.0 / np.polyfit(np.array(ts), np.log(y), 1)[0]
+    f_eq, _ = _f_inside(gpu, geom, 8 * tau_ex, (0.1 * R) ** 2 / (6 * D), n)
+    return tau, f_eq
+
+
+def test_permeable_slab_exchange_time(gpu):
+    L, kappa = 20e-6, 20e-6
+    tau_ex = _tau_slab(L, kappa)
+    tau, f_eq = _exchange(gpu, gpu.PermeableSlab1D(L, kappa), tau_ex, 0.5, L / 2, 500_000)
+    print(f"slab: tau {tau * 1e3:.2f} ms vs exact {tau_ex * 1e3:.2f} ms; f_eq {f_eq:.4f}")
+    assert abs(tau - tau_ex) / tau_ex < 0.02
+    assert abs(f_eq - 0.5) < 0.02
+
+
+@pytest.mark.parametrize("kind,feq", [("cylinder", 0.25), ("sphere", 0.125)])
+def test_permeable_shell_exchange_time(gpu, kind, feq):
+    R, kappa = 10e-6, 20e-6
+    tau_ex = _tau_shell(R, 2 * R, kappa, kind)
+    tau, f_eq = _exchange(gpu, gpu.PermeableShell(R, 2 * R, kappa, kind=kind), tau_ex, feq, R,
+                          500_000)
+    print(f"{kind} shell: tau {tau * 1e3:.2f} ms vs exact {tau_ex * 1e3:.2f} ms; f_eq {f_eq:.4f}")
+    assert abs(tau - tau_ex) / tau_ex < 0.02
+    assert abs(f_eq - feq) < 0.02
+
+
+def test_ellipsoid_sphere_limit_and_anisotropy(gpu):
+    """a = b = c reproduces Sphere(a) within Monte-Carlo error; a prolate
+    ellipsoid attenuates more along its long axis than across it."""
+    wf = ds.set_b(ds.pgse(0.01, 0.03, 0.05, np.eye(3), 200, slew_rate=np.inf), [2e9] * 3)
+    n = 200_000
+    Se = gpu.simulate(n, D, wf, gpu.Ellipsoid((4e-6, 4e-6, 4e-6)), seed=1)
+    Ss = gpu.simulate(n, D, wf, gpu.Sphere(4e-6), seed=2)
+    assert np.all(np.abs(Se - Ss) <= 4 * np.sqrt(2 * 0.5 / n))
+    Sp = gpu.simulate(n, D, wf, gpu.Ellipsoid((2e-6, 2e-6, 8e-6)), seed=3)
+    assert Sp[2] < Sp[0] - 0.05 and abs(Sp[0] - Sp[1]) < 4 * np.sqrt(2 * 0.5 / n)

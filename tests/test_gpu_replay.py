@@ -34,7 +34,14 @@ GEOMS = [
     ("sph", lambda: ds.Sphere(2.5e-6, surface_relaxivity_t2=1e-5)),
     ("sph_perm", lambda: ds.Sphere(3e-6, permeability=5e-5)),
     ("pcyl", lambda: packed_cyl(rho=1e-5)),
+    ("ell_rho", lambda: ds.Ellipsoid((3e-6, 2e-6, 4e-6), surface_relaxivity_t2=1e-5)),
+    ("ell_perm", lambda: ds.Ellipsoid((3e-6, 2e-6, 4e-6), permeability=3e-5)),
+    ("slab", lambda: ds.PermeableSlab1D(8e-6, 3e-5)),
+    ("shell", lambda: ds.PermeableShell(3e-6, 6e-6, 3e-5)),
+    ("shell_cyl", lambda: ds.PermeableShell(3e-6, 6e-6, 3e-5, kind="cylinder",
+                                            orientation=(0.6, 0.0, 0.8))),
 ]
+PERMEABLE = ("sph_perm", "slab", "shell", "shell_cyl")
 
 
 @pytest.mark.parametrize("name,factory", GEOMS, ids=[g[0] for g in GEOMS])
@@ -59,13 +66,16 @@ def test_producer_bit_exact(gpu, oracle, name, factory, save_relax):
     if save_relax:
         np.testing.assert_array_equal(dlog.view(np.uint16),
                                       o["dlog_full"].astype(np.float16).view(np.uint16))
-        if name == "sph_perm":
-            occ = comp.astype(np.float64) * pp["sub_steps"]
-            assert np.all(np.abs(occ - np.round(occ)) < 0.02 * pp["sub_steps"])
+        if name == "ell_perm":                  # permeable, no radius: float16 zeros
+            assert comp.dtype == np.float16 and not comp.any()
+        elif name in PERMEABLE:
+            assert comp.dtype == np.float16
+            np.testing.assert_array_equal(comp.view(np.uint16),
+                                          o["occ_full"].astype(np.float16).view(np.uint16))
             assert 0 < comp.astype(float).mean() < 1
         else:
             assert comp.dtype == np.int8 and not comp.any()
-        if name in ("cyl_rho", "sph", "pcyl"):
+        if name in ("cyl_rho", "sph", "pcyl", "ell_rho"):
             assert (dlog < 0).any()
 
 

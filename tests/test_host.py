@@ -23,8 +23,9 @@ def test_plan_constants_match_oracle_restatement(oracle, name, factory, T2, T1, 
     o = oracle.oracle_plan(g, D, dt, T2, T1)
     assert p.n_sub == o["n_sub"] and p.wall == o["wall"] and p.kind == o["kind"]
     for k in ("step_len", "gamma_dt_sub", "dt_sub", "rho_over_D", "kappa_over_D",
-              "inv_T2", "inv_T1", "eps", "nudge", "size", "cell"):
+              "inv_T2", "inv_T1", "eps", "nudge", "size", "cell", "aux"):
         assert np.float32(getattr(p, k)) == np.float32(o[k]), k
+    assert [np.float32(v) for v in p.axes] == [np.float32(v) for v in o["axes"]]
 
 
 def test_substep_rules_closed_form():

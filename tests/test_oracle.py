@@ -34,7 +34,7 @@ def _wave(n_t=1, n_meas=1, dt=1e-4):
     return ds.Waveform(G=np.zeros((n_meas, n_t, 3), np.float32), dt=dt, echo_idx=n_t - 1)
 
 
-def _one_step(oracle, geom, r0, normal, u=None, T2=None, wall_plan=None):
+def _one_step(oracle, geom, r0, normal, u=None, T2=None, wall_plan=None, per_sample=False):
     """Single walker, single sub-step probe of the oracle's step."""
     wf = _wave()
     plan = oracle.oracle_plan(geom, D, wf.dt, T2)
@@ -46,7 +46,7 @@ def _one_step(oracle, geom, r0, normal, u=None, T2=None, wall_plan=None):
     if u is not None:
         kw["inject_uniforms"] = np.asarray([[u]], np.float32)
     res = oracle.run_oracle(geom, D, wf, 1, r0=np.asarray(r0, np.float32).reshape(1, 3),
-                            plan=plan, **kw)
+                            plan=plan, per_sample=per_sample, **kw)
     return res, plan
 
 
@@ -272,3 +272,137 @@ def test_permeable_weights_only_on_reflection(oracle):
     res = oracle.run_oracle(geom, D, wf, 500, seed=2)
     assert res["n_cross"].sum() > 100
     assert np.mean(res["log_w"] != 0.0) < 0.01
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_ellipsoid_normal_incidence_along_each_axis(oracle, axis):
+    """At a semi-axis end the outward normal is that axis: a walker moving straight
+    out by s beyond the wall lands at 2c - x0 - s, with d_perp = s - (c - x0)."""
+    semi = (3e-6, 2e-6, 4e-6)
+    rho = 1e-5
+    geom = ds.Ellipsoid(semi, surface_relaxivity_t2=rho)
+    s = oracle.oracle_plan(geom, D, _wave().dt)["step_len"]
+    c = semi[axis]
+    x0 = c - 0.4 * s
+    r0 = [0.0, 0.0, 0.0]
+    r0[axis] = x0
+    n = [0.0, 0.0, 0.0]
+    n[axis] = 1.0
+    res, _ = _one_step(oracle, geom, r0, n)
+    x = float(res["final_pos"][0, axis])
+    assert abs(x - (2 * c - x0 - s)) < 1e-6 * c + 1e-12      # float32 rounding
+    assert np.all(res["final_pos"][0, [k for k in range(3) if k != axis]] == 0.0)
+    expect = -2.0 * (rho / D) * (s - (c - x0))
+    assert abs(float(res["log_w"][0]) - expect) < 2e-5 * abs(expect) + 1e-9
+
+
+def test_ellipsoid_and_shell_containment(oracle):
+    """Random walks stay inside a reflecting ellipsoid and inside the shell's
+    outer wall (r < R_out) at every sample."""
+    wf = _wave(n_t=40)
+    for geom in (ds.Ellipsoid((3e-6, 1.5e-6, 2e-6)),
+                 ds.PermeableShell(1.5e-6, 3e-6, 1e-4),
+                 ds.PermeableShell(1.5e-6, 3e-6, 1e-4, kind="cylinder")):
+        res = oracle.run_oracle(geom, D, wf, 400, seed=3, per_sample=True)
+        pos = res["pos_full"].astype(np.float64)
+        if isinstance(geom, ds.Ellipsoid):
+            q = np.sum((pos / geom.semiaxes) ** 2, axis=-1)
+            assert q.max() < 1.0
+        else:
+            rad = pos if geom.kind == "sphere" else pos[..., :2]
+            assert np.linalg.norm(rad, axis=-1).max() < geom.r_outer
+            assert res["n_hits"] > 0
+
+
+@pytest.mark.parametrize("u,transmit", [(0.0, True), (0.9999999, False)])
+def test_slab_membrane_decision_and_fold(oracle, u, transmit):
+    """Crossing x = L/2: transmitted straight through, or mirrored to L - x_new with
+    dlog = -2 (rho/D) |x_new - L/2|; walls at 0 and L fold (geometries.py:3270-3300)."""
+    L, kappa, rho = 8e-6, 3e-5, 2e-5
+    geom = ds.PermeableSlab1D(L, kappa, surface_relaxivity_t2=rho)
+    s = oracle.oracle_plan(geom, D, _wave().dt)["step_len"]
+    x0 = L / 2 - 0.3 * s
+    res, _ = _one_step(oracle, geom, [x0, 1e-7, 0.0], [1.0, 0.0, 0.0], u=u)
+    xn = np.float32(x0) + np.float32(s)
+    if transmit:
+        assert float(res["final_pos"][0, 0]) == float(np.float32(xn))
+        assert res["n_cross"][0] == 1 and res["log_w"][0] == 0.0
+    else:
+        assert abs(float(res["final_pos"][0, 0]) - (L - float(xn))) < 1e-12
+        expect = -2.0 * (rho / D) * (float(xn) - L / 2)
+        assert abs(float(res["log_w"][0]) - expect) < 1e-5 * abs(expect)
+    # outer wall: fold back inside [0, L]
+    res2, _ = _one_step(oracle, geom, [0.2 * s, 0.0, 0.0], [-1.0, 0.0, 0.0], u=u)
+    assert abs(float(res2["final_pos"][0, 0]) - 0.8 * s) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["sphere", "cylinder"])
+@pytest.mark.parametrize("u,transmit", [(0.0, True), (0.9999999, False)])
+def test_shell_membrane_and_outer_wall(oracle, kind, u, transmit):
+    """Outward through the membrane at R_in: transmitted straight, or mirrored to
+    2 R_in - x0 - s (nudged back inside); at the outer wall always mirrored to
+    2 R_out - x0 - s (geometries.py:3349-3426)."""
+    Rin, Rout, kappa = 2e-6, 4e-6, 5e-5
+    geom = ds.PermeableShell(Rin, Rout, kappa, kind=kind)
+    s = oracle.oracle_plan(geom, D, _wave().dt)["step_len"]
+    x0 = Rin - 0.3 * s
+    res, _ = _one_step(oracle, geom, [x0, 0.0, 0.0], [1.0, 0.0, 0.0], u=u)
+    x = float(res["final_pos"][0, 0])
+    if transmit:
+        assert abs(x - (x0 + s)) < 1e-12 and res["n_cross"][0] == 1
+    else:
+        assert abs(x - (2 * Rin - x0 - s)) < 2e-4 * Rin and x < Rin
+    x1 = Rout - 0.3 * s
+    res2, _ = _one_step(oracle, geom, [x1, 0.0, 0.0], [1.0, 0.0, 0.0], u=u)
+    assert abs(float(res2["final_pos"][0, 0]) - (2 * Rout - x1 - s)) < 2e-4 * Rout
+    assert res2["n_cross"][0] == 0
+
+
+@pytest.mark.parametrize("u,transmit", [(0.0, True), (0.9999999, False)])
+def test_producer_occupancy_rule(oracle, u, transmit):
+    """Occupancy of the producer's compartment-1 channel (dmipy_sim/core.py:1166-1185):
+    through a membrane it follows the transmit decision; it keys on the geometry's
+    radius, so a slab walker with x < L/2 but |r| >= L/2, and a cylindrical-shell
+    walker inside R_in radially but with |r| >= R_in, count as outside; an Ellipsoid
+    (no radius) always reads 0."""
+    L, kappa = 8e-6, 3e-5
+    slab = ds.PermeableSlab1D(L, kappa)
+    s = oracle.oracle_plan(slab, D, _wave().dt)["step_len"]
+    res, _ = _one_step(oracle, slab, [L / 2 - 0.3 * s, 0.0, 0.0], [1.0, 0.0, 0.0], u=u,
+                       per_sample=True)
+    assert res["occ_full"][0, 0] == (1.0 if transmit else 0.0)
+    assert res["occ_full"][0, 0] == res["comp_full"][0, 0]
+    res, _ = _one_step(oracle, slab, [0.1 * L, 0.6 * L, 0.0], [0.0, 0.0, 1.0], u=u,
+                       per_sample=True)
+    assert res["comp_full"][0, 0] == 0 and res["occ_full"][0, 0] == 1.0
+    Rin = 2e-6
+    for kind in ("sphere", "cylinder"):
+        shell = ds.PermeableShell(Rin, 2 * Rin, kappa, kind=kind)
+        s = oracle.oracle_plan(shell, D, _wave().dt)["step_len"]
+        res, _ = _one_step(oracle, shell, [Rin - 0.3 * s, 0.0, 0.0], [1.0, 0.0, 0.0], u=u,
+                           per_sample=True)
+        assert res["occ_full"][0, 0] == (1.0 if transmit else 0.0)
+    cyl_shell = ds.PermeableShell(Rin, 2 * Rin, kappa, kind="cylinder")
+    res, _ = _one_step(oracle, cyl_shell, [0.5 * Rin, 0.0, 1.5 * Rin], [0.0, 1.0, 0.0], u=u,
+                       per_sample=True)
+    assert res["comp_full"][0, 0] == 0 and res["occ_full"][0, 0] == 1.0
+    ell = ds.Ellipsoid((3e-6, 2e-6, 4e-6), permeability=kappa)
+    res, _ = _one_step(oracle, ell, [0.0, 0.0, 3.9e-6], [0.0, 0.0, 1.0], u=0.0, per_sample=True)
+    assert res["occ_full"][0, 0] == 0.0
+
+
+def test_producer_occupancy_is_mean_of_sub_step_compartments(oracle):
+    """With several sub-steps per sample the channel is the fraction of sub-steps in
+    compartment 1; for a permeable Sphere the rule equals the walker's compartment,
+    so a one-sub-step walk's per-sample compartments reproduce it exactly."""
+    geom = ds.Sphere(3e-6, permeability=5e-4)
+    wf = _wave(n_t=60, dt=2e-5)
+    p = oracle.oracle_plan(geom, D, wf.dt)
+    p.update(n_sub=1, dt_sub=float(np.float32(wf.dt)))
+    res = oracle.run_oracle(geom, D, wf, 1000, seed=2, plan=p, per_sample=True)
+    np.testing.assert_array_equal(res["occ_full"], res["comp_full"].astype(np.float32))
+    assert 0 < res["occ_full"].mean() < 1
+    p4 = dict(p, n_sub=4, step_len=float(np.float32(p["step_len"]) / np.float32(2.0)))
+    res4 = oracle.run_oracle(geom, D, wf, 1000, seed=2, plan=p4, per_sample=True)
+    occ = res4["occ_full"] * 4
+    np.testing.assert_array_equal(occ, np.round(occ))
