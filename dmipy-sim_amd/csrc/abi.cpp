@@ -104,6 +104,8 @@ int batch_hip_fail(hipError_t e, const char* what) { return hip_fail(e, what); }
 // ---------------------------------------------------------------------------
 struct dmw_plan {
   dmw_geometry geom{};
+  dmw_myelin myelin{};
+  double inv_norm = 1.0;     // 1 / (sum of spin weights): 1/N except for the myelinated cylinder
   dmw_run run{};
   int n_meas = 0, n_t = 0, n_echo = 0;
   int mb = 4, n_chunks = 1, n_meas_pad = 4;
@@ -165,7 +167,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (wave->n_meas < 1 || wave->n_t < 1 || !wave->G) return fail(DMW_EINVAL, "empty waveform");
   if (run->n_sub < 1) return fail(DMW_EINVAL, "n_sub must be >= 1");
   if ((long long)wave->n_t * run->n_sub >= (1LL << 32)) return fail(DMW_EINVAL, "n_t * n_sub too large");
-  if (geom->kind < DMW_FREE || geom->kind > DMW_PERMEABLE_SHELL_CYL) return fail(DMW_EINVAL, "unknown geometry kind");
+  if (geom->kind < DMW_FREE || geom->kind > DMW_MYELINATED_CYLINDER) return fail(DMW_EINVAL, "unknown geometry kind");
   if (geom->wall < DMW_WALL_REFLECT || geom->wall > DMW_WALL_PERMEATE) return fail(DMW_EINVAL, "unknown wall mode");
   if (geom->kind == DMW_FREE && geom->wall != DMW_WALL_REFLECT)
     return fail(DMW_EUNSUPPORTED, "FreeDiffusion has no wall");
@@ -177,6 +179,18 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if ((geom->kind == DMW_PERMEABLE_SHELL || geom->kind == DMW_PERMEABLE_SHELL_CYL) &&
       !(geom->aux > geom->size && geom->size > 0.0f))
     return fail(DMW_EINVAL, "PermeableShell needs 0 < r_inner < r_outer");
+  const bool myel = geom->kind == DMW_MYELINATED_CYLINDER;
+  if (myel) {
+    if (!geom->myelin) return fail(DMW_EINVAL, "MyelinatedCylinder needs its compartment constants");
+    if (geom->wall != DMW_WALL_REFLECT)
+      return fail(DMW_EUNSUPPORTED, "MyelinatedCylinder handles its membranes itself (wall must be reflect)");
+    if (run->n_sub != 1) return fail(DMW_EINVAL, "MyelinatedCylinder takes one step per sample (n_sub = 1)");
+    if (!(geom->aux > geom->size && geom->size > 0.0f))
+      return fail(DMW_EINVAL, "outer_radius must be > inner_radius");
+    const int64_t* ns = geom->myelin->n_start;
+    if (!run->r0 && !(0 <= ns[0] && ns[0] <= ns[1] && ns[1] <= run->n_walkers))
+      return fail(DMW_EINVAL, "MyelinatedCylinder start counts must satisfy 0 <= n0 <= n01 <= n_walkers");
+  }
   const bool packed = geom->kind == DMW_PACKED_CYLINDERS || geom->kind == DMW_PACKED_SPHERES;
   if (packed && (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f)))
     return fail(DMW_EINVAL, "packed substrate needs objects and a positive cell side");
@@ -188,7 +202,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   }
   const long long N = run->n_walkers;
   const long long S = (long long)wave->n_t * run->n_sub;
-  if (run->inj_normals && geom->wall == DMW_WALL_PERMEATE && !run->inj_uniforms)
+  if (run->inj_normals && (geom->wall == DMW_WALL_PERMEATE || myel) && !run->inj_uniforms)
     return fail(DMW_EINVAL, "injected walk on a permeable wall needs injected uniforms");
 
   hipStream_t st;
@@ -197,7 +211,9 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
 
   dmw_plan* P = new dmw_plan();
   P->geom = *geom;
+  P->geom.myelin = nullptr;
   P->run = *run;
+  P->inv_norm = 1.0 / (double)N;
   P->n_meas = wave->n_meas;
   P->n_t = wave->n_t;
   P->n_echo = wave->n_echo;
@@ -254,6 +270,35 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.ay = geom->axes[1];
   a.az = geom->axes[2];
   a.aux = geom->aux;
+  if (myel) {
+    const dmw_myelin& m = *geom->myelin;
+    P->myelin = m;
+    P->geom.myelin = &P->myelin;
+    for (int c = 0; c < 3; ++c) {
+      a.m_step[c] = m.step_len[c];
+      a.m_q2[c] = m.dt_over_T2[c];
+      a.m_wf[c] = m.spin_weight[c];
+    }
+    for (int k = 0; k < 4; ++k) a.m_k2[k] = m.two_kappa_over_D[k];
+    a.m_n0 = m.n_start[0];
+    a.m_n01 = m.n_start[1];
+    // Ensemble normalisation: sum of the start compartments' spin weights, with
+    // the kernel's own start rule (index ranges, or classify_position of r0).
+    double wsum = 0.0;
+    if (run->r0) {
+      const float R0 = geom->size * geom->size, R1 = geom->aux * geom->aux;
+      for (long long w = 0; w < N; ++w) {
+        const float x = run->r0[w * 3], y = run->r0[w * 3 + 1];
+        const float rr = x * x + y * y;
+        wsum += (double)m.spin_weight[rr < R0 ? 0 : (rr < R1 ? 1 : 2)];
+      }
+    } else {
+      wsum = (double)m.n_start[0] * m.spin_weight[0] +
+             (double)(m.n_start[1] - m.n_start[0]) * m.spin_weight[1] +
+             (double)(N - m.n_start[1]) * m.spin_weight[2];
+    }
+    P->inv_norm = 1.0 / wsum;
+  }
   a.n_obj = 0;
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
@@ -353,6 +398,7 @@ static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid,
     case DMW_PERMEABLE_SLAB: return dmw::launch_walk_slab(wall, mb, kahan, grid, lds, st, a);
     case DMW_PERMEABLE_SHELL: return dmw::launch_walk_shell(wall, mb, kahan, grid, lds, st, a);
     case DMW_PERMEABLE_SHELL_CYL: return dmw::launch_walk_shellc(wall, mb, kahan, grid, lds, st, a);
+    case DMW_MYELINATED_CYLINDER: return dmw::launch_walk_myel(wall, mb, kahan, grid, lds, st, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -402,7 +448,7 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a), "walk kernel launch");
   HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
   // CPMG returns the echo rows; a plain walk returns the signal after the last sample.
-  const double inv_n = 1.0 / (double)N;
+  const double inv_n = P->inv_norm;
   double* sig = P->sig.as<double>();
   if (P->n_echo > 0) {
     HIP_TRY(dmw::launch_reduce_partials(P->echo_partials.as<double>(), P->n_waves,
@@ -493,6 +539,8 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
   *out = nullptr;
   const long long N = run->n_walkers;
   if (N < 1) return fail(DMW_EINVAL, "n_walkers must be >= 1");
+  if (geom->kind == DMW_MYELINATED_CYLINDER)
+    return fail(DMW_EUNSUPPORTED, "MyelinatedCylinder uses a dedicated fused step kernel (no replay walk)");
   const bool has_radius = geom->kind == DMW_CYLINDER || geom->kind == DMW_SPHERE ||
                           geom->kind == DMW_PERMEABLE_SLAB || geom->kind == DMW_PERMEABLE_SHELL ||
                           geom->kind == DMW_PERMEABLE_SHELL_CYL;

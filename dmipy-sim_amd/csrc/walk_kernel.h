@@ -50,7 +50,8 @@ enum GeomKind {
   G_ELL = 6,    // Ellipsoid: axes = 1/a^2, 1/b^2, 1/c^2; aux = 1 - NUDGE/min(a,b,c)
   G_SLAB = 7,   // PermeableSlab1D: size = L, membrane at L/2
   G_SHELL = 8,  // PermeableShell, spherical: size = R_in, aux = R_out
-  G_SHELLC = 9  // PermeableShell, cylindrical (axis z in the native frame)
+  G_SHELLC = 9, // PermeableShell, cylindrical (axis z in the native frame)
+  G_MYEL = 10   // MyelinatedCylinder: size = R_in, aux = R_out (axis z in the native frame)
 };
 enum WallKind { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
@@ -85,6 +86,11 @@ struct WalkArgs {
   long long n_walkers;
   uint32_t seed_lo, seed_hi, instance;
   float ax, ay, az, aux;     // Ellipsoid 1/semi-axis^2 and clamp factor; shell R_out
+  // MyelinatedCylinder, per compartment (intra, sheath, extra): step length, dt/T2,
+  // spin weight; 2 kappa/D_leaving for (inner from 0, inner from 1, outer from 1,
+  // outer from 2); walkers [0, m_n0) start intra, [m_n0, m_n01) in the sheath.
+  float m_step[3], m_q2[3], m_wf[3], m_k2[4];
+  long long m_n0, m_n01;
   const float* r0;           // device (3, N) or null
   const float* inj_n;        // device (S, 3, N) or null, S = n_t * n_sub
   const float* inj_u;        // device (S, N) or null
@@ -556,6 +562,73 @@ __device__ __forceinline__ float shell_step(Pos& p, float sx, float sy, float sz
 }
 
 // ---------------------------------------------------------------------------
+// MyelinatedCylinder (physics.py:513-668): compartments 0 = intra-axonal
+// (r < R_in), 1 = sheath, 2 = extra-axonal (r >= R_out) in the native frame,
+// carried in Pos::cx (the start compartment, which selects the spin weight,
+// in Pos::cy).  One step per waveform sample with the compartment's own step
+// length; a step that ends across a membrane of its compartment transmits
+// with p = min(1, 2 kappa/D_leaving |r_new - R|), else it is mirrored to
+// 2R - r_new radially; then the wrong-side clamps to R -+ NUDGE.  At most one
+// membrane can be crossed per step, so one uniform decides it.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pick3(int c, float v0, float v1, float v2) {
+  return c == 0 ? v0 : (c == 1 ? v1 : v2);
+}
+
+__device__ __forceinline__ int myel_classify(float x, float y, const WalkArgs& a) {
+  const float rr = x * x + y * y;
+  return rr < a.size * a.size ? 0 : (rr < a.aux * a.aux ? 1 : 2);
+}
+
+// Rescale (x, y) to radius Rt when the wrong-side test holds (geometries'
+// final_xy_hat * (R -+ NUDGE), physics.py:625-653).
+__device__ __forceinline__ void myel_clamp(float& x, float& y, float& n, bool wrong, float Rt) {
+  if (wrong) {
+    const float s = fmaxf(n, 1e-20f);
+    x = (x / s) * Rt;
+    y = (y / s) * Rt;
+    n = sqrtf(x * x + y * y);
+  }
+}
+
+__device__ __forceinline__ void myel_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                          const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
+  const float Rin = a.size, Rout = a.aux, NUDGE = a.nudge;
+  const int c = p.cx;
+  float nx = p.x + sx, ny = p.y + sy;
+  const float nz = p.z + sz;
+  const float n = sqrtf(nx * nx + ny * ny);
+  const bool in_out = (c == 0) && (n >= Rin), in_in = (c == 1) && (n < Rin);
+  const bool out_out = (c == 1) && (n >= Rout), out_in = (c == 2) && (n < Rout);
+  int nc = c;
+  float nn = n;
+  if (in_out || in_in || out_out || out_in) {
+    hits++;
+    const bool inner = in_out || in_in;
+    const float Rk = inner ? Rin : Rout;
+    const float k2 = inner ? (c == 0 ? a.m_k2[0] : a.m_k2[1]) : (c == 1 ? a.m_k2[2] : a.m_k2[3]);
+    const float pr = fminf(1.0f, k2 * fabsf(n - Rk));
+    if (ps.get() < pr) {
+      ncross++;
+      nc = inner ? (in_out ? 1 : 0) : (out_out ? 2 : 1);
+    } else {
+      const float s = fmaxf(n, 1e-20f);
+      const float hx = nx / s, hy = ny / s;
+      const float rr = 2.0f * Rk - n;
+      nx = hx * rr;
+      ny = hy * rr;
+      nn = sqrtf(nx * nx + ny * ny);
+    }
+  }
+  myel_clamp(nx, ny, nn, (nc == 0) && (nn >= Rin), Rin - NUDGE);
+  myel_clamp(nx, ny, nn, (nc == 1) && (nn < Rin), Rin + NUDGE);
+  myel_clamp(nx, ny, nn, (nc == 1) && (nn >= Rout), Rout - NUDGE);
+  myel_clamp(nx, ny, nn, (nc == 2) && (nn < Rout), Rout + NUDGE);
+  p.x = nx; p.y = ny; p.z = nz;
+  p.cx = nc;
+}
+
+// ---------------------------------------------------------------------------
 // Packed cylinders (2-D cell) and packed spheres (3-D cell).
 // ---------------------------------------------------------------------------
 struct Tables {
@@ -764,6 +837,8 @@ __device__ __forceinline__ int classify(const Pos& p, const WalkArgs& a, const T
     return p.x < 0.5f * a.size ? 0 : 1;
   } else if (GEOM == G_SHELL || GEOM == G_SHELLC) {
     return shell_norm<GEOM == G_SHELLC>(p.x, p.y, p.z) < a.size ? 0 : 1;
+  } else if (GEOM == G_MYEL) {
+    return p.cx;  // the carried compartment (core.py:486-530)
   } else if (GEOM == G_PCYL || GEOM == G_PSPH) {
     constexpr int DIM = (GEOM == G_PCYL) ? 2 : 3;
     const int gc = cell_of<DIM>(p, a);
@@ -832,6 +907,8 @@ __device__ __forceinline__ float wall_step(Pos& p, float sx, float sy, float sz,
     dlog = slab_step<WALL>(p, sx, sy, sz, a, ps, hits, ncross);
   } else if (GEOM == G_SHELL || GEOM == G_SHELLC) {
     dlog = shell_step<GEOM == G_SHELLC, WALL>(p, sx, sy, sz, a, ps, hits, ncross);
+  } else if (GEOM == G_MYEL) {
+    myel_step(p, sx, sy, sz, a, ps, hits, ncross);
   } else {
     if (WALL == W_PERMEATE) {
       dlog = packed_permeate<DIM>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
@@ -851,6 +928,13 @@ template <int GEOM>
 __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const Tables& tb) {
   Pos p{0.0f, 0.0f, 0.0f, 0, 0, 0};
   if (GEOM == G_FREE) return p;
+  // MyelinatedCylinder: the start compartment is fixed by the walker index (the
+  // reference allots round(N V_c / V) walkers per compartment in index order,
+  // extra-axonal space bounded at 2 R_out, geometries.py:921-1008).
+  const int c0 = (GEOM == G_MYEL) ? ((long long)w < a.m_n0 ? 0 : ((long long)w < a.m_n01 ? 1 : 2)) : 0;
+  const float r_hi = (GEOM == G_MYEL) ? pick3(c0, a.size, a.aux, 2.0f * a.aux) : 0.0f;
+  p.cx = c0;
+  p.cy = c0;
   for (uint32_t attempt = 0; attempt < 4096u; ++attempt) {
     const u32x4 r = philox4x32_10({w, a.instance, attempt, TAG_INIT}, a.seed_lo, a.seed_hi);
     const float u0 = u01(r.x), u1 = u01(r.y), u2 = u01(r.z);
@@ -874,6 +958,10 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
     } else if (GEOM == G_SLAB) {
       p.x = u0 * (0.5f * a.size);
       return p;
+    } else if (GEOM == G_MYEL) {
+      p.x = (2.0f * u0 - 1.0f) * r_hi;
+      p.y = (2.0f * u1 - 1.0f) * r_hi;
+      if (myel_classify(p.x, p.y, a) == c0 && p.x * p.x + p.y * p.y < r_hi * r_hi) return p;
     } else if (GEOM == G_ELL) {
       // unit ball scaled by the semi-axes (uniform: constant Jacobian)
       const float bx = 2.0f * u0 - 1.0f, by = 2.0f * u1 - 1.0f, bz = 2.0f * u2 - 1.0f;
@@ -1164,6 +1252,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   if (a.r0) {
     p = Pos{a.r0[w], a.r0[N + w], a.r0[2 * N + w], 0, 0, 0};
     if (PACKED) fold<DIM>(p, a);
+    if (GEOM == G_MYEL) { p.cx = myel_classify(p.x, p.y, a); p.cy = p.cx; }
   } else {
     p = sample_start<GEOM>(w, a, tb);
   }
@@ -1195,7 +1284,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const float nx = a.inj_n[base], ny = a.inj_n[base + N], nz = a.inj_n[base + 2 * N];
         const float nrm = sqrtf((nx * nx + ny * ny) + nz * nz);
         ux = nx / nrm; uy = ny / nrm; uz = nz / nrm;
-        if (WALL == W_PERMEATE && a.inj_u) uinj = a.inj_u[(size_t)gs * N + w];
+        if ((WALL == W_PERMEATE || GEOM == G_MYEL) && a.inj_u) uinj = a.inj_u[(size_t)gs * N + w];
       } else {
         if (!(gs & 1u) || (t == 0 && s == 0)) {
           rnd = philox4x32_10({w, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
@@ -1204,12 +1293,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const uint32_t rb = (gs & 1u) ? rnd.w : rnd.y;
         iso_direction(ra, rb, &ux, &uy, &uz);
       }
-      const float sx = ux * a.step_len, sy = uy * a.step_len, sz = uz * a.step_len;
+      const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
+      const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
       const PowlesSrc ps{w, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
       float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
       if (WALL != W_REFLECT) dlog = dlog * chi;
-      if (a.has_t2) dlog = dlog - (a.dt_sub * a.inv_T2) * chi;
+      if (a.has_t2) {
+        // MyelinatedCylinder: dt/T2 of the compartment after the step (physics.py:657-659)
+        const float q2 = (GEOM == G_MYEL) ? pick3(p.cx, a.m_q2[0], a.m_q2[1], a.m_q2[2])
+                                          : a.dt_sub * a.inv_T2;
+        dlog = dlog - q2 * chi;
+      }
       if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       logw = logw + dlog;
       if (CHANNELS && a.dlog_h) lw_sample = lw_sample + dlog;
@@ -1264,7 +1359,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       if (CHANNELS && a.comp_h) a.comp_h[(size_t)t * N + w] = f16_bits(occ / fsub);
     }
     if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
-      const float wgt = expf(logw);
+      float wgt = expf(logw);
+      if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
       ph.emit(wgt, active, lane,
               a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
       ++next_echo;
@@ -1272,7 +1368,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   }
 
   if (a.comp_final && !a.comp_full) comp = classify<GEOM>(p, a, tb);
-  const float wgt = expf(logw);
+  float wgt = expf(logw);
+  // MyelinatedCylinder: water-fraction spin weight of the start compartment (_ens, core.py:426-429, 489)
+  if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
   ph.emit(wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
   if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
   if (active) {

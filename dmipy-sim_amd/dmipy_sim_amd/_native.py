@@ -19,11 +19,18 @@ _ip = C.POINTER(C.c_int32)
 _up = C.POINTER(C.c_uint32)
 
 
+class Myelin_t(C.Structure):
+    _fields_ = [("step_len", C.c_float * 3), ("two_kappa_over_D", C.c_float * 4),
+                ("dt_over_T2", C.c_float * 3), ("spin_weight", C.c_float * 3),
+                ("n_start", C.c_int64 * 2)]
+
+
 class Geometry_t(C.Structure):
     _fields_ = [("kind", C.c_int32), ("wall", C.c_int32), ("n_obj", C.c_int32),
                 ("size", C.c_float), ("cell", C.c_float), ("eps", C.c_float),
                 ("nudge", C.c_float), ("rho_over_D", C.c_float), ("kappa_over_D", C.c_float),
-                ("centers", _fp), ("radii", _fp), ("axes", C.c_float * 3), ("aux", C.c_float)]
+                ("centers", _fp), ("radii", _fp), ("axes", C.c_float * 3), ("aux", C.c_float),
+                ("myelin", C.POINTER(Myelin_t))]
 
 
 class Waveform_t(C.Structure):

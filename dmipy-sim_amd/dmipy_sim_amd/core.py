@@ -36,6 +36,8 @@ def _replay_gap(geometry, *, return_positions, return_compartments,
         return "return_compartments is a fused single-pass internal (no replay equivalent)"
     if return_walker_signals:
         return "return_walker_signals is a fused single-pass internal (no replay equivalent)"
+    if getattr(geometry, '_is_myelinated', False):
+        return "MyelinatedCylinder uses a dedicated fused step kernel (no replay walk)"
     if getattr(geometry, 'permeability', None) is not None:
         return ("membrane permeability is single-pass walk semantics "
                 "(fused-only; scalar replay has no exchange reservoir)")
@@ -95,7 +97,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
         if rot is not None:
             inject_normals = np.ascontiguousarray(
                 np.einsum('ij,wsj->wsi', rot.astype(np.float64), inject_normals), dtype=np.float32)
-        if plan.wall == WALL_PERMEATE:
+        if plan.wall == WALL_PERMEATE or plan.myelin is not None:
             if inject_uniforms is None:
                 raise ValueError("inject_uniforms required on a permeable wall")
             inject_uniforms = np.ascontiguousarray(inject_uniforms, dtype=np.float32).reshape(N, S)
@@ -106,7 +108,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     if not meas_block:
         meas_block = autotune.cached_block(plan, n_meas, kahan)
 
-    gd = plan.geometry_desc()
+    gd = plan.geometry_desc(n_walkers=N)
     G = np.ascontiguousarray(G, dtype=np.float32)   # referenced by wd until the call returns
     wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
                         chi=nat.fptr(chi), n_echo=0 if echo is None else len(echo),
@@ -196,6 +198,10 @@ def simulate(
         if gap is not None:
             raise NotImplementedError(f"engine='replay' cannot serve this run: {gap}. "
                                       "Use engine='fused' (or 'auto').")
+    if return_positions == 'full' and getattr(geometry, '_is_myelinated', False):
+        raise NotImplementedError(   # core.py:459-463
+            "return_positions='full' is supported for standard geometries "
+            "(including Mesh), not MyelinatedCylinder / PackedMyelinatedCylinders.")
 
     check_gpu(n_walkers, require_gpu, what="simulate")
 
@@ -434,7 +440,7 @@ class PlannedWalk:
         elif not meas_block:
             meas_block = autotune.cached_block(plan, n_meas, kahan)
         self.meas_block = int(meas_block)
-        gd = plan.geometry_desc()
+        gd = plan.geometry_desc(n_walkers=int(n_walkers))
         wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(self._G), chi=nat.fptr(self._chi),
                             n_echo=0, echo_idx=None)
         rd = nat.Run_t(n_walkers=int(n_walkers), seed=0, n_sub=plan.n_sub,

@@ -16,12 +16,20 @@ the HIP walk kernel; these objects carry the parameters to it.
 * ``PackedCylinders`` -- dmipy_sim/geometries.py:1604-2242
 * ``pack_spheres``    -- dmipy_sim/geometries.py:2249-2340
 * ``PackedSpheres``   -- dmipy_sim/geometries.py:2343-2786
+* ``Ellipsoid``       -- dmipy_sim/geometries.py:1137-1425
+* ``MyelinatedCylinder`` -- dmipy_sim/geometries.py:847-1136
+* ``PermeableSlab1D`` -- dmipy_sim/geometries.py:3221-3300
+* ``PermeableShell``  -- dmipy_sim/geometries.py:3303-3431
 """
 import numpy as np
 
 # Geometry kind codes shared with include/dmipy_walk.h (enum dmw_geom_kind).
 KIND_FREE, KIND_BOX1D, KIND_CYLINDER, KIND_SPHERE, KIND_PACKED_CYL, KIND_PACKED_SPH = range(6)
-KIND_ELLIPSOID, KIND_SLAB, KIND_SHELL, KIND_SHELL_CYL = range(6, 10)
+KIND_ELLIPSOID, KIND_SLAB, KIND_SHELL, KIND_SHELL_CYL, KIND_MYELIN = range(6, 11)
+
+# Water per unit myelin-sheath volume, the default myelin proton density of
+# MyelinatedCylinder.water_fractions (dmipy_sim/substrate/biophysical_constants.py:523-530).
+MYELIN_WATER_PROTON_DENSITY = 0.40
 
 
 def _rng_from_key(key):
@@ -400,6 +408,116 @@ class PackedSpheres(Geometry):
 
     def volume_fraction(self):
         return float((4.0 / 3.0) * np.pi * np.sum(self._radii_np ** 3) / self._L_float ** 3)
+
+
+class MyelinatedCylinder(Geometry):
+    """Three concentric compartments along ``orientation``: 0 intra-axonal
+    (r < inner_radius), 1 myelin sheath, 2 extra-axonal (r >= outer_radius), each
+    with its own isotropic diffusivity and optional T2, and a permeability per
+    membrane (dmipy_sim/geometries.py:847-1136; step physics.py:420-679).
+
+    ``water_fractions`` are per-compartment proton densities used as ensemble
+    spin weights of each walker's start compartment; the default is
+    (1, MYELIN_WATER_PROTON_DENSITY, 1).  ``D_myelin`` defaults to 0 (a stuck
+    pool).  ``simulate`` takes no diffusivity for this geometry, ignores its T2
+    argument (T2 is per compartment here) and walks one step per sample.
+    """
+    _walk_kind = KIND_MYELIN
+    _is_myelinated = True
+
+    def __init__(self, inner_radius, outer_radius, orientation,
+                 D_intra, D_extra, D_myelin=0.0,
+                 kappa_inner=None, kappa_outer=None,
+                 T2_intra=None, T2_myelin=None, T2_extra=None,
+                 water_fractions=None):
+        if outer_radius <= inner_radius:
+            raise ValueError("outer_radius must be > inner_radius")
+        self.inner_radius = float(inner_radius)
+        self.outer_radius = float(outer_radius)
+        self.D_intra = float(D_intra)
+        self.D_myelin = float(D_myelin)
+        self.D_extra = float(D_extra)
+        self.kappa_inner = _opt_float(kappa_inner)
+        self.kappa_outer = _opt_float(kappa_outer)
+        self.T2_intra = _opt_float(T2_intra)
+        self.T2_myelin = _opt_float(T2_myelin)
+        self.T2_extra = _opt_float(T2_extra)
+        if water_fractions is None:
+            water_fractions = (1.0, MYELIN_WATER_PROTON_DENSITY, 1.0)
+        self.water_fractions = tuple(float(w) for w in water_fractions)
+        o = np.asarray(orientation, dtype=np.float64)
+        self.orientation = (o / np.linalg.norm(o)).astype(np.float32)
+        R = _rotation_to_z(self.orientation)
+        self._R = R.astype(np.float32)
+        self._R_inv = R.T.astype(np.float32)
+        self._is_identity_rotation = bool(np.allclose(R, np.eye(3)))
+
+    def _areas(self):
+        """Cross-section areas of the three compartments, the extra-axonal one
+        bounded at 2 R_out (the placement convention of the reference)."""
+        Ri, Ro = self.inner_radius, self.outer_radius
+        return (np.pi * Ri ** 2, np.pi * (Ro ** 2 - Ri ** 2), np.pi * ((2.0 * Ro) ** 2 - Ro ** 2))
+
+    def start_counts(self, n_walkers):
+        """Walkers starting in each compartment, allotted by area in index order:
+        round(N a_c / sum a) for intra and sheath, the rest extra-axonal."""
+        a = self._areas()
+        tot = a[0] + a[1] + a[2]
+        n_intra = int(round(n_walkers * a[0] / tot))
+        n_myelin = int(round(n_walkers * a[1] / tot))
+        return n_intra, n_myelin, n_walkers - n_intra - n_myelin
+
+    def init_positions(self, n_walkers, key=0):
+        """Host placement (uniform in each compartment's cross-section, z = 0, lab
+        frame); also sets ``_init_compartments``.  The GPU walk places walkers
+        itself with the same allotment."""
+        rng = _rng_from_key(key)
+        counts = self.start_counts(n_walkers)
+        Ri, Ro = self.inner_radius, self.outer_radius
+        bands = ((0.0, Ri), (Ri, Ro), (Ro, 2.0 * Ro))
+        pos = np.zeros((n_walkers, 3), np.float32)
+        comp = np.zeros(n_walkers, np.int32)
+        i = 0
+        for c, (n_c, (lo, hi)) in enumerate(zip(counts, bands)):
+            got, have = [], 0
+            while have < n_c:
+                xy = rng.uniform(-hi, hi, (n_c * 3, 2))
+                r = np.linalg.norm(xy, axis=1)
+                xy = xy[(r >= lo) & (r < hi)]
+                got.append(xy)
+                have += len(xy)
+            if n_c:
+                pos[i:i + n_c, :2] = np.concatenate(got)[:n_c]
+                comp[i:i + n_c] = c
+            i += n_c
+        self._init_compartments = comp
+        return (self._R_inv.astype(np.float64) @ pos.T.astype(np.float64)).T.astype(np.float32)
+
+    def classify_position(self, r):
+        """0 intra, 1 sheath, 2 extra by the squared axial-plane distance in float32."""
+        r = np.asarray(r, dtype=np.float32)
+        rc = r if self._is_identity_rotation else (r.astype(np.float64) @ self._R.T.astype(np.float64)).astype(np.float32)
+        rr = rc[..., 0] * rc[..., 0] + rc[..., 1] * rc[..., 1]
+        Ri, Ro = np.float32(self.inner_radius), np.float32(self.outer_radius)
+        return np.where(rr < Ri * Ri, 0, np.where(rr < Ro * Ro, 1, 2)).astype(np.int32)
+
+    def volume(self, compartment, L=1.0):
+        names = ('intra', 'myelin', 'extra')
+        if compartment not in names:
+            raise ValueError(
+                f"compartment must be 'intra', 'myelin', or 'extra'; got {compartment!r}")
+        return self._areas()[names.index(compartment)] * float(L)
+
+    def surface_area(self, compartment, L=1.0):
+        Ri, Ro = self.inner_radius, self.outer_radius
+        per = {'intra': Ri, 'myelin': Ri + Ro, 'extra': Ro}
+        if compartment not in per:
+            raise ValueError(
+                f"compartment must be 'intra', 'myelin', or 'extra'; got {compartment!r}")
+        return 2.0 * np.pi * per[compartment] * float(L)
+
+    def volume_fraction(self, compartment):
+        return self.volume(compartment) / (np.pi * (2.0 * self.outer_radius) ** 2)
 
 
 class Ellipsoid(Geometry):

@@ -16,6 +16,7 @@ bit for bit on identical increments:
 * EPS, NUDGE   ``f32(1e-7*R)``, ``f32(1e-4*R)`` with R the radius, or the
                smallest radius of a packing (geometries.py:503-504, 1694-1696)
 """
+import ctypes as C
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -133,11 +134,13 @@ class WalkPlan:
     rot: np.ndarray = None       # lab -> native rotation (float32 3x3) or None
     extra: dict = field(default_factory=dict)
     axes: tuple = (0.0, 0.0, 0.0)  # Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2
-    aux: float = 0.0               # Ellipsoid clamp factor; shell outer radius
+    aux: float = 0.0               # Ellipsoid clamp factor; shell / myelin outer radius
+    myelin: dict = None            # MyelinatedCylinder per-compartment constants
 
-    def geometry_desc(self, wall=None, rho_over_D=None, kappa_over_D=None):
+    def geometry_desc(self, wall=None, rho_over_D=None, kappa_over_D=None, n_walkers=None):
         """The dmw_geometry descriptor of this plan (optionally with another wall
-        mode or wall constants, as the walk-once producer uses)."""
+        mode or wall constants, as the walk-once producer uses).  The myelinated
+        cylinder's start allotment depends on ``n_walkers``."""
         from . import _native as nat
         c = None if self.centers is None else np.ascontiguousarray(self.centers, np.float32)
         r = None if self.radii is None else np.ascontiguousarray(self.radii, np.float32)
@@ -151,6 +154,22 @@ class WalkPlan:
         for i in range(3):
             g.axes[i] = self.axes[i]
         g.aux = self.aux
+        if self.myelin is not None:
+            if n_walkers is None:
+                raise ValueError("the myelinated cylinder's descriptor needs n_walkers")
+            m = self.myelin
+            md = nat.Myelin_t()
+            for i in range(3):
+                md.step_len[i] = m["step_len"][i]
+                md.dt_over_T2[i] = m["dt_over_T2"][i]
+                md.spin_weight[i] = m["spin_weight"][i]
+            for i in range(4):
+                md.two_kappa_over_D[i] = m["two_kappa_over_D"][i]
+            n0, n1, _ = m["start_counts"](int(n_walkers))
+            md.n_start[0] = n0
+            md.n_start[1] = n0 + n1
+            self.extra["_myelin_keep"] = md
+            g.myelin = C.pointer(md)
         return g
 
 
@@ -159,6 +178,38 @@ def _rotation_of(geometry):
     if R is None or getattr(geometry, '_is_identity_rotation', True):
         return None
     return np.asarray(R, dtype=np.float32)
+
+
+def _myelin_plan(geometry, dt, T1):
+    """Constants of make_myelin_step_fn (physics.py:420-512): one step per sample;
+    float32 step lengths sqrt(6 D_c dt); 2 kappa / max(D_leaving, 1e-30) per
+    membrane side; dt/T2 per compartment (1e6 s for an unset T2 once any is set);
+    EPS/NUDGE from the inner radius; T1 on the stored intervals.  The simulate()
+    T2 argument does not apply (core.py:485-489)."""
+    g = geometry
+    dt32 = f32(dt)
+    D = [f32(g.D_intra), f32(g.D_myelin), f32(g.D_extra)]
+    step = [float(np.sqrt((f32(6.0) * d) * dt32, dtype=np.float32)) for d in D]
+    k_in = f32(g.kappa_inner) if g.kappa_inner is not None else f32(0.0)
+    k_out = f32(g.kappa_outer) if g.kappa_outer is not None else f32(0.0)
+    floor = f32(1e-30)
+    two_k = [float(f32(2.0) * (k / np.maximum(D[c], floor)))
+             for k, c in ((k_in, 0), (k_in, 1), (k_out, 1), (k_out, 2))]
+    t2s = (g.T2_intra, g.T2_myelin, g.T2_extra)
+    has_t2 = any(t is not None for t in t2s)
+    q2 = [float(dt32 / f32(t if t is not None else 1e6)) for t in t2s]
+    has_t1 = T1 is not None
+    myelin = dict(step_len=step, two_kappa_over_D=two_k, dt_over_T2=q2,
+                  spin_weight=[float(f32(w)) for w in g.water_fractions],
+                  start_counts=g.start_counts)
+    return WalkPlan(kind=geo.KIND_MYELIN, wall=WALL_REFLECT, n_sub=1,
+                    size=float(f32(g.inner_radius)), cell=0.0,
+                    eps=float(f32(1e-7 * g.inner_radius)), nudge=float(f32(1e-4 * g.inner_radius)),
+                    rho_over_D=0.0, kappa_over_D=0.0, step_len=0.0,
+                    gamma_dt_sub=float(f32(GAMMA * dt)), dt_sub=float(dt32),
+                    inv_T2=0.0, inv_T1=float(f32(1.0 / T1)) if has_t1 else 0.0,
+                    has_t2=has_t2, has_t1=has_t1, has_weight=True,
+                    rot=_rotation_of(geometry), aux=float(f32(g.outer_radius)), myelin=myelin)
 
 
 def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fused"):
@@ -172,6 +223,8 @@ def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fu
     if kind is None:
         raise NotImplementedError(
             f"geometry {type(geometry).__name__} is not supported by the HIP walk")
+    if kind == geo.KIND_MYELIN:
+        return _myelin_plan(geometry, dt, T1)
     if diffusivity is None:
         raise ValueError("diffusivity is required for standard geometries")
     D = float(diffusivity)

@@ -47,7 +47,8 @@ enum dmw_geom_kind {
   DMW_ELLIPSOID = 6,         /* Ellipsoid          geometries.py:1137-1425 */
   DMW_PERMEABLE_SLAB = 7,    /* PermeableSlab1D    geometries.py:3221-3300 */
   DMW_PERMEABLE_SHELL = 8,   /* PermeableShell, kind='sphere'   geometries.py:3303-3431 */
-  DMW_PERMEABLE_SHELL_CYL = 9 /* PermeableShell, kind='cylinder' (axis z in the native frame) */
+  DMW_PERMEABLE_SHELL_CYL = 9, /* PermeableShell, kind='cylinder' (axis z in the native frame) */
+  DMW_MYELINATED_CYLINDER = 10 /* MyelinatedCylinder geometries.py:847-1136, step physics.py:420-679 */
 };
 
 enum dmw_wall {
@@ -65,6 +66,23 @@ enum dmw_status {
   DMW_EHIP = 5               /* any other HIP failure     -> RuntimeError        */
 };
 
+/* MyelinatedCylinder compartments 0 = intra-axonal, 1 = myelin sheath, 2 =
+ * extra-axonal (make_myelin_step_fn, physics.py:420-679).  One step per waveform
+ * sample (run->n_sub must be 1); run->step_len and run->inv_T2 are unused.  The
+ * signal is sum_w spin_weight[c0(w)] e^{log_w} e^{i phi} / sum_w spin_weight[c0(w)]
+ * (_ens, core.py:426-429, 486-530), c0 the start compartment: by walker index
+ * when walkers are placed in-kernel (geometries.py:921-1008 allots them in
+ * index order), else classify_position of run->r0 (geometries.py:1035-1052). */
+typedef struct dmw_myelin {
+  float step_len[3];         /* float32 sqrt(6 D_c dt) per compartment */
+  float two_kappa_over_D[4]; /* float32 2 kappa / max(D_leaving, 1e-30): inner membrane
+                                left from 0, from 1; outer membrane left from 1, from 2 */
+  float dt_over_T2[3];       /* float32 dt / T2_c (applied when run->has_t2) */
+  float spin_weight[3];      /* water fractions (proton densities) */
+  int64_t n_start[2];        /* walkers [0, n_start[0]) start intra-axonal,
+                                [n_start[0], n_start[1]) in the sheath, the rest outside */
+} dmw_myelin;
+
 /* Geometry and wall constants (float32 values rounded as the reference rounds
  * them, see make_step_fn physics.py:229-290 and the EPS/NUDGE definitions
  * geometries.py:503-504, 1694-1696). */
@@ -81,8 +99,10 @@ typedef struct dmw_geometry {
   const float* centers;      /* (n_obj, 2) cylinders or (n_obj, 3) spheres */
   const float* radii;        /* (n_obj) */
   float axes[3];             /* Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2 */
-  float aux;                 /* Ellipsoid: 1 - NUDGE/min(a,b,c) (clamp); shells: R_out.
-                                size = R_in for shells, the slab length for the slab */
+  float aux;                 /* Ellipsoid: 1 - NUDGE/min(a,b,c) (clamp); shells and the
+                                myelinated cylinder: R_out.  size = R_in for those, the
+                                slab length for the slab */
+  const dmw_myelin* myelin;  /* DMW_MYELINATED_CYLINDER only, else NULL */
 } dmw_geometry;
 
 /* Gradient waveform, Waveform.G of dmipy_sim/waveforms.py:39-90. */

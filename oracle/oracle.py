@@ -49,7 +49,9 @@ class OracleArgs(C.Structure):
                 ("inj_n", _fp), ("inj_u", _fp), ("cell_form", C.c_int32),
                 ("phase64", C.c_int32), ("threads", C.c_int32),
                 ("walker_begin", C.c_int64), ("walker_end", C.c_int64),
-                ("instance", C.c_uint32), ("axes", C.c_float * 3), ("aux", C.c_float)]
+                ("instance", C.c_uint32), ("axes", C.c_float * 3), ("aux", C.c_float),
+                ("m_step", C.c_float * 3), ("m_q2", C.c_float * 3), ("m_wf", C.c_float * 3),
+                ("m_k2", C.c_float * 4), ("m_n0", C.c_int64), ("m_n01", C.c_int64)]
 
 
 class OracleOut(C.Structure):
@@ -104,9 +106,46 @@ def _radius(g):
     return None
 
 
+def _myelin_plan(g, dt, T1):
+    """MyelinatedCylinder (make_myelin_step_fn, physics.py:455-512): one step per
+    sample, sqrt(float32(6) D_c dt) steps, 2 kappa / max(D_leaving, 1e-30),
+    dt/T2_c with 1e6 s for unset T2s, EPS/NUDGE of R_in; start allotment by
+    cross-section area with extra-axonal space bounded at 2 R_out
+    (geometries.py:921-946)."""
+    dt32 = f32(dt)
+    Ds = [f32(g.D_intra), f32(g.D_myelin), f32(g.D_extra)]
+    ki = f32(0.0) if g.kappa_inner is None else f32(g.kappa_inner)
+    ko = f32(0.0) if g.kappa_outer is None else f32(g.kappa_outer)
+    k2 = []
+    for k, Dl in ((ki, Ds[0]), (ki, Ds[1]), (ko, Ds[1]), (ko, Ds[2])):
+        k2.append(float(f32(2.0) * f32(k / max(Dl, f32(1e-30)))))
+    T2s = [g.T2_intra, g.T2_myelin, g.T2_extra]
+    Ri, Ro = g.inner_radius, g.outer_radius
+    area = [math.pi * Ri ** 2, math.pi * (Ro ** 2 - Ri ** 2), math.pi * ((2 * Ro) ** 2 - Ro ** 2)]
+    return dict(kind=10, wall=W_REFLECT, n_sub=1, rho_over_D=0.0, kappa_over_D=0.0, step_len=0.0,
+                gamma_dt_sub=float(f32(GAMMA * dt)), dt_sub=float(dt32), inv_T2=0.0,
+                inv_T1=float(f32(1.0 / T1)) if T1 is not None else 0.0,
+                has_t2=any(t is not None for t in T2s), has_t1=T1 is not None,
+                size=float(f32(Ri)), cell=0.0, eps=float(f32(1e-7 * Ri)),
+                nudge=float(f32(1e-4 * Ri)), centers=None, radii=None, axes=[0.0, 0.0, 0.0],
+                aux=float(f32(Ro)),
+                m_step=[float(np.sqrt(f32(f32(6.0) * d) * dt32)) for d in Ds],
+                m_q2=[float(f32(dt32 / f32(1e6 if t is None else t))) for t in T2s],
+                m_wf=[float(f32(w)) for w in g.water_fractions], m_k2=k2, m_area=area)
+
+
+def myelin_start_counts(p, N):
+    """Walkers starting intra-axonal and in the sheath (the rest start outside)."""
+    a = p["m_area"]
+    tot = a[0] + a[1] + a[2]
+    return int(round(N * a[0] / tot)), int(round(N * a[1] / tot))
+
+
 def oracle_plan(geom, D, dt, T2=None, T1=None):
     """Step constants of one walk, restated from the reference (see module doc)."""
     name = type(geom).__name__
+    if name == "MyelinatedCylinder":
+        return _myelin_plan(geom, dt, T1)
     kind = KIND[name]
     if name == "PermeableShell" and geom.kind == "cylinder":
         kind = 9
@@ -206,6 +245,15 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                    instance=int(instance), aux=float(p.get("aux", 0.0)))
     for k in range(3):
         A.axes[k] = float(p.get("axes", [0.0, 0.0, 0.0])[k])
+    if p["kind"] == 10:
+        for k in range(3):
+            A.m_step[k] = p["m_step"][k]
+            A.m_q2[k] = p["m_q2"][k]
+            A.m_wf[k] = p["m_wf"][k]
+        for k in range(4):
+            A.m_k2[k] = p["m_k2"][k]
+        n0, n1 = myelin_start_counts(p, N)
+        A.m_n0, A.m_n01 = n0, n0 + n1
     rows = 1 if echo is None else len(echo)
     res = {"signal": np.zeros(rows * n_meas), "signal_im": np.zeros(rows * n_meas),
            "final_pos": np.zeros((N, 3), np.float32), "comp_origin": np.zeros(N, np.int32),

@@ -71,7 +71,13 @@ typedef struct {
   int64_t walker_end;      /* 0 = all */
   uint32_t instance;       /* second Philox counter word (substrate instance) */
   float axes[3];           /* Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2 */
-  float aux;               /* Ellipsoid: 1 - NUDGE/min semi-axis; shell: R_out */
+  float aux;               /* Ellipsoid: 1 - NUDGE/min semi-axis; shell, myelin: R_out */
+  /* MyelinatedCylinder, per compartment (intra, sheath, extra) */
+  float m_step[3];         /* float32 sqrt(6 D_c dt) */
+  float m_q2[3];           /* float32 dt / T2_c */
+  float m_wf[3];           /* spin weights (water fractions) */
+  float m_k2[4];           /* 2 kappa / max(D_leaving, 1e-30): inner from 0, 1; outer from 1, 2 */
+  int64_t m_n0, m_n01;     /* start compartments by walker index */
 } oracle_args;
 
 typedef struct {
@@ -91,7 +97,7 @@ typedef struct {
 } oracle_out;
 
 enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5,
-       K_ELL = 6, K_SLAB = 7, K_SHELL = 8, K_SHELLC = 9 };
+       K_ELL = 6, K_SLAB = 7, K_SHELL = 8, K_SHELLC = 9, K_MYEL = 10 };
 enum { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
 /* ---------------------------------------------------------------- Philox */
@@ -658,6 +664,66 @@ static float shell_step(float r[3], const float st[3], float u, const oracle_arg
   return dlog;
 }
 
+/* ---------------------------------------------------------- MyelinatedCylinder
+   make_myelin_step_fn (physics.py:513-668) in the cylinder frame: the compartment's
+   step; crossing flags from the compartment and |r_new_xy|; transmission
+   probabilities min(1, 2 kappa/D_leaving |r - R|); inner mirror, then the outer
+   mirror on the recomputed norm; the four wrong-side clamps, each on a freshly
+   computed norm.  The reference draws separate uniforms for the two membranes; at
+   most one membrane is crossed per step, so one uniform u serves both here. */
+static float xy_norm(const float r[3]) { return sqrtf(r[0] * r[0] + r[1] * r[1]); }
+
+static void myel_clamp(float r[3], int cond, float Rt) {
+  const float n = xy_norm(r);
+  const float s = fmaxf(n, 1e-20f);
+  if (cond) { r[0] = (r[0] / s) * Rt; r[1] = (r[1] / s) * Rt; }
+}
+
+static void myel_step(float r[3], const float dir[3], float u, const oracle_args* A, int* comp,
+                      uint32_t* hits, uint32_t* ncross) {
+  const int c = *comp;
+  const float Rin = A->size, Rout = A->aux, NUDGE = A->nudge;
+  const float sl = c == 0 ? A->m_step[0] : (c == 1 ? A->m_step[1] : A->m_step[2]);
+  float rn[3] = {r[0] + dir[0] * sl, r[1] + dir[1] * sl, r[2] + dir[2] * sl};
+  const float n = xy_norm(rn);
+  const int cio = c == 0 && n >= Rin, cii = c == 1 && n < Rin;
+  const int coo = c == 1 && n >= Rout, coi = c == 2 && n < Rout;
+  const int ci = cio || cii, co = coo || coi;
+  const float p_in = fminf(1.0f, (c == 0 ? A->m_k2[0] : A->m_k2[1]) * fabsf(n - Rin));
+  const float p_out = fminf(1.0f, (c == 1 ? A->m_k2[2] : A->m_k2[3]) * fabsf(n - Rout));
+  const int t_in = ci && (u < p_in), t_out = co && (u < p_out);
+  if (ci || co) (*hits)++;
+  int nc = c;
+  {
+    const float s = fmaxf(n, 1e-20f);
+    const float hx = rn[0] / s, hy = rn[1] / s;
+    const float rr = 2.0f * Rin - n;
+    if (ci && !t_in) { rn[0] = hx * rr; rn[1] = hy * rr; }
+    if (t_in) nc = cio ? 1 : 0;
+  }
+  {
+    const float n2 = xy_norm(rn);
+    const float s = fmaxf(n2, 1e-20f);
+    const float hx = rn[0] / s, hy = rn[1] / s;
+    const float rr = 2.0f * Rout - n2;
+    if (co && !t_out) { rn[0] = hx * rr; rn[1] = hy * rr; }
+    if (t_out) nc = coo ? 2 : 1;
+  }
+  if (t_in || t_out) (*ncross)++;
+  myel_clamp(rn, nc == 0 && xy_norm(rn) >= Rin, Rin - NUDGE);
+  myel_clamp(rn, nc == 1 && xy_norm(rn) < Rin, Rin + NUDGE);
+  myel_clamp(rn, nc == 1 && xy_norm(rn) >= Rout, Rout - NUDGE);
+  myel_clamp(rn, nc == 2 && xy_norm(rn) < Rout, Rout + NUDGE);
+  r[0] = rn[0]; r[1] = rn[1]; r[2] = rn[2];
+  *comp = nc;
+}
+
+/* classify_position of MyelinatedCylinder (geometries.py:1035-1052). */
+static int myel_classify(const float r[3], const oracle_args* A) {
+  const float rr = r[0] * r[0] + r[1] * r[1];
+  return rr < A->size * A->size ? 0 : (rr < A->aux * A->aux ? 1 : 2);
+}
+
 /* Producer compartment occupancy rule (dmipy_sim/core.py:1166-1185): keyed on the
    geometry's `radius`; Cylinder by the axial-plane norm, Sphere / slab (radius L/2) /
    shells (radius R_in) by the 3-D norm; no radius (Ellipsoid, others): 0. */
@@ -725,6 +791,13 @@ static void sample_start(const oracle_args* A, uint32_t w, float r[3]) {
     } else if (A->kind == K_SLAB) {
       r[0] = u0 * (0.5f * A->size); r[1] = 0.0f; r[2] = 0.0f;
       return;
+    } else if (A->kind == K_MYEL) {
+      /* uniform in the start compartment's annulus, the extra-axonal one bounded
+         at 2 R_out (geometries.py:921-1008); compartment by walker index */
+      const int c0 = (int64_t)w < A->m_n0 ? 0 : ((int64_t)w < A->m_n01 ? 1 : 2);
+      const float rh = c0 == 0 ? A->size : (c0 == 1 ? A->aux : 2.0f * A->aux);
+      r[0] = (2.0f * u0 - 1.0f) * rh; r[1] = (2.0f * u1 - 1.0f) * rh; r[2] = 0.0f;
+      if (myel_classify(r, A) == c0 && r[0] * r[0] + r[1] * r[1] < rh * rh) return;
     } else if (A->kind == K_ELL) {
       const float b[3] = {2.0f * u0 - 1.0f, 2.0f * u1 - 1.0f, 2.0f * u2 - 1.0f};
       for (int k = 0; k < 3; ++k) r[k] = b[k] / sqrtf(A->axes[k]);
@@ -779,6 +852,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
   const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
   double* acc = (double*)calloc((size_t)rows * M * 2, sizeof(double));
   uint64_t hits_total = 0;
+  double wsum = 0.0;      /* sum of spin weights (_ens denominator, core.py:426-429) */
   if (!acc) return 1;
 #ifdef _OPENMP
   if (A->threads > 0) omp_set_num_threads(A->threads);
@@ -789,6 +863,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
     float* phi = (float*)malloc(sizeof(float) * (M > 0 ? M : 1));
     double* phi64 = (double*)malloc(sizeof(double) * (M > 0 ? M : 1));
     uint64_t hits_loc = 0;
+    double wsum_loc = 0.0;
 #pragma omp for schedule(dynamic, 64)
     for (int64_t w = w0; w < w1; ++w) {
       float r[3];
@@ -801,7 +876,10 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
           if (k != 0.0f) { r[d] = r[d] - L * k; cell[d] += (int)k; }
         }
       }
-      int comp = classify(r, A);
+      const int myel = A->kind == K_MYEL;
+      int mstate = myel ? myel_classify(r, A) : 0;   /* carried compartment */
+      const int c0 = mstate;
+      int comp = myel ? mstate : classify(r, A);
       if (O->comp_origin) O->comp_origin[w] = comp;
       for (int m = 0; m < M; ++m) { phi[m] = 0.0f; phi64[m] = 0.0; }
       float logw = 0.0f;
@@ -825,16 +903,22 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
               philox10(ctr, key[0], key[1], rnd);
             }
             oracle_direction((gs & 1) ? rnd[2] : rnd[0], (gs & 1) ? rnd[3] : rnd[1], dir);
-            if (A->wall == W_PERMEATE) {
+            if (A->wall == W_PERMEATE || myel) {
               const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)gs, 2u};
               uint32_t o[4];
               philox10(ctr, key[0], key[1], o);
               u = unit_u(o[0]);
             }
           }
-          float dlog = substep(r, dir, u, A, &hits, &ncross);
+          float dlog = 0.0f;
+          if (myel) myel_step(r, dir, u, A, &mstate, &hits, &ncross);
+          else dlog = substep(r, dir, u, A, &hits, &ncross);
           if (A->wall != W_REFLECT) dlog = dlog * chi;
-          if (A->has_t2) dlog = dlog - (A->dt_sub * A->inv_T2) * chi;
+          if (A->has_t2) {
+            /* myelin: dt / T2 of the compartment after the step (physics.py:657-659) */
+            const float q2 = myel ? A->m_q2[mstate] : A->dt_sub * A->inv_T2;
+            dlog = dlog - q2 * chi;
+          }
           if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
           logw = logw + dlog;
           dlog_sample = dlog_sample + dlog;
@@ -861,10 +945,10 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             O->pos_full[((size_t)w * T + t) * 3 + d] = (packed && A->cell_form) ? r[d] + L * (float)cell[d] : r[d];
         if (O->dlog_full) O->dlog_full[(size_t)w * T + t] = dlog_sample;
         if (O->occ_full) O->occ_full[(size_t)w * T + t] = occ / (float)NS;
-        if (O->comp_full || (O->comp_final && t == T - 1)) comp = classify(r, A);
+        if (O->comp_full || (O->comp_final && t == T - 1)) comp = myel ? mstate : classify(r, A);
         if (O->comp_full) O->comp_full[(size_t)w * T + t] = comp;
         if (e_next < A->n_echo && A->echo_idx[e_next] == t) {
-          const double wg = exp((double)logw);
+          const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0);
           for (int m = 0; m < M; ++m) {
             const double ph = A->phase64 ? phi64[m] : (double)phi[m];
             loc[((size_t)e_next * M + m) * 2] += wg * cos(ph);
@@ -874,7 +958,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
         }
       }
       if (A->n_echo == 0) {
-        const double wg = exp((double)logw);
+        const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0);
         for (int m = 0; m < M; ++m) {
           const double ph = A->phase64 ? phi64[m] : (double)phi[m];
           loc[m * 2] += wg * cos(ph);
@@ -890,17 +974,19 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       if (O->phi)
         for (int m = 0; m < M; ++m) O->phi[w * M + m] = A->phase64 ? (float)phi64[m] : phi[m];
       hits_loc += hits;
+      wsum_loc += A->kind == K_MYEL ? (double)A->m_wf[c0] : 1.0;
     }
 #pragma omp critical
     {
       for (int i = 0; i < rows * M * 2; ++i) acc[i] += loc[i];
       hits_total += hits_loc;
+      wsum += wsum_loc;
     }
     free(loc);
     free(phi);
     free(phi64);
   }
-  const double inv = 1.0 / (double)(w1 - w0);
+  const double inv = 1.0 / wsum;
   for (int i = 0; i < rows * M; ++i) {
     O->signal[i] = acc[2 * i] * inv;
     if (O->signal_im) O->signal_im[i] = acc[2 * i + 1] * inv;

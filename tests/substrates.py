@@ -66,6 +66,21 @@ CASES = [
 ]
 
 
+# MyelinatedCylinder runs on the single-walk path only (not the batched launch);
+# axial here, the rotated frame is covered in test_gpu_myelin.py.
+MYELIN_CASES = [
+    ("myel", lambda: ds.MyelinatedCylinder(3e-6, 5e-6, (0, 0, 1), D_intra=2e-9, D_extra=1.5e-9,
+                                           D_myelin=4e-10), None, None),
+    ("myel_perm", lambda: ds.MyelinatedCylinder(
+        3e-6, 5e-6, (0, 0, 1), D_intra=2e-9, D_extra=2e-9, D_myelin=5e-10, kappa_inner=5e-5,
+        kappa_outer=3e-5, T2_intra=0.08, T2_myelin=0.015, T2_extra=0.1), None, None),
+    ("myel_outer", lambda: ds.MyelinatedCylinder(
+        2e-6, 3.5e-6, (0, 0, 1), D_intra=2e-9, D_extra=2e-9, kappa_outer=1e-4,
+        T2_myelin=0.02, water_fractions=(1.0, 0.3, 0.8)), None, None),
+]
+WALK_CASES = CASES + MYELIN_CASES
+
+
 def start_positions(geom, n, seed):
     return geom.init_positions(n, seed)
 

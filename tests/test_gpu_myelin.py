@@ -1,0 +1,150 @@
+"""MyelinatedCylinder on the GPU: in-kernel start placement and walk bit-exact
+against the oracle, compartment tracking, and the reference's own physics checks
+(tests/test_myelin.py B-E, tests/test_compartment_tagging.py) at their sizes."""
+import numpy as np
+import pytest
+
+import dmipy_sim_amd as ds
+from substrates import small_pgse
+
+pytestmark = pytest.mark.gpu
+
+SEED = 42
+
+
+def _pgse_b(delta, Delta, n_t, direction, bvals):
+    wf = ds.pgse(delta, Delta, 1.0, np.asarray([direction], np.float32), n_t)
+    return ds.set_b(ds.tile_waveform(wf, len(bvals)), np.asarray(bvals, float))
+
+
+def _b0(n_t, TE):
+    return ds.Waveform(G=np.zeros((1, n_t, 3), np.float32), dt=TE / (n_t - 1), echo_idx=n_t - 1)
+
+
+def _geom(**kw):
+    base = dict(inner_radius=3e-6, outer_radius=5e-6, orientation=[0, 0, 1], D_intra=2e-9,
+                D_extra=2e-9)
+    base.update(kw)
+    return ds.MyelinatedCylinder(**base)
+
+
+@pytest.mark.parametrize("orientation", [(0, 0, 1), (0.6, 0.0, 0.8)], ids=["axial", "rotated"])
+def test_in_kernel_start_and_walk_bit_exact(gpu, oracle, orientation):
+    """No r0: both sides place walkers from the Philox stream with the area allotment
+    and walk them; positions, compartments, crossings and log-weights bit-exact,
+    spin-weighted signals within 1e-5."""
+    g = _geom(orientation=orientation, D_myelin=6e-10, kappa_inner=8e-5, kappa_outer=5e-5,
+              T2_intra=0.07, T2_myelin=0.015, T2_extra=0.09, water_fractions=(1.0, 0.4, 0.8))
+    wf = small_pgse()
+    plan = ds.make_walk_plan(g, None, wf.dt)
+    G_native = np.einsum('ij,mtj->mti', plan.rot.astype(np.float64), wf.G).astype(np.float32) \
+        if plan.rot is not None else None
+    N = 1500
+    print(f"myelin start+walk orientation={orientation} N={N}", flush=True)
+    r = gpu.walk(g, None, wf, N, seed=9, outputs=("final_pos", "comp_origin", "comp_final",
+                                                   "n_cross", "log_w"))
+    o = oracle.run_oracle(g, None, wf, N, seed=9, G_native=G_native)
+    pos = r["final_pos"]
+    if plan.rot is not None:
+        pos = (plan.rot.astype(np.float64) @ pos.T.astype(np.float64)).T
+        np.testing.assert_allclose(pos, o["final_pos"], rtol=0, atol=5e-12)
+    else:
+        np.testing.assert_array_equal(pos, o["final_pos"])
+    np.testing.assert_array_equal(r["comp_origin"], o["comp_origin"])
+    np.testing.assert_array_equal(r["comp_final"], o["comp_final"])
+    np.testing.assert_array_equal(r["n_cross"], o["n_cross"])
+    np.testing.assert_array_equal(r["log_w"], o["log_w"])
+    assert np.max(np.abs(r["signal"] - o["signal"])) <= 1e-5
+    n0, n1, n2 = g.start_counts(N)
+    np.testing.assert_array_equal(np.bincount(r["comp_origin"], minlength=3), [n0, n1, n2])
+    assert r["n_cross"].sum() > 0
+
+
+def test_compartment_tracking_and_volume_fractions(gpu):
+    """'final' equals the last sample of 'full'; impermeable walls keep every walker in
+    its start compartment; start fractions match volume_fraction within 0.01; with
+    permeable membranes walkers do change compartment
+    (reference tests/test_compartment_tagging.py:92-109, 312-343)."""
+    wf = ds.Waveform(G=np.zeros((2, 50, 3), np.float32), dt=1e-4, echo_idx=49)
+    mc = _geom(D_intra=1e-9)
+    _, o_full, c_full = gpu.simulate(500, None, wf, mc, seed=1, return_compartments="full")
+    _, o_fin, c_fin = gpu.simulate(500, None, wf, mc, seed=1, return_compartments="final")
+    np.testing.assert_array_equal(o_full, o_fin)
+    np.testing.assert_array_equal(c_full[:, -1], c_fin)
+    assert np.all(c_full == o_full[:, None])
+    mc1 = _geom(D_intra=1e-9, water_fractions=(1.0, 1.0, 1.0))
+    _, orig, _ = gpu.simulate(100_000, None, wf, mc1, seed=0, return_compartments="final")
+    for cid, label in enumerate(("intra", "myelin", "extra")):
+        assert abs((orig == cid).mean() - mc1.volume_fraction(label)) < 0.01
+    mcp = _geom(D_intra=1e-9, kappa_inner=1e-4, kappa_outer=1e-4)
+    _, orig, seq = gpu.simulate(2000, None, wf, mcp, seed=0, return_compartments="full")
+    assert np.any(seq != orig[:, None], axis=1).mean() > 0.01
+
+
+def test_intra_only_matches_cylinder(gpu):
+    """water_fractions (1, 0, 0) with impermeable membranes is the Cylinder(R_in)
+    signal, per b-value, within 4 combined MC sigma (reference test_myelin.py B)."""
+    n = 80_000
+    tol = max(4.0 * np.sqrt(2) / np.sqrt(n), 0.020)
+    mc = _geom(inner_radius=5e-6, outer_radius=7e-6, water_fractions=(1.0, 0.0, 0.0))
+    cyl = ds.Cylinder(5e-6, [0, 0, 1])
+    for k, d in enumerate(([1, 0, 0], [0, 0, 1])):
+        wf = _pgse_b(10e-3, 30e-3, 2000, d, [0.5e9, 1.0e9, 2.0e9])
+        s_m = gpu.simulate(n, None, wf, mc, seed=SEED + k)
+        s_c = gpu.simulate(n, 2e-9, wf, cyl, seed=SEED + k)
+        print(f"intra-only dir {d}: {s_m} vs cylinder {s_c}", flush=True)
+        assert np.all(np.abs(s_m - s_c) <= tol)
+
+
+def test_extra_only_short_time_is_free_along_axis(gpu):
+    """Extra-axonal walkers only, short Delta, gradient along the axis: ADC within 5 %
+    of D_extra (reference test_myelin.py C)."""
+    D_extra = 2e-9
+    mc = _geom(D_extra=D_extra, water_fractions=(0.0, 0.0, 1.0))
+    b = np.array([0.0, 0.3e9, 0.6e9, 1.0e9, 1.5e9])
+    wf = _pgse_b(1e-3, 2e-3, 2000, [0, 0, 1], b)
+    s = gpu.simulate(100_000, None, wf, mc, seed=SEED)
+    adc = -np.polyfit(b[1:], np.log(s[1:]) - np.log(s[0]), 1)[0]
+    assert abs(adc - D_extra) / D_extra < 0.05, adc
+
+
+def test_exchange_changes_signal(gpu):
+    """A highly permeable inner membrane (with mobile myelin water) attenuates the
+    perpendicular signal beyond 3 MC sigma relative to the impermeable axon
+    (reference test_myelin.py D)."""
+    kw = dict(D_myelin=2e-9, water_fractions=(1.0, 0.0, 0.0))
+    wf = _pgse_b(10e-3, 40e-3, 2000, [1, 0, 0], [1.0e9, 2.0e9])
+    n = 100_000
+    s_no = gpu.simulate(n, None, wf, _geom(**kw), seed=SEED)
+    s_ex = gpu.simulate(n, None, wf, _geom(kappa_inner=1.0, **kw), seed=SEED)
+    assert np.any(np.abs(s_no - s_ex) > 3.0 / np.sqrt(n)), (s_no, s_ex)
+    assert np.all(s_ex < s_no)
+
+
+@pytest.mark.parametrize("TE", [20e-3, 40e-3, 60e-3, 80e-3])
+def test_t2_weighted_b0_mixture(gpu, TE):
+    """b = 0, impermeable: S = sum_c f_c exp(-TE/T2_c) with f_c the area x water
+    fraction weights (reference test_myelin.py E), within 0.01."""
+    Ri, Ro = 3e-6, 5e-6
+    T2 = np.array([80e-3, 15e-3, 100e-3])
+    w = np.array([1.0, 0.15, 1.0])
+    vol = np.array([Ri ** 2, Ro ** 2 - Ri ** 2, (2 * Ro) ** 2 - Ro ** 2]) * w
+    f = vol / vol.sum()
+    mc = _geom(T2_intra=T2[0], T2_myelin=T2[1], T2_extra=T2[2], water_fractions=tuple(w))
+    s = gpu.simulate(100_000, None, _b0(2000, TE), mc, seed=SEED)
+    expect = float(np.sum(f * np.exp(-TE / T2)))
+    assert abs(s[0] - expect) < max(0.01, 3.0 / np.sqrt(100_000)), (s[0], expect)
+
+
+def test_myelin_only_single_exponential_and_unity(gpu):
+    """All weight in the (stuck) sheath: every walker decays by dt/T2_myelin at each of
+    the n_t samples, S = exp(-n_t dt/T2_myelin) (to the float32 accumulation of the
+    log-weight, 1e-4 relative); without T2 the b = 0 signal is 1 for any weighting."""
+    TE, n_t = 30e-3, 2000
+    wf = _b0(n_t, TE)
+    mc = _geom(T2_myelin=15e-3, water_fractions=(0.0, 1.0, 0.0))
+    s = gpu.simulate(100_000, None, wf, mc, seed=SEED)
+    expect = np.exp(-n_t * wf.dt / 15e-3)
+    assert abs(s[0] - expect) < 1e-4 * expect
+    s1 = gpu.simulate(20_000, None, _b0(500, TE), _geom(water_fractions=(0.3, 0.5, 0.2)), seed=1)
+    assert abs(s1[0] - 1.0) < 1e-6

@@ -8,12 +8,12 @@ import pytest
 
 import dmipy_sim_amd as ds
 from dmipy_sim_amd import _native
-from substrates import CASES, D
+from substrates import CASES, WALK_CASES, D
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.mark.parametrize("name,factory,T2,T1", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("name,factory,T2,T1", WALK_CASES, ids=[c[0] for c in WALK_CASES])
 @pytest.mark.parametrize("dt", [1e-4, 2.5e-5, 5.005e-5])
 def test_plan_constants_match_oracle_restatement(oracle, name, factory, T2, T1, dt):
     """The product's plan and the oracle's independent restatement of
@@ -26,6 +26,13 @@ def test_plan_constants_match_oracle_restatement(oracle, name, factory, T2, T1, 
               "inv_T2", "inv_T1", "eps", "nudge", "size", "cell", "aux"):
         assert np.float32(getattr(p, k)) == np.float32(o[k]), k
     assert [np.float32(v) for v in p.axes] == [np.float32(v) for v in o["axes"]]
+    if p.myelin is not None:
+        for pk, ok in (("step_len", "m_step"), ("two_kappa_over_D", "m_k2"),
+                       ("dt_over_T2", "m_q2"), ("spin_weight", "m_wf")):
+            assert [np.float32(v) for v in p.myelin[pk]] == [np.float32(v) for v in o[ok]], pk
+        for N in (1, 7, 1000, 12345):
+            n0, n1, _ = p.myelin["start_counts"](N)
+            assert (n0, n1) == oracle.myelin_start_counts(o, N)
 
 
 def test_substep_rules_closed_form():
@@ -281,3 +288,19 @@ def test_autotune_candidates_and_cache(tmp_path):
     autotune.load(str(path))
     assert autotune.cached_block(plan, 128, False) == 64
     autotune.clear()
+
+
+def test_myelinated_cylinder_unsupported_paths_raise():
+    """Full position export and the replay engine are not served for the
+    myelinated cylinder (core.py:91, 459-463); the batched launch and the
+    walk-once producer refuse it too.  Raised before any GPU work."""
+    mc = ds.MyelinatedCylinder(3e-6, 5e-6, [0, 0, 1], D_intra=2e-9, D_extra=2e-9)
+    wf = ds.Waveform(G=np.zeros((1, 20, 3), np.float32), dt=1e-4, echo_idx=19)
+    with pytest.raises(NotImplementedError, match="return_positions='full'"):
+        ds.simulate(100, None, wf, mc, return_positions="full")
+    with pytest.raises(NotImplementedError, match="dedicated fused step kernel"):
+        ds.simulate(100, None, wf, mc, engine="replay")
+    with pytest.raises(ValueError):
+        ds.MyelinatedCylinder(5e-6, 3e-6, [0, 0, 1], D_intra=2e-9, D_extra=2e-9)
+    assert mc.water_fractions == (1.0, 0.4, 1.0) and mc.D_myelin == 0.0
+    assert abs(sum(mc.volume_fraction(c) for c in ("intra", "myelin", "extra")) - 1.0) < 1e-12

@@ -406,3 +406,78 @@ def test_producer_occupancy_is_mean_of_sub_step_compartments(oracle):
     res4 = oracle.run_oracle(geom, D, wf, 1000, seed=2, plan=p4, per_sample=True)
     occ = res4["occ_full"] * 4
     np.testing.assert_array_equal(occ, np.round(occ))
+
+
+# ------------------------------------------------------------ MyelinatedCylinder
+def _myel(**kw):
+    base = dict(inner_radius=2e-6, outer_radius=3.5e-6, orientation=(0, 0, 1), D_intra=2e-9,
+                D_extra=2e-9, D_myelin=1e-9)
+    base.update(kw)
+    return ds.MyelinatedCylinder(**base)
+
+
+@pytest.mark.parametrize("u,transmit", [(0.0, True), (0.9999999, False)])
+def test_myelin_membranes_transmit_or_mirror(oracle, u, transmit):
+    """Crossing R_in outward from the axon and R_out inward from outside: transmitted
+    straight on (compartment changes), or mirrored radially to 2R - |r_new|
+    (physics.py:560-611)."""
+    g = _myel(kappa_inner=1e-4, kappa_outer=1e-4)
+    p = oracle.oracle_plan(g, None, _wave().dt)
+    Ri, Ro = np.float32(g.inner_radius), np.float32(g.outer_radius)
+    s0, s2 = np.float32(p["m_step"][0]), np.float32(p["m_step"][2])
+    x0 = float(Ri - np.float32(0.3) * s0)
+    res, _ = _one_step(oracle, g, [x0, 0.0, 0.0], [1.0, 0.0, 0.0], u=u)
+    xn = np.float32(x0) + s0
+    if transmit:
+        assert res["final_pos"][0, 0] == xn and res["comp_final"][0] == 1 and res["n_cross"][0] == 1
+    else:
+        assert res["final_pos"][0, 0] == np.float32(2.0) * Ri - xn and res["comp_final"][0] == 0
+    x1 = float(Ro + np.float32(0.3) * s2)
+    res, _ = _one_step(oracle, g, [x1, 0.0, 0.0], [-1.0, 0.0, 0.0], u=u)
+    xn = np.float32(x1) - s2
+    if transmit:
+        assert res["final_pos"][0, 0] == xn and res["comp_final"][0] == 1
+    else:
+        assert res["final_pos"][0, 0] == np.float32(2.0) * Ro - xn and res["comp_final"][0] == 2
+
+
+def test_myelin_overshoot_mirror_then_clamp(oracle):
+    """A step of 3.5 R_in from the axis mirrors to radius -1.5 R_in (through the axis),
+    which the compartment-0 clamp pulls back to R_in - NUDGE (physics.py:621-631)."""
+    Ri = 1e-6
+    dt = _wave().dt
+    D = (3.5 * Ri) ** 2 / (6.0 * dt)
+    g = _myel(inner_radius=Ri, outer_radius=2e-6, D_intra=D)
+    p = oracle.oracle_plan(g, None, dt)
+    s0 = np.float32(p["m_step"][0])
+    assert abs(float(s0) / Ri - 3.5) < 1e-6
+    res, _ = _one_step(oracle, g, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], u=0.5)
+    target = np.float32(Ri) - np.float32(p["nudge"])
+    assert res["final_pos"][0, 0] == -target and res["final_pos"][0, 1] == 0.0
+    assert res["comp_final"][0] == 0
+
+
+def test_myelin_start_allotment_stuck_pool_and_weighted_t2(oracle):
+    """In-kernel starts: compartment counts are round(N A_c / A) by cross-section
+    area (extra-axonal bounded at 2 R_out); a D_myelin = 0 pool never moves; with
+    impermeable membranes the b = 0 signal is the spin-weighted mixture
+    sum_c n_c w_c exp(-n_t dt/T2_c) / sum_c n_c w_c (_ens, core.py:426-429)."""
+    g = _myel(D_myelin=0.0, T2_intra=0.08, T2_myelin=0.012, T2_extra=0.1,
+              water_fractions=(1.0, 0.4, 0.7))
+    n_t, dt, N = 60, 2e-4, 3000
+    wf = ds.Waveform(G=np.zeros((1, n_t, 3), np.float32), dt=dt, echo_idx=n_t - 1)
+    res = oracle.run_oracle(g, None, wf, N, seed=4, per_sample=True)
+    Ri, Ro = g.inner_radius, g.outer_radius
+    A = np.array([Ri ** 2, Ro ** 2 - Ri ** 2, 4 * Ro ** 2 - Ro ** 2])
+    n0, n1 = int(round(N * A[0] / A.sum())), int(round(N * A[1] / A.sum()))
+    counts = np.array([n0, n1, N - n0 - n1])
+    np.testing.assert_array_equal(np.bincount(res["comp_origin"], minlength=3), counts)
+    assert np.all(res["comp_origin"][:n0] == 0) and np.all(res["comp_origin"][n0:n0 + n1] == 1)
+    sheath = res["comp_origin"] == 1
+    pos = res["pos_full"][sheath]
+    assert np.all(pos == pos[:, :1, :])
+    assert np.all(res["comp_final"] == res["comp_origin"])
+    w = np.array([1.0, 0.4, 0.7])
+    T2 = np.array([0.08, 0.012, 0.1])
+    expect = np.sum(counts * w * np.exp(-n_t * dt / T2)) / np.sum(counts * w)
+    assert abs(res["signal"][0] - expect) < 2e-6 * expect
