@@ -15,6 +15,9 @@
 //   PackedCyl.     geometries.py:1746-2016 (reflect*), 2018-2183 (permeate),
 //                  2185-2214 (classify)
 //   PackedSph.     geometries.py:2445-2631, 2633-2756, 2758-2771
+//   Ellipsoid      geometries.py:1137-1425
+//   Slab / Shell   geometries.py:3221-3300, 3303-3431
+//   Myelinated     physics.py:420-679 (make_myelin_step_fn), geometries.py:847-1136
 //   ensemble       core.py:426-429, stimulated echo 734-735 (host)
 // with these deliberate differences, none of which changes a position or a wall
 // decision:
