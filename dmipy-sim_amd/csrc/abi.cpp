@@ -114,12 +114,12 @@ struct dmw_plan {
   unsigned grid_x = 0;
   size_t lds = 0;
   dmw::WalkArgs args{};
-  DevBuf Gs, GsT, chi, echo, objs, cstart, citems, r0, inj_n, inj_u;
+  DevBuf Gs, GsT, chi, echo, objs, cstart, citems, r0, inj_n, inj_u, pm_start;
   DevBuf partials, echo_partials, sig;
   DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~dmw_plan() {
-    for (DevBuf* b : {&Gs, &GsT, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &partials,
+    for (DevBuf* b : {&Gs, &GsT, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &pm_start, &partials,
                       &echo_partials, &sig, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
                       &o_logw, &o_phi, &o_counters})
       b->release();
@@ -167,7 +167,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (wave->n_meas < 1 || wave->n_t < 1 || !wave->G) return fail(DMW_EINVAL, "empty waveform");
   if (run->n_sub < 1) return fail(DMW_EINVAL, "n_sub must be >= 1");
   if ((long long)wave->n_t * run->n_sub >= (1LL << 32)) return fail(DMW_EINVAL, "n_t * n_sub too large");
-  if (geom->kind < DMW_FREE || geom->kind > DMW_MYELINATED_CYLINDER) return fail(DMW_EINVAL, "unknown geometry kind");
+  if (geom->kind < DMW_FREE || geom->kind > DMW_PACKED_MYELINATED) return fail(DMW_EINVAL, "unknown geometry kind");
   if (geom->wall < DMW_WALL_REFLECT || geom->wall > DMW_WALL_PERMEATE) return fail(DMW_EINVAL, "unknown wall mode");
   if (geom->kind == DMW_FREE && geom->wall != DMW_WALL_REFLECT)
     return fail(DMW_EUNSUPPORTED, "FreeDiffusion has no wall");
@@ -191,6 +191,27 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     if (!run->r0 && !(0 <= ns[0] && ns[0] <= ns[1] && ns[1] <= run->n_walkers))
       return fail(DMW_EINVAL, "MyelinatedCylinder start counts must satisfy 0 <= n0 <= n01 <= n_walkers");
   }
+  const bool pmyel = geom->kind == DMW_PACKED_MYELINATED;
+  if (pmyel) {
+    const dmw_packed_myelin* pm = geom->packed_myelin;
+    if (!pm || !pm->inner_radii || !pm->step_len || !pm->two_kappa_over_D || !pm->start)
+      return fail(DMW_EINVAL, "PackedMyelinatedCylinders needs its per-axon tables");
+    if (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f))
+      return fail(DMW_EINVAL, "PackedMyelinatedCylinders needs axons and a positive cell side");
+    if (geom->wall != DMW_WALL_REFLECT)
+      return fail(DMW_EUNSUPPORTED, "PackedMyelinatedCylinders handles its walls itself (wall must be reflect)");
+    if (run->r0)
+      return fail(DMW_EUNSUPPORTED, "simulate(r0=...) is unsupported for PackedMyelinatedCylinders; it "
+                                    "initialises walkers (and their compartments) from `seed`");
+    for (int k = 0; k < geom->n_obj; ++k)
+      if (!(pm->inner_radii[k] > 0.0f && geom->radii[k] > pm->inner_radii[k]))
+        return fail(DMW_EINVAL, "PackedMyelinatedCylinders needs 0 < inner < outer radius per axon");
+    const int64_t* st0 = pm->start;
+    if (st0[0] != 0) return fail(DMW_EINVAL, "start boundaries must begin at 0");
+    for (int j = 0; j < 2 * geom->n_obj; ++j)
+      if (st0[j + 1] < st0[j]) return fail(DMW_EINVAL, "start boundaries must be ascending");
+    if (st0[2 * geom->n_obj] > run->n_walkers) return fail(DMW_EINVAL, "start boundaries exceed n_walkers");
+  }
   const bool packed = geom->kind == DMW_PACKED_CYLINDERS || geom->kind == DMW_PACKED_SPHERES;
   if (packed && (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f)))
     return fail(DMW_EINVAL, "packed substrate needs objects and a positive cell side");
@@ -202,7 +223,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   }
   const long long N = run->n_walkers;
   const long long S = (long long)wave->n_t * run->n_sub;
-  if (run->inj_normals && (geom->wall == DMW_WALL_PERMEATE || myel) && !run->inj_uniforms)
+  if (run->inj_normals && (geom->wall == DMW_WALL_PERMEATE || myel || pmyel) && !run->inj_uniforms)
     return fail(DMW_EINVAL, "injected walk on a permeable wall needs injected uniforms");
 
   hipStream_t st;
@@ -212,6 +233,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   dmw_plan* P = new dmw_plan();
   P->geom = *geom;
   P->geom.myelin = nullptr;
+  P->geom.packed_myelin = nullptr;
   P->run = *run;
   P->inv_norm = 1.0 / (double)N;
   P->n_meas = wave->n_meas;
@@ -300,6 +322,44 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     P->inv_norm = 1.0 / wsum;
   }
   a.n_obj = 0;
+  if (pmyel) {
+    const dmw_packed_myelin& pm = *geom->packed_myelin;
+    const int n = geom->n_obj;
+    float smax = 0.0f;
+    std::vector<float> rows((size_t)n * 12, 0.0f);
+    for (int k = 0; k < n; ++k) {
+      float* r0 = &rows[(size_t)k * 4];
+      r0[0] = geom->centers[k * 2];
+      r0[1] = geom->centers[k * 2 + 1];
+      r0[2] = pm.inner_radii[k];
+      r0[3] = geom->radii[k];
+      float* r1 = &rows[((size_t)n + k) * 4];
+      for (int c = 0; c < 3; ++c) {
+        r1[c] = pm.step_len[k * 3 + c];
+        smax = std::max(smax, r1[c]);
+      }
+      float* r2 = &rows[((size_t)2 * n + k) * 4];
+      for (int c = 0; c < 4; ++c) r2[c] = pm.two_kappa_over_D[k * 4 + c];
+    }
+    Grid G = build_grid(*geom, smax, 2);
+    TRY(upload(P->objs, rows.data(), rows.size() * 4, st));
+    TRY(upload(P->cstart, G.start.data(), G.start.size() * 4, st));
+    TRY(upload(P->citems, G.items.data(), std::max<size_t>(1, G.items.size()) * 4, st));
+    TRY(upload(P->pm_start, pm.start, ((size_t)2 * n + 1) * 8, st));
+    a.n_obj = n;
+    a.grid_n = G.n;
+    a.grid_h = G.h;
+    a.grid_inv_h = 1.0f / G.h;
+    a.n_items = (int)G.items.size();
+    a.exact_minimage = 1;
+    P->lds = (size_t)n * 48 + (G.start.size() + G.items.size()) * 4;
+    for (int c = 0; c < 3; ++c) a.pm_q2[c] = pm.dt_over_T2[c];
+    a.pm_rhoD = pm.rho_over_D;
+    a.pm_wmy = pm.myelin_weight;
+    a.pm_start = P->pm_start.as<long long>();
+    const long long n_my = pm.start[2 * n] - pm.start[n];
+    P->inv_norm = 1.0 / ((double)(N - n_my) + (double)n_my * (double)pm.myelin_weight);
+  }
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
     Grid G = build_grid(*geom, run->step_len, dim);
@@ -399,6 +459,7 @@ static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid,
     case DMW_PERMEABLE_SHELL: return dmw::launch_walk_shell(wall, mb, kahan, grid, lds, st, a);
     case DMW_PERMEABLE_SHELL_CYL: return dmw::launch_walk_shellc(wall, mb, kahan, grid, lds, st, a);
     case DMW_MYELINATED_CYLINDER: return dmw::launch_walk_myel(wall, mb, kahan, grid, lds, st, a);
+    case DMW_PACKED_MYELINATED: return dmw::launch_walk_pmyel(wall, mb, kahan, grid, lds, st, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -541,6 +602,8 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
   if (N < 1) return fail(DMW_EINVAL, "n_walkers must be >= 1");
   if (geom->kind == DMW_MYELINATED_CYLINDER)
     return fail(DMW_EUNSUPPORTED, "MyelinatedCylinder uses a dedicated fused step kernel (no replay walk)");
+  if (geom->kind == DMW_PACKED_MYELINATED)
+    return fail(DMW_EUNSUPPORTED, "PackedMyelinatedCylinders trajectories are not provided by this producer");
   const bool has_radius = geom->kind == DMW_CYLINDER || geom->kind == DMW_SPHERE ||
                           geom->kind == DMW_PERMEABLE_SLAB || geom->kind == DMW_PERMEABLE_SHELL ||
                           geom->kind == DMW_PERMEABLE_SHELL_CYL;

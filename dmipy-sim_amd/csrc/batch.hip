@@ -271,8 +271,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   const bool have_chi = sets[0].chi != nullptr;
   for (int i = 0; i < n_inst; ++i) {
     const dmw_instance& I = inst[i];
-    if (I.geom.kind == DMW_MYELINATED_CYLINDER)
-      return dmw::batch_fail(DMW_EUNSUPPORTED, "MyelinatedCylinder runs on the single-walk path, not the batched launch");
+    if (I.geom.kind == DMW_MYELINATED_CYLINDER || I.geom.kind == DMW_PACKED_MYELINATED)
+      return dmw::batch_fail(DMW_EUNSUPPORTED, "myelinated geometries run on the single-walk path, not the batched launch");
     if (I.geom.kind < DMW_FREE || I.geom.kind > DMW_PERMEABLE_SHELL_CYL) return dmw::batch_fail(DMW_EINVAL, "unknown geometry kind");
     if (I.geom.wall < 0 || I.geom.wall > 2 || I.n_walkers < 1 || I.n_sub < 1 || I.wave_set < 0 || I.wave_set >= n_sets)
       return dmw::batch_fail(DMW_EINVAL, "bad instance");

@@ -54,7 +54,8 @@ enum GeomKind {
   G_SLAB = 7,   // PermeableSlab1D: size = L, membrane at L/2
   G_SHELL = 8,  // PermeableShell, spherical: size = R_in, aux = R_out
   G_SHELLC = 9, // PermeableShell, cylindrical (axis z in the native frame)
-  G_MYEL = 10   // MyelinatedCylinder: size = R_in, aux = R_out (axis z in the native frame)
+  G_MYEL = 10,  // MyelinatedCylinder: size = R_in, aux = R_out (axis z in the native frame)
+  G_PMYEL = 11  // PackedMyelinatedCylinders: objs rows (cx, cy, R_in, R_out), steps, 2 kappa/D
 };
 enum WallKind { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
@@ -94,6 +95,10 @@ struct WalkArgs {
   // outer from 2); walkers [0, m_n0) start intra, [m_n0, m_n01) in the sheath.
   float m_step[3], m_q2[3], m_wf[3], m_k2[4];
   long long m_n0, m_n01;
+  // PackedMyelinatedCylinders: dt_sub/T2 per class (intra, myelin, extra), wall
+  // relaxivity / D_ref, myelin spin weight, start boundaries (2 n_obj + 1, device).
+  float pm_q2[3], pm_rhoD, pm_wmy;
+  const long long* pm_start;
   const float* r0;           // device (3, N) or null
   const float* inj_n;        // device (S, 3, N) or null, S = n_t * n_sub
   const float* inj_u;        // device (S, N) or null
@@ -823,6 +828,157 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
   return (-2.0f * a.rhoD) * dperp;
 }
 
+// ---------------------------------------------------------------------------
+// PackedMyelinatedCylinders (make_packed_myelin_traj_step_fn, physics.py:682-978,
+// without magnetisation transfer, as the fused forward calls it).  Walker
+// compartment: Pos::cx = class (0 intra, 1 myelin, 2 extra), Pos::cy = axon
+// (0 for extra-axonal walkers, whose membrane tests use axon 0 as the reference's
+// k_cyl does), Pos::cz = start class.  LDS rows per axon k: objs[k] = (cx, cy,
+// R_in, R_out), objs[n + k] = step lengths (intra, myelin, extra), objs[2n + k] =
+// 2 kappa/D (inner from intra, inner from myelin, outer from myelin, outer from
+// extra).  Minimum images use true division, as the reference does.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pm_wrap(float q, float L) { return q - L * floorf(q / L + 0.5f); }
+
+__device__ __forceinline__ Pos pm_point(float x, float y) { return Pos{x, y, 0.0f, 0, 0, 0}; }
+
+// The exterior clamp's target: argmin over the axons listed for the cell of (x, y)
+// of d^2/R_out^2 among axons whose outer disc contains the point (first index on
+// ties; -1 when none does), physics.py:872-882.
+__device__ __forceinline__ int pm_nearest_container(float x, float y, const WalkArgs& a,
+                                                    const Tables& tb, uint32_t& ncand) {
+  const float L = a.cell;
+  const int gc = cell_of<2>(pm_point(x, y), a);
+  const int beg = tb.start[gc], end = tb.start[gc + 1];
+  ncand += (uint32_t)(end - beg);
+  float best = 1.0f;
+  int kb = -1;
+  for (int i = beg; i < end; ++i) {
+    const int j = tb.items[i];
+    const float4 g = tb.objs[j];
+    const float fx = pm_wrap(x - g.x, L), fy = pm_wrap(y - g.y, L);
+    const float d2 = fx * fx + fy * fy;
+    const float RR = g.w * g.w;
+    const float pen = (g.w > 0.0f) ? (d2 < RR ? d2 / (RR + 1e-30f) : 1.0f) : 1.0f;
+    if (pen < best) { best = pen; kb = j; }
+  }
+  return kb;
+}
+
+// One sub-step; returns the unit (rho/D = 1) boundary log-weight.
+__device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz, const WalkArgs& a,
+                                            const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
+                                            uint32_t& ncross, uint32_t& ncand) {
+  const int n = a.n_obj;
+  const float L = a.cell, NUDGE = a.nudge;
+  const int cls = p.cx, k = p.cy;
+  const float4 geo = tb.objs[k];
+  const float4 stp = tb.objs[n + k];
+  const float4 k2 = tb.objs[2 * n + k];
+  const bool is_intra = cls == 0, is_myelin = cls == 1, is_extra = cls == 2;
+  const float sl = is_intra ? stp.x : (is_myelin ? stp.y : stp.z);
+  const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
+  const float Rin = geo.z, Rout = geo.w;
+  float qx = pm_wrap((p.x + sx) - geo.x, L), qy = pm_wrap((p.y + sy) - geo.y, L);
+  const float nrm = sqrtf(qx * qx + qy * qy);
+  const bool cio = is_intra && nrm >= Rin, cii = is_myelin && nrm < Rin;
+  const bool coo = is_myelin && nrm >= Rout, coi = is_extra && nrm < Rout;
+  const bool ci = cio || cii, co = coo || coi;
+  const float dpi = fabsf(nrm - Rin), dpo = fabsf(nrm - Rout);
+  bool t_in = false, t_out = false;
+  if (ci || co) {
+    hits++;
+    const float u = ps.get();
+    if (ci) t_in = u < fminf(1.0f, (is_intra ? k2.x : k2.y) * dpi);
+    else t_out = u < fminf(1.0f, (is_myelin ? k2.z : k2.w) * dpo);
+  }
+  int ncls = cls, nk = k;
+  float dlog = 0.0f;
+  // inner membrane: transmit, or mirror radially to 2 R_in - |q| (physics.py:812-830)
+  if (ci) {
+    if (t_in) {
+      ncls = cio ? 1 : 0;
+    } else {
+      const float s = fmaxf(nrm, 1e-20f);
+      const float hx = qx / s, hy = qy / s;
+      const float rr = 2.0f * Rin - nrm;
+      qx = hx * rr;
+      qy = hy * rr;
+      dlog = dlog + (-2.0f * dpi);
+    }
+  }
+  // outer membrane, on the recomputed norm (physics.py:832-851)
+  if (co) {
+    if (t_out) {
+      ncls = coo ? 2 : 1;
+      nk = coo ? 0 : k;
+    } else {
+      const float n2 = sqrtf(qx * qx + qy * qy);
+      const float s = fmaxf(n2, 1e-20f);
+      const float hx = qx / s, hy = qy / s;
+      const float rr = 2.0f * Rout - n2;
+      qx = hx * rr;
+      qy = hy * rr;
+      dlog = dlog + (-2.0f * dpo);
+    }
+  }
+  if (t_in || t_out) ncross++;
+  float xa = pm_wrap(qx + geo.x, L), ya = pm_wrap(qy + geo.y, L);
+  // extra-axonal walkers: mirror off the outer wall of the nearest containing axon
+  // (physics.py:853-890)
+  if (is_extra) {
+    const int kc = pm_nearest_container(xa, ya, a, tb, ncand);
+    if (kc >= 0) {
+      const float4 gc = tb.objs[kc];
+      const float cqx = pm_wrap(xa - gc.x, L), cqy = pm_wrap(ya - gc.y, L);
+      const float dcl = sqrtf(cqx * cqx + cqy * cqy);
+      const float cnx = xa - cqx, cny = ya - cqy;
+      const float drefl = 2.0f * gc.w - dcl;
+      const float den = fmaxf(dcl, NUDGE);
+      xa = cnx + (cqx * drefl) / den;
+      ya = cny + (cqy * drefl) / den;
+      dlog = dlog + (-2.0f * fmaxf(gc.w - dcl, 0.0f));
+    }
+  }
+  // wrong-side clamps of untransmitted intra / myelin walkers (physics.py:892-916)
+  const bool same = !(t_in || t_out);
+  if (same && (is_intra || is_myelin)) {
+    float ex = pm_wrap(xa - geo.x, L), ey = pm_wrap(ya - geo.y, L);
+    float de = sqrtf(ex * ex + ey * ey);
+    if (is_intra && de >= Rin) {
+      const float s = fmaxf(de, 1e-20f);
+      xa = geo.x + (ex / s) * (Rin - NUDGE);
+      ya = geo.y + (ey / s) * (Rin - NUDGE);
+    }
+    if (is_myelin && de < Rin) {
+      const float s = fmaxf(de, 1e-20f);
+      xa = geo.x + (ex / s) * (Rin + NUDGE);
+      ya = geo.y + (ey / s) * (Rin + NUDGE);
+      ex = pm_wrap(xa - geo.x, L);
+      ey = pm_wrap(ya - geo.y, L);
+      de = sqrtf(ex * ex + ey * ey);
+    }
+    if (is_myelin && de >= Rout) {
+      const float s = fmaxf(de, 1e-20f);
+      xa = geo.x + (ex / s) * (Rout - NUDGE);
+      ya = geo.y + (ey / s) * (Rout - NUDGE);
+    }
+  }
+  // intra / myelin walkers are re-classified against their own axon (physics.py:926-949)
+  if (!is_extra) {
+    const float fx = pm_wrap(xa - geo.x, L), fy = pm_wrap(ya - geo.y, L);
+    const float d2 = fx * fx + fy * fy;
+    if (d2 < Rin * Rin) { ncls = 0; nk = k; }
+    else if (d2 < Rout * Rout) { ncls = 1; nk = k; }
+  }
+  p.x = xa;
+  p.y = ya;
+  p.z = p.z + sz;
+  p.cx = ncls;
+  p.cy = nk;
+  return dlog;
+}
+
 // Compartment of a position (classify_position): 0/1 for Cylinder and Sphere
 // (geometries.py:180-184, 803-813), 0 or the largest 1-based index of a
 // containing object for packed substrates (geometries.py:2185-2214, 2758-2771).
@@ -840,8 +996,8 @@ __device__ __forceinline__ int classify(const Pos& p, const WalkArgs& a, const T
     return p.x < 0.5f * a.size ? 0 : 1;
   } else if (GEOM == G_SHELL || GEOM == G_SHELLC) {
     return shell_norm<GEOM == G_SHELLC>(p.x, p.y, p.z) < a.size ? 0 : 1;
-  } else if (GEOM == G_MYEL) {
-    return p.cx;  // the carried compartment (core.py:486-530)
+  } else if (GEOM == G_MYEL || GEOM == G_PMYEL) {
+    return p.cx;  // the carried compartment (core.py:486-530; packed: class, core.py:549-570)
   } else if (GEOM == G_PCYL || GEOM == G_PSPH) {
     constexpr int DIM = (GEOM == G_PCYL) ? 2 : 3;
     const int gc = cell_of<DIM>(p, a);
@@ -938,6 +1094,25 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
   const float r_hi = (GEOM == G_MYEL) ? pick3(c0, a.size, a.aux, 2.0f * a.aux) : 0.0f;
   p.cx = c0;
   p.cy = c0;
+  // PackedMyelinatedCylinders: segment j of the start boundaries is intra of axon j
+  // (j < n), myelin of axon j - n (j < 2n), extra-axonal past the last boundary
+  // (geometries.py:3112-3210: area allotment in this order).
+  int pm_cls = 2, pm_k = 0;
+  if (GEOM == G_PMYEL) {
+    const int n2 = 2 * a.n_obj;
+    if ((long long)w < a.pm_start[n2]) {
+      int lo = 0, hi = n2;                       // first j with start[j] > w, minus 1
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (a.pm_start[mid] <= (long long)w) lo = mid; else hi = mid;
+      }
+      pm_cls = lo < a.n_obj ? 0 : 1;
+      pm_k = lo < a.n_obj ? lo : lo - a.n_obj;
+    }
+    p.cx = pm_cls;
+    p.cy = pm_k;
+    p.cz = pm_cls;
+  }
   for (uint32_t attempt = 0; attempt < 4096u; ++attempt) {
     const u32x4 r = philox4x32_10({w, a.instance, attempt, TAG_INIT}, a.seed_lo, a.seed_hi);
     const float u0 = u01(r.x), u1 = u01(r.y), u2 = u01(r.z);
@@ -965,6 +1140,31 @@ __device__ __forceinline__ Pos sample_start(uint32_t w, const WalkArgs& a, const
       p.x = (2.0f * u0 - 1.0f) * r_hi;
       p.y = (2.0f * u1 - 1.0f) * r_hi;
       if (myel_classify(p.x, p.y, a) == c0 && p.x * p.x + p.y * p.y < r_hi * r_hi) return p;
+    } else if (GEOM == G_PMYEL) {
+      if (pm_cls == 2) {
+        // uniform in the cell, outside every outer disc
+        p.x = (u0 - 0.5f) * a.cell;
+        p.y = (u1 - 0.5f) * a.cell;
+        bool outside = true;
+        const int gc = cell_of<2>(p, a);
+        for (int i = tb.start[gc]; i < tb.start[gc + 1]; ++i) {
+          const float4 g = tb.objs[tb.items[i]];
+          const float fx = pm_wrap(p.x - g.x, a.cell), fy = pm_wrap(p.y - g.y, a.cell);
+          outside = outside && (fx * fx + fy * fy > g.w * g.w);
+        }
+        if (outside) return p;
+      } else {
+        const float4 g = tb.objs[pm_k];
+        const float rh = pm_cls == 0 ? g.z : g.w;
+        const float qx = (2.0f * u0 - 1.0f) * rh, qy = (2.0f * u1 - 1.0f) * rh;
+        const float d2 = qx * qx + qy * qy;
+        const bool ok = pm_cls == 0 ? d2 < rh * rh : (d2 >= g.z * g.z && d2 < rh * rh);
+        if (ok) {
+          p.x = qx + g.x;
+          p.y = qy + g.y;
+          return p;
+        }
+      }
     } else if (GEOM == G_ELL) {
       // unit ball scaled by the semi-axes (uniform: constant Jacobian)
       const float bx = 2.0f * u0 - 1.0f, by = 2.0f * u1 - 1.0f, bz = 2.0f * u2 - 1.0f;
@@ -1221,6 +1421,8 @@ constexpr bool uses_mfma() { return !(KAHAN || MB < 32); }
 template <int GEOM, int WALL, int MB, bool KAHAN>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void walk_kernel(const WalkArgs a) {
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
+  constexpr bool TABLES = PACKED || GEOM == G_PMYEL;
+  constexpr int OBJ_ROWS = (GEOM == G_PMYEL) ? 3 : 1;
   constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
   // The float16 trajectory channels of the walk-once producer are compiled into
   // the 4-measurement variant only (the producer launches with that block), so
@@ -1229,12 +1431,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   extern __shared__ float4 lds[];
 
   Tables tb{nullptr, nullptr, nullptr};
-  if (PACKED) {
+  if (TABLES) {
     const int n_cells = (DIM == 2) ? a.grid_n * a.grid_n : a.grid_n * a.grid_n * a.grid_n;
     float4* objs = lds;
-    int* start = reinterpret_cast<int*>(lds + a.n_obj);
+    int* start = reinterpret_cast<int*>(lds + OBJ_ROWS * a.n_obj);
     int* items = start + n_cells + 1;
-    for (int i = threadIdx.x; i < a.n_obj; i += kBlock) objs[i] = a.objs[i];
+    for (int i = threadIdx.x; i < OBJ_ROWS * a.n_obj; i += kBlock) objs[i] = a.objs[i];
     for (int i = threadIdx.x; i <= n_cells; i += kBlock) start[i] = a.cell_start[i];
     for (int i = threadIdx.x; i < a.n_items; i += kBlock) items[i] = a.cell_items[i];
     __syncthreads();
@@ -1272,11 +1474,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const float fsub = (float)n_sub;
   u32x4 rnd{0u, 0u, 0u, 0u};
   int next_echo = 0;
+  // PackedMyelinatedCylinders: the phase uses the position unwrapped by the
+  // minimum-image displacement of every sub-step (physics.py:1040-1047).
+  float uwx = p.x, uwy = p.y, uwz = p.z;
 
   for (int t = 0; t < a.n_t; ++t) {
     ph.prefetch(a, t, m0, lane);
     const float chi = a.chi ? a.chi[t] : 1.0f;
-    const float x0 = p.x, y0 = p.y, z0 = p.z;
+    const float x0 = (GEOM == G_PMYEL) ? uwx : p.x;
+    const float y0 = (GEOM == G_PMYEL) ? uwy : p.y;
+    const float z0 = (GEOM == G_PMYEL) ? uwz : p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     float lw_sample = 0.0f, occ = 0.0f;
     for (int s = 0; s < n_sub; ++s) {
@@ -1287,7 +1494,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const float nx = a.inj_n[base], ny = a.inj_n[base + N], nz = a.inj_n[base + 2 * N];
         const float nrm = sqrtf((nx * nx + ny * ny) + nz * nz);
         ux = nx / nrm; uy = ny / nrm; uz = nz / nrm;
-        if ((WALL == W_PERMEATE || GEOM == G_MYEL) && a.inj_u) uinj = a.inj_u[(size_t)gs * N + w];
+        if ((WALL == W_PERMEATE || GEOM == G_MYEL || GEOM == G_PMYEL) && a.inj_u)
+          uinj = a.inj_u[(size_t)gs * N + w];
       } else {
         if (!(gs & 1u) || (t == 0 && s == 0)) {
           rnd = philox4x32_10({w, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
@@ -1296,33 +1504,54 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const uint32_t rb = (gs & 1u) ? rnd.w : rnd.y;
         iso_direction(ra, rb, &ux, &uy, &uz);
       }
-      const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
-      const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
       const PowlesSrc ps{w, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
-      float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
-      // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
-      if (WALL != W_REFLECT) dlog = dlog * chi;
-      if (a.has_t2) {
-        // MyelinatedCylinder: dt/T2 of the compartment after the step (physics.py:657-659)
-        const float q2 = (GEOM == G_MYEL) ? pick3(p.cx, a.m_q2[0], a.m_q2[1], a.m_q2[2])
-                                          : a.dt_sub * a.inv_T2;
-        dlog = dlog - q2 * chi;
+      float dlog;
+      if (GEOM == G_PMYEL) {
+        const float px0 = p.x, py0 = p.y, pz0 = p.z;
+        const float db = pmyel_step(p, ux, uy, uz, a, tb, ps, hits, ncross, ncand);
+        float dx = p.x - px0, dy = p.y - py0;
+        const float dz = p.z - pz0;
+        dx = dx - L * rintf(dx / L);
+        dy = dy - L * rintf(dy / L);
+        uwx = uwx + dx;
+        uwy = uwy + dy;
+        uwz = uwz + dz;
+        // per-class T2 (transverse), T1 (stored), wall relaxivity (transverse),
+        // in the reference's order (physics.py:1049-1060)
+        dlog = 0.0f;
+        if (a.has_t2) dlog = dlog - pick3(p.cx, a.pm_q2[0], a.pm_q2[1], a.pm_q2[2]) * chi;
+        if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
+        if (a.pm_rhoD > 0.0f) dlog = dlog + (a.pm_rhoD * db) * chi;
+      } else {
+        const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
+        const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
+        dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+        // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
+        if (WALL != W_REFLECT) dlog = dlog * chi;
+        if (a.has_t2) {
+          // MyelinatedCylinder: dt/T2 of the compartment after the step (physics.py:657-659)
+          const float q2 = (GEOM == G_MYEL) ? pick3(p.cx, a.m_q2[0], a.m_q2[1], a.m_q2[2])
+                                            : a.dt_sub * a.inv_T2;
+          dlog = dlog - q2 * chi;
+        }
+        if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       }
-      if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       logw = logw + dlog;
       if (CHANNELS && a.dlog_h) lw_sample = lw_sample + dlog;
       if (CHANNELS && a.comp_h) occ = occ + outside_norm<GEOM>(p, a);
       if (n_sub > 1) {
-        ax = ax + (p.x - x0);
-        ay = ay + (p.y - y0);
-        az = az + (p.z - z0);
+        ax = ax + (((GEOM == G_PMYEL) ? uwx : p.x) - x0);
+        ay = ay + (((GEOM == G_PMYEL) ? uwy : p.y) - y0);
+        az = az + (((GEOM == G_PMYEL) ? uwz : p.z) - z0);
       }
     }
     // Sum of the n_sub sub-step positions (unfolded), then the phase update.
     float Px, Py, Pz;
     {
       float ox = x0, oy = y0, oz = z0;
-      float ex = p.x, ey = p.y, ez = p.z;
+      float ex = (GEOM == G_PMYEL) ? uwx : p.x;
+      float ey = (GEOM == G_PMYEL) ? uwy : p.y;
+      float ez = (GEOM == G_PMYEL) ? uwz : p.z;
       if (PACKED) {
         ox = ox + L * (float)p.cx; oy = oy + L * (float)p.cy; oz = oz + L * (float)p.cz;
         ex = ex + L * (float)p.cx; ey = ey + L * (float)p.cy; ez = ez + L * (float)p.cz;
@@ -1364,6 +1593,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     if (next_echo < a.n_echo && a.echo_idx[next_echo] == t) {
       float wgt = expf(logw);
       if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
+      if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
       ph.emit(wgt, active, lane,
               a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
       ++next_echo;
@@ -1374,6 +1604,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   float wgt = expf(logw);
   // MyelinatedCylinder: water-fraction spin weight of the start compartment (_ens, core.py:426-429, 489)
   if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
+  // PackedMyelinatedCylinders: myelin water carries its proton density (core.py:552-553)
+  if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
   ph.emit(wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
   if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
   if (active) {

@@ -32,6 +32,8 @@ hipError_t launch_walk_shellc(int wall, int mb, bool kahan, dim3 grid, size_t ld
                               const WalkArgs& a);
 hipError_t launch_walk_myel(int wall, int mb, bool kahan, dim3 grid, size_t lds, hipStream_t st,
                             const WalkArgs& a);
+hipError_t launch_walk_pmyel(int wall, int mb, bool kahan, dim3 grid, size_t lds, hipStream_t st,
+                             const WalkArgs& a);
 
 // Fixed-order float64 sum of per-wave partials: out[c] = scale * sum_w in[w*n_cols + c].
 hipError_t launch_reduce_partials(const double* in, long long n_rows, int n_cols, double scale,

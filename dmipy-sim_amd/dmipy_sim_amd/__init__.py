@@ -10,6 +10,7 @@ from .waveforms import (Waveform, pgse, ogse, cpmg, calc_b, set_b, rotate_wavefo
                         tile_waveform, fibonacci_directions)
 from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders, Ellipsoid,
                          PermeableSlab1D, PermeableShell, MyelinatedCylinder,
+                         PackedMyelinatedCylinders, pack_myelinated_cylinders,
                          PackedSpheres, pack_cylinders, pack_spheres, pack_dense)
 from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
                       collision_sub_steps, replay_sub_steps)
@@ -25,7 +26,8 @@ __all__ = [
     "GAMMA", "DEFAULT_SLEW_RATE", "Waveform", "pgse", "ogse", "cpmg", "calc_b", "set_b",
     "rotate_waveform", "tile_waveform", "fibonacci_directions", "FreeDiffusion", "Box1D",
     "Sphere", "Cylinder", "PackedCylinders", "PackedSpheres", "Ellipsoid", "PermeableSlab1D",
-    "PermeableShell", "MyelinatedCylinder", "pack_cylinders",
+    "PermeableShell", "MyelinatedCylinder", "PackedMyelinatedCylinders",
+    "pack_myelinated_cylinders", "pack_cylinders",
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",

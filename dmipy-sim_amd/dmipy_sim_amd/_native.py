@@ -25,12 +25,18 @@ class Myelin_t(C.Structure):
                 ("n_start", C.c_int64 * 2)]
 
 
+class PackedMyelin_t(C.Structure):
+    _fields_ = [("inner_radii", _fp), ("step_len", _fp), ("two_kappa_over_D", _fp),
+                ("dt_over_T2", C.c_float * 3), ("rho_over_D", C.c_float),
+                ("myelin_weight", C.c_float), ("start", C.POINTER(C.c_int64))]
+
+
 class Geometry_t(C.Structure):
     _fields_ = [("kind", C.c_int32), ("wall", C.c_int32), ("n_obj", C.c_int32),
                 ("size", C.c_float), ("cell", C.c_float), ("eps", C.c_float),
                 ("nudge", C.c_float), ("rho_over_D", C.c_float), ("kappa_over_D", C.c_float),
                 ("centers", _fp), ("radii", _fp), ("axes", C.c_float * 3), ("aux", C.c_float),
-                ("myelin", C.POINTER(Myelin_t))]
+                ("myelin", C.POINTER(Myelin_t)), ("packed_myelin", C.POINTER(PackedMyelin_t))]
 
 
 class Waveform_t(C.Structure):

@@ -38,6 +38,9 @@ def _replay_gap(geometry, *, return_positions, return_compartments,
         return "return_walker_signals is a fused single-pass internal (no replay equivalent)"
     if getattr(geometry, '_is_myelinated', False):
         return "MyelinatedCylinder uses a dedicated fused step kernel (no replay walk)"
+    if getattr(geometry, '_is_packed_myelinated', False):
+        return ("PackedMyelinatedCylinders fused single-reflection kernel is not "
+                "position-parity with the multi-bounce replay walk")
     if getattr(geometry, 'permeability', None) is not None:
         return ("membrane permeability is single-pass walk semantics "
                 "(fused-only; scalar replay has no exchange reservoir)")
@@ -97,7 +100,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
         if rot is not None:
             inject_normals = np.ascontiguousarray(
                 np.einsum('ij,wsj->wsi', rot.astype(np.float64), inject_normals), dtype=np.float32)
-        if plan.wall == WALL_PERMEATE or plan.myelin is not None:
+        if plan.wall == WALL_PERMEATE or plan.myelin is not None or plan.packed_myelin is not None:
             if inject_uniforms is None:
                 raise ValueError("inject_uniforms required on a permeable wall")
             inject_uniforms = np.ascontiguousarray(inject_uniforms, dtype=np.float32).reshape(N, S)
@@ -198,10 +201,16 @@ def simulate(
         if gap is not None:
             raise NotImplementedError(f"engine='replay' cannot serve this run: {gap}. "
                                       "Use engine='fused' (or 'auto').")
-    if return_positions == 'full' and getattr(geometry, '_is_myelinated', False):
+    _myelinated = (getattr(geometry, '_is_myelinated', False)
+                   or getattr(geometry, '_is_packed_myelinated', False))
+    if return_positions == 'full' and _myelinated:
         raise NotImplementedError(   # core.py:459-463
             "return_positions='full' is supported for standard geometries "
             "(including Mesh), not MyelinatedCylinder / PackedMyelinatedCylinders.")
+    if r0 is not None and getattr(geometry, '_is_packed_myelinated', False):
+        raise NotImplementedError(   # core.py:542-546
+            "simulate(r0=...) is unsupported for PackedMyelinatedCylinders; it "
+            "initialises walkers (and their compartments) from `seed` via init_positions.")
 
     check_gpu(n_walkers, require_gpu, what="simulate")
 

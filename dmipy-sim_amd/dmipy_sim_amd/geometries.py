@@ -18,6 +18,7 @@ the HIP walk kernel; these objects carry the parameters to it.
 * ``PackedSpheres``   -- dmipy_sim/geometries.py:2343-2786
 * ``Ellipsoid``       -- dmipy_sim/geometries.py:1137-1425
 * ``MyelinatedCylinder`` -- dmipy_sim/geometries.py:847-1136
+* ``pack_myelinated_cylinders``, ``PackedMyelinatedCylinders`` -- dmipy_sim/geometries.py:2787-3220
 * ``PermeableSlab1D`` -- dmipy_sim/geometries.py:3221-3300
 * ``PermeableShell``  -- dmipy_sim/geometries.py:3303-3431
 """
@@ -25,7 +26,7 @@ import numpy as np
 
 # Geometry kind codes shared with include/dmipy_walk.h (enum dmw_geom_kind).
 KIND_FREE, KIND_BOX1D, KIND_CYLINDER, KIND_SPHERE, KIND_PACKED_CYL, KIND_PACKED_SPH = range(6)
-KIND_ELLIPSOID, KIND_SLAB, KIND_SHELL, KIND_SHELL_CYL, KIND_MYELIN = range(6, 11)
+KIND_ELLIPSOID, KIND_SLAB, KIND_SHELL, KIND_SHELL_CYL, KIND_MYELIN, KIND_PACKED_MYELIN = range(6, 12)
 
 # Water per unit myelin-sheath volume, the default myelin proton density of
 # MyelinatedCylinder.water_fractions (dmipy_sim/substrate/biophysical_constants.py:523-530).
@@ -287,6 +288,179 @@ def pack_dense(radii, target_vf, seed=0, dim=2, gap=0.02, max_iter=200_000):
     else:
         vf = float((4.0 / 3.0) * np.pi * np.sum(radii ** 3) / L ** 3)
     return c, L, vf
+
+
+def pack_myelinated_cylinders(inner_radii, g_ratios, target_packing, cell_size=None, seed=0,
+                              max_attempts=100_000):
+    """RSA placement of myelinated cylinders (outer radius = inner / g-ratio) in a
+    periodic square, outer boundaries disjoint (dmipy_sim/geometries.py:2787-2857).
+    ``cell_size`` defaults to sqrt(pi sum R_out^2 / target_packing).  Returns
+    ``(inner_radii, g_ratios, centers (N, 2))``."""
+    inner = np.asarray(inner_radii, dtype=np.float64).ravel()
+    n = len(inner)
+    g = np.broadcast_to(np.asarray(g_ratios, dtype=np.float64).ravel(), (n,)).copy()
+    outer = inner / g
+    if cell_size is None:
+        if target_packing is None:
+            raise ValueError("Provide either cell_size or target_packing.")
+        cell_size = float(np.sqrt(np.pi * np.sum(outer ** 2) / target_packing))
+    centers = _rsa(outer, float(cell_size), seed, max_attempts, 2, "cylinder")
+    return inner, g, centers
+
+
+class PackedMyelinatedCylinders:
+    """Periodic square cell of myelinated axons sharing one axis
+    (dmipy_sim/geometries.py:2860-3220; fused step physics.py:682-1062).
+
+    Compartments: 0 extra-axonal, 1..N_max intra-axonal of axon k (k+1), N_max+1..2
+    N_max myelin of axon k; ``simulate`` reports them as 0 intra, 1 myelin, 2 extra.
+    Per-axon diffusivities, permeabilities and wall relaxivities broadcast from
+    scalars; T2 per compartment class (the first axon's values apply); myelin water
+    carries the spin weight ``_myelin_proton_density``.  Walker placement, the
+    walk and the surface-relaxivity sub-steps run in the HIP kernel.
+    """
+    _is_packed_myelinated = True
+    _walk_kind = KIND_PACKED_MYELIN
+
+    def __init__(self, inner_radii, g_ratios, centers, cell_size, N_max=128,
+                 orientation=(0., 0., 1.), D_intra=2e-9, D_myelin=0.0, D_extra=2e-9,
+                 T2_intra=None, T2_myelin=None, T2_extra=None, kappa_inner=0.0,
+                 kappa_outer=0.0, rho_inner=0.0, rho_outer=0.0):
+        inner = np.asarray(inner_radii, dtype=np.float64).ravel()
+        n = len(inner)
+        g = np.broadcast_to(np.asarray(g_ratios, dtype=np.float64).ravel(), (n,)).copy()
+        centers = np.asarray(centers, dtype=np.float64)
+        if centers.shape != (n, 2):
+            raise ValueError(f"centers shape {centers.shape} must be ({n}, 2)")
+        if N_max < n:
+            raise ValueError(f"N_max={N_max} < N_actual={n}")
+        self.N_actual = n
+        self.N_max = int(N_max)
+        self._cell_size = float(cell_size)
+        self._L_float = float(cell_size)
+        self._myelin_proton_density = MYELIN_WATER_PROTON_DENSITY
+
+        def bcast(x, name, opt=False, zero_none=False):
+            if x is None:
+                if opt:
+                    return None
+                x = 0.0
+            x = np.asarray(x, dtype=np.float64).ravel()
+            if x.size == 1:
+                x = np.broadcast_to(x, (n,)).copy()
+            elif x.size != n:
+                raise ValueError(f"{name} must be scalar or length {n}, got {x.size}")
+            out = np.zeros(self.N_max)
+            out[:n] = x
+            return out
+
+        self._inner_radii_np = bcast(inner, 'inner_radii')
+        self._outer_radii_np = bcast(inner / g, 'outer_radii')
+        c = np.zeros((self.N_max, 2))
+        c[:n] = centers
+        self._centers_np = c
+        self._D_intra_np = bcast(D_intra, 'D_intra')
+        self._D_myelin_np = bcast(D_myelin, 'D_myelin')
+        self._D_extra_np = bcast(D_extra, 'D_extra')
+        self._kappa_inner_np = bcast(kappa_inner, 'kappa_inner')
+        self._kappa_outer_np = bcast(kappa_outer, 'kappa_outer')
+        self._rho_inner_np = bcast(rho_inner, 'rho_inner')
+        self._rho_outer_np = bcast(rho_outer, 'rho_outer')
+        self._has_t2 = T2_intra is not None or T2_myelin is not None or T2_extra is not None
+        big = np.full(self.N_max, 1e6)
+        self._T2_np = tuple(
+            (bcast(t, name, opt=True) if t is not None else big) if self._has_t2 else None
+            for t, name in ((T2_intra, 'T2_intra'), (T2_myelin, 'T2_myelin'),
+                            (T2_extra, 'T2_extra')))
+        o = np.asarray(orientation, dtype=np.float64)
+        self.orientation = (o / np.linalg.norm(o)).astype(np.float32)
+        R = _rotation_to_z(self.orientation)
+        self._R = R.astype(np.float32)
+        self._R_inv = R.T.astype(np.float32)
+        self._is_identity_rotation = bool(np.allclose(R, np.eye(3)))
+        nz = self._inner_radii_np[self._inner_radii_np > 0]
+        ref_r = float(np.min(nz)) if len(nz) else 1e-6
+        self._eps = np.float32(1e-7 * ref_r)
+        self._nudge = np.float32(1e-4 * ref_r)
+        self.min_gap = _min_gap(self._centers_np[:n], self._outer_radii_np[:n], self._L_float)
+
+    def volume_fraction(self, compartment):
+        L, n = self._L_float, self.N_actual
+        ri, ro = self._inner_radii_np[:n], self._outer_radii_np[:n]
+        if compartment == 'intra':
+            return float(np.pi * np.sum(ri ** 2) / L ** 2)
+        if compartment == 'myelin':
+            return float(np.pi * np.sum(ro ** 2 - ri ** 2) / L ** 2)
+        if compartment == 'extra':
+            return float((L ** 2 - np.pi * np.sum(ro ** 2)) / L ** 2)
+        raise ValueError(
+            f"compartment must be 'intra', 'myelin', or 'extra'; got {compartment!r}")
+
+    def start_counts(self, n_walkers):
+        """Walkers per compartment by area, in placement order: intra of each axon,
+        myelin of each axon, then extra-axonal (the remainder; a negative remainder
+        is trimmed from the intra counts in axon order)."""
+        L, n = self._L_float, self.N_actual
+        ri, ro = self._inner_radii_np[:n], self._outer_radii_np[:n]
+        a_in = np.pi * ri ** 2
+        a_my = np.pi * (ro ** 2 - ri ** 2)
+        a_ex = L * L - np.pi * np.sum(ro ** 2)
+        tot = a_ex + np.sum(a_in) + np.sum(a_my)
+        n_in = np.array([int(round(n_walkers * a / tot)) for a in a_in], dtype=np.int64)
+        n_my = np.array([int(round(n_walkers * a / tot)) for a in a_my], dtype=np.int64)
+        n_ex = n_walkers - int(n_in.sum()) - int(n_my.sum())
+        if n_ex < 0:
+            excess = -n_ex
+            for k in range(n):
+                trim = min(excess, int(n_in[k]))
+                n_in[k] -= trim
+                excess -= trim
+                if excess == 0:
+                    break
+            n_ex = 0
+        return n_in, n_my, n_ex
+
+    def init_positions(self, n_walkers, key=0):
+        """Host placement in the cell (lab frame, z = 0) with the area allotment;
+        sets ``_init_compartments`` (0 extra, k+1 intra, N_max+k+1 myelin).  The GPU
+        walk places its walkers itself."""
+        rng = _rng_from_key(key)
+        n, L = self.N_actual, self._L_float
+        ri, ro, c = self._inner_radii_np[:n], self._outer_radii_np[:n], self._centers_np[:n]
+        n_in, n_my, _ = self.start_counts(n_walkers)
+        pos = np.zeros((n_walkers, 3), np.float32)
+        comp = np.zeros(n_walkers, np.int32)
+        i = 0
+        for counts, lo_r, hi_r, base in ((n_in, None, ri, 1), (n_my, ri, ro, self.N_max + 1)):
+            for k in range(n):
+                nk = int(counts[k])
+                if nk == 0:
+                    continue
+                got, have = [], 0
+                while have < nk:
+                    xy = rng.uniform(-hi_r[k], hi_r[k], (max(nk * 4, 64), 2))
+                    d = np.linalg.norm(xy, axis=1)
+                    keep = d < hi_r[k] if lo_r is None else (d >= lo_r[k]) & (d < hi_r[k])
+                    got.append(xy[keep])
+                    have += int(keep.sum())
+                pos[i:i + nk, :2] = np.concatenate(got)[:nk] + c[k]
+                comp[i:i + nk] = base + k
+                i += nk
+        m = n_walkers - i
+        got, have = [], 0
+        while have < m:
+            xy = rng.uniform(-L / 2.0, L / 2.0, (max(m * 4, 1024), 2))
+            out = np.ones(len(xy), bool)
+            for k in range(n):
+                d = xy - c[k]
+                d -= L * np.round(d / L)
+                out &= np.sum(d ** 2, axis=1) > ro[k] ** 2
+            got.append(xy[out])
+            have += int(out.sum())
+        if m > 0:
+            pos[i:, :2] = np.concatenate(got)[:m]
+        self._init_compartments = comp
+        return (self._R_inv.astype(np.float64) @ pos.T.astype(np.float64)).T.astype(np.float32)
 
 
 def _min_gap(centers, radii, L):

@@ -136,6 +136,7 @@ class WalkPlan:
     axes: tuple = (0.0, 0.0, 0.0)  # Ellipsoid: float32 1/a^2, 1/b^2, 1/c^2
     aux: float = 0.0               # Ellipsoid clamp factor; shell / myelin outer radius
     myelin: dict = None            # MyelinatedCylinder per-compartment constants
+    packed_myelin: dict = None     # PackedMyelinatedCylinders per-axon tables
 
     def geometry_desc(self, wall=None, rho_over_D=None, kappa_over_D=None, n_walkers=None):
         """The dmw_geometry descriptor of this plan (optionally with another wall
@@ -170,6 +171,21 @@ class WalkPlan:
             md.n_start[1] = n0 + n1
             self.extra["_myelin_keep"] = md
             g.myelin = C.pointer(md)
+        if self.packed_myelin is not None:
+            if n_walkers is None:
+                raise ValueError("the packed myelinated descriptor needs n_walkers")
+            m = self.packed_myelin
+            n_in, n_my, _ = m["start_counts"](int(n_walkers))
+            start = np.ascontiguousarray(
+                np.concatenate([[0], np.cumsum(np.concatenate([n_in, n_my]))]), dtype=np.int64)
+            pm = nat.PackedMyelin_t(inner_radii=nat.fptr(m["inner"]), step_len=nat.fptr(m["step_len"]),
+                                    two_kappa_over_D=nat.fptr(m["two_kappa_over_D"]),
+                                    rho_over_D=m["rho_over_D"], myelin_weight=m["myelin_weight"],
+                                    start=start.ctypes.data_as(C.POINTER(C.c_int64)))
+            for i in range(3):
+                pm.dt_over_T2[i] = m["dt_over_T2"][i]
+            self.extra["_pmyelin_keep"] = (pm, start)
+            g.packed_myelin = C.pointer(pm)
         return g
 
 
@@ -212,6 +228,53 @@ def _myelin_plan(geometry, dt, T1):
                     rot=_rotation_of(geometry), aux=float(f32(g.outer_radius)), myelin=myelin)
 
 
+def _packed_myelin_plan(geometry, dt, T1):
+    """Constants of make_packed_myelin_step_fn (physics.py:979-1031) and its walk
+    (682-724): wall relaxivity rho = max over the inner/outer walls, rho/D_ref with
+    D_ref the largest intra/extra diffusivity and the surface sub-step rule on the
+    extra-axonal pore when rho > 0 (else one step per sample); per-axon float32 step
+    lengths sqrt(6 D dt_sub) and 2 kappa / max(D_leaving, 1e-30); dt_sub/T2 per class
+    from the first axon; NUDGE of the smallest inner radius."""
+    g = geometry
+    n = g.N_actual
+    rho = max(float(np.max(g._rho_inner_np)), float(np.max(g._rho_outer_np)))
+    if rho > 0.0:
+        D_ref = float(max(np.max(g._D_intra_np), np.max(g._D_extra_np)))
+        rho_over_D = float(f32(rho / D_ref))
+        n_sub = surface_sub_steps(g, D_ref, dt)
+    else:
+        rho_over_D, n_sub = 0.0, 1
+    dt_sub = dt / n_sub
+    dt32 = f32(dt_sub)
+    Ds = [g._D_intra_np[:n].astype(np.float32), g._D_myelin_np[:n].astype(np.float32),
+          g._D_extra_np[:n].astype(np.float32)]
+    step = np.stack([np.sqrt((f32(6.0) * d) * dt32).astype(np.float32) for d in Ds], axis=1)
+    floor = f32(1e-30)
+    ki = g._kappa_inner_np[:n].astype(np.float32)
+    ko = g._kappa_outer_np[:n].astype(np.float32)
+    k2 = np.stack([f32(2.0) * (ki / np.maximum(Ds[0], floor)),
+                   f32(2.0) * (ki / np.maximum(Ds[1], floor)),
+                   f32(2.0) * (ko / np.maximum(Ds[1], floor)),
+                   f32(2.0) * (ko / np.maximum(Ds[2], floor))], axis=1).astype(np.float32)
+    q2 = [0.0, 0.0, 0.0]
+    if g._has_t2:
+        q2 = [float(dt32 / f32(t[0])) for t in g._T2_np]
+    has_t1 = T1 is not None
+    pm = dict(inner=np.ascontiguousarray(g._inner_radii_np[:n], np.float32),
+              step_len=np.ascontiguousarray(step), two_kappa_over_D=np.ascontiguousarray(k2),
+              dt_over_T2=q2, rho_over_D=rho_over_D,
+              myelin_weight=float(f32(g._myelin_proton_density)), start_counts=g.start_counts)
+    return WalkPlan(kind=geo.KIND_PACKED_MYELIN, wall=WALL_REFLECT, n_sub=int(n_sub), size=0.0,
+                    cell=float(f32(g._L_float)), eps=float(g._eps), nudge=float(g._nudge),
+                    rho_over_D=0.0, kappa_over_D=0.0, step_len=0.0,
+                    gamma_dt_sub=float(f32(GAMMA * dt_sub)), dt_sub=float(dt32),
+                    inv_T2=0.0, inv_T1=float(f32(1.0 / T1)) if has_t1 else 0.0,
+                    has_t2=bool(g._has_t2), has_t1=has_t1, has_weight=True,
+                    centers=np.ascontiguousarray(g._centers_np[:n], np.float32),
+                    radii=np.ascontiguousarray(g._outer_radii_np[:n], np.float32),
+                    rot=_rotation_of(geometry), packed_myelin=pm)
+
+
 def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fused"):
     """Resolve wall routine, sub-steps and float32 constants for one walk.
 
@@ -225,6 +288,8 @@ def make_walk_plan(geometry, diffusivity, dt, T2=None, T1=None, substep_rule="fu
             f"geometry {type(geometry).__name__} is not supported by the HIP walk")
     if kind == geo.KIND_MYELIN:
         return _myelin_plan(geometry, dt, T1)
+    if kind == geo.KIND_PACKED_MYELIN:
+        return _packed_myelin_plan(geometry, dt, T1)
     if diffusivity is None:
         raise ValueError("diffusivity is required for standard geometries")
     D = float(diffusivity)

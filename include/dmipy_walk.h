@@ -48,7 +48,9 @@ enum dmw_geom_kind {
   DMW_PERMEABLE_SLAB = 7,    /* PermeableSlab1D    geometries.py:3221-3300 */
   DMW_PERMEABLE_SHELL = 8,   /* PermeableShell, kind='sphere'   geometries.py:3303-3431 */
   DMW_PERMEABLE_SHELL_CYL = 9, /* PermeableShell, kind='cylinder' (axis z in the native frame) */
-  DMW_MYELINATED_CYLINDER = 10 /* MyelinatedCylinder geometries.py:847-1136, step physics.py:420-679 */
+  DMW_MYELINATED_CYLINDER = 10, /* MyelinatedCylinder geometries.py:847-1136, step physics.py:420-679 */
+  DMW_PACKED_MYELINATED = 11   /* PackedMyelinatedCylinders geometries.py:2860-3220,
+                                  step physics.py:682-1062 */
 };
 
 enum dmw_wall {
@@ -83,6 +85,28 @@ typedef struct dmw_myelin {
                                 [n_start[0], n_start[1]) in the sheath, the rest outside */
 } dmw_myelin;
 
+/* PackedMyelinatedCylinders (DMW_PACKED_MYELINATED): n_obj axons in the periodic
+ * square of side `cell`, geometry centers (n_obj, 2) and radii = OUTER radii; the
+ * compartment of a walker is (class, axon) with class 0 intra, 1 myelin, 2 extra
+ * (an extra-axonal walker refers to axon 0, as the reference's k_cyl does).
+ * run->n_sub sub-steps per sample (the surface rule when rho > 0), run->step_len
+ * and run->inv_T2 unused.  The phase uses the unwrapped position; the signal is
+ * sum_w s(w) e^{log_w} e^{i phi} / sum_w s(w), s = myelin_weight for walkers
+ * starting in myelin, else 1 (core.py:548-556).  Walkers are placed in-kernel:
+ * [start[j], start[j+1]) intra of axon j for j < n_obj, myelin of axon j - n_obj
+ * for n_obj <= j < 2 n_obj, the rest [start[2 n_obj], n_walkers) extra-axonal. */
+typedef struct dmw_packed_myelin {
+  const float* inner_radii;      /* (n_obj) */
+  const float* step_len;         /* (n_obj, 3) float32 sqrt(6 D dt_sub): intra, myelin, extra */
+  const float* two_kappa_over_D; /* (n_obj, 4) float32 2 kappa / max(D_leaving, 1e-30): inner
+                                    membrane left from intra, from myelin; outer membrane
+                                    left from myelin, from extra */
+  float dt_over_T2[3];           /* float32 dt_sub / T2 per class (applied when run->has_t2) */
+  float rho_over_D;              /* wall relaxivity / D_ref (0 = none) */
+  float myelin_weight;           /* spin weight of myelin water */
+  const int64_t* start;          /* (2 n_obj + 1) ascending walker-index boundaries */
+} dmw_packed_myelin;
+
 /* Geometry and wall constants (float32 values rounded as the reference rounds
  * them, see make_step_fn physics.py:229-290 and the EPS/NUDGE definitions
  * geometries.py:503-504, 1694-1696). */
@@ -103,6 +127,7 @@ typedef struct dmw_geometry {
                                 myelinated cylinder: R_out.  size = R_in for those, the
                                 slab length for the slab */
   const dmw_myelin* myelin;  /* DMW_MYELINATED_CYLINDER only, else NULL */
+  const dmw_packed_myelin* packed_myelin; /* DMW_PACKED_MYELINATED only, else NULL */
 } dmw_geometry;
 
 /* Gradient waveform, Waveform.G of dmipy_sim/waveforms.py:39-90. */

@@ -51,7 +51,9 @@ class OracleArgs(C.Structure):
                 ("walker_begin", C.c_int64), ("walker_end", C.c_int64),
                 ("instance", C.c_uint32), ("axes", C.c_float * 3), ("aux", C.c_float),
                 ("m_step", C.c_float * 3), ("m_q2", C.c_float * 3), ("m_wf", C.c_float * 3),
-                ("m_k2", C.c_float * 4), ("m_n0", C.c_int64), ("m_n01", C.c_int64)]
+                ("m_k2", C.c_float * 4), ("m_n0", C.c_int64), ("m_n01", C.c_int64),
+                ("pm_inner", _fp), ("pm_step", _fp), ("pm_k2", _fp), ("pm_q2", C.c_float * 3),
+                ("pm_rhoD", C.c_float), ("pm_wmy", C.c_float), ("pm_start", C.POINTER(C.c_int64))]
 
 
 class OracleOut(C.Structure):
@@ -134,6 +136,77 @@ def _myelin_plan(g, dt, T1):
                 m_wf=[float(f32(w)) for w in g.water_fractions], m_k2=k2, m_area=area)
 
 
+def _packed_myelin_plan(g, dt, T1):
+    """PackedMyelinatedCylinders (make_packed_myelin_step_fn, physics.py:979-1031, and
+    the walk's constants, 708-724): rho = max wall relaxivity, rho/D_ref with D_ref =
+    max(D_intra, D_extra) and the surface sub-step rule on the extra-axonal pore
+    area/perimeter (physics.py:51-69, 163-188); per-axon sqrt(f32(6) D dt_sub) and
+    2 kappa/max(D_leaving, 1e-30); dt_sub/T2 of the first axon per class; NUDGE of
+    the smallest inner radius (geometries.py:3067-3070)."""
+    n = g.N_actual
+    rho = max(float(np.max(g._rho_inner_np)), float(np.max(g._rho_outer_np)))
+    n_sub, rD = 1, 0.0
+    if rho > 0.0:
+        D_ref = float(max(np.max(g._D_intra_np), np.max(g._D_extra_np)))
+        rD = float(f32(rho / D_ref))
+        frac = getattr(g, "surface_substep_frac", None)
+        frac = 8.0 if frac is None else frac
+        if frac and frac > 0:
+            ro = g._outer_radii_np[g._outer_radii_np > 0]
+            area = g._cell_size ** 2 - float(np.sum(math.pi * ro ** 2))
+            perim = float(np.sum(2.0 * math.pi * ro))
+            Rc = area / perim
+            n_sub = max(1, math.ceil(dt / ((Rc / frac) ** 2 / (6.0 * D_ref))))
+    dt_sub = dt / n_sub
+    d32 = f32(dt_sub)
+    step = np.zeros((n, 3), np.float32)
+    k2 = np.zeros((n, 4), np.float32)
+    for k in range(n):
+        Dk = [f32(g._D_intra_np[k]), f32(g._D_myelin_np[k]), f32(g._D_extra_np[k])]
+        for c in range(3):
+            step[k, c] = np.sqrt(f32(f32(6.0) * Dk[c]) * d32)
+        ki, ko = f32(g._kappa_inner_np[k]), f32(g._kappa_outer_np[k])
+        for j, (kap, Dl) in enumerate(((ki, Dk[0]), (ki, Dk[1]), (ko, Dk[1]), (ko, Dk[2]))):
+            k2[k, j] = f32(2.0) * f32(kap / max(Dl, f32(1e-30)))
+    q2 = [0.0, 0.0, 0.0]
+    if g._has_t2:
+        q2 = [float(f32(d32 / f32(t[0]))) for t in g._T2_np]
+    ri = g._inner_radii_np[:n]
+    return dict(kind=11, wall=W_REFLECT, n_sub=int(n_sub), rho_over_D=0.0, kappa_over_D=0.0,
+                step_len=0.0, gamma_dt_sub=float(f32(GAMMA * dt_sub)), dt_sub=float(d32),
+                inv_T2=0.0, inv_T1=float(f32(1.0 / T1)) if T1 is not None else 0.0,
+                has_t2=bool(g._has_t2), has_t1=T1 is not None, size=0.0,
+                cell=float(f32(g._cell_size)), eps=float(f32(1e-7 * float(ri.min()))),
+                nudge=float(f32(1e-4 * float(ri.min()))),
+                centers=np.ascontiguousarray(g._centers_np[:n], np.float32),
+                radii=np.ascontiguousarray(g._outer_radii_np[:n], np.float32),
+                axes=[0.0, 0.0, 0.0], aux=0.0, pm_inner=np.ascontiguousarray(ri, np.float32),
+                pm_step=step, pm_k2=k2, pm_q2=q2, pm_rhoD=rD,
+                pm_wmy=float(f32(g._myelin_proton_density)), pm_geom=g)
+
+
+def packed_myelin_start(p, N):
+    """Start boundaries: intra of each axon, myelin of each axon (area allotment,
+    geometries.py:3112-3137), the remainder extra-axonal."""
+    g = p["pm_geom"]
+    n = g.N_actual
+    L = g._cell_size
+    ri, ro = g._inner_radii_np[:n], g._outer_radii_np[:n]
+    a_in = [math.pi * r ** 2 for r in ri]
+    a_my = [math.pi * (o ** 2 - r ** 2) for r, o in zip(ri, ro)]
+    tot = (L * L - math.pi * float(np.sum(ro ** 2))) + sum(a_in) + sum(a_my)
+    n_in = [int(round(N * a / tot)) for a in a_in]
+    n_my = [int(round(N * a / tot)) for a in a_my]
+    excess = sum(n_in) + sum(n_my) - N
+    k = 0
+    while excess > 0 and k < n:
+        t = min(excess, n_in[k])
+        n_in[k] -= t
+        excess -= t
+        k += 1
+    return np.ascontiguousarray(np.cumsum([0] + n_in + n_my), dtype=np.int64)
+
+
 def myelin_start_counts(p, N):
     """Walkers starting intra-axonal and in the sheath (the rest start outside)."""
     a = p["m_area"]
@@ -146,6 +219,8 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
     name = type(geom).__name__
     if name == "MyelinatedCylinder":
         return _myelin_plan(geom, dt, T1)
+    if name == "PackedMyelinatedCylinders":
+        return _packed_myelin_plan(geom, dt, T1)
     kind = KIND[name]
     if name == "PermeableShell" and geom.kind == "cylinder":
         kind = 9
@@ -254,6 +329,18 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
             A.m_k2[k] = p["m_k2"][k]
         n0, n1 = myelin_start_counts(p, N)
         A.m_n0, A.m_n01 = n0, n0 + n1
+    keep = []
+    if p["kind"] == 11:
+        start = packed_myelin_start(p, N)
+        keep += [start, p["pm_step"], p["pm_k2"], p["pm_inner"]]
+        A.pm_inner = fp(p["pm_inner"])
+        A.pm_step = fp(np.ascontiguousarray(p["pm_step"]))
+        A.pm_k2 = fp(np.ascontiguousarray(p["pm_k2"]))
+        for k in range(3):
+            A.pm_q2[k] = p["pm_q2"][k]
+        A.pm_rhoD = p["pm_rhoD"]
+        A.pm_wmy = p["pm_wmy"]
+        A.pm_start = start.ctypes.data_as(C.POINTER(C.c_int64))
     rows = 1 if echo is None else len(echo)
     res = {"signal": np.zeros(rows * n_meas), "signal_im": np.zeros(rows * n_meas),
            "final_pos": np.zeros((N, 3), np.float32), "comp_origin": np.zeros(N, np.int32),

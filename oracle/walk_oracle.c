@@ -78,6 +78,14 @@ typedef struct {
   float m_wf[3];           /* spin weights (water fractions) */
   float m_k2[4];           /* 2 kappa / max(D_leaving, 1e-30): inner from 0, 1; outer from 1, 2 */
   int64_t m_n0, m_n01;     /* start compartments by walker index */
+  /* PackedMyelinatedCylinders: centers (n_obj, 2), radii = outer radii */
+  const float* pm_inner;   /* (n_obj) inner radii */
+  const float* pm_step;    /* (n_obj, 3) step lengths intra, myelin, extra */
+  const float* pm_k2;      /* (n_obj, 4) 2 kappa/D: inner from intra, from myelin; outer from myelin, from extra */
+  float pm_q2[3];          /* dt_sub / T2 per class: intra, myelin, extra */
+  float pm_rhoD;           /* wall relaxivity / D_ref */
+  float pm_wmy;            /* myelin spin weight */
+  const int64_t* pm_start; /* (2 n_obj + 1) start boundaries */
 } oracle_args;
 
 typedef struct {
@@ -97,7 +105,7 @@ typedef struct {
 } oracle_out;
 
 enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5,
-       K_ELL = 6, K_SLAB = 7, K_SHELL = 8, K_SHELLC = 9, K_MYEL = 10 };
+       K_ELL = 6, K_SLAB = 7, K_SHELL = 8, K_SHELLC = 9, K_MYEL = 10, K_PMYEL = 11 };
 enum { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
 /* ---------------------------------------------------------------- Philox */
@@ -718,6 +726,159 @@ static void myel_step(float r[3], const float dir[3], float u, const oracle_args
   *comp = nc;
 }
 
+/* ------------------------------------------------ PackedMyelinatedCylinders
+   make_packed_myelin_traj_step_fn (physics.py:726-978, no magnetisation transfer) in
+   the reference's compartment encoding: 0 extra, k+1 intra of axon k, n+k+1 myelin
+   of axon k (N_max = n here).  Every membrane quantity refers to the walker's own
+   axon k_cyl (axon 0 for extra-axonal walkers); the exterior clamp scans every axon.
+   One uniform u decides whichever membrane is crossed.  Returns the unit
+   (rho/D = 1) boundary log-weight. */
+static float wrapL(float q, float L) { return q - L * floorf(q / L + 0.5f); }
+
+static int pm_to3(int cid, int n) { return cid == 0 ? 2 : (cid > n ? 1 : 0); }
+
+static float pm_step_oracle(float r[3], const float dir[3], float u, const oracle_args* A, int* cid,
+                            uint32_t* hits, uint32_t* ncross) {
+  const int n = A->n_obj;
+  const float L = A->cell, NUDGE = A->nudge;
+  const int c = *cid;
+  const int is_extra = c == 0, is_intra = c >= 1 && c <= n, is_myelin = c > n;
+  int k = is_intra ? c - 1 : (is_myelin ? c - (n + 1) : 0);
+  if (k < 0) k = 0;
+  const float* st = A->pm_step + 3 * k;
+  const float sl = is_intra ? st[0] : (is_myelin ? st[1] : st[2]);
+  const float step[3] = {dir[0] * sl, dir[1] * sl, dir[2] * sl};
+  const float cx = A->centers[2 * k], cy = A->centers[2 * k + 1];
+  const float Rin = A->pm_inner[k], Rout = A->radii[k];
+  float q[2] = {wrapL((r[0] + step[0]) - cx, L), wrapL((r[1] + step[1]) - cy, L)};
+  const float nrm = sqrtf(q[0] * q[0] + q[1] * q[1]);
+  const int cio = is_intra && nrm >= Rin, cii = is_myelin && nrm < Rin;
+  const int coo = is_myelin && nrm >= Rout, coi = is_extra && nrm < Rout;
+  const int ci = cio || cii, co = coo || coi;
+  const float dpi = fabsf(nrm - Rin), dpo = fabsf(nrm - Rout);
+  const float* k2 = A->pm_k2 + 4 * k;
+  const float p_in = fminf(1.0f, (is_intra ? k2[0] : k2[1]) * dpi);
+  const float p_out = fminf(1.0f, (is_myelin ? k2[2] : k2[3]) * dpo);
+  const int t_in = ci && u < p_in, t_out = co && u < p_out;
+  if (ci || co) (*hits)++;
+  int nc = c;
+  float db = 0.0f;
+  {
+    const float s = fmaxf(nrm, 1e-20f);
+    const float h0 = q[0] / s, h1 = q[1] / s;
+    const float rr = 2.0f * Rin - nrm;
+    if (ci && !t_in) { q[0] = h0 * rr; q[1] = h1 * rr; }
+    if (t_in) nc = cio ? n + k + 1 : k + 1;
+    db = db + ((ci && !t_in) ? -2.0f * dpi : 0.0f);
+  }
+  {
+    const float n2 = sqrtf(q[0] * q[0] + q[1] * q[1]);
+    const float s = fmaxf(n2, 1e-20f);
+    const float h0 = q[0] / s, h1 = q[1] / s;
+    const float rr = 2.0f * Rout - n2;
+    if (co && !t_out) { q[0] = h0 * rr; q[1] = h1 * rr; }
+    if (t_out) nc = coo ? 0 : n + k + 1;
+    db = db + ((co && !t_out) ? -2.0f * dpo : 0.0f);
+  }
+  if (t_in || t_out) (*ncross)++;
+  float xy[2] = {wrapL(q[0] + cx, L), wrapL(q[1] + cy, L)};
+  /* exterior clamp: argmin over every axon of d^2/R_out^2 inside its outer disc */
+  int kcl = 0;
+  float best = 0.0f;
+  for (int j = 0; j < n; ++j) {
+    const float fx = wrapL(xy[0] - A->centers[2 * j], L), fy = wrapL(xy[1] - A->centers[2 * j + 1], L);
+    const float d2 = fx * fx + fy * fy;
+    const float Ro = A->radii[j];
+    const float pen = Ro > 0.0f ? (d2 < Ro * Ro ? d2 / (Ro * Ro + 1e-30f) : 1.0f) : 1.0f;
+    if (j == 0 || pen < best) { best = pen; kcl = j; }
+  }
+  if (is_extra && best < 1.0f) {
+    const float Rc = A->radii[kcl];
+    const float qx = wrapL(xy[0] - A->centers[2 * kcl], L), qy = wrapL(xy[1] - A->centers[2 * kcl + 1], L);
+    const float dcl = sqrtf(qx * qx + qy * qy);
+    const float nx = xy[0] - qx, ny = xy[1] - qy;
+    const float drefl = 2.0f * Rc - dcl;
+    xy[0] = nx + qx * drefl / fmaxf(dcl, NUDGE);
+    xy[1] = ny + qy * drefl / fmaxf(dcl, NUDGE);
+    db = db + -2.0f * fmaxf(Rc - dcl, 0.0f);
+  }
+  /* wrong-side clamps of untransmitted intra / myelin walkers */
+  const int same = nc == c;
+  {
+    const float ex = wrapL(xy[0] - cx, L), ey = wrapL(xy[1] - cy, L);
+    const float de = sqrtf(ex * ex + ey * ey);
+    const float s = fmaxf(de, 1e-20f);
+    if (is_intra && same && de >= Rin) { xy[0] = cx + ex / s * (Rin - NUDGE); xy[1] = cy + ey / s * (Rin - NUDGE); }
+  }
+  {
+    const float ex = wrapL(xy[0] - cx, L), ey = wrapL(xy[1] - cy, L);
+    const float de = sqrtf(ex * ex + ey * ey);
+    const float s = fmaxf(de, 1e-20f);
+    if (is_myelin && same && de < Rin) { xy[0] = cx + ex / s * (Rin + NUDGE); xy[1] = cy + ey / s * (Rin + NUDGE); }
+  }
+  {
+    const float ex = wrapL(xy[0] - cx, L), ey = wrapL(xy[1] - cy, L);
+    const float de = sqrtf(ex * ex + ey * ey);
+    const float s = fmaxf(de, 1e-20f);
+    if (is_myelin && same && de >= Rout) { xy[0] = cx + ex / s * (Rout - NUDGE); xy[1] = cy + ey / s * (Rout - NUDGE); }
+  }
+  r[0] = xy[0];
+  r[1] = xy[1];
+  r[2] = r[2] + step[2];
+  /* intra / myelin walkers re-classified against their own axon */
+  const float fx = wrapL(r[0] - cx, L), fy = wrapL(r[1] - cy, L);
+  const float d2 = fx * fx + fy * fy;
+  if (!is_extra) {
+    if (d2 < Rin * Rin) nc = k + 1;
+    else if (d2 < Rout * Rout) nc = n + k + 1;
+  }
+  *cid = nc;
+  return db;
+}
+
+/* Start placement: segment j of pm_start is intra of axon j (j < n), myelin of axon
+   j - n (j < 2n); walkers past the last boundary are extra-axonal, uniform in the
+   cell outside every outer disc (geometries.py:3112-3210). */
+static int pm_start_oracle(const oracle_args* A, uint32_t w, float r[3]) {
+  const int n = A->n_obj;
+  const float L = A->cell;
+  int cid = 0, seg = -1;
+  for (int j = 0; j < 2 * n; ++j)
+    if ((int64_t)w >= A->pm_start[j] && (int64_t)w < A->pm_start[j + 1]) seg = j;
+  if (seg >= 0) cid = seg < n ? seg + 1 : n + (seg - n) + 1;
+  const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
+  r[0] = r[1] = r[2] = 0.0f;
+  for (uint32_t att = 0; att < 4096u; ++att) {
+    const uint32_t ctr[4] = {w, A->instance, att, 1u};
+    uint32_t o[4];
+    philox10(ctr, key[0], key[1], o);
+    const float u0 = unit_u(o[0]), u1 = unit_u(o[1]);
+    if (cid == 0) {
+      r[0] = (u0 - 0.5f) * L;
+      r[1] = (u1 - 0.5f) * L;
+      int out = 1;
+      for (int j = 0; j < n; ++j) {
+        const float fx = wrapL(r[0] - A->centers[2 * j], L), fy = wrapL(r[1] - A->centers[2 * j + 1], L);
+        if (!(fx * fx + fy * fy > A->radii[j] * A->radii[j])) out = 0;
+      }
+      if (out) return cid;
+    } else {
+      const int k = cid <= n ? cid - 1 : cid - n - 1;
+      const float Rin = A->pm_inner[k], Rout = A->radii[k];
+      const float rh = cid <= n ? Rin : Rout;
+      const float qx = (2.0f * u0 - 1.0f) * rh, qy = (2.0f * u1 - 1.0f) * rh;
+      const float d2 = qx * qx + qy * qy;
+      const int ok = cid <= n ? d2 < rh * rh : (d2 >= Rin * Rin && d2 < rh * rh);
+      if (ok) {
+        r[0] = qx + A->centers[2 * k];
+        r[1] = qy + A->centers[2 * k + 1];
+        return cid;
+      }
+    }
+  }
+  return cid;
+}
+
 /* classify_position of MyelinatedCylinder (geometries.py:1035-1052). */
 static int myel_classify(const float r[3], const oracle_args* A) {
   const float rr = r[0] * r[0] + r[1] * r[1];
@@ -868,8 +1029,13 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
     for (int64_t w = w0; w < w1; ++w) {
       float r[3];
       int cell[3] = {0, 0, 0};
-      if (A->r0) { r[0] = A->r0[w * 3]; r[1] = A->r0[w * 3 + 1]; r[2] = A->r0[w * 3 + 2]; }
+      const int pmyel = A->kind == K_PMYEL;
+      int pcid = 0;                                   /* packed myelin: compartment id */
+      if (pmyel) pcid = pm_start_oracle(A, (uint32_t)w, r);
+      else if (A->r0) { r[0] = A->r0[w * 3]; r[1] = A->r0[w * 3 + 1]; r[2] = A->r0[w * 3 + 2]; }
       else sample_start(A, (uint32_t)w, r);
+      const int pc0 = pm_to3(pcid, A->n_obj);
+      float uw[3] = {r[0], r[1], r[2]};               /* packed myelin: unwrapped position */
       if (packed && A->cell_form) {
         for (int d = 0; d < dim; ++d) {
           const float k = floorf(r[d] / L + 0.5f);
@@ -879,7 +1045,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       const int myel = A->kind == K_MYEL;
       int mstate = myel ? myel_classify(r, A) : 0;   /* carried compartment */
       const int c0 = mstate;
-      int comp = myel ? mstate : classify(r, A);
+      int comp = myel ? mstate : (pmyel ? pc0 : classify(r, A));
       if (O->comp_origin) O->comp_origin[w] = comp;
       for (int m = 0; m < M; ++m) { phi[m] = 0.0f; phi64[m] = 0.0; }
       float logw = 0.0f;
@@ -903,7 +1069,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
               philox10(ctr, key[0], key[1], rnd);
             }
             oracle_direction((gs & 1) ? rnd[2] : rnd[0], (gs & 1) ? rnd[3] : rnd[1], dir);
-            if (A->wall == W_PERMEATE || myel) {
+            if (A->wall == W_PERMEATE || myel || pmyel) {
               const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)gs, 2u};
               uint32_t o[4];
               philox10(ctr, key[0], key[1], o);
@@ -911,20 +1077,34 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             }
           }
           float dlog = 0.0f;
-          if (myel) myel_step(r, dir, u, A, &mstate, &hits, &ncross);
-          else dlog = substep(r, dir, u, A, &hits, &ncross);
-          if (A->wall != W_REFLECT) dlog = dlog * chi;
-          if (A->has_t2) {
-            /* myelin: dt / T2 of the compartment after the step (physics.py:657-659) */
-            const float q2 = myel ? A->m_q2[mstate] : A->dt_sub * A->inv_T2;
-            dlog = dlog - q2 * chi;
+          if (pmyel) {
+            /* make_packed_myelin_step_fn (physics.py:1033-1060): the walk, the unwrapped
+               displacement, then T2 of the new class, T1, and the walls' relaxivity */
+            const float r_old[3] = {r[0], r[1], r[2]};
+            const float db = pm_step_oracle(r, dir, u, A, &pcid, &hits, &ncross);
+            float dr[3] = {r[0] - r_old[0], r[1] - r_old[1], r[2] - r_old[2]};
+            for (int d = 0; d < 2; ++d) dr[d] = dr[d] - L * rintf(dr[d] / L);
+            for (int d = 0; d < 3; ++d) uw[d] = uw[d] + dr[d];
+            if (A->has_t2) dlog = dlog - A->pm_q2[pm_to3(pcid, A->n_obj)] * chi;
+            if (A->has_t1) dlog = dlog - A->dt_sub * A->inv_T1 * (1.0f - chi);
+            if (A->pm_rhoD > 0.0f) dlog = dlog + A->pm_rhoD * db * chi;
+          } else {
+            if (myel) myel_step(r, dir, u, A, &mstate, &hits, &ncross);
+            else dlog = substep(r, dir, u, A, &hits, &ncross);
+            if (A->wall != W_REFLECT) dlog = dlog * chi;
+            if (A->has_t2) {
+              /* myelin: dt / T2 of the compartment after the step (physics.py:657-659) */
+              const float q2 = myel ? A->m_q2[mstate] : A->dt_sub * A->inv_T2;
+              dlog = dlog - q2 * chi;
+            }
+            if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
           }
-          if (A->has_t1) dlog = dlog - (A->dt_sub * A->inv_T1) * (1.0f - chi);
           logw = logw + dlog;
           dlog_sample = dlog_sample + dlog;
           if (O->occ_full) occ = occ + occupancy_outside(r, A);
           /* phase of every measurement, right-endpoint rule */
           float ru[3] = {r[0], r[1], r[2]};
+          if (pmyel) { ru[0] = uw[0]; ru[1] = uw[1]; ru[2] = uw[2]; }
           if (packed && A->cell_form)
             for (int d = 0; d < 3; ++d) ru[d] = r[d] + L * (float)cell[d];
           for (int m = 0; m < M; ++m) {
@@ -945,10 +1125,12 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             O->pos_full[((size_t)w * T + t) * 3 + d] = (packed && A->cell_form) ? r[d] + L * (float)cell[d] : r[d];
         if (O->dlog_full) O->dlog_full[(size_t)w * T + t] = dlog_sample;
         if (O->occ_full) O->occ_full[(size_t)w * T + t] = occ / (float)NS;
-        if (O->comp_full || (O->comp_final && t == T - 1)) comp = myel ? mstate : classify(r, A);
+        if (O->comp_full || (O->comp_final && t == T - 1))
+          comp = myel ? mstate : (pmyel ? pm_to3(pcid, A->n_obj) : classify(r, A));
         if (O->comp_full) O->comp_full[(size_t)w * T + t] = comp;
         if (e_next < A->n_echo && A->echo_idx[e_next] == t) {
-          const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0);
+          const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0) *
+                            ((pmyel && pc0 == 1) ? (double)A->pm_wmy : 1.0);
           for (int m = 0; m < M; ++m) {
             const double ph = A->phase64 ? phi64[m] : (double)phi[m];
             loc[((size_t)e_next * M + m) * 2] += wg * cos(ph);
@@ -958,7 +1140,8 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
         }
       }
       if (A->n_echo == 0) {
-        const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0);
+        const double wg = exp((double)logw) * (myel ? (double)A->m_wf[c0] : 1.0) *
+                          ((pmyel && pc0 == 1) ? (double)A->pm_wmy : 1.0);
         for (int m = 0; m < M; ++m) {
           const double ph = A->phase64 ? phi64[m] : (double)phi[m];
           loc[m * 2] += wg * cos(ph);
@@ -974,7 +1157,8 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       if (O->phi)
         for (int m = 0; m < M; ++m) O->phi[w * M + m] = A->phase64 ? (float)phi64[m] : phi[m];
       hits_loc += hits;
-      wsum_loc += A->kind == K_MYEL ? (double)A->m_wf[c0] : 1.0;
+      wsum_loc += A->kind == K_MYEL ? (double)A->m_wf[c0]
+                                    : ((pmyel && pc0 == 1) ? (double)A->pm_wmy : 1.0);
     }
 #pragma omp critical
     {

@@ -148,3 +148,96 @@ def test_myelin_only_single_exponential_and_unity(gpu):
     assert abs(s[0] - expect) < 1e-4 * expect
     s1 = gpu.simulate(20_000, None, _b0(500, TE), _geom(water_fractions=(0.3, 0.5, 0.2)), seed=1)
     assert abs(s1[0] - 1.0) < 1e-6
+
+
+# ------------------------------------------------------ PackedMyelinatedCylinders
+def _pmyel(n=8, packing=0.5, **kw):
+    rng = np.random.default_rng(0)
+    ri, gr, c = ds.pack_myelinated_cylinders(rng.uniform(1e-6, 2e-6, n), 0.7, packing, seed=1)
+    L = float(np.sqrt(np.pi * np.sum((ri / gr) ** 2) / packing))
+    return ds.PackedMyelinatedCylinders(ri, gr, c, L, N_max=16, **kw)
+
+
+PM_CASES = [
+    ("plain_t2", dict(D_myelin=3e-10, T2_intra=0.08, T2_myelin=0.015, T2_extra=0.1), None),
+    ("perm_rho_t1", dict(D_myelin=5e-10, kappa_inner=3e-5, kappa_outer=2e-5, rho_outer=2e-5,
+                         rho_inner=1e-5, T2_myelin=0.02), 0.9),
+]
+
+
+@pytest.mark.parametrize("name,kw,T1", PM_CASES, ids=[c[0] for c in PM_CASES])
+def test_packed_myelin_bit_exact(gpu, oracle, name, kw, T1):
+    """In-kernel placement (grid-accelerated exterior test) and the walk (per-axon
+    membranes, exterior mirror over the grid cell's axons, clamps, unwrapped phase,
+    per-class T2, T1 on stored intervals, wall relaxivity with its sub-steps) against
+    the oracle's all-axon restatement: bit-exact states, signals within 1e-5."""
+    g = _pmyel(**kw)
+    wf = small_pgse()
+    chi = np.ones(wf.G.shape[1], np.float32)
+    chi[12:18] = 0.0
+    wf.chi_perp = chi
+    N = 1500
+    plan = ds.make_walk_plan(g, None, wf.dt, T1=T1)
+    print(f"packed myelin {name} n_sub={plan.n_sub} N={N}", flush=True)
+    r = gpu.walk(g, None, wf, N, seed=7, T1=T1, outputs=("final_pos", "comp_origin", "comp_final",
+                                                         "n_cross", "log_w", "signal_im"))
+    o = oracle.run_oracle(g, None, wf, N, seed=7, T1=T1)
+    np.testing.assert_array_equal(r["final_pos"], o["final_pos"])
+    np.testing.assert_array_equal(r["comp_origin"], o["comp_origin"])
+    np.testing.assert_array_equal(r["comp_final"], o["comp_final"])
+    np.testing.assert_array_equal(r["n_cross"], o["n_cross"])
+    np.testing.assert_array_equal(r["log_w"], o["log_w"])
+    assert np.max(np.abs(r["signal"] - o["signal"])) <= 1e-5
+    assert np.max(np.abs(r["signal_im"] - o["signal_im"])) <= 1e-5
+    if "kappa_inner" in kw:
+        assert r["n_cross"].sum() > 0
+        assert (plan.n_sub > 1) and np.any(r["log_w"] < 0)
+
+
+def test_packed_myelin_single_axon_intra_matches_cylinder(gpu):
+    """One axon in a wide cell with extra-axonal and myelin water suppressed by a 1 us
+    T2: S = f_intra S_Cylinder(R_in) per b-value (f_intra the intra share of the
+    spin weights), within 4 combined MC sigma or 0.02 (reference test_myelin.py B)."""
+    ri, gr, c = ds.pack_myelinated_cylinders([4e-6], 0.7, 0.1, seed=0)
+    L = float(np.sqrt(np.pi * (ri[0] / gr[0]) ** 2 / 0.1))
+    g = ds.PackedMyelinatedCylinders(ri, gr, c, L, N_max=4, T2_myelin=1e-6, T2_extra=1e-6)
+    N = 400_000
+    n_in, n_my, n_ex = g.start_counts(N)
+    f_in = n_in.sum() / (n_in.sum() + 0.4 * n_my.sum() + n_ex)
+    cyl = ds.Cylinder(4e-6, [0, 0, 1])
+    tol = max(4.0 * np.sqrt(2) / np.sqrt(n_in.sum()), 0.02)
+    for k, d in enumerate(([1, 0, 0], [0, 0, 1])):
+        wf = _pgse_b(10e-3, 30e-3, 2000, d, [0.5e9, 1.0e9, 2.0e9])
+        s_pm = gpu.simulate(N, None, wf, g, seed=SEED + k) / f_in
+        s_c = gpu.simulate(int(n_in.sum()), 2e-9, wf, cyl, seed=SEED + k)
+        print(f"packed single axon dir {d}: {s_pm} vs cylinder {s_c}", flush=True)
+        assert np.all(np.abs(s_pm - s_c) <= tol)
+
+
+def test_packed_myelin_extra_axonal_free_along_axis(gpu):
+    """Extra-axonal water only (intra and myelin suppressed by a 1 us T2): along the
+    axis the walls do not hinder, so S(b) / S(0) = exp(-b D_extra) (ADC within 3 %)."""
+    D_extra = 2e-9
+    g = _pmyel(D_extra=D_extra, T2_intra=1e-6, T2_myelin=1e-6)
+    b = np.array([0.0, 0.5e9, 1.0e9, 1.5e9])
+    wf = _pgse_b(5e-3, 20e-3, 1000, [0, 0, 1], b)
+    s = gpu.simulate(200_000, None, wf, g, seed=SEED)
+    adc = -np.polyfit(b[1:], np.log(s[1:]) - np.log(s[0]), 1)[0]
+    assert abs(adc - D_extra) / D_extra < 0.03, adc
+
+
+def test_packed_myelin_fractions_and_exchange(gpu):
+    """Start fractions equal the volume fractions within 0.01 (reference
+    test_compartment_tagging.py:169-184 for the single axon); 'final' is the last
+    sample of 'full'; permeable walls let walkers change compartment."""
+    wf = ds.Waveform(G=np.zeros((1, 60, 3), np.float32), dt=1e-4, echo_idx=59)
+    g = _pmyel()
+    _, orig, _ = gpu.simulate(100_000, None, wf, g, seed=0, return_compartments="final")
+    for cid, label in enumerate(("intra", "myelin", "extra")):
+        assert abs((orig == cid).mean() - g.volume_fraction(label)) < 0.01
+    gp = _pmyel(D_myelin=1e-9, kappa_inner=1e-4, kappa_outer=1e-4)
+    _, o_full, c_full = gpu.simulate(3000, None, wf, gp, seed=2, return_compartments="full")
+    _, o_fin, c_fin = gpu.simulate(3000, None, wf, gp, seed=2, return_compartments="final")
+    np.testing.assert_array_equal(o_full, o_fin)
+    np.testing.assert_array_equal(c_full[:, -1], c_fin)
+    assert np.any(c_full != o_full[:, None], axis=1).mean() > 0.01

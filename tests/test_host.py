@@ -304,3 +304,53 @@ def test_myelinated_cylinder_unsupported_paths_raise():
         ds.MyelinatedCylinder(5e-6, 3e-6, [0, 0, 1], D_intra=2e-9, D_extra=2e-9)
     assert mc.water_fractions == (1.0, 0.4, 1.0) and mc.D_myelin == 0.0
     assert abs(sum(mc.volume_fraction(c) for c in ("intra", "myelin", "extra")) - 1.0) < 1e-12
+
+
+def _pmyel_geom(**kw):
+    rng = np.random.default_rng(0)
+    ri, gr, c = ds.pack_myelinated_cylinders(rng.uniform(1e-6, 2e-6, 8), 0.7, 0.5, seed=1)
+    L = float(np.sqrt(np.pi * np.sum((ri / gr) ** 2) / 0.5))
+    return ds.PackedMyelinatedCylinders(ri, gr, c, L, N_max=16, **kw)
+
+
+@pytest.mark.parametrize("kw", [
+    dict(),
+    dict(D_myelin=3e-10, T2_intra=0.08, T2_extra=0.1, kappa_inner=2e-5, kappa_outer=1e-5),
+    dict(rho_outer=2e-5, rho_inner=1e-5, D_extra=[1.5e-9, 2e-9, 2.5e-9, 2e-9, 2e-9, 2e-9, 2e-9, 2e-9],
+         T2_myelin=0.015),
+], ids=["plain", "perm_t2", "rho_per_axon"])
+@pytest.mark.parametrize("dt", [1e-4, 2.5e-5])
+def test_packed_myelin_plan_matches_oracle(oracle, kw, dt):
+    """Product plan and the oracle's restatement agree on the surface sub-steps, the
+    per-axon float32 tables, the per-class dt/T2 and the start boundaries."""
+    g = _pmyel_geom(**kw)
+    p = ds.make_walk_plan(g, None, dt, T1=0.9)
+    o = oracle.oracle_plan(g, None, dt, T1=0.9)
+    assert p.n_sub == o["n_sub"] and p.kind == o["kind"] == 11
+    for k in ("gamma_dt_sub", "dt_sub", "inv_T1", "nudge", "cell"):
+        assert np.float32(getattr(p, k)) == np.float32(o[k]), k
+    pm = p.packed_myelin
+    np.testing.assert_array_equal(pm["step_len"], o["pm_step"])
+    np.testing.assert_array_equal(pm["two_kappa_over_D"], o["pm_k2"])
+    np.testing.assert_array_equal(pm["inner"], o["pm_inner"])
+    assert [np.float32(v) for v in pm["dt_over_T2"]] == [np.float32(v) for v in o["pm_q2"]]
+    assert np.float32(pm["rho_over_D"]) == np.float32(o["pm_rhoD"])
+    assert (p.n_sub > 1) == ("rho_outer" in kw)
+    for N in (1, 500, 12345):
+        n_in, n_my, _ = pm["start_counts"](N)
+        mine = np.concatenate([[0], np.cumsum(np.concatenate([n_in, n_my]))])
+        np.testing.assert_array_equal(mine, oracle.packed_myelin_start(o, N))
+
+
+def test_packed_myelin_unsupported_paths_raise():
+    g = _pmyel_geom()
+    wf = ds.Waveform(G=np.zeros((1, 20, 3), np.float32), dt=1e-4, echo_idx=19)
+    with pytest.raises(NotImplementedError, match="r0"):
+        ds.simulate(10, None, wf, g, r0=np.zeros((10, 3)))
+    with pytest.raises(NotImplementedError, match="return_positions='full'"):
+        ds.simulate(10, None, wf, g, return_positions="full")
+    with pytest.raises(NotImplementedError, match="PackedMyelinatedCylinders"):
+        ds.simulate(10, None, wf, g, engine="replay")
+    with pytest.raises(ValueError):
+        ds.PackedMyelinatedCylinders([1e-6, 2e-6], 0.7, np.zeros((3, 2)), 1e-5)
+    assert abs(sum(g.volume_fraction(c) for c in ("intra", "myelin", "extra")) - 1.0) < 1e-12

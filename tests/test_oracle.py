@@ -481,3 +481,68 @@ def test_myelin_start_allotment_stuck_pool_and_weighted_t2(oracle):
     T2 = np.array([0.08, 0.012, 0.1])
     expect = np.sum(counts * w * np.exp(-n_t * dt / T2)) / np.sum(counts * w)
     assert abs(res["signal"][0] - expect) < 2e-6 * expect
+
+
+# ------------------------------------------------------ PackedMyelinatedCylinders
+def _pmyel(n=8, seed=1, **kw):
+    rng = np.random.default_rng(0)
+    ri, gr, c = ds.pack_myelinated_cylinders(rng.uniform(1e-6, 2e-6, n), 0.7, 0.5, seed=seed)
+    L = float(np.sqrt(np.pi * np.sum((ri / gr) ** 2) / 0.5))
+    return ds.PackedMyelinatedCylinders(ri, gr, c, L, N_max=16, **kw), L
+
+
+def _axon_dist(g, L, pos):
+    n = g.N_actual
+    d = pos[..., None, :2].astype(np.float64) - g._centers_np[:n]
+    d -= L * np.floor(d / L + 0.5)
+    return np.sqrt(np.sum(d ** 2, -1))
+
+
+def test_packed_myelin_allotment_pools_and_weighted_t2(oracle):
+    """Start classes follow the area allotment in index order (intra of each axon,
+    myelin of each axon, the rest extra-axonal); D_myelin = 0 water stays put up
+    to the cell re-wrap's rounding; intra walkers never leave their axon; with
+    impermeable walls the b = 0 signal is the spin-weighted mixture
+    sum_c n_c w_c exp(-n_t dt/T2_c) / sum_c n_c w_c with w = (1, 0.4, 1)."""
+    g, L = _pmyel(D_myelin=0.0, T2_intra=0.08, T2_myelin=0.015, T2_extra=0.1)
+    n_t, dt, N = 100, 1e-4, 2000
+    wf = ds.Waveform(G=np.zeros((1, n_t, 3), np.float32), dt=dt, echo_idx=n_t - 1)
+    res = oracle.run_oracle(g, None, wf, N, seed=3, per_sample=True)
+    n_in, n_my, n_ex = g.start_counts(N)
+    co = res["comp_origin"]
+    np.testing.assert_array_equal(np.bincount(co, minlength=3), [n_in.sum(), n_my.sum(), n_ex])
+    assert np.all(co[:n_in.sum()] == 0) and np.all(co[n_in.sum():n_in.sum() + n_my.sum()] == 1)
+    pos = res["pos_full"]
+    assert np.abs(pos[co == 1] - pos[co == 1][:, :1]).max() < 1e-11
+    dist = _axon_dist(g, L, pos[co == 0])
+    assert np.all(np.min(dist - g._inner_radii_np[:g.N_actual], axis=-1) < 0)
+    assert np.all(res["comp_final"] == co)
+    cnt = np.bincount(co, minlength=3)
+    w = np.array([1.0, 0.4, 1.0])
+    T2 = np.array([0.08, 0.015, 0.1])
+    expect = np.sum(cnt * w * np.exp(-n_t * dt / T2)) / np.sum(cnt * w)
+    assert abs(res["signal"][0] - expect) < 1e-6 * expect
+
+
+def test_packed_myelin_phase_uses_unwrapped_position(oracle):
+    """With G nonzero at one sample only, phi = GAMMA dt G . r_unwrapped(t); the
+    unwrapped position moves by at most three steps between samples (a step plus
+    the exterior mirror, which moves a walker by twice its penetration), differs
+    from the in-cell position by whole cells, and some walkers do cross the cell
+    edge (physics.py:1040-1047)."""
+    g, L = _pmyel(D_extra=3e-9)
+    n_t, dt, N = 400, 1e-4, 1500
+    G = np.zeros((4, n_t, 3), np.float32)
+    G[0, -1, 0] = G[1, -2, 0] = 0.05
+    G[2, -1, 1] = G[3, -2, 1] = 0.05
+    wf = ds.Waveform(G=G, dt=dt, echo_idx=n_t - 1)
+    p = oracle.oracle_plan(g, None, dt)
+    res = oracle.run_oracle(g, None, wf, N, seed=5, per_sample=True, phase64=True)
+    uw = res["phi"].astype(np.float64) / (p["gamma_dt_sub"] * 0.05)     # (N, 4)
+    ux_t, ux_p, uy_t, uy_p = uw.T
+    step = float(p["pm_step"][:, 2].max())
+    assert np.all(np.hypot(ux_t - ux_p, uy_t - uy_p) <= 3.0 * step * 1.001 + 1e-12)
+    pos = res["pos_full"][:, -1, :2].astype(np.float64)
+    m = np.stack([ux_t - pos[:, 0], uy_t - pos[:, 1]], 1) / L
+    assert np.all(np.abs(m - np.round(m)) < 1e-3)
+    assert np.any(np.round(m) != 0)
