@@ -33,15 +33,18 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float bf2f(__bf16 v) { return (float)v; }
 __device__ __forceinline__ __bf16 f2bf(float v) { return (__bf16)v; }
 
-// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi).
-__device__ __forceinline__ float gelu(float x) {
+// GELU, tanh form: 0.5 x (1 + tanh(c (x + 0.044715 x^3))), c = sqrt(2/pi), evaluated
+// through the identity 0.5 (1 + tanh(u)) = 1 / (1 + exp(-2u)) = s: GELU = x s and
+// GELU' = s + 2 x s (1 - s) c (1 + 0.134145 x^2).  exp via the hardware exp2
+// (relative error ~1e-7, far below the bf16 rounding of the activations).
+__device__ __forceinline__ float gelu_sig(float x) {
   const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
-  return 0.5f * x * (1.0f + tanhf(u));
+  return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-2.8853900818f * u));   // -2u log2(e)
 }
+__device__ __forceinline__ float gelu(float x) { return x * gelu_sig(x); }
 __device__ __forceinline__ float gelu_grad(float x) {
-  const float u = 0.7978845608f * (x + 0.044715f * x * x * x);
-  const float th = tanhf(u);
-  return 0.5f * (1.0f + th) + 0.5f * x * (1.0f - th * th) * 0.7978845608f * (1.0f + 0.134145f * x * x);
+  const float s = gelu_sig(x);
+  return s + 2.0f * x * s * (1.0f - s) * 0.7978845608f * (1.0f + 0.134145f * x * x);
 }
 
 enum Epi { EPI_FWD = 0, EPI_DX = 1, EPI_DW = 2 };
@@ -135,6 +138,84 @@ __global__ __launch_bounds__(256) void gemm_bf16(int M, int N, int K, const __bf
       }
 }
 
+// Inference GEMM: Y = bf16(GELU(A . W + bias)) with W given transposed (Wt[n][k],
+// row-major, ldb = K) so both operand tiles load as contiguous 16-byte rows.
+// 128 x 128 tiles, K in steps of 32, 256 threads = 2 x 2 wavefronts of 64 x 64
+// (4 x 4 MFMA 16x16x32 tiles each), double-buffered LDS fed by a register
+// prefetch of the next K step.  Per output the MFMA sequence (ascending K steps of
+// 32) and the epilogue are those of gemm_bf16<EPI_FWD>, so the two paths give the
+// same bits.  Tile order is XCD-aware: consecutive workgroups land on the 8 XCDs in
+// turn, so each XCD is given whole rows of N tiles and reuses its A strip from L2.
+// M a multiple of 128, N of 128, K of 32 (checked on the host).
+constexpr int BM = 128, BN = 128, BK = 32, BLDK = 40;
+
+__global__ __launch_bounds__(256) void gemm_fwd_big(int M, int N, int K, const __bf16* __restrict__ A, int lda,
+                                                    const __bf16* __restrict__ Wt, int ldb,
+                                                    const float* __restrict__ bias, __bf16* __restrict__ Y,
+                                                    int ldy) {
+  __shared__ __bf16 As[2][BM][BLDK];
+  __shared__ __bf16 Bs[2][BN][BLDK];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nN = N / BN, nMt = M / BM;
+  int tile = blockIdx.x;
+  if (nMt % 8 == 0) {
+    const int xcd = blockIdx.x & 7, j = blockIdx.x >> 3;
+    tile = ((j / nN) * 8 + xcd) * nN + j % nN;
+  }
+  const int m0 = (tile / nN) * BM, n0 = (tile % nN) * BN;
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  f32x4 acc[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const int lr = tid >> 2, lc = (tid & 3) * 8;
+  const __bf16* a0 = A + (size_t)(m0 + lr) * lda + lc;
+  const __bf16* a1 = A + (size_t)(m0 + lr + 64) * lda + lc;
+  const __bf16* b0 = Wt + (size_t)(n0 + lr) * ldb + lc;
+  const __bf16* b1 = Wt + (size_t)(n0 + lr + 64) * ldb + lc;
+  bf16x8 ra0 = *reinterpret_cast<const bf16x8*>(a0), ra1 = *reinterpret_cast<const bf16x8*>(a1);
+  bf16x8 rb0 = *reinterpret_cast<const bf16x8*>(b0), rb1 = *reinterpret_cast<const bf16x8*>(b1);
+  *reinterpret_cast<bf16x8*>(&As[0][lr][lc]) = ra0;
+  *reinterpret_cast<bf16x8*>(&As[0][lr + 64][lc]) = ra1;
+  *reinterpret_cast<bf16x8*>(&Bs[0][lr][lc]) = rb0;
+  *reinterpret_cast<bf16x8*>(&Bs[0][lr + 64][lc]) = rb1;
+  __syncthreads();
+  const int nk = K / BK;
+  const int kq = (lane >> 4) * 8, rq = lane & 15;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    const bool more = kt + 1 < nk;
+    if (more) {
+      const int ko = (kt + 1) * BK;
+      ra0 = *reinterpret_cast<const bf16x8*>(a0 + ko);
+      ra1 = *reinterpret_cast<const bf16x8*>(a1 + ko);
+      rb0 = *reinterpret_cast<const bf16x8*>(b0 + ko);
+      rb1 = *reinterpret_cast<const bf16x8*>(b1 + ko);
+    }
+    bf16x8 af[4], bfr[4];
+    for (int i = 0; i < 4; ++i) af[i] = *reinterpret_cast<const bf16x8*>(&As[cur][wm + 16 * i + rq][kq]);
+    for (int j = 0; j < 4; ++j) bfr[j] = *reinterpret_cast<const bf16x8*>(&Bs[cur][wn + 16 * j + rq][kq]);
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    if (more) {
+      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr][lc]) = ra0;
+      *reinterpret_cast<bf16x8*>(&As[cur ^ 1][lr + 64][lc]) = ra1;
+      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr][lc]) = rb0;
+      *reinterpret_cast<bf16x8*>(&Bs[cur ^ 1][lr + 64][lc]) = rb1;
+    }
+    __syncthreads();
+  }
+  for (int j = 0; j < 4; ++j) {
+    const int n = n0 + wn + 16 * j + (lane & 15);
+    const float bn = bias[n];
+    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+        Y[(size_t)m * ldy + n] = f2bf(gelu(acc[i][j][r] + bn));
+      }
+  }
+}
+
 // y[b] = Y[b] . w + b0; with targets also the squared error and dy = 2 (y - t) / B.
 __global__ void out_fwd_kernel(int B, int H, const __bf16* __restrict__ Y, const float* __restrict__ w,
                                const float* __restrict__ b0, const float* __restrict__ t,
@@ -214,7 +295,8 @@ __global__ void out_bwd_w_sum_kernel(int B, int H, int n_chunks, const float* __
 __global__ void adam_kernel(long long n, float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m1, float* __restrict__ m2, float lr, float b1, float b2,
                             float eps, float c1, float c2, const int* __restrict__ bf_index,
-                            __bf16* __restrict__ pbf) {
+                            __bf16* __restrict__ pbf, const int* __restrict__ bfT_index,
+                            __bf16* __restrict__ pbfT) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -226,15 +308,22 @@ __global__ void adam_kernel(long long n, float* __restrict__ p, const float* __r
   const float np = p[i] - upd;
   p[i] = np;
   const int j = bf_index[i];
-  if (j >= 0) pbf[j] = f2bf(np);
+  if (j >= 0) {
+    pbf[j] = f2bf(np);
+    pbfT[bfT_index[i]] = f2bf(np);
+  }
 }
 
 __global__ void to_bf16_kernel(long long n, const float* __restrict__ p, const int* __restrict__ bf_index,
-                               __bf16* __restrict__ pbf) {
+                               __bf16* __restrict__ pbf, const int* __restrict__ bfT_index,
+                               __bf16* __restrict__ pbfT) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int j = bf_index[i];
-  if (j >= 0) pbf[j] = f2bf(p[i]);
+  if (j >= 0) {
+    pbf[j] = f2bf(p[i]);
+    pbfT[bfT_index[i]] = f2bf(p[i]);
+  }
 }
 
 // Gather a training batch: rows idx[b] of the resident dataset.
@@ -269,6 +358,8 @@ struct dmw_mlp {
   float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
   int* bf_index = nullptr;
   __bf16* pbf = nullptr;
+  int* bfT_index = nullptr;  // W_l element (k, n) -> its place in the transposed copy (n, k)
+  __bf16* pbfT = nullptr;    // bf16 W_l^T, the inference GEMM's B operand
   std::vector<__bf16*> Y;   // Y[0] = input batch (B x k0), Y[l+1] = GELU(Z_l) (B x H)
   std::vector<float*> Z;    // Z[l] (B x H)
   __bf16 *dZa = nullptr, *dZb = nullptr;
@@ -321,11 +412,20 @@ hipError_t gemm(hipStream_t st, int M, int N, int K, const __bf16* A, int lda, c
 unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 
 // Forward pass of a batch already in m->Y[0]; logits in m->out; with targets
-// also dy and squared errors.
+// also dy and squared errors.  Without targets (inference) the pre-activations
+// are not kept, and batches of 128-row tiles take the large-tile GEMM.
 int forward(dmw_mlp* m, int B, const float* targets_dev) {
   hipStream_t st = m->st;
+  const bool big = targets_dev == nullptr && B % dmw::BM == 0 && m->H % dmw::BN == 0;
   for (int l = 0; l < m->depth; ++l) {
     const int K = l == 0 ? m->k0 : m->H;
+    if (big) {
+      const unsigned n_tiles = (unsigned)((B / dmw::BM) * (m->H / dmw::BN));
+      hipLaunchKernelGGL(dmw::gemm_fwd_big, dim3(n_tiles), dim3(256), 0, st, B, m->H, K, m->Y[l], K,
+                         m->pbfT + m->wbf_off[l], K, m->p + m->b_off[l], m->Y[l + 1], m->H);
+      MLP_TRY(hipGetLastError(), "mlp forward gemm");
+      continue;
+    }
     MLP_TRY((gemm<dmw::EPI_FWD, false, false>(st, B, m->H, K, m->Y[l], K, m->pbf + m->wbf_off[l], m->H,
                                               m->p + m->b_off[l], m->Z[l], m->Y[l + 1], nullptr, nullptr,
                                               m->H)),
@@ -432,6 +532,8 @@ int dmw_mlp_create(dmw_mlp** out, int32_t n_in, int32_t width, int32_t depth, in
   MC(m->alloc(&m->m2, m->n_params), "hipMalloc");
   MC(m->alloc(&m->bf_index, m->n_params), "hipMalloc");
   MC(m->alloc(&m->pbf, m->n_bf), "hipMalloc");
+  MC(m->alloc(&m->bfT_index, m->n_params), "hipMalloc");
+  MC(m->alloc(&m->pbfT, m->n_bf), "hipMalloc");
   m->Y.resize(depth + 1);
   m->Z.resize(depth);
   MC(m->alloc(&m->Y[0], (size_t)max_batch * m->k0), "hipMalloc");
@@ -455,6 +557,14 @@ int dmw_mlp_create(dmw_mlp** out, int32_t n_in, int32_t width, int32_t depth, in
     for (long long i = 0; i < (long long)K * width; ++i) idx[m->w_off[l] + i] = (int)(m->wbf_off[l] + i);
   }
   MC(hipMemcpy(m->bf_index, idx.data(), idx.size() * 4, hipMemcpyHostToDevice), "H2D");
+  std::vector<int> idxT(m->n_params, -1);
+  for (int l = 0; l < depth; ++l) {
+    const int K = l == 0 ? m->k0 : width;
+    for (int k = 0; k < K; ++k)
+      for (int n = 0; n < width; ++n)
+        idxT[m->w_off[l] + (long long)k * width + n] = (int)(m->wbf_off[l] + (long long)n * K + k);
+  }
+  MC(hipMemcpy(m->bfT_index, idxT.data(), idxT.size() * 4, hipMemcpyHostToDevice), "H2D");
   std::vector<float> zeros(m->n_params, 0.0f);
   MC(hipMemcpy(m->p, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
   MC(hipMemcpy(m->m1, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
@@ -499,7 +609,7 @@ int dmw_mlp_set_params(dmw_mlp* m, const float* params, int64_t n) {
   MLP_TRY(hipMemcpy(m->m2, zeros.data(), zeros.size() * 4, hipMemcpyHostToDevice), "H2D");
   m->adam_t = 0;
   hipLaunchKernelGGL(dmw::to_bf16_kernel, dim3(blocks(m->n_params, 256)), dim3(256), 0, m->st, m->n_params, m->p,
-                     m->bf_index, m->pbf);
+                     m->bf_index, m->pbf, m->bfT_index, m->pbfT);
   MLP_TRY(hipGetLastError(), "to_bf16");
   MLP_TRY(hipStreamSynchronize(m->st), "mlp set params");
   return DMW_OK;
@@ -617,7 +727,7 @@ int dmw_mlp_train(dmw_mlp* m, int32_t batch, int32_t n_steps, const int32_t* ind
     m->adam_t += 1;
     const float c1 = 1.0f - std::pow(beta1, (float)m->adam_t), c2 = 1.0f - std::pow(beta2, (float)m->adam_t);
     hipLaunchKernelGGL(dmw::adam_kernel, dim3(blocks(m->n_params, 256)), dim3(256), 0, m->st, m->n_params, m->p,
-                       m->g, m->m1, m->m2, lr[s], beta1, beta2, eps, c1, c2, m->bf_index, m->pbf);
+                       m->g, m->m1, m->m2, lr[s], beta1, beta2, eps, c1, c2, m->bf_index, m->pbf, m->bfT_index, m->pbfT);
     MLP_TRY(hipGetLastError(), "adam");
     if (losses) {
       MLP_TRY(hipMemcpyAsync(sq_host.data(), m->sq, (size_t)batch * 4, hipMemcpyDeviceToHost, m->st), "D2H");
