@@ -43,6 +43,7 @@
 #include <utility>
 
 #include "dmw_rng.h"
+#include "rn_arith.h"
 
 #pragma clang fp contract(off)
 
@@ -199,23 +200,23 @@ __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float
                                              uint32_t& hits) {
   const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
   const float RR = R * R;
-  const float sl = sqrtf(sx * sx + sy * sy);
+  const float sl = sqrt_rn(sx * sx + sy * sy);
   float dx = 0.0f, dy = 0.0f;
-  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; }
+  if (sl > 0.0f) { dx = div_rn(sx, sl); dy = div_rn(sy, sl); }
   float rem = sl, dsum = 0.0f;
   for (int it = 0; it < 10; ++it) {
     const float dp = dx * x + dy * y;
     const float disc = fmaxf(dp * dp - ((x * x + y * y) - RR), 0.0f);
-    const float sq = sqrtf(disc);
+    const float sq = sqrt_rn(disc);
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem) && (sl > 0.0f))) break;
     const float hx = x + d * dx, hy = y + d * dy;
-    const float nx = hx / R, ny = hy / R;
+    const float nx = div_rn(hx, R), ny = div_rn(hy, R);
     const float tdd = 2.0f * (dx * nx + dy * ny);
     float rx = dx - tdd * nx, ry = dy - tdd * ny;
-    const float rn = sqrtf(rx * rx + ry * ry);
-    rx = rx / rn; ry = ry / rn;
-    dsum = dsum + (rem - d) * (sq / R);
+    const float rn = sqrt_rn(rx * rx + ry * ry);
+    rx = div_rn(rx, rn); ry = div_rn(ry, rn);
+    dsum = dsum + (rem - d) * (div_rn(sq, R));
     x = hx - NUDGE * nx;
     y = hy - NUDGE * ny;
     dx = rx; dy = ry;
@@ -224,11 +225,11 @@ __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float
   }
   const float rm = fmaxf(rem, 0.0f);
   float fx = x + dx * rm, fy = y + dy * rm;
-  const float fn = sqrtf(fx * fx + fy * fy);
+  const float fn = sqrt_rn(fx * fx + fy * fy);
   if (fn >= R) {
     const float s = R - NUDGE;
-    fx = (fx * s) / fn;
-    fy = (fy * s) / fn;
+    fx = div_rn(fx * s, fn);
+    fy = div_rn(fy * s, fn);
   }
   x = fx; y = fy;
   return dsum;
@@ -239,23 +240,23 @@ __device__ __forceinline__ float cyl_permeate(float& x, float& y, float sx, floa
                                               const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
   const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
   const float RR = R * R;
-  const float sl = sqrtf(sx * sx + sy * sy);
+  const float sl = sqrt_rn(sx * sx + sy * sy);
   float dx = 0.0f, dy = 0.0f;
-  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; }
+  if (sl > 0.0f) { dx = div_rn(sx, sl); dy = div_rn(sy, sl); }
   const float dp = dx * x + dy * y;
   const float rr = x * x + y * y;
   const float disc = dp * dp - (rr - RR);
-  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const float sq = sqrt_rn(fmaxf(disc, 0.0f));
   const bool inside = rr < RR;
   const float t_hit = inside ? (-dp + sq) : (-dp - sq);
   const bool any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
   if (!any_hit) { x = x + sx; y = y + sy; return 0.0f; }
   hits++;
   const float hx = x + t_hit * dx, hy = y + t_hit * dy;
-  const float hl = fmaxf(sqrtf(hx * hx + hy * hy), 1e-30f);
-  const float nx = hx / hl, ny = hy / hl;
+  const float hl = fmaxf(sqrt_rn(hx * hx + hy * hy), 1e-30f);
+  const float nx = div_rn(hx, hl), ny = div_rn(hy, hl);
   const float remaining = sl - t_hit;
-  const float dperp = remaining * (sq / R);
+  const float dperp = remaining * (div_rn(sq, R));
   const float p = fminf(1.0f, (2.0f * a.kappaD) * dperp);
   if (ps.get() < p) {
     x = x + sx; y = y + sy; ncross++;
@@ -263,18 +264,18 @@ __device__ __forceinline__ float cyl_permeate(float& x, float& y, float sx, floa
   }
   const float tdd = 2.0f * (dx * nx + dy * ny);
   float rx = dx - tdd * nx, ry = dy - tdd * ny;
-  const float rn = sqrtf(rx * rx + ry * ry);
-  rx = rx / rn; ry = ry / rn;
+  const float rn = sqrt_rn(rx * rx + ry * ry);
+  rx = div_rn(rx, rn); ry = div_rn(ry, rn);
   const float sxh = R * nx, syh = R * ny;
   const float qx = inside ? sxh - NUDGE * nx : sxh + NUDGE * nx;
   const float qy = inside ? syh - NUDGE * ny : syh + NUDGE * ny;
   const float go = fmaxf(remaining - NUDGE, 0.0f);
   float fx = qx + rx * go, fy = qy + ry * go;
-  const float fnorm = sqrtf(fx * fx + fy * fy);
+  const float fnorm = sqrt_rn(fx * fx + fy * fy);
   const float fsafe = fmaxf(fnorm, 1e-30f);
   const float target = inside ? R - NUDGE : R + NUDGE;
   const bool wrong = inside ? (fnorm >= R) : (fnorm <= R);
-  if (wrong) { fx = (fx * target) / fsafe; fy = (fy * target) / fsafe; }
+  if (wrong) { fx = div_rn(fx * target, fsafe); fy = div_rn(fy * target, fsafe); }
   x = fx; y = fy;
   return (-2.0f * a.rhoD) * dperp;
 }
@@ -286,23 +287,23 @@ __device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float s
                                              uint32_t& hits) {
   const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
   const float RR = R * R;
-  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
-  float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float sl = sqrt_rn((sx * sx + sy * sy) + sz * sz);
+  float dx = div_rn(sx, sl), dy = div_rn(sy, sl), dz = div_rn(sz, sl);
   float x = p.x, y = p.y, z = p.z;
   float rem = sl, dsum = 0.0f;
   for (int it = 0; it < 10; ++it) {
     const float dp = (dx * x + dy * y) + dz * z;
     const float disc = fmaxf(dp * dp - (((x * x + y * y) + z * z) - RR), 0.0f);
-    const float sq = sqrtf(disc);
+    const float sq = sqrt_rn(disc);
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem))) break;
     const float hx = x + d * dx, hy = y + d * dy, hz = z + d * dz;
-    const float nx = hx / R, ny = hy / R, nz = hz / R;
+    const float nx = div_rn(hx, R), ny = div_rn(hy, R), nz = div_rn(hz, R);
     const float tdd = 2.0f * ((dx * nx + dy * ny) + dz * nz);
     float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
-    const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
-    rx = rx / rn; ry = ry / rn; rz = rz / rn;
-    dsum = dsum + (rem - d) * (sq / R);
+    const float rn = sqrt_rn((rx * rx + ry * ry) + rz * rz);
+    rx = div_rn(rx, rn); ry = div_rn(ry, rn); rz = div_rn(rz, rn);
+    dsum = dsum + (rem - d) * (div_rn(sq, R));
     x = hx - NUDGE * nx; y = hy - NUDGE * ny; z = hz - NUDGE * nz;
     dx = rx; dy = ry; dz = rz;
     rem = (rem - d) - NUDGE;
@@ -310,10 +311,10 @@ __device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float s
   }
   const float rm = fmaxf(rem, 0.0f);
   float fx = x + dx * rm, fy = y + dy * rm, fz = z + dz * rm;
-  const float fn = sqrtf((fx * fx + fy * fy) + fz * fz);
+  const float fn = sqrt_rn((fx * fx + fy * fy) + fz * fz);
   if (fn >= R) {
     const float s = R - NUDGE;
-    fx = (fx * s) / fn; fy = (fy * s) / fn; fz = (fz * s) / fn;
+    fx = div_rn(fx * s, fn); fy = div_rn(fy * s, fn); fz = div_rn(fz * s, fn);
   }
   p.x = fx; p.y = fy; p.z = fz;
   return dsum;
@@ -324,22 +325,22 @@ __device__ __forceinline__ float sph_permeate(Pos& p, float sx, float sy, float 
                                               const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
   const float R = a.size, EPS = a.eps, NUDGE = a.nudge;
   const float RR = R * R;
-  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
-  const float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float sl = sqrt_rn((sx * sx + sy * sy) + sz * sz);
+  const float dx = div_rn(sx, sl), dy = div_rn(sy, sl), dz = div_rn(sz, sl);
   const float x = p.x, y = p.y, z = p.z;
   const float dp = (dx * x + dy * y) + dz * z;
   const float rr = (x * x + y * y) + z * z;
   const float disc = dp * dp - (rr - RR);
-  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const float sq = sqrt_rn(fmaxf(disc, 0.0f));
   const bool inside = rr < RR;
   const float t_hit = inside ? (-dp + sq) : (-dp - sq);
   const bool any_hit = (disc > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
   if (!any_hit) { p.x = x + sx; p.y = y + sy; p.z = z + sz; return 0.0f; }
   hits++;
   const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
-  const float nx = hx / R, ny = hy / R, nz = hz / R;
+  const float nx = div_rn(hx, R), ny = div_rn(hy, R), nz = div_rn(hz, R);
   const float remaining = sl - t_hit;
-  const float dperp = remaining * (sq / R);
+  const float dperp = remaining * (div_rn(sq, R));
   const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
   if (ps.get() < pr) {
     p.x = x + sx; p.y = y + sy; p.z = z + sz; ncross++;
@@ -347,8 +348,8 @@ __device__ __forceinline__ float sph_permeate(Pos& p, float sx, float sy, float 
   }
   const float tdd = 2.0f * ((dx * nx + dy * ny) + dz * nz);
   float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
-  const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
-  rx = rx / rn; ry = ry / rn; rz = rz / rn;
+  const float rn = sqrt_rn((rx * rx + ry * ry) + rz * rz);
+  rx = div_rn(rx, rn); ry = div_rn(ry, rn); rz = div_rn(rz, rn);
   const float qx = inside ? hx - NUDGE * nx : hx + NUDGE * nx;
   const float qy = inside ? hy - NUDGE * ny : hy + NUDGE * ny;
   const float qz = inside ? hz - NUDGE * nz : hz + NUDGE * nz;
@@ -678,9 +679,9 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
                                                 const Tables& tb, uint32_t& hits, uint32_t& ncand) {
   const float EPS = a.eps, NUDGE = a.nudge, L = a.cell;
   if (DIM == 2) (void)sz;
-  const float sl = (DIM == 2) ? sqrtf(sx * sx + sy * sy) : sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float sl = (DIM == 2) ? sqrt_rn(sx * sx + sy * sy) : sqrt_rn((sx * sx + sy * sy) + sz * sz);
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; if (DIM == 3) dz = sz / sl; }
+  if (sl > 0.0f) { dx = div_rn(sx, sl); dy = div_rn(sy, sl); if (DIM == 3) dz = div_rn(sz, sl); }
   const int gc = cell_of<DIM>(p, a);
   const int beg = tb.start[gc], end = tb.start[gc + 1];
   ncand += (uint32_t)(end - beg);
@@ -693,7 +694,7 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
     const float RR = o.w * o.w;
     const float dp = dot_d<DIM>(dx, dy, dz, qx, qy, qz);
     const float disc = dp * dp - (dot_d<DIM>(qx, qy, qz, qx, qy, qz) - RR);
-    const float t = -dp - sqrtf(fmaxf(disc, 0.0f));
+    const float t = -dp - sqrt_rn(fmaxf(disc, 0.0f));
     const bool valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
     if (valid && t < best_t) {
       best_t = t; best_disc = disc; bqx = qx; bqy = qy; bqz = qz; bR = o.w; hit = true;
@@ -705,13 +706,13 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
     hits++;
     const float ts = best_t;
     const float qhx = bqx + ts * dx, qhy = bqy + ts * dy, qhz = bqz + ts * dz;
-    const float nx = qhx / bR, ny = qhy / bR, nz = (DIM == 3) ? qhz / bR : 0.0f;
+    const float nx = div_rn(qhx, bR), ny = div_rn(qhy, bR), nz = (DIM == 3) ? div_rn(qhz, bR) : 0.0f;
     const float tdd = 2.0f * dot_d<DIM>(dx, dy, dz, nx, ny, nz);
     float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = (DIM == 3) ? dz - tdd * nz : 0.0f;
-    const float rn = sqrtf(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
+    const float rn = sqrt_rn(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
     if (rn > 0.0f) {
       const float rs = fmaxf(rn, 1e-30f);
-      rx = rx / rs; ry = ry / rs; rz = rz / rs;
+      rx = div_rn(rx, rs); ry = div_rn(ry, rs); rz = div_rn(rz, rs);
     } else {
       rx = 0.0f; ry = 0.0f; rz = 0.0f;
     }
@@ -720,7 +721,7 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
     fx = ((p.x + ts * dx) + NUDGE * nx) + rx * go;
     fy = ((p.y + ts * dy) + NUDGE * ny) + ry * go;
     if (DIM == 3) fz = ((p.z + ts * dz) + NUDGE * nz) + rz * go;
-    dperp = remaining * (sqrtf(fmaxf(best_disc, 0.0f)) / bR);
+    dperp = remaining * div_rn(sqrt_rn(fmaxf(best_disc, 0.0f)), bR);
   }
   // Safety clamp on the folded end point, over the same candidates
   // (geometries.py:1858-1877).
@@ -737,13 +738,13 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
     const float d2 = dot_d<DIM>(qx, qy, qz, qx, qy, qz);
     const float RR = o.w * o.w;
     if (d2 < RR) {
-      const float pen = d2 / RR;
+      const float pen = div_rn(d2, RR);
       if (pen < best_pen) { best_pen = pen; cqx = qx; cqy = qy; cqz = qz; cR = o.w; }
     }
   }
   if (best_pen < 1.0f) {
-    const float dcl = sqrtf(dot_d<DIM>(cqx, cqy, cqz, cqx, cqy, cqz));
-    const float scale = (cR + NUDGE) / fmaxf(dcl, NUDGE);
+    const float dcl = sqrt_rn(dot_d<DIM>(cqx, cqy, cqz, cqx, cqy, cqz));
+    const float scale = div_rn(cR + NUDGE, fmaxf(dcl, NUDGE));
     const float sm1 = scale - 1.0f;
     fx = fx + cqx * sm1;
     fy = fy + cqy * sm1;
@@ -761,9 +762,9 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
                                                  const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
                                                  uint32_t& ncross, uint32_t& ncand) {
   const float EPS = a.eps, NUDGE = a.nudge;
-  const float sl = (DIM == 2) ? sqrtf(sx * sx + sy * sy) : sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float sl = (DIM == 2) ? sqrt_rn(sx * sx + sy * sy) : sqrt_rn((sx * sx + sy * sy) + sz * sz);
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  if (sl > 0.0f) { dx = sx / sl; dy = sy / sl; if (DIM == 3) dz = sz / sl; }
+  if (sl > 0.0f) { dx = div_rn(sx, sl); dy = div_rn(sy, sl); if (DIM == 3) dz = div_rn(sz, sl); }
   const int gc = cell_of<DIM>(p, a);
   const int beg = tb.start[gc], end = tb.start[gc + 1];
   ncand += (uint32_t)(end - beg);
@@ -778,7 +779,7 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
     const bool inside = dist2 < RR;
     const float dp = dot_d<DIM>(dx, dy, dz, qx, qy, qz);
     const float disc = dp * dp - (dist2 - RR);
-    const float sq = sqrtf(fmaxf(disc, 0.0f));
+    const float sq = sqrt_rn(fmaxf(disc, 0.0f));
     const float t = inside ? (-dp + sq) : (-dp - sq);
     const bool valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
     if (valid && t < best_t) {
@@ -793,10 +794,10 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
   hits++;
   const float ts = best_t;
   const float hx = bqx + ts * dx, hy = bqy + ts * dy, hz = (DIM == 3) ? bqz + ts * dz : 0.0f;
-  const float hl = fmaxf(sqrtf(dot_d<DIM>(hx, hy, hz, hx, hy, hz)), 1e-30f);
-  const float nx = hx / hl, ny = hy / hl, nz = (DIM == 3) ? hz / hl : 0.0f;
+  const float hl = fmaxf(sqrt_rn(dot_d<DIM>(hx, hy, hz, hx, hy, hz)), 1e-30f);
+  const float nx = div_rn(hx, hl), ny = div_rn(hy, hl), nz = (DIM == 3) ? div_rn(hz, hl) : 0.0f;
   const float remaining = sl - ts;
-  const float dperp = remaining * (sqrtf(fmaxf(best_disc, 0.0f)) / bR);
+  const float dperp = remaining * div_rn(sqrt_rn(fmaxf(best_disc, 0.0f)), bR);
   const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
   if (ps.get() < pr) {
     p.x = p.x + sx; p.y = p.y + sy; p.z = p.z + sz; ncross++;
@@ -804,10 +805,10 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
   }
   const float tdd = 2.0f * dot_d<DIM>(dx, dy, dz, nx, ny, nz);
   float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = (DIM == 3) ? dz - tdd * nz : 0.0f;
-  const float rn = sqrtf(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
+  const float rn = sqrt_rn(dot_d<DIM>(rx, ry, rz, rx, ry, rz));
   if (rn > 0.0f) {
     const float rs = fmaxf(rn, 1e-30f);
-    rx = rx / rs; ry = ry / rs; rz = rz / rs;
+    rx = div_rn(rx, rs); ry = div_rn(ry, rs); rz = div_rn(rz, rs);
   } else {
     rx = 0.0f; ry = 0.0f; rz = 0.0f;
   }
@@ -817,11 +818,11 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
   const float uz = hit_inside ? shz - NUDGE * nz : shz + NUDGE * nz;
   const float go = fmaxf(remaining - NUDGE, 0.0f);
   float fx = ux + rx * go, fy = uy + ry * go, fz = (DIM == 3) ? uz + rz * go : 0.0f;
-  const float fn = sqrtf(dot_d<DIM>(fx, fy, fz, fx, fy, fz));
+  const float fn = sqrt_rn(dot_d<DIM>(fx, fy, fz, fx, fy, fz));
   const float fs = fmaxf(fn, 1e-30f);
   const float target = hit_inside ? bR - NUDGE : bR + NUDGE;
   const bool wrong = hit_inside ? (fn >= bR) : (fn <= bR);
-  if (wrong) { fx = (fx * target) / fs; fy = (fy * target) / fs; if (DIM == 3) fz = (fz * target) / fs; }
+  if (wrong) { fx = div_rn(fx * target, fs); fy = div_rn(fy * target, fs); if (DIM == 3) fz = div_rn(fz * target, fs); }
   p.x = p.x + (fx - bqx);
   p.y = p.y + (fy - bqy);
   p.z = (DIM == 3) ? p.z + (fz - bqz) : p.z + sz;
