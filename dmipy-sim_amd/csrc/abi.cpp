@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/dmipy_walk.h"
+#include "envelope.h"
 #include "walk_kernel.h"
 #include "walk_launch.h"
 #include "batch.h"
@@ -179,6 +180,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if ((geom->kind == DMW_PERMEABLE_SHELL || geom->kind == DMW_PERMEABLE_SHELL_CYL) &&
       !(geom->aux > geom->size && geom->size > 0.0f))
     return fail(DMW_EINVAL, "PermeableShell needs 0 < r_inner < r_outer");
+  if (const char* why = dmw::outside_float_envelope(*geom, run->step_len)) return fail(DMW_EINVAL, why);
   const bool myel = geom->kind == DMW_MYELINATED_CYLINDER;
   if (myel) {
     if (!geom->myelin) return fail(DMW_EINVAL, "MyelinatedCylinder needs its compartment constants");

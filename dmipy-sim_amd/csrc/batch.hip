@@ -21,6 +21,7 @@
 
 #include "../../include/dmipy_walk.h"
 #include "batch.h"
+#include "envelope.h"
 #include "grid.h"
 #include "walk_kernel.h"
 
@@ -281,6 +282,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
         ((I.geom.kind == DMW_PERMEABLE_SHELL || I.geom.kind == DMW_PERMEABLE_SHELL_CYL) &&
          I.geom.wall == DMW_WALL_SURFACE))
       return dmw::batch_fail(DMW_EUNSUPPORTED, "wall mode not available for this geometry");
+    if (const char* why = dmw::outside_float_envelope(I.geom, I.step_len)) return dmw::batch_fail(DMW_EINVAL, why);
     const bool packed = I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES;
     if (packed && (I.geom.n_obj < 1 || !I.geom.centers || !I.geom.radii || !(I.geom.cell > 0.0f)))
       return dmw::batch_fail(DMW_EINVAL, "packed instance needs objects and a positive cell side");

@@ -19,6 +19,8 @@
 
 #include <stdint.h>
 
+#include "rn_arith.h"
+
 #if defined(__HIPCC__)
 #define DMW_HD __host__ __device__ __forceinline__
 #else
@@ -77,10 +79,12 @@ DMW_HD void sincospi_quarter(float f, float* s, float* c) {
   *c = fmaf(x, pc, 1.0f);
 }
 
-// Unit vector uniformly distributed on the sphere from two 32-bit words.
-DMW_HD void iso_direction(uint32_t a, uint32_t b, float* ux, float* uy, float* uz) {
+// Unit vector uniformly distributed on the sphere from two 32-bit words.  The
+// square root's argument is 0 or at least 2^-22 (z on a 2^-23 grid), inside the
+// range where sqrt_rn equals sqrtf.
+__device__ __forceinline__ void iso_direction(uint32_t a, uint32_t b, float* ux, float* uy, float* uz) {
   const float z = 1.0f - 2.0f * u01(a);           // exact, in (-1, 1]
-  const float s = sqrtf(fmaxf(1.0f - z * z, 0.0f));
+  const float s = sqrt_rn(fmaxf(1.0f - z * z, 0.0f));
   const float t = 2.0f * u01(b);                  // azimuth / pi in [0, 2)
   const float q = rintf(2.0f * t);                // quadrant 0..4
   const float f = t - 0.5f * q;                   // exact, |f| <= 1/4

@@ -372,25 +372,25 @@ __device__ __forceinline__ float ell_form(float ux, float uy, float uz, float vx
 __device__ __forceinline__ float ell_reflect(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
                                              uint32_t& hits) {
   const float EPS = a.eps, NUDGE = a.nudge;
-  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
-  float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float sl = sqrt_rn((sx * sx + sy * sy) + sz * sz);
+  float dx = div_rn(sx, sl), dy = div_rn(sy, sl), dz = div_rn(sz, sl);
   float x = p.x, y = p.y, z = p.z, rem = sl, dsum = 0.0f;
   for (int it = 0; it < 10; ++it) {
     const float A = ell_form(dx, dy, dz, dx, dy, dz, a);
     const float B = ell_form(x, y, z, dx, dy, dz, a);
     const float C = ell_form(x, y, z, x, y, z, a) - 1.0f;
     const float disc = fmaxf(B * B - A * C, 0.0f);
-    const float d = (-B + sqrtf(disc)) / A;
+    const float d = div_rn(-B + sqrt_rn(disc), A);
     if (!((d > EPS) && (d < rem))) break;
     const float hx = x + d * dx, hy = y + d * dy, hz = z + d * dz;
     const float mx = hx * a.ax, my = hy * a.ay, mz = hz * a.az;
-    const float mn = sqrtf((mx * mx + my * my) + mz * mz);
-    const float nx = mx / mn, ny = my / mn, nz = mz / mn;
+    const float mn = sqrt_rn((mx * mx + my * my) + mz * mz);
+    const float nx = div_rn(mx, mn), ny = div_rn(my, mn), nz = div_rn(mz, mn);
     const float cosa = (dx * nx + dy * ny) + dz * nz;
     const float tdd = 2.0f * cosa;
     float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
-    const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
-    rx = rx / rn; ry = ry / rn; rz = rz / rn;
+    const float rn = sqrt_rn((rx * rx + ry * ry) + rz * rz);
+    rx = div_rn(rx, rn); ry = div_rn(ry, rn); rz = div_rn(rz, rn);
     dsum = dsum + (rem - d) * cosa;
     x = hx - NUDGE * nx; y = hy - NUDGE * ny; z = hz - NUDGE * nz;
     dx = rx; dy = ry; dz = rz;
@@ -399,9 +399,9 @@ __device__ __forceinline__ float ell_reflect(Pos& p, float sx, float sy, float s
   }
   const float rm = fmaxf(rem, 0.0f);
   float fx = x + dx * rm, fy = y + dy * rm, fz = z + dz * rm;
-  const float qx = fx * sqrtf(a.ax), qy = fy * sqrtf(a.ay), qz = fz * sqrtf(a.az);
-  const float q = sqrtf((qx * qx + qy * qy) + qz * qz);
-  if (q >= 1.0f) { fx = (fx * a.aux) / q; fy = (fy * a.aux) / q; fz = (fz * a.aux) / q; }
+  const float qx = fx * sqrt_rn(a.ax), qy = fy * sqrt_rn(a.ay), qz = fz * sqrt_rn(a.az);
+  const float q = sqrt_rn((qx * qx + qy * qy) + qz * qz);
+  if (q >= 1.0f) { fx = div_rn(fx * a.aux, q); fy = div_rn(fy * a.aux, q); fz = div_rn(fz * a.aux, q); }
   p.x = fx; p.y = fy; p.z = fz;
   return dsum;
 }
@@ -410,23 +410,23 @@ __device__ __forceinline__ float ell_reflect(Pos& p, float sx, float sy, float s
 __device__ __forceinline__ float ell_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
                                               const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
   const float EPS = a.eps, NUDGE = a.nudge;
-  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
-  const float dx = sx / sl, dy = sy / sl, dz = sz / sl;
+  const float sl = sqrt_rn((sx * sx + sy * sy) + sz * sz);
+  const float dx = div_rn(sx, sl), dy = div_rn(sy, sl), dz = div_rn(sz, sl);
   const float x = p.x, y = p.y, z = p.z;
   const float A = ell_form(dx, dy, dz, dx, dy, dz, a);
   const float B = ell_form(x, y, z, dx, dy, dz, a);
   const float C = ell_form(x, y, z, x, y, z, a) - 1.0f;
   const float disc_raw = B * B - A * C;
-  const float sq = sqrtf(fmaxf(disc_raw, 0.0f));
+  const float sq = sqrt_rn(fmaxf(disc_raw, 0.0f));
   const bool inside = C < 0.0f;
-  const float t_hit = inside ? (-B + sq) / A : (-B - sq) / A;
+  const float t_hit = inside ? div_rn(-B + sq, A) : div_rn(-B - sq, A);
   const bool any_hit = (disc_raw > 0.0f) && (t_hit > EPS) && (t_hit < sl) && (sl > 0.0f);
   if (!any_hit) { p.x = x + sx; p.y = y + sy; p.z = z + sz; return 0.0f; }
   hits++;
   const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
   const float mx = hx * a.ax, my = hy * a.ay, mz = hz * a.az;
-  const float mn = sqrtf((mx * mx + my * my) + mz * mz);
-  const float nx = mx / mn, ny = my / mn, nz = mz / mn;
+  const float mn = sqrt_rn((mx * mx + my * my) + mz * mz);
+  const float nx = div_rn(mx, mn), ny = div_rn(my, mn), nz = div_rn(mz, mn);
   const float remaining = sl - t_hit;
   const float dn = (dx * nx + dy * ny) + dz * nz;
   const float dperp = remaining * fabsf(dn);
@@ -437,8 +437,8 @@ __device__ __forceinline__ float ell_permeate(Pos& p, float sx, float sy, float 
   }
   const float tdd = 2.0f * dn;
   float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
-  const float rn = sqrtf((rx * rx + ry * ry) + rz * rz);
-  rx = rx / rn; ry = ry / rn; rz = rz / rn;
+  const float rn = sqrt_rn((rx * rx + ry * ry) + rz * rz);
+  rx = div_rn(rx, rn); ry = div_rn(ry, rn); rz = div_rn(rz, rn);
   const float ux = inside ? -nx : nx, uy = inside ? -ny : ny, uz = inside ? -nz : nz;
   const float qx = hx + NUDGE * ux, qy = hy + NUDGE * uy, qz = hz + NUDGE * uz;
   const float go = fmaxf(remaining - NUDGE, 0.0f);
@@ -493,15 +493,15 @@ __device__ __forceinline__ float slab_step(Pos& p, float sx, float sy, float sz,
 // ---------------------------------------------------------------------------
 template <bool CYLK>
 __device__ __forceinline__ float shell_norm(float x, float y, float z) {
-  return CYLK ? sqrtf((x * x + y * y) + 0.0f * 0.0f) : sqrtf((x * x + y * y) + z * z);
+  return CYLK ? sqrt_rn((x * x + y * y) + 0.0f * 0.0f) : sqrt_rn((x * x + y * y) + z * z);
 }
 
 template <bool CYLK>
 __device__ __forceinline__ float shell_first_t(float b, float r2, float dd2, float Rk, float EPS, float sl) {
   const float c = r2 - Rk * Rk;
   const float disc = b * b - dd2 * c;
-  const float sq = sqrtf(fmaxf(disc, 0.0f));
-  float t1 = (-b - sq) / dd2, t2 = (-b + sq) / dd2;
+  const float sq = sqrt_rn(fmaxf(disc, 0.0f));
+  float t1 = div_rn(-b - sq, dd2), t2 = div_rn(-b + sq, dd2);
   t1 = (disc > 0.0f && t1 > EPS && t1 < sl) ? t1 : 1e30f;
   t2 = (disc > 0.0f && t2 > EPS && t2 < sl) ? t2 : 1e30f;
   return fminf(t1, t2);
@@ -511,9 +511,9 @@ template <bool CYLK, int WALL>
 __device__ __forceinline__ float shell_step(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
                                             const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross) {
   const float Rin = a.size, Rout = a.aux, EPS = a.eps, NUDGE = a.nudge;
-  const float sl = sqrtf((sx * sx + sy * sy) + sz * sz);
+  const float sl = sqrt_rn((sx * sx + sy * sy) + sz * sz);
   const float den = fmaxf(sl, EPS);
-  const float dx = sx / den, dy = sy / den, dz = sz / den;
+  const float dx = div_rn(sx, den), dy = div_rn(sy, den), dz = div_rn(sz, den);
   const float x = p.x, y = p.y, z = p.z;
   const float rz = CYLK ? 0.0f : z, dzr = CYLK ? 0.0f : dz;       // radial components
   const float b = (dx * x + dy * y) + dzr * rz;
@@ -530,8 +530,8 @@ __device__ __forceinline__ float shell_step(Pos& p, float sx, float sy, float sz
     hits++;
     const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
     const float hzr = CYLK ? 0.0f : hz;
-    const float hn = fmaxf(sqrtf((hx * hx + hy * hy) + hzr * hzr), EPS);
-    const float nx = hx / hn, ny = hy / hn, nz = hzr / hn;
+    const float hn = fmaxf(sqrt_rn((hx * hx + hy * hy) + hzr * hzr), EPS);
+    const float nx = div_rn(hx, hn), ny = div_rn(hy, hn), nz = div_rn(hzr, hn);
     const float remaining = sl - t_hit;
     const float dn = (dx * nx + dy * ny) + dz * nz;
     const float dperp = remaining * fabsf(dn);
@@ -558,13 +558,13 @@ __device__ __forceinline__ float shell_step(Pos& p, float sx, float sy, float sz
   }
   // never outside the reflecting wall (mirror the overshoot back)
   const float ozr = CYLK ? 0.0f : oz;
-  const float rm = sqrtf((ox * ox + oy * oy) + ozr * ozr);
+  const float rm = sqrt_rn((ox * ox + oy * oy) + ozr * ozr);
   const float over = rm - Rout;
   if (over > 0.0f) {
     const float rs = fmaxf(rm, EPS), t2 = 2.0f * over;
-    ox = ox - t2 * (ox / rs);
-    oy = oy - t2 * (oy / rs);
-    oz = oz - t2 * (ozr / rs);
+    ox = ox - t2 * (div_rn(ox, rs));
+    oy = oy - t2 * (div_rn(oy, rs));
+    oz = oz - t2 * (div_rn(ozr, rs));
   }
   p.x = ox; p.y = oy; p.z = oz;
   return dlog;
@@ -594,9 +594,9 @@ __device__ __forceinline__ int myel_classify(float x, float y, const WalkArgs& a
 __device__ __forceinline__ void myel_clamp(float& x, float& y, float& n, bool wrong, float Rt) {
   if (wrong) {
     const float s = fmaxf(n, 1e-20f);
-    x = (x / s) * Rt;
-    y = (y / s) * Rt;
-    n = sqrtf(x * x + y * y);
+    x = (div_rn(x, s)) * Rt;
+    y = (div_rn(y, s)) * Rt;
+    n = sqrt_rn(x * x + y * y);
   }
 }
 
@@ -606,7 +606,7 @@ __device__ __forceinline__ void myel_step(Pos& p, float sx, float sy, float sz, 
   const int c = p.cx;
   float nx = p.x + sx, ny = p.y + sy;
   const float nz = p.z + sz;
-  const float n = sqrtf(nx * nx + ny * ny);
+  const float n = sqrt_rn(nx * nx + ny * ny);
   const bool in_out = (c == 0) && (n >= Rin), in_in = (c == 1) && (n < Rin);
   const bool out_out = (c == 1) && (n >= Rout), out_in = (c == 2) && (n < Rout);
   int nc = c;
@@ -622,11 +622,11 @@ __device__ __forceinline__ void myel_step(Pos& p, float sx, float sy, float sz, 
       nc = inner ? (in_out ? 1 : 0) : (out_out ? 2 : 1);
     } else {
       const float s = fmaxf(n, 1e-20f);
-      const float hx = nx / s, hy = ny / s;
+      const float hx = div_rn(nx, s), hy = div_rn(ny, s);
       const float rr = 2.0f * Rk - n;
       nx = hx * rr;
       ny = hy * rr;
-      nn = sqrtf(nx * nx + ny * ny);
+      nn = sqrt_rn(nx * nx + ny * ny);
     }
   }
   myel_clamp(nx, ny, nn, (nc == 0) && (nn >= Rin), Rin - NUDGE);
@@ -839,7 +839,7 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
 // 2 kappa/D (inner from intra, inner from myelin, outer from myelin, outer from
 // extra).  Minimum images use true division, as the reference does.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float pm_wrap(float q, float L) { return q - L * floorf(q / L + 0.5f); }
+__device__ __forceinline__ float pm_wrap(float q, float L) { return q - L * floorf(div_rn(q, L) + 0.5f); }
 
 __device__ __forceinline__ Pos pm_point(float x, float y) { return Pos{x, y, 0.0f, 0, 0, 0}; }
 
@@ -860,7 +860,7 @@ __device__ __forceinline__ int pm_nearest_container(float x, float y, const Walk
     const float fx = pm_wrap(x - g.x, L), fy = pm_wrap(y - g.y, L);
     const float d2 = fx * fx + fy * fy;
     const float RR = g.w * g.w;
-    const float pen = (g.w > 0.0f) ? (d2 < RR ? d2 / (RR + 1e-30f) : 1.0f) : 1.0f;
+    const float pen = (g.w > 0.0f) ? (d2 < RR ? div_rn(d2, RR + 1e-30f) : 1.0f) : 1.0f;
     if (pen < best) { best = pen; kb = j; }
   }
   return kb;
@@ -881,7 +881,7 @@ __device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz
   const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
   const float Rin = geo.z, Rout = geo.w;
   float qx = pm_wrap((p.x + sx) - geo.x, L), qy = pm_wrap((p.y + sy) - geo.y, L);
-  const float nrm = sqrtf(qx * qx + qy * qy);
+  const float nrm = sqrt_rn(qx * qx + qy * qy);
   const bool cio = is_intra && nrm >= Rin, cii = is_myelin && nrm < Rin;
   const bool coo = is_myelin && nrm >= Rout, coi = is_extra && nrm < Rout;
   const bool ci = cio || cii, co = coo || coi;
@@ -901,7 +901,7 @@ __device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz
       ncls = cio ? 1 : 0;
     } else {
       const float s = fmaxf(nrm, 1e-20f);
-      const float hx = qx / s, hy = qy / s;
+      const float hx = div_rn(qx, s), hy = div_rn(qy, s);
       const float rr = 2.0f * Rin - nrm;
       qx = hx * rr;
       qy = hy * rr;
@@ -914,9 +914,9 @@ __device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz
       ncls = coo ? 2 : 1;
       nk = coo ? 0 : k;
     } else {
-      const float n2 = sqrtf(qx * qx + qy * qy);
+      const float n2 = sqrt_rn(qx * qx + qy * qy);
       const float s = fmaxf(n2, 1e-20f);
-      const float hx = qx / s, hy = qy / s;
+      const float hx = div_rn(qx, s), hy = div_rn(qy, s);
       const float rr = 2.0f * Rout - n2;
       qx = hx * rr;
       qy = hy * rr;
@@ -932,12 +932,12 @@ __device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz
     if (kc >= 0) {
       const float4 gc = tb.objs[kc];
       const float cqx = pm_wrap(xa - gc.x, L), cqy = pm_wrap(ya - gc.y, L);
-      const float dcl = sqrtf(cqx * cqx + cqy * cqy);
+      const float dcl = sqrt_rn(cqx * cqx + cqy * cqy);
       const float cnx = xa - cqx, cny = ya - cqy;
       const float drefl = 2.0f * gc.w - dcl;
       const float den = fmaxf(dcl, NUDGE);
-      xa = cnx + (cqx * drefl) / den;
-      ya = cny + (cqy * drefl) / den;
+      xa = cnx + div_rn(cqx * drefl, den);
+      ya = cny + div_rn(cqy * drefl, den);
       dlog = dlog + (-2.0f * fmaxf(gc.w - dcl, 0.0f));
     }
   }
@@ -945,24 +945,24 @@ __device__ __forceinline__ float pmyel_step(Pos& p, float ux, float uy, float uz
   const bool same = !(t_in || t_out);
   if (same && (is_intra || is_myelin)) {
     float ex = pm_wrap(xa - geo.x, L), ey = pm_wrap(ya - geo.y, L);
-    float de = sqrtf(ex * ex + ey * ey);
+    float de = sqrt_rn(ex * ex + ey * ey);
     if (is_intra && de >= Rin) {
       const float s = fmaxf(de, 1e-20f);
-      xa = geo.x + (ex / s) * (Rin - NUDGE);
-      ya = geo.y + (ey / s) * (Rin - NUDGE);
+      xa = geo.x + (div_rn(ex, s)) * (Rin - NUDGE);
+      ya = geo.y + (div_rn(ey, s)) * (Rin - NUDGE);
     }
     if (is_myelin && de < Rin) {
       const float s = fmaxf(de, 1e-20f);
-      xa = geo.x + (ex / s) * (Rin + NUDGE);
-      ya = geo.y + (ey / s) * (Rin + NUDGE);
+      xa = geo.x + (div_rn(ex, s)) * (Rin + NUDGE);
+      ya = geo.y + (div_rn(ey, s)) * (Rin + NUDGE);
       ex = pm_wrap(xa - geo.x, L);
       ey = pm_wrap(ya - geo.y, L);
-      de = sqrtf(ex * ex + ey * ey);
+      de = sqrt_rn(ex * ex + ey * ey);
     }
     if (is_myelin && de >= Rout) {
       const float s = fmaxf(de, 1e-20f);
-      xa = geo.x + (ex / s) * (Rout - NUDGE);
-      ya = geo.y + (ey / s) * (Rout - NUDGE);
+      xa = geo.x + (div_rn(ex, s)) * (Rout - NUDGE);
+      ya = geo.y + (div_rn(ey, s)) * (Rout - NUDGE);
     }
   }
   // intra / myelin walkers are re-classified against their own axon (physics.py:926-949)
@@ -1512,8 +1512,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         const float db = pmyel_step(p, ux, uy, uz, a, tb, ps, hits, ncross, ncand);
         float dx = p.x - px0, dy = p.y - py0;
         const float dz = p.z - pz0;
-        dx = dx - L * rintf(dx / L);
-        dy = dy - L * rintf(dy / L);
+        dx = dx - L * rintf(div_rn(dx, L));
+        dy = dy - L * rintf(div_rn(dy, L));
         uwx = uwx + dx;
         uwy = uwy + dy;
         uwz = uwz + dz;

@@ -354,3 +354,23 @@ def test_packed_myelin_unsupported_paths_raise():
     with pytest.raises(ValueError):
         ds.PackedMyelinatedCylinders([1e-6, 2e-6], 0.7, np.zeros((3, 2)), 1e-5)
     assert abs(sum(g.volume_fraction(c) for c in ("intra", "myelin", "extra")) - 1.0) < 1e-12
+
+
+def test_float_envelope_rejects_sub_angstrom_lengths():
+    """The wall routines' exact unscaled arithmetic holds for lengths >= 1e-10 and
+    steps >= 1e-11 in the caller's unit; smaller ones are refused with ValueError
+    before any GPU work (FreeDiffusion and Box1D use no square roots)."""
+    from dmipy_sim_amd import _native as nat
+    wf = ds.Waveform(G=np.zeros((1, 5, 3), np.float32), dt=1e-4, echo_idx=4)
+    plan = ds.make_walk_plan(ds.Cylinder(5e-11, (0, 0, 1)), D, wf.dt)
+    gd = plan.geometry_desc(n_walkers=64)
+    G = np.zeros((1, 5, 3), np.float32)
+    wd = nat.Waveform_t(n_meas=1, n_t=5, G=nat.fptr(G), chi=None, n_echo=0, echo_idx=None)
+    rd = nat.Run_t(n_walkers=64, seed=0, n_sub=plan.n_sub, step_len=plan.step_len,
+                   gamma_dt_sub=plan.gamma_dt_sub, dt_sub=plan.dt_sub, inv_T2=0.0, inv_T1=0.0,
+                   has_t2=0, has_t1=0, kahan=0, meas_block=0, r0=None, inj_normals=None,
+                   inj_uniforms=None)
+    import ctypes as C
+    h = C.c_void_p()
+    rc = nat.lib().dmw_plan_create(C.byref(h), gd, wd, rd)
+    assert rc == 1 and b"float32 range" in nat.lib().dmw_last_error()
