@@ -381,6 +381,11 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     a.exact_minimage = G.exact_minimage ? 1 : 0;
     P->lds = (size_t)geom->n_obj * 16 + (G.start.size() + G.items.size()) * 4;
   }
+  // dynamic LDS (object tables and grid) plus the kernel's 4 KB phase-operand stage
+  if (P->lds + 4096 > 65536) {
+    delete P;
+    return fail(DMW_EUNSUPPORTED, "object tables and neighbour grid exceed the 60 KB of LDS per workgroup");
+  }
   a.objs = P->objs.as<float4>();
   a.cell_start = P->cstart.as<int>();
   a.cell_items = P->citems.as<int>();

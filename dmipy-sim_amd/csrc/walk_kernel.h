@@ -1340,13 +1340,20 @@ struct PhaseMFMA {
       bv[nt] = gt[16 * nt];
     });
   }
+  // A operands: lane (k = lane/16, j = lane%16) needs component k (x, y, z, then 0) of
+  // walker 16 wt + j.  Every lane stages (x, y, z, 0) of its own walker in LDS and reads
+  // the one word it needs -- one 16-byte store and four 4-byte loads per sample, no
+  // cross-lane select (conflict-free: the 16 readers of a row step 16 bytes apart).
   __device__ __forceinline__ void update(const WalkArgs&, int, int, int lane, float Px, float Py, float Pz) {
+    __shared__ float4 stage[kBlock];
     const int kq = lane >> 4, jq = lane & 15;
+    __builtin_amdgcn_wave_barrier();
+    stage[threadIdx.x] = float4{Px, Py, Pz, 0.0f};
+    __builtin_amdgcn_wave_barrier();
+    const float* sw = reinterpret_cast<const float*>(stage + (threadIdx.x & ~(kWave - 1)));
     static_for<4>([&](auto wc) {
       constexpr int wt = decltype(wc)::value;
-      const int src = 16 * wt + jq;
-      const float xs = __shfl(Px, src, kWave), ys = __shfl(Py, src, kWave), zs = __shfl(Pz, src, kWave);
-      const float av = kq == 0 ? xs : (kq == 1 ? ys : (kq == 2 ? zs : 0.0f));
+      const float av = sw[(16 * wt + jq) * 4 + kq];
       static_for<NT>([&](auto nc) {
         constexpr int nt = decltype(nc)::value;
         acc[wt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nt], acc[wt][nt], 0, 0, 0);
