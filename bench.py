@@ -37,7 +37,7 @@ PEAK_FP32_TFLOPS = 157.3
 # device-to-device copy (FETCH x 2.000, WRITE x 1.000; profiles/walk_kernel_cylinder_pgse.md).
 # None where not measured.
 TRAFFIC_BYTES = {
-    "cylinder_pgse": 17_463_893 + 32_172_842,
+    "cylinder_pgse": 17_487_531 + 32_172_842,
     "free_pgse": None,
     "packed_cylinders_free_waveform": None,
 }
