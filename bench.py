@@ -31,6 +31,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "walker-steps/s (1 GPU); signals/s surrogate vs MC at RMSE≤5e-3 S/S0"
 PEAK_FP32_TFLOPS = 157.3
+# Phase product of blocks of 32 or more measurements: float16 hi + lo parts of the
+# displacement sums and of G on the matrix cores, float32 accumulation.
+PHASE_MFMA = "fp16 hi/lo split on matrix cores, fp32 accumulate"
 
 # HBM bytes per walk launch from rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE
 # runs of this script, each scaled by the counter's reading on a 256 MiB
@@ -129,7 +132,7 @@ def run_batch(args):
     config = {"workload": "batch_4096_sphere_cylinder", "instances": len(inst),
               "walkers": int(bw.n_walkers.sum()), "n_t": bw.n_t, "n_meas": bw.n_meas,
               "walker_steps_per_launch": total_ws, "graph": True,
-              "phase": "fp32", "ensemble_sum": "fp64"}
+              "phase": PHASE_MFMA, "ensemble_sum": "fp64"}
     line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
@@ -429,7 +432,7 @@ def main():
     config = {"workload": cfg["name"], "walkers": cfg["n_walkers"], "n_t": int(wf.G.shape[1]),
               "n_meas": int(wf.G.shape[0]), "n_sub": plan.n_sub,
               "geometry": type(cfg["geometry"]).__name__, "diffusivity": cfg["diffusivity"],
-              "phase": "kahan" if pw.kahan else "fp32", "ensemble_sum": "fp64"}
+              "phase": "kahan fp32 (VALU)" if pw.kahan else PHASE_MFMA, "ensemble_sum": "fp64"}
     if sliced:
         config["time_slice"] = f"first {sliced} of 2000 samples"
     line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,

@@ -20,6 +20,7 @@
 #include "walk_launch.h"
 #include "batch.h"
 #include "grid.h"
+#include "phase16.h"
 #include "traj.h"
 
 namespace {
@@ -115,12 +116,12 @@ struct dmw_plan {
   unsigned grid_x = 0;
   size_t lds = 0;
   dmw::WalkArgs args{};
-  DevBuf Gs, GsT, chi, echo, objs, cstart, citems, r0, inj_n, inj_u, pm_start;
+  DevBuf Gs, GsH, Gsum, chi, echo, objs, cstart, citems, r0, inj_n, inj_u, pm_start;
   DevBuf partials, echo_partials, sig;
   DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~dmw_plan() {
-    for (DevBuf* b : {&Gs, &GsT, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &pm_start, &partials,
+    for (DevBuf* b : {&Gs, &GsH, &Gsum, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &pm_start, &partials,
                       &echo_partials, &sig, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
                       &o_logw, &o_phi, &o_counters})
       b->release();
@@ -261,27 +262,6 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     }                 \
   } while (0)
 
-  // Waveform: (n_meas, n_t, 3) -> (n_t, n_meas_pad, 3) scaled by gamma*dt_sub.
-  {
-    std::vector<float> gs((size_t)wave->n_t * P->n_meas_pad * 3, 0.0f);
-    const double gdt = run->gamma_dt_sub;
-    for (int m = 0; m < wave->n_meas; ++m)
-      for (int t = 0; t < wave->n_t; ++t)
-        for (int c = 0; c < 3; ++c)
-          gs[((size_t)t * P->n_meas_pad + m) * 3 + c] =
-              (float)(gdt * (double)wave->G[((size_t)m * wave->n_t + t) * 3 + c]);
-    TRY(upload(P->Gs, gs.data(), gs.size() * 4, st));
-    // Component-major copy for the matrix-core phase update: (n_t, 4, n_meas_pad).
-    std::vector<float> gt((size_t)wave->n_t * 4 * P->n_meas_pad, 0.0f);
-    for (int t = 0; t < wave->n_t; ++t)
-      for (int c = 0; c < 3; ++c)
-        for (int m = 0; m < P->n_meas_pad; ++m)
-          gt[((size_t)t * 4 + c) * P->n_meas_pad + m] = gs[((size_t)t * P->n_meas_pad + m) * 3 + c];
-    TRY(upload(P->GsT, gt.data(), gt.size() * 4, st));
-  }
-  if (wave->chi) TRY(upload(P->chi, wave->chi, (size_t)wave->n_t * 4, st));
-  if (wave->n_echo > 0) TRY(upload(P->echo, wave->echo_idx, (size_t)wave->n_echo * 4, st));
-
   dmw::WalkArgs& a = P->args;
   a.size = geom->size;
   a.cell = geom->cell;
@@ -324,6 +304,21 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     P->inv_norm = 1.0 / wsum;
   }
   a.n_obj = 0;
+  // LDS per workgroup: object tables and grid (dynamic) plus, for the matrix-core
+  // phase update, 8 KB of operand stage and start points and 128 x mb bytes of
+  // waveform operands.  The grid is coarsened to fit; only if even one cell does
+  // not fit (and no meas_block was requested) is a narrower block taken.
+  auto static_lds = [&](int mb) -> size_t { return (mb >= 32 && !P->kahan) ? 8192 + 128 * (size_t)mb : 0; };
+  auto fit_grid = [&](float step, int dim, size_t obj_bytes, Grid& G) -> bool {
+    for (;;) {
+      const size_t budget = 65536 - static_lds(P->mb);
+      G = build_grid(*geom, step, dim, budget, obj_bytes);
+      if ((size_t)geom->n_obj * obj_bytes + (G.start.size() + G.items.size()) * 4 <= budget) return true;
+      if (run->meas_block != 0 || P->mb <= 16) return false;
+      P->mb = P->mb > 32 ? P->mb / 2 : 16;
+    }
+  };
+  const char* lds_msg = "object tables and neighbour grid exceed the LDS of a workgroup";
   if (pmyel) {
     const dmw_packed_myelin& pm = *geom->packed_myelin;
     const int n = geom->n_obj;
@@ -343,7 +338,11 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
       float* r2 = &rows[((size_t)2 * n + k) * 4];
       for (int c = 0; c < 4; ++c) r2[c] = pm.two_kappa_over_D[k * 4 + c];
     }
-    Grid G = build_grid(*geom, smax, 2);
+    Grid G;
+    if (!fit_grid(smax, 2, 48, G)) {
+      delete P;
+      return fail(DMW_EUNSUPPORTED, lds_msg);
+    }
     TRY(upload(P->objs, rows.data(), rows.size() * 4, st));
     TRY(upload(P->cstart, G.start.data(), G.start.size() * 4, st));
     TRY(upload(P->citems, G.items.data(), std::max<size_t>(1, G.items.size()) * 4, st));
@@ -364,7 +363,11 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   }
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
-    Grid G = build_grid(*geom, run->step_len, dim);
+    Grid G;
+    if (!fit_grid(run->step_len, dim, 16, G)) {
+      delete P;
+      return fail(DMW_EUNSUPPORTED, lds_msg);
+    }
     std::vector<float> objs((size_t)geom->n_obj * 4, 0.0f);
     for (int k = 0; k < geom->n_obj; ++k) {
       for (int d = 0; d < dim; ++d) objs[k * 4 + d] = geom->centers[k * dim + d];
@@ -381,11 +384,31 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     a.exact_minimage = G.exact_minimage ? 1 : 0;
     P->lds = (size_t)geom->n_obj * 16 + (G.start.size() + G.items.size()) * 4;
   }
-  // dynamic LDS (object tables and grid) plus the kernel's 4 KB phase-operand stage
-  if (P->lds + 4096 > 65536) {
-    delete P;
-    return fail(DMW_EUNSUPPORTED, "object tables and neighbour grid exceed the 60 KB of LDS per workgroup");
+  P->n_chunks = (wave->n_meas + P->mb - 1) / P->mb;
+  P->n_meas_pad = P->n_chunks * P->mb;
+  double g_scale = 1.0;
+  // Waveform: (n_meas, n_t, 3) -> (n_t, n_meas_pad, 3) scaled by gamma*dt_sub.
+  {
+    std::vector<float> gs((size_t)wave->n_t * P->n_meas_pad * 3, 0.0f);
+    const double gdt = run->gamma_dt_sub;
+    for (int m = 0; m < wave->n_meas; ++m)
+      for (int t = 0; t < wave->n_t; ++t)
+        for (int c = 0; c < 3; ++c)
+          gs[((size_t)t * P->n_meas_pad + m) * 3 + c] =
+              (float)(gdt * (double)wave->G[((size_t)m * wave->n_t + t) * 3 + c]);
+    TRY(upload(P->Gs, gs.data(), gs.size() * 4, st));
+    // float16 hi/lo operands and column sums of the matrix-core phase update
+    g_scale = dmw::phase_g_scale(gs.data(), gs.size());
+    std::vector<uint2> gh;
+    dmw::phase_half_table(gs.data(), wave->n_t, P->n_meas_pad, g_scale, gh);
+    TRY(upload(P->GsH, gh.data(), gh.size() * sizeof(uint2), st));
+    std::vector<float> gsum;
+    dmw::phase_column_sums(gs.data(), wave->n_t, P->n_meas_pad, wave->echo_idx, wave->n_echo, gsum);
+    TRY(upload(P->Gsum, gsum.data(), gsum.size() * 4, st));
   }
+  if (wave->chi) TRY(upload(P->chi, wave->chi, (size_t)wave->n_t * 4, st));
+  if (wave->n_echo > 0) TRY(upload(P->echo, wave->echo_idx, (size_t)wave->n_echo * 4, st));
+
   a.objs = P->objs.as<float4>();
   a.cell_start = P->cstart.as<int>();
   a.cell_items = P->citems.as<int>();
@@ -400,7 +423,19 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.n_meas = wave->n_meas;
   a.n_meas_pad = P->n_meas_pad;
   a.Gs = P->Gs.as<float>();
-  a.GsT = P->GsT.as<float>();
+  a.GsH = P->GsH.as<uint2>();
+  a.Gsum = P->Gsum.as<float>();
+  {
+    double step_max = run->step_len;
+    if (myel)
+      for (int c = 0; c < 3; ++c) step_max = std::max(step_max, (double)a.m_step[c]);
+    if (pmyel)
+      for (int k = 0; k < 3 * geom->n_obj; ++k)
+        step_max = std::max(step_max, (double)geom->packed_myelin->step_len[k]);
+    const double p_scale = dmw::phase_p_scale(wave->n_t, run->n_sub, step_max, 1.0);
+    a.p_scale = (float)p_scale;
+    a.inv_pg_scale = (float)(1.0 / (p_scale * g_scale));
+  }
   a.chi = wave->chi ? P->chi.as<float>() : nullptr;
   a.n_echo = wave->n_echo;
   a.echo_idx = wave->n_echo > 0 ? P->echo.as<int>() : nullptr;
@@ -534,6 +569,9 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   float ms = 0.0f;
   HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
   out->kernel_ms = ms;
+  for (double v : hs)
+    if (std::isnan(v))
+      return fail(DMW_EHIP, "walk produced a NaN signal (non-finite input, or a phase operand beyond the float16 range)");
   for (int r = 0; r < rows; ++r)
     for (int m = 0; m < P->n_meas; ++m) {
       out->signal[(size_t)r * P->n_meas + m] = hs[((size_t)r * P->n_meas_pad + m) * 2];

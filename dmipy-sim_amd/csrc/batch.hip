@@ -23,6 +23,7 @@
 #include "batch.h"
 #include "envelope.h"
 #include "grid.h"
+#include "phase16.h"
 #include "walk_kernel.h"
 
 namespace dmw {
@@ -35,7 +36,9 @@ struct InstDev {
   int grid_n, exact_minimage, n_obj;
   int obj_off, start_off, items_off;
   long long n_walkers;
-  long long g_off, gt_off;   // offsets (floats) of the instance's waveform set
+  long long g_off;          // offset (floats) of the instance's waveform set
+  long long gh_off, gsum_off;  // offsets of its float16 phase operands (uint2) and column sums (floats)
+  float p_scale, inv_pg_scale;  // matrix-core phase scales (powers of two)
   int unit_begin, unit_end;
   unsigned long long seed_offset;
   unsigned int stream;       // Philox instance word
@@ -47,7 +50,8 @@ struct BatchArgs {
   const int2* units;        // (instance, tile) in queue order
   int n_units;
   const float* G;           // sets x (n_t, n_meas_pad, 3) T/m, not scaled
-  const float* GT;          // sets x (n_t, 4, n_meas_pad) component-major, row 3 = 0
+  const uint2* GH;          // sets x (n_t, 2, n_meas_pad) float16 x4 hi/lo parts of G * g_scale(set)
+  const float* Gsum;        // sets x (3, n_meas_pad): sum over t of G
   const float* chi;
   int n_t, n_meas, n_meas_pad;
   const float4* objs;       // concatenated object tables of packed instances
@@ -78,7 +82,10 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.ax = I.ax; a.ay = I.ay; a.az = I.az; a.aux = I.aux;
   a.instance = inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
   a.Gs = b.G + I.g_off;
-  a.GsT = b.GT + I.gt_off;
+  a.GsH = b.GH + I.gh_off;
+  a.Gsum = b.Gsum + I.gsum_off;
+  a.p_scale = I.p_scale;
+  a.inv_pg_scale = I.inv_pg_scale;
   a.n_meas_pad = b.n_meas_pad;
   Tables tb{nullptr, nullptr, nullptr};
   if (PACKED) tb = Tables{b.objs + I.obj_off, b.starts + I.start_off, b.items + I.items_off};
@@ -90,6 +97,12 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   const int n_sub = I.n_sub;
   const float fsub = (float)n_sub;
   const float L = I.cell;
+  // start point of the matrix-core phase, in the units of the update's operand (x gdt)
+  if (PACKED)
+    ph.set_ref(a, 0, lane, (fsub * (p.x + L * (float)p.cx)) * I.gdt, (fsub * (p.y + L * (float)p.cy)) * I.gdt,
+               (fsub * (p.z + L * (float)p.cz)) * I.gdt);
+  else
+    ph.set_ref(a, 0, lane, (fsub * p.x) * I.gdt, (fsub * p.y) * I.gdt, (fsub * p.z) * I.gdt);
   u32x4 rnd{0u, 0u, 0u, 0u};
   for (int t = 0; t < b.n_t; ++t) {
     ph.prefetch(a, t, 0, lane);
@@ -129,7 +142,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
     if (PACKED) fold<DIM>(p, a);
   }
   const float wgt = expf(logw);
-  ph.emit(wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
+  ph.emit(a, 0, wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
   (void)hits; (void)ncross; (void)ncand;
 }
 
@@ -318,7 +331,9 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     d.cell = I.geom.cell;
     d.inv_cell = I.geom.cell > 0.0f ? 1.0f / I.geom.cell : 0.0f;
     d.g_off = (long long)I.wave_set * n_t * Mp * 3;
-    d.gt_off = (long long)I.wave_set * n_t * 4 * Mp;
+    d.gh_off = (long long)I.wave_set * n_t * 2 * ((Mp + 31) / 32 * 32);  // phase_half_table's size per set
+    d.gsum_off = (long long)I.wave_set * 3 * Mp;
+    d.p_scale = (float)dmw::phase_p_scale(n_t, I.n_sub, I.step_len, I.gamma_dt_sub);
     d.grid_n = 1; d.grid_inv_h = 0.0f; d.exact_minimage = 0; d.n_obj = 0;
     d.obj_off = d.start_off = d.items_off = 0;
     if (I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES) {
@@ -358,15 +373,28 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   std::vector<int2> q_units(B->n_units);
   for (int k = 0; k < B->n_units; ++k) q_units[k] = units[order[k]];
 
-  std::vector<float> G((size_t)n_sets * n_t * Mp * 3, 0.0f), GT((size_t)n_sets * n_t * 4 * Mp, 0.0f);
+  std::vector<float> G((size_t)n_sets * n_t * Mp * 3, 0.0f);
   for (int s = 0; s < n_sets; ++s)
     for (int m = 0; m < n_meas; ++m)
       for (int t = 0; t < n_t; ++t)
-        for (int c = 0; c < 3; ++c) {
-          const float v = sets[s].G[((size_t)m * n_t + t) * 3 + c];
-          G[(((size_t)s * n_t + t) * Mp + m) * 3 + c] = v;
-          GT[(((size_t)s * n_t + t) * 4 + c) * Mp + m] = v;
-        }
+        for (int c = 0; c < 3; ++c)
+          G[(((size_t)s * n_t + t) * Mp + m) * 3 + c] = sets[s].G[((size_t)m * n_t + t) * 3 + c];
+  // float16 hi/lo phase operands and column sums per waveform set
+  std::vector<uint2> GH;
+  std::vector<float> Gsum;
+  std::vector<double> g_scale(n_sets);
+  for (int s = 0; s < n_sets; ++s) {
+    const float* gs = G.data() + (size_t)s * n_t * Mp * 3;
+    g_scale[s] = dmw::phase_g_scale(gs, (size_t)n_t * Mp * 3);
+    std::vector<uint2> gh;
+    std::vector<float> sums;
+    dmw::phase_half_table(gs, n_t, Mp, g_scale[s], gh);
+    dmw::phase_column_sums(gs, n_t, Mp, nullptr, 0, sums);
+    GH.insert(GH.end(), gh.begin(), gh.end());
+    Gsum.insert(Gsum.end(), sums.begin(), sums.end());
+  }
+  for (int i = 0; i < n_inst; ++i)
+    hi[i].inv_pg_scale = (float)(1.0 / ((double)hi[i].p_scale * g_scale[inst[i].wave_set]));
 
   int n_cu = 256;
   hipDeviceProp_t prop;
@@ -378,7 +406,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     delete B;
     return dmw::batch_hip_fail(err, what);
   };
-  void *d_units, *d_uid, *d_G, *d_GT, *d_chi = nullptr, *d_objs, *d_starts, *d_items;
+  void *d_units, *d_uid, *d_G, *d_GH, *d_Gsum, *d_chi = nullptr, *d_objs, *d_starts, *d_items;
 #define BT(x, what)                                       \
   do {                                                    \
     hipError_t err_ = (x);                                \
@@ -389,7 +417,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   BT(B->upload(&d_units, q_units.data(), q_units.size() * sizeof(int2)), "batch upload");
   BT(B->upload(&d_uid, order.data(), order.size() * sizeof(int)), "batch upload");
   BT(B->upload(&d_G, G.data(), G.size() * 4), "batch upload");
-  BT(B->upload(&d_GT, GT.data(), GT.size() * 4), "batch upload");
+  BT(B->upload(&d_GH, GH.data(), GH.size() * sizeof(uint2)), "batch upload");
+  BT(B->upload(&d_Gsum, Gsum.data(), Gsum.size() * 4), "batch upload");
   if (have_chi) BT(B->upload(&d_chi, sets[0].chi, (size_t)n_t * 4), "batch upload");
   BT(B->upload(&d_objs, objs.data(), objs.size() * sizeof(float4)), "batch upload");
   BT(B->upload(&d_starts, starts.data(), starts.size() * 4), "batch upload");
@@ -408,7 +437,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   a.unit_id = (const int*)d_uid;
   a.n_units = B->n_units;
   a.G = (const float*)d_G;
-  a.GT = (const float*)d_GT;
+  a.GH = (const uint2*)d_GH;
+  a.Gsum = (const float*)d_Gsum;
   a.chi = have_chi ? (const float*)d_chi : nullptr;
   a.n_t = n_t;
   a.n_meas = n_meas;
@@ -451,6 +481,9 @@ int dmw_batch_execute(dmw_batch* B, uint64_t seed, double* signals, dmw_batch_st
   if (e != hipSuccess) return dmw::batch_hip_fail(e, "D2H");
   e = hipStreamSynchronize(B->st);
   if (e != hipSuccess) return dmw::batch_hip_fail(e, "batch walk");
+  for (double v : h)
+    if (std::isnan(v))
+      return dmw::batch_fail(DMW_EHIP, "batch produced a NaN signal (non-finite input, or a phase operand beyond the float16 range)");
   for (int i = 0; i < B->n_inst; ++i)
     for (int m = 0; m < B->n_meas; ++m) signals[(size_t)i * B->n_meas + m] = h[((size_t)i * B->n_meas_pad + m) * 2];
   if (stats) {

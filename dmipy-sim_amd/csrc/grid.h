@@ -11,7 +11,8 @@ namespace {
 
 // Periodic neighbour grid of a packed substrate: every cell lists, in ascending
 // object order, the objects whose disc/ball inflated by the largest single-step
-// reach intersects the cell.
+// reach intersects the cell.  The grid is coarsened until the object table
+// (obj_bytes per object) and the grid fit in max_bytes, or it is a single cell.
 struct Grid {
   int n = 1;
   float h = 0.0f;
@@ -19,7 +20,8 @@ struct Grid {
   bool exact_minimage = false;
 };
 
-Grid build_grid(const dmw_geometry& g, float step_len, int dim) {
+Grid build_grid(const dmw_geometry& g, float step_len, int dim, size_t max_bytes = 60 * 1024,
+                size_t obj_bytes = 16) {
   Grid G;
   const int n_obj = g.n_obj;
   const double L = g.cell;
@@ -75,9 +77,9 @@ Grid build_grid(const dmw_geometry& g, float step_len, int dim) {
       G.items.insert(G.items.end(), lists[i].begin(), lists[i].end());
     }
     G.start[n_cells] = (int)G.items.size();
-    const size_t lds = (size_t)n_obj * 16 + (G.start.size() + G.items.size()) * 4;
+    const size_t lds = (size_t)n_obj * obj_bytes + (G.start.size() + G.items.size()) * 4;
     G.exact_minimage = (rmax + inflate + h * std::sqrt((double)dim) + tol) >= 0.45 * L;
-    if (lds <= 60 * 1024 || n == 1) return G;
+    if (lds <= max_bytes || n == 1) return G;
     n = std::max(1, n * 3 / 4);
   }
 }

@@ -30,7 +30,10 @@
 //     once per waveform sample;
 //   * the phase is advanced once per waveform sample with the sum of the n_sub
 //     sub-step positions (G is constant across the sub-steps of a sample,
-//     physics.py:287-289), optionally Kahan-compensated;
+//     physics.py:287-289), optionally Kahan-compensated; on the matrix cores the
+//     walker operand is that sum less n_sub times the start point, both operands
+//     split into float16 hi + lo parts (relative residual below 2^-21), and the
+//     start point's phase (start . sum_t G) is added back at the end;
 //   * the ensemble sum is float64, reduced in a fixed order.
 // All float32 wall arithmetic is written operation by operation in the
 // reference's order and compiled without contraction.
@@ -83,7 +86,14 @@ struct WalkArgs {
   // waveform
   int n_t, n_meas, n_meas_pad;
   const float* Gs;           // device (n_t, n_meas_pad, 3), scaled by gamma*dt_sub
-  const float* GsT;          // device (n_t, 4, n_meas_pad): same, component-major, row 3 = 0
+  // Matrix-core phase operands: Gs * g_scale split into float16 hi and lo parts,
+  // in 16-byte per-lane chunks (n_t, 2, n_meas_pad / 32, 16) x (2 x (x, y, z, 0));
+  // Gsum = sums over t of Gs between emission points, (n_echo + 1, 3, n_meas_pad).
+  // The walker operand is its displacement sum from the start point times
+  // p_scale (both scales powers of two, chosen so neither operand exceeds 2^14).
+  const uint2* GsH;
+  const float* Gsum;
+  float p_scale, inv_pg_scale;  // inv_pg_scale = 1 / (p_scale * g_scale)
   const float* chi;          // device (n_t) or null
   int n_echo;
   const int* echo_idx;       // device (n_echo)
@@ -1299,7 +1309,9 @@ struct PhaseVALU {
       }
     });
   }
-  __device__ __forceinline__ void emit(float wgt, bool active, int lane, double* dst) const {
+  __device__ __forceinline__ void set_ref(const WalkArgs&, int, int, float, float, float) {}
+  __device__ __forceinline__ void add_segment(const WalkArgs&, int, int, int) {}
+  __device__ __forceinline__ void emit(const WalkArgs&, int, float wgt, bool active, int lane, double* dst) const {
     emit_partials<MB, KAHAN>(phi, kc, wgt, active, lane, dst);
   }
   __device__ __forceinline__ void store_phi(const WalkArgs& a, int m0, long long wave_base, bool active,
@@ -1314,54 +1326,140 @@ struct PhaseVALU {
 };
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __fp16 fp16x2v __attribute__((ext_vector_type(2)));  // v_cvt_pkrtz result
+typedef _Float16 half4v __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2v __attribute__((ext_vector_type(2)));
 
-// Matrix-core form: per sample the wave computes the (64 walkers x 4) . (4 x MB)
-// product Phi += P . Gs(t) with v_mfma_f32_16x16x4f32 (K = x, y, z, 0).
-// Accumulator tile (wt, nt) holds walkers 16 wt + 4 (lane/16) + r (r = 0..3) and
-// measurement 16 nt + lane%16; A operands are gathered across lanes, B operands
-// are one coalesced load per lane per measurement tile (Gs^T layout).
+// Per-lane start point of the matrix-core phase (n_sub times the unfolded start
+// position, times the caller's position factor), one slot per thread.
+__device__ __forceinline__ float4* phase_ref_slots() {
+  __shared__ float4 ref[kBlock];
+  return ref;
+}
+
+// Waveform operands of the current sample, NP 16-byte chunks per thread, landed
+// by global->LDS loads (no registers held across the sub-steps).
+template <int NP>
+__device__ __forceinline__ uint4* phase_b_slots() {
+  __shared__ uint4 b[kBlock * NP];
+  return b;
+}
+
+// Matrix-core form: per sample the wave computes the (64 walkers x 16) . (16 x MB)
+// product Phi += A . B with v_mfma_f32_16x16x16f16.  With d the walker's
+// displacement sum (P - n_sub * start) times p_scale and g = Gs(t) times g_scale,
+// both split into float16 hi + lo parts, the K = 16 slots hold
+//   A: (d_hi, 0) (d_hi, 0) (d_lo, 0) (d_lo, 0)     B: (g_hi, 0) (g_lo, 0) (g_hi, 0) (g_lo, 0)
+// (each (v, 0) = x, y, z, 0), i.e. all four hi/lo products, so the product
+// misses only the split residuals (relative 2^-21).  Float16 products are exact
+// and accumulate in float32.  The float32 form (v_mfma_f32_16x16x4f32) runs on
+// the vector ALUs' issue slots and cost four times as many cycles.  Accumulator
+// tile (wt, nt) holds walkers 16 wt + 4 (lane/16) + r (r = 0..3) and measurement
+// 16 nt + lane%16, in units of 1 / (p_scale g_scale); it starts at the start
+// point's own phase up to the first emission point, start . sum_{t <= e_0} Gs(t),
+// and after each echo emission takes the next segment's, so at every emission it
+// carries the whole phase.
+//
+// Range: per sub-step a walker moves at most one step (three for the packed
+// myelin mirror), so |d| <= 4 n_sub (n_t n_sub) step_max before scaling; the host
+// picks p_scale so that bound maps to 2^14, a factor of four below the float16
+// limit, and g_scale so max |Gs| maps to 2^14.
 template <int MB>
 struct PhaseMFMA {
   static constexpr int NT = MB / 16;
+  static constexpr int NP = MB / 32;  // 16-byte operand chunks per lane: column tiles 2p, 2p + 1
   floatx4 acc[4][NT];
-  float bv[NT];
-  __device__ __forceinline__ void init() {
+  __device__ __forceinline__ void init() {}
+  // Records the lane's start point (n_sub x unfolded start, times the caller's
+  // position factor) and starts every accumulator at that point's phase over the
+  // first segment.
+  __device__ __forceinline__ void set_ref(const WalkArgs& a, int m0, int lane, float rx, float ry, float rz) {
+    __builtin_amdgcn_wave_barrier();
+    phase_ref_slots()[threadIdx.x] = float4{rx, ry, rz, 0.0f};
     static_for<4>([&](auto wc) {
       static_for<NT>([&](auto nc) {
         acc[decltype(wc)::value][decltype(nc)::value] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
       });
     });
+    add_segment(a, m0, lane, 0);
   }
-  // B operands of sample t: lane (k = lane/16, j = lane%16) reads Gs^T[t][k][m0 + 16 nt + j].
-  __device__ __forceinline__ void prefetch(const WalkArgs& a, int t, int m0, int lane) {
-    const float* gt = a.GsT + ((size_t)t * 4 + (lane >> 4)) * a.n_meas_pad + m0 + (lane & 15);
-    static_for<NT>([&](auto nc) {
-      constexpr int nt = decltype(nc)::value;
-      bv[nt] = gt[16 * nt];
-    });
-  }
-  // A operands: lane (k = lane/16, j = lane%16) needs component k (x, y, z, then 0) of
-  // walker 16 wt + j.  Every lane stages (x, y, z, 0) of its own walker in LDS and reads
-  // the one word it needs -- one 16-byte store and four 4-byte loads per sample, no
-  // cross-lane select (conflict-free: the 16 readers of a row step 16 bytes apart).
-  __device__ __forceinline__ void update(const WalkArgs&, int, int, int lane, float Px, float Py, float Pz) {
-    __shared__ float4 stage[kBlock];
+  // acc += start . (sum of Gs over segment seg), in accumulator units.
+  __device__ __forceinline__ void add_segment(const WalkArgs& a, int m0, int lane, int seg) {
     const int kq = lane >> 4, jq = lane & 15;
     __builtin_amdgcn_wave_barrier();
-    stage[threadIdx.x] = float4{Px, Py, Pz, 0.0f};
-    __builtin_amdgcn_wave_barrier();
-    const float* sw = reinterpret_cast<const float*>(stage + (threadIdx.x & ~(kWave - 1)));
-    static_for<4>([&](auto wc) {
-      constexpr int wt = decltype(wc)::value;
-      const float av = sw[(16 * wt + jq) * 4 + kq];
-      static_for<NT>([&](auto nc) {
-        constexpr int nt = decltype(nc)::value;
-        acc[wt][nt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[nt], acc[wt][nt], 0, 0, 0);
+    const float4* ref = phase_ref_slots() + (threadIdx.x & ~(kWave - 1));
+    const float* gs = a.Gsum + (size_t)seg * 3 * a.n_meas_pad;
+    const float pg = 1.0f / a.inv_pg_scale;  // exact: a power of two
+    static_for<NT>([&](auto nc) {
+      constexpr int nt = decltype(nc)::value;
+      // start points re-read per column tile (inside the walk loop, at echoes,
+      // holding all 16 would spill)
+      asm volatile("" ::: "memory");
+      const int m = m0 + 16 * nt + jq;
+      const float sx = gs[m], sy = gs[a.n_meas_pad + m], sz = gs[2 * a.n_meas_pad + m];
+      static_for<4>([&](auto wc) {
+        constexpr int wt = decltype(wc)::value;
+        static_for<4>([&](auto rc) {
+          constexpr int r = decltype(rc)::value;
+          const float4 q = ref[16 * wt + 4 * kq + r];
+          acc[wt][nt][r] = acc[wt][nt][r] + ((q.x * sx + q.y * sy) + q.z * sz) * pg;
+        });
       });
     });
   }
-  __device__ __forceinline__ void emit(float wgt, bool active, int lane, double* dst) const {
+  // B operands of sample t: lane (k = lane/16, j = lane%16) needs part k%2 (hi, lo)
+  // of measurements m0 + 16 nt + j.  The table stores them as 16-byte chunks (column
+  // tiles 2p and 2p + 1 of one lane), copied straight into the wave's LDS slots at
+  // the start of the sample and read back in update().
+  __device__ __forceinline__ void prefetch(const WalkArgs& a, int t, int m0, int lane) {
+    const uint4* src = reinterpret_cast<const uint4*>(a.GsH) +
+                       (((size_t)t * 2 + ((lane >> 4) & 1)) * (a.n_meas_pad / 32) + m0 / 32) * 16 + (lane & 15);
+    uint4* dst = phase_b_slots<NP>() + (threadIdx.x >> 6) * (kWave * NP);
+    static_for<NP>([&](auto pc) {
+      constexpr int pj = decltype(pc)::value;
+      __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)(src + 16 * pj),
+                                       (void __attribute__((address_space(3)))*)(dst + kWave * pj), 16, 0, 0);
+    });
+  }
+  // A operands: every lane splits its own displacement sum and stages
+  // (d_hi, 0, d_lo, 0) (16 bytes) in LDS; lane (k, j) then reads the hi (k < 2) or
+  // lo (k >= 2) half of walker 16 wt + j -- four 8-byte loads per sample, no
+  // bank conflicts (32 distinct 8-byte words per wave).
+  __device__ __forceinline__ void update(const WalkArgs& a, int, int, int lane, float Px, float Py, float Pz) {
+    __shared__ uint4 stage[kBlock];
     const int kq = lane >> 4, jq = lane & 15;
+    const float4 r = phase_ref_slots()[threadIdx.x];
+    const float dx = (Px - r.x) * a.p_scale, dy = (Py - r.y) * a.p_scale, dz = (Pz - r.z) * a.p_scale;
+    const fp16x2v hxy = __builtin_amdgcn_cvt_pkrtz(dx, dy);
+    const fp16x2v hz = __builtin_amdgcn_cvt_pkrtz(dz, 0.0f);
+    const fp16x2v lxy = __builtin_amdgcn_cvt_pkrtz(dx - (float)hxy.x, dy - (float)hxy.y);
+    const fp16x2v lz = __builtin_amdgcn_cvt_pkrtz(dz - (float)hz.x, 0.0f);
+    __builtin_amdgcn_wave_barrier();
+    stage[threadIdx.x] = make_uint4(__builtin_bit_cast(uint32_t, hxy), __builtin_bit_cast(uint32_t, hz),
+                                    __builtin_bit_cast(uint32_t, lxy), __builtin_bit_cast(uint32_t, lz));
+    __builtin_amdgcn_wave_barrier();
+    const u32x2v* sw = reinterpret_cast<const u32x2v*>(stage + (threadIdx.x & ~(kWave - 1)));
+    const uint4* bs = phase_b_slots<NP>() + (threadIdx.x >> 6) * (kWave * NP) + lane;
+    half4v bv[NT];
+    static_for<NP>([&](auto pc) {
+      constexpr int pj = decltype(pc)::value;
+      const uint4 c = bs[kWave * pj];
+      bv[2 * pj] = __builtin_bit_cast(half4v, u32x2v{c.x, c.y});
+      bv[2 * pj + 1] = __builtin_bit_cast(half4v, u32x2v{c.z, c.w});
+    });
+    static_for<4>([&](auto wc) {
+      constexpr int wt = decltype(wc)::value;
+      const half4v av = __builtin_bit_cast(half4v, sw[(16 * wt + jq) * 2 + (kq >> 1)]);
+      static_for<NT>([&](auto nc) {
+        constexpr int nt = decltype(nc)::value;
+        acc[wt][nt] = __builtin_amdgcn_mfma_f32_16x16x16f16(av, bv[nt], acc[wt][nt], 0, 0, 0);
+      });
+    });
+  }
+  __device__ __forceinline__ void emit(const WalkArgs& a, int, float wgt, bool active, int lane,
+                                       double* dst) const {
+    const int kq = lane >> 4, jq = lane & 15;
+    const float inv = a.inv_pg_scale;
     float wv[4][4];
     static_for<4>([&](auto wc) {
       constexpr int wt = decltype(wc)::value;
@@ -1381,7 +1479,7 @@ struct PhaseMFMA {
         static_for<4>([&](auto rc) {
           constexpr int r = decltype(rc)::value;
           float sn, cs;
-          phase_sincos(acc[wt][nt][r], &sn, &cs);
+          phase_sincos(acc[wt][nt][r] * inv, &sn, &cs);
           re += (double)(wv[wt][r] * cs);
           im += (double)(wv[wt][r] * sn);
         });
@@ -1408,7 +1506,7 @@ struct PhaseMFMA {
         static_for<NT>([&](auto nc) {
           constexpr int nt = decltype(nc)::value;
           const int m = m0 + 16 * nt + jq;
-          if (w < N && m < a.n_meas) a.phi[(size_t)m * N + w] = acc[wt][nt][r];
+          if (w < N && m < a.n_meas) a.phi[(size_t)m * N + w] = acc[wt][nt][r] * a.inv_pg_scale;
         });
       });
     });
@@ -1485,6 +1583,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   // PackedMyelinatedCylinders: the phase uses the position unwrapped by the
   // minimum-image displacement of every sub-step (physics.py:1040-1047).
   float uwx = p.x, uwy = p.y, uwz = p.z;
+  // Start point of the matrix-core phase: n_sub times the unfolded start position.
+  if (PACKED)
+    ph.set_ref(a, m0, lane, fsub * (p.x + L * (float)p.cx), fsub * (p.y + L * (float)p.cy),
+               fsub * (p.z + L * (float)p.cz));
+  else
+    ph.set_ref(a, m0, lane, fsub * p.x, fsub * p.y, fsub * p.z);
 
   for (int t = 0; t < a.n_t; ++t) {
     ph.prefetch(a, t, m0, lane);
@@ -1602,9 +1706,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       float wgt = expf(logw);
       if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
       if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
-      ph.emit(wgt, active, lane,
+      ph.emit(a, m0, wgt, active, lane,
               a.echo_partials + (((size_t)wave_id * a.n_echo + next_echo) * a.n_meas_pad + m0) * 2);
       ++next_echo;
+      ph.add_segment(a, m0, lane, next_echo);
     }
   }
 
@@ -1614,7 +1719,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
   // PackedMyelinatedCylinders: myelin water carries its proton density (core.py:552-553)
   if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
-  ph.emit(wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
+  ph.emit(a, m0, wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
   if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
   if (active) {
     if (primary) {

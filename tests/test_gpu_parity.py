@@ -133,3 +133,80 @@ def test_cpmg_echoes_match_oracle(gpu, oracle):
                           echo_indices=wf.echo_indices)
     assert g["signal"].shape == (6, 2)
     assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+
+
+def _unbalanced(wf, factor=1.5):
+    """The waveform with its first half scaled: a nonzero net moment sum_t G."""
+    G = wf.G.copy()
+    G[:, : G.shape[1] // 2] *= factor
+    return ds.Waveform(G=G, dt=wf.dt, echo_idx=wf.echo_idx)
+
+
+@pytest.mark.parametrize("mb", [32, 128])
+@pytest.mark.parametrize("where", ["cylinder_axis", "packed_far_cell"])
+def test_matrix_phase_far_start_net_moment(gpu, oracle, mb, where):
+    """Matrix-core phase (float16 hi/lo split of displacement sums) with start points
+    far from the origin and a waveform whose net moment is not zero, so the start
+    point's own phase (start . sum_t G) is a large part of every walker's phase."""
+    wf = _unbalanced(small_pgse())
+    if where == "cylinder_axis":
+        geom = ds.Cylinder(3e-6, (0, 0, 1), surface_relaxivity_t2=2e-5)
+        r0 = inside_starts(geom, 256, 7)
+        r0[:, 2] += 2e-5                      # 20 um along the free axis
+    else:
+        from substrates import packed_cyl
+        geom = packed_cyl(kappa=5e-5)
+        r0 = inside_starts(geom, 256, 7)
+        r0[:, 0] += 3 * geom._L_float         # three cells away: cell index 3 after the fold
+    plan = ds.make_walk_plan(geom, D, wf.dt)
+    rng = np.random.default_rng(7)
+    S = wf.G.shape[1] * plan.n_sub
+    kw = {"inject_normals": rng.standard_normal((256, S, 3)).astype(np.float32),
+          "inject_uniforms": rng.random((256, S)).astype(np.float32)}
+    print(f"matrix phase: {where} mb={mb}", flush=True)
+    g = gpu.walk(geom, D, wf, 256, r0=r0, outputs=("final_pos", "phi", "signal_im"), meas_block=mb, **kw)
+    o = oracle.run_oracle(geom, D, wf, 256, r0=r0, **kw)
+    np.testing.assert_array_equal(g["final_pos"], o["final_pos"])
+    assert np.abs(o["phi"]).max() > 3.0, "start-point phase too small to test"
+    np.testing.assert_allclose(g["phi"], o["phi"], atol=2e-4, rtol=1e-5)
+    assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+    assert np.max(np.abs(g["signal_im"] - o["signal_im"])) <= 1e-5
+
+
+def test_cpmg_echoes_on_matrix_cores(gpu, oracle):
+    """Echo rows emitted from the matrix-core accumulators (32-measurement block)."""
+    wf = ds.cpmg(6, 10e-3, 0.02, [[1, 0, 0], [0, 1, 0]], n_t_per_echo=20)
+    geom = ds.Sphere(4e-6, surface_relaxivity_t2=1e-5)
+    n = 512
+    r0 = geom.init_positions(n, 1)
+    plan = ds.make_walk_plan(geom, D, wf.dt, T2=0.07)
+    rng = np.random.default_rng(1)
+    nrm = rng.standard_normal((n, wf.G.shape[1] * plan.n_sub, 3)).astype(np.float32)
+    g = gpu.walk(geom, D, wf, n, T2=0.07, r0=r0, inject_normals=nrm, echo_indices=wf.echo_indices,
+                 meas_block=32)
+    o = oracle.run_oracle(geom, D, wf, n, T2=0.07, r0=r0, inject_normals=nrm,
+                          echo_indices=wf.echo_indices)
+    assert g["signal"].shape == (6, 2)
+    assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+
+
+def test_large_tables_take_a_narrower_block(gpu, oracle):
+    """2025 cylinders: the object table and neighbour grid leave too little LDS for the
+    128-measurement block, so the plan takes a narrower one; results are unchanged."""
+    k = 45
+    h = 4.5e-6
+    xs = (np.arange(k) + 0.5) * h
+    centers = np.stack(np.meshgrid(xs, xs, indexing="ij"), -1).reshape(-1, 2)
+    radii = np.full(len(centers), 1.6e-6)
+    geom = ds.PackedCylinders(radii, centers, k * h, surface_relaxivity_t2=2e-5)
+    wf = small_pgse(n_t=12, n_dirs=32)       # 128 measurements
+    n = 256
+    r0 = geom.init_positions(n, 5)
+    plan = ds.make_walk_plan(geom, D, wf.dt)
+    rng = np.random.default_rng(5)
+    nrm = rng.standard_normal((n, wf.G.shape[1] * plan.n_sub, 3)).astype(np.float32)
+    print(f"large tables: n_obj={len(radii)} n_sub={plan.n_sub}", flush=True)
+    g = gpu.walk(geom, D, wf, n, r0=r0, inject_normals=nrm, outputs=("final_pos",))
+    o = oracle.run_oracle(geom, D, wf, n, r0=r0, inject_normals=nrm)
+    np.testing.assert_array_equal(g["final_pos"], o["final_pos"])
+    assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
