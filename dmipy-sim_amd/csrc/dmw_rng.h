@@ -90,11 +90,14 @@ __device__ __forceinline__ void iso_direction(uint32_t a, uint32_t b, float* ux,
   const float f = t - 0.5f * q;                   // exact, |f| <= 1/4
   float sf, cf;
   sincospi_quarter(f, &sf, &cf);
-  const int qi = ((int)q) & 3;
-  float c = cf, sn = sf;
-  if (qi == 1) { c = -sf; sn = cf; }
-  else if (qi == 2) { c = -cf; sn = -sf; }
-  else if (qi == 3) { c = sf; sn = -cf; }
+  // quadrant qi: (cos, sin)(pi t) = (cf, sf), (-sf, cf), (-cf, -sf), (sf, -cf) for
+  // qi = 0..3 -- odd quadrants swap the pair, the sign of cos flips for qi = 1, 2
+  // and that of sin for qi = 2, 3 (sign flips are exact; no branches)
+  const uint32_t qi = (uint32_t)(int)q;
+  const bool swap = (qi & 1u) != 0u;
+  const float c0 = swap ? sf : cf, s0 = swap ? cf : sf;
+  const float c = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, c0) ^ (((qi + 1u) & 2u) << 30));
+  const float sn = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, s0) ^ ((qi & 2u) << 30));
   *ux = s * c;
   *uy = s * sn;
   *uz = z;

@@ -11,15 +11,22 @@
 //
 //   sqrt_rn(x)   exact (equal to sqrtf) for x == +0 and for finite x >= 2^-96;
 //   div_rn(x, y) exact (equal to x / y) for finite x, normal y, and a quotient whose
-//                exponent stays inside [-100, +100]; x == +-0 keeps the IEEE sign.
+//                exponent stays inside [-100, +100]; x == +-0 keeps the IEEE sign;
+//   div_rn_r(x, y, r)  the same quotient for a divisor known in advance, given its
+//                correctly rounded reciprocal r = RN(1/y): the hardware reciprocal
+//                and its Newton step are skipped.  q0 = x r is within two ulps of
+//                x/y, one residual step makes it faithful, and the last step rounds
+//                correctly (Markstein: faithful q, exact residual x - y q, r = RN(1/y)).
 //
-// For those inputs they execute the compiler's own instruction sequence minus the
-// steps that are identities there, and they are checked against sqrtf (every float in
-// the range) and against IEEE division (random operand pairs) on the GPU by
-// tests/gpu_selftest/arith_check.hip.
+// For those inputs sqrt_rn and div_rn execute the compiler's own instruction sequence
+// minus the steps that are identities there.  All three are checked against sqrtf
+// (every float in the range) and against IEEE division (random operand pairs) on the
+// GPU by tests/gpu_selftest/arith_check.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #pragma clang fp contract(off)
 
@@ -47,6 +54,32 @@ __device__ __forceinline__ float div_rn(float x, float y) {
   const float m1 = __builtin_fmaf(-y, q1, x);
   const float q = __builtin_fmaf(m1, r, q1);
   return x == 0.0f ? q0 : q;                                       // +-0 with the IEEE sign
+}
+
+__device__ __forceinline__ float div_rn_r(float x, float y, float r) {
+  const float q0 = x * r;
+  const float m0 = __builtin_fmaf(-y, q0, x);
+  const float q1 = __builtin_fmaf(m0, r, q0);
+  const float m1 = __builtin_fmaf(-y, q1, x);
+  const float q = __builtin_fmaf(m1, r, q1);
+  return x == 0.0f ? q0 : q;                                       // +-0 with the IEEE sign
+}
+
+// Host: RN(1/y) for div_rn_r.  1/y in float64 rounded to float32 can miss by one
+// ulp when 1/y lies within 2^-53 of a float32 midpoint, so the neighbour with the
+// smallest residual |1 - y r| is taken (y r is exact in float64: 24 + 24 bits).
+inline float recip_rn(float y) {
+  const float r0 = (float)(1.0 / (double)y);
+  float best = r0;
+  double res = std::fabs(1.0 - (double)y * (double)r0);
+  for (const float c : {std::nextafter(r0, -INFINITY), std::nextafter(r0, INFINITY)}) {
+    const double rc = std::fabs(1.0 - (double)y * (double)c);
+    if (rc < res) {
+      best = c;
+      res = rc;
+    }
+  }
+  return best;
 }
 
 }  // namespace dmw

@@ -71,6 +71,7 @@ constexpr int kWave = 64;
 struct WalkArgs {
   // geometry
   float size, cell, inv_cell, eps, nudge, rhoD, kappaD;
+  float inv_size;            // RN(1 / size): reciprocal for the known-divisor quotient
   int n_obj;
   const float4* objs;        // device (cx, cy, cz, R), native frame
   int grid_n;                // packed: grid cells per axis
@@ -221,12 +222,12 @@ __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem) && (sl > 0.0f))) break;
     const float hx = x + d * dx, hy = y + d * dy;
-    const float nx = div_rn(hx, R), ny = div_rn(hy, R);
+    const float nx = div_rn_r(hx, R, a.inv_size), ny = div_rn_r(hy, R, a.inv_size);
     const float tdd = 2.0f * (dx * nx + dy * ny);
     float rx = dx - tdd * nx, ry = dy - tdd * ny;
     const float rn = sqrt_rn(rx * rx + ry * ry);
     rx = div_rn(rx, rn); ry = div_rn(ry, rn);
-    dsum = dsum + (rem - d) * (div_rn(sq, R));
+    dsum = dsum + (rem - d) * (div_rn_r(sq, R, a.inv_size));
     x = hx - NUDGE * nx;
     y = hy - NUDGE * ny;
     dx = rx; dy = ry;
@@ -266,7 +267,7 @@ __device__ __forceinline__ float cyl_permeate(float& x, float& y, float sx, floa
   const float hl = fmaxf(sqrt_rn(hx * hx + hy * hy), 1e-30f);
   const float nx = div_rn(hx, hl), ny = div_rn(hy, hl);
   const float remaining = sl - t_hit;
-  const float dperp = remaining * (div_rn(sq, R));
+  const float dperp = remaining * (div_rn_r(sq, R, a.inv_size));
   const float p = fminf(1.0f, (2.0f * a.kappaD) * dperp);
   if (ps.get() < p) {
     x = x + sx; y = y + sy; ncross++;
@@ -308,12 +309,13 @@ __device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float s
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem))) break;
     const float hx = x + d * dx, hy = y + d * dy, hz = z + d * dz;
-    const float nx = div_rn(hx, R), ny = div_rn(hy, R), nz = div_rn(hz, R);
+    const float nx = div_rn_r(hx, R, a.inv_size), ny = div_rn_r(hy, R, a.inv_size);
+    const float nz = div_rn_r(hz, R, a.inv_size);
     const float tdd = 2.0f * ((dx * nx + dy * ny) + dz * nz);
     float rx = dx - tdd * nx, ry = dy - tdd * ny, rz = dz - tdd * nz;
     const float rn = sqrt_rn((rx * rx + ry * ry) + rz * rz);
     rx = div_rn(rx, rn); ry = div_rn(ry, rn); rz = div_rn(rz, rn);
-    dsum = dsum + (rem - d) * (div_rn(sq, R));
+    dsum = dsum + (rem - d) * (div_rn_r(sq, R, a.inv_size));
     x = hx - NUDGE * nx; y = hy - NUDGE * ny; z = hz - NUDGE * nz;
     dx = rx; dy = ry; dz = rz;
     rem = (rem - d) - NUDGE;
@@ -348,9 +350,10 @@ __device__ __forceinline__ float sph_permeate(Pos& p, float sx, float sy, float 
   if (!any_hit) { p.x = x + sx; p.y = y + sy; p.z = z + sz; return 0.0f; }
   hits++;
   const float hx = x + t_hit * dx, hy = y + t_hit * dy, hz = z + t_hit * dz;
-  const float nx = div_rn(hx, R), ny = div_rn(hy, R), nz = div_rn(hz, R);
+  const float nx = div_rn_r(hx, R, a.inv_size), ny = div_rn_r(hy, R, a.inv_size);
+  const float nz = div_rn_r(hz, R, a.inv_size);
   const float remaining = sl - t_hit;
-  const float dperp = remaining * (div_rn(sq, R));
+  const float dperp = remaining * (div_rn_r(sq, R, a.inv_size));
   const float pr = fminf(1.0f, (2.0f * a.kappaD) * dperp);
   if (ps.get() < pr) {
     p.x = x + sx; p.y = y + sy; p.z = z + sz; ncross++;

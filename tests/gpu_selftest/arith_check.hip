@@ -1,12 +1,14 @@
 // arith_check.hip -- test infrastructure: checks the walk's unscaled correctly
-// rounded square root and quotient (dmipy-sim_amd/csrc/rn_arith.h) against the
+// rounded square root and quotients (dmipy-sim_amd/csrc/rn_arith.h) against the
 // compiler's IEEE sqrtf and '/' on the GPU, bit for bit.
 //   arith_check_sqrt: every float bit pattern in [lo, hi) (grid-stride);
 //   arith_check_div:  n operand pairs from a counter hash, each operand with a
 //                     random sign, mantissa and an exponent in [-emax, emax]; pairs
 //                     whose quotient exponent leaves [-100, 100] are skipped; one
 //                     pair in 16 uses an all-ones mantissa divisor, one in 64 a zero
-//                     numerator.
+//                     numerator.  known_divisor = 1 checks div_rn_r with the
+//                     reciprocal RN(1/y) taken from IEEE division;
+//   arith_check_recip (host only): recip_rn(y) against IEEE 1.0f / y.
 // Mismatches are counted; the first one found is reported.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,7 +41,7 @@ __global__ void sqrt_kernel(uint32_t lo, uint32_t hi, unsigned long long* bad, u
   }
 }
 
-__global__ void div_kernel(uint64_t n, uint32_t seed, int emax, unsigned long long* bad,
+__global__ void div_kernel(uint64_t n, uint32_t seed, int emax, int known_divisor, unsigned long long* bad,
                            unsigned long long* skipped, uint32_t* first) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -55,7 +57,7 @@ __global__ void div_kernel(uint64_t n, uint32_t seed, int emax, unsigned long lo
       atomicAdd(skipped, 1ull);
       continue;
     }
-    const float got = dmw::div_rn(x, y);
+    const float got = known_divisor ? dmw::div_rn_r(x, y, 1.0f / y) : dmw::div_rn(x, y);
     if (rb != __builtin_bit_cast(uint32_t, got)) {
       if (atomicAdd(bad, 1ull) == 0ull) {
         first[0] = __builtin_bit_cast(uint32_t, x);
@@ -66,13 +68,13 @@ __global__ void div_kernel(uint64_t n, uint32_t seed, int emax, unsigned long lo
 }
 
 int run(unsigned long long* out_bad, unsigned long long* out_skipped, uint32_t* out_first,
-        bool is_div, uint32_t lo, uint32_t hi, uint64_t n, uint32_t seed, int emax) {
+        bool is_div, uint32_t lo, uint32_t hi, uint64_t n, uint32_t seed, int emax, int known_divisor) {
   unsigned long long* d = nullptr;
   uint32_t* f = nullptr;
   if (hipMalloc(&d, 16) != hipSuccess || hipMalloc(&f, 8) != hipSuccess) return 1;
   if (hipMemset(d, 0, 16) != hipSuccess || hipMemset(f, 0, 8) != hipSuccess) return 1;
   if (is_div)
-    hipLaunchKernelGGL(div_kernel, dim3(8192), dim3(256), 0, 0, n, seed, emax, d, d + 1, f);
+    hipLaunchKernelGGL(div_kernel, dim3(8192), dim3(256), 0, 0, n, seed, emax, known_divisor, d, d + 1, f);
   else
     hipLaunchKernelGGL(sqrt_kernel, dim3(8192), dim3(256), 0, 0, lo, hi, d, f);
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) return 1;
@@ -89,10 +91,37 @@ int run(unsigned long long* out_bad, unsigned long long* out_skipped, uint32_t* 
 }  // namespace
 
 extern "C" int arith_check_sqrt(uint32_t lo, uint32_t hi, unsigned long long* bad, uint32_t* first) {
-  return run(bad, nullptr, first, false, lo, hi, 0, 0, 0);
+  return run(bad, nullptr, first, false, lo, hi, 0, 0, 0, 0);
 }
 
-extern "C" int arith_check_div(uint64_t n, uint32_t seed, int emax, unsigned long long* bad,
+extern "C" int arith_check_div(uint64_t n, uint32_t seed, int emax, int known_divisor, unsigned long long* bad,
                                unsigned long long* skipped, uint32_t* first) {
-  return run(bad, skipped, first, true, 0, 0, n, seed, emax);
+  return run(bad, skipped, first, true, 0, 0, n, seed, emax, known_divisor);
+}
+
+// Host: recip_rn(y) == 1.0f / y (IEEE float32 division of the host compiler) for n
+// random normal y with exponents in [-emax, emax], plus every mantissa of [1, 2)
+// when exhaustive is set.
+extern "C" int arith_check_recip(uint64_t n, uint32_t seed, int emax, int exhaustive, unsigned long long* bad,
+                                 uint32_t* first) {
+  unsigned long long nb = 0;
+  auto check = [&](float y) {
+    volatile float one = 1.0f;
+    const float ref = one / y;
+    const float got = dmw::recip_rn(y);
+    if (__builtin_bit_cast(uint32_t, ref) != __builtin_bit_cast(uint32_t, got)) {
+      if (nb++ == 0) *first = __builtin_bit_cast(uint32_t, y);
+    }
+  };
+  uint64_t v = 0x9E3779B97F4A7C15ull ^ ((uint64_t)seed << 32);
+  for (uint64_t i = 0; i < n; ++i) {
+    v ^= v << 13; v ^= v >> 7; v ^= v << 17;
+    const uint32_t h = (uint32_t)v, g = (uint32_t)(v >> 32);
+    const int e = (int)(g % (uint32_t)(2 * emax + 1)) - emax;
+    check(__builtin_bit_cast(float, (h & 0x807FFFFFu) | ((uint32_t)(e + 127) << 23)));
+  }
+  if (exhaustive)
+    for (uint32_t m = 0; m < (1u << 23); ++m) check(__builtin_bit_cast(float, 0x3F800000u | m));
+  *bad = nb;
+  return 0;
 }

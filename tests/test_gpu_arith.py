@@ -1,7 +1,7 @@
-"""The walk's unscaled correctly rounded square root and quotient
+"""The walk's unscaled correctly rounded square root and quotients
 (dmipy-sim_amd/csrc/rn_arith.h) against the compiler's IEEE sqrtf and '/' on the
 GPU, bit for bit: every float in the square root's range, and 2^31 + 2^31 random
-operand pairs for the quotient (tests/gpu_selftest/arith_check.hip)."""
+operand pairs for each quotient form (tests/gpu_selftest/arith_check.hip)."""
 import ctypes as C
 import os
 
@@ -19,7 +19,7 @@ def arith(gpu):
     lib = C.CDLL(LIB)
     lib.arith_check_sqrt.argtypes = [C.c_uint32, C.c_uint32, C.POINTER(C.c_ulonglong),
                                      C.POINTER(C.c_uint32)]
-    lib.arith_check_div.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_ulonglong),
+    lib.arith_check_div.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_ulonglong),
                                     C.POINTER(C.c_ulonglong), C.POINTER(C.c_uint32)]
     return lib
 
@@ -33,12 +33,14 @@ def test_sqrt_rn_exhaustive(arith):
         assert bad.value == 0, f"{bad.value} mismatches, first at bits {first[0]:#x}"
 
 
+@pytest.mark.parametrize("known", [0, 1], ids=["div_rn", "div_rn_r"])
 @pytest.mark.parametrize("emax,seed", [(60, 1), (20, 2)])
-def test_div_rn_random(arith, emax, seed):
+def test_div_rn_random(arith, emax, seed, known):
     """Random signs, mantissas (one divisor in 16 with an all-ones mantissa, one
-    numerator in 64 zero) and exponents in [-emax, emax]."""
+    numerator in 64 zero) and exponents in [-emax, emax]; div_rn_r with the divisor's
+    correctly rounded reciprocal."""
     bad, skipped, first = C.c_ulonglong(0), C.c_ulonglong(0), (C.c_uint32 * 2)()
-    print(f"div check emax={emax} seed={seed}", flush=True)
-    assert arith.arith_check_div(1 << 31, seed, emax, C.byref(bad), C.byref(skipped), first) == 0
+    print(f"div check emax={emax} seed={seed} known_divisor={known}", flush=True)
+    assert arith.arith_check_div(1 << 31, seed, emax, known, C.byref(bad), C.byref(skipped), first) == 0
     assert bad.value == 0, (f"{bad.value} mismatches, first x={first[0]:#x} y={first[1]:#x}")
     assert skipped.value < (1 << 31) // 20     # quotients beyond 2^+-100 are outside the contract

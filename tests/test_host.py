@@ -374,3 +374,19 @@ def test_float_envelope_rejects_sub_angstrom_lengths():
     h = C.c_void_p()
     rc = nat.lib().dmw_plan_create(C.byref(h), gd, wd, rd)
     assert rc == 1 and b"float32 range" in nat.lib().dmw_last_error()
+
+
+def test_recip_rn_matches_ieee_division():
+    """The host's correctly rounded reciprocal (the known divisor's RN(1/y) for the
+    walk's div_rn_r) equals IEEE 1.0f / y: every mantissa of [1, 2) and 2e6 random
+    normal floats (host code of tests/gpu_selftest/arith_check.hip, no GPU call)."""
+    import ctypes as C
+    lib_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_selftest", "libarith_check.so")
+    if not os.path.exists(lib_path):
+        pytest.fail("tests/gpu_selftest/libarith_check.so is not built (run build())")
+    lib = C.CDLL(lib_path)
+    lib.arith_check_recip.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_ulonglong),
+                                      C.POINTER(C.c_uint32)]
+    bad, first = C.c_ulonglong(0), C.c_uint32(0)
+    assert lib.arith_check_recip(2_000_000, 3, 120, 1, C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, f"{bad.value} mismatches, first y bits {first.value:#x}"
