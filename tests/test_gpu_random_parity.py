@@ -2,7 +2,8 @@
 time steps, relaxivities and permeabilities, GPU against the oracle on the same
 Philox stream.  Spreads the operands of the wall arithmetic over several decades
 (positions, step lengths, radii from 0.3 to 30 um; steps from R/40 to R) so the
-exact-arithmetic routines are exercised far beyond the fixed parity cases."""
+exact-arithmetic routines are exercised far beyond the fixed parity cases; half the
+trials run the phase product on the matrix cores (float16 hi/lo split operands)."""
 import numpy as np
 import pytest
 
@@ -42,12 +43,21 @@ def test_random_walks_bit_exact(gpu, oracle, kind):
         T2 = float(rng.uniform(0.02, 0.1)) if rng.random() < 0.3 else None
         seed = int(rng.integers(1 << 30))
         plan = ds.make_walk_plan(g, D, dt, T2=T2)
-        print(f"{kind} trial {trial}: n_sub={plan.n_sub} wall={plan.wall} n_t={n_t}", flush=True)
+        # odd trials take the matrix-core phase update (32-measurement block; the random
+        # G has a nonzero net moment, so the start point's phase is exercised too)
+        mb = 32 if trial % 2 else 0
+        print(f"{kind} trial {trial}: n_sub={plan.n_sub} wall={plan.wall} n_t={n_t} meas_block={mb}",
+              flush=True)
         r = gpu.walk(g, D, wf, 256, seed=seed, T2=T2, outputs=("final_pos", "comp_final", "n_cross",
-                                                                "log_w"))
+                                                                "log_w", "phi"), meas_block=mb)
         o = oracle.run_oracle(g, D, wf, 256, seed=seed, T2=T2)
         np.testing.assert_array_equal(r["final_pos"], o["final_pos"], err_msg=f"trial {trial}")
         np.testing.assert_array_equal(r["comp_final"], o["comp_final"])
         np.testing.assert_array_equal(r["n_cross"], o["n_cross"])
         np.testing.assert_array_equal(r["log_w"], o["log_w"])
         assert np.max(np.abs(r["signal"] - o["signal"])) <= 1e-5
+        # per-walker phases: both sides sum n_t x n_sub float32 terms in different orders
+        # (hundreds of sub-steps of absolute positions here), so the bar is relative to
+        # the largest phase; a layout or start-point error would be of the phase's order
+        bar = 2e-4 + 1e-3 * float(np.abs(o["phi"]).max())
+        np.testing.assert_allclose(r["phi"], o["phi"], atol=bar, rtol=0, err_msg=f"trial {trial}")
