@@ -108,6 +108,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   for (int t = 0; t < b.n_t; ++t) {
     ph.prefetch(a, t, 0, lane);
     const float chi = b.chi ? b.chi[t] : 1.0f;
+    const float q2chi = I.has_t2 ? (I.dt_sub * I.inv_T2) * chi : 0.0f;  // zero when off (x - 0 == x)
     const float x0 = p.x, y0 = p.y, z0 = p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
     for (int s = 0; s < n_sub; ++s) {
@@ -120,13 +121,11 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
       const PowlesSrc ps{w, inst, gs, seed_lo, seed_hi, 0.0f, false};
       float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       if (WALL != W_REFLECT) dlog = dlog * chi;
-      if (I.has_t2) dlog = dlog - (I.dt_sub * I.inv_T2) * chi;
+      dlog = dlog - q2chi;
       logw = logw + dlog;
-      if (n_sub > 1) {
-        ax = ax + (p.x - x0);
-        ay = ay + (p.y - y0);
-        az = az + (p.z - z0);
-      }
+      ax = ax + (p.x - x0);   // read only when n_sub > 1
+      ay = ay + (p.y - y0);
+      az = az + (p.z - z0);
     }
     float ox = x0, oy = y0, oz = z0, ex = p.x, ey = p.y, ez = p.z;
     if (PACKED) {

@@ -1596,6 +1596,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   for (int t = 0; t < a.n_t; ++t) {
     ph.prefetch(a, t, m0, lane);
     const float chi = a.chi ? a.chi[t] : 1.0f;
+    // relaxation per sub-step: T2 while transverse, T1 while stored (physics.py:307-311)
+    const float q2chi = a.has_t2 ? (a.dt_sub * a.inv_T2) * chi : 0.0f;
+    const float q1chi = a.has_t1 ? (a.dt_sub * a.inv_T1) * (1.0f - chi) : 0.0f;
     const float x0 = (GEOM == G_PMYEL) ? uwx : p.x;
     const float y0 = (GEOM == G_PMYEL) ? uwy : p.y;
     const float z0 = (GEOM == G_PMYEL) ? uwz : p.z;
@@ -1643,22 +1646,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
         dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
         // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
         if (WALL != W_REFLECT) dlog = dlog * chi;
-        if (a.has_t2) {
+        if (GEOM == G_MYEL) {
           // MyelinatedCylinder: dt/T2 of the compartment after the step (physics.py:657-659)
-          const float q2 = (GEOM == G_MYEL) ? pick3(p.cx, a.m_q2[0], a.m_q2[1], a.m_q2[2])
-                                            : a.dt_sub * a.inv_T2;
-          dlog = dlog - q2 * chi;
+          if (a.has_t2) dlog = dlog - pick3(p.cx, a.m_q2[0], a.m_q2[1], a.m_q2[2]) * chi;
+          if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
+        } else {
+          // per-sample constants, zero when the term is off (x - 0 == x, signed zeros too)
+          dlog = (dlog - q2chi) - q1chi;
         }
-        if (a.has_t1) dlog = dlog - (a.dt_sub * a.inv_T1) * (1.0f - chi);
       }
       logw = logw + dlog;
       if (CHANNELS && a.dlog_h) lw_sample = lw_sample + dlog;
       if (CHANNELS && a.comp_h) occ = occ + outside_norm<GEOM>(p, a);
-      if (n_sub > 1) {
-        ax = ax + (((GEOM == G_PMYEL) ? uwx : p.x) - x0);
-        ay = ay + (((GEOM == G_PMYEL) ? uwy : p.y) - y0);
-        az = az + (((GEOM == G_PMYEL) ? uwz : p.z) - z0);
-      }
+      // (accumulated unconditionally: only read when n_sub > 1; a select on the
+      // run-time n_sub would cost more than the sums)
+      ax = ax + (((GEOM == G_PMYEL) ? uwx : p.x) - x0);
+      ay = ay + (((GEOM == G_PMYEL) ? uwy : p.y) - y0);
+      az = az + (((GEOM == G_PMYEL) ? uwz : p.z) - z0);
     }
     // Sum of the n_sub sub-step positions (unfolded), then the phase update.
     float Px, Py, Pz;
