@@ -205,6 +205,20 @@ __device__ __forceinline__ float box_step(Pos& p, float sx, float sy, float sz, 
 }
 
 // ---------------------------------------------------------------------------
+// Bounce-loop shortcut after a reflection.  With the walker inside (c = |p|^2 - R^2
+// < 0, as computed by the iteration) and heading inward (dp = u . p <= -2^-60, so
+// dp^2 does not underflow), the iteration's float32 quantities satisfy
+// disc >= fl(dp^2), sqrt_rn(disc) >= |dp| (1 - 2^-23) and d = -dp + sq >= 2 |dp|
+// (1 - 2^-22).  A remaining path rem <= fl(2 |dp| (1 - 2^-20)) is then below d, so
+// the iteration would find no hit and leave every value unchanged: it can be skipped
+// (its square root is the saving).  Only used after a hit, where it decides for
+// almost every lane that reflected.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool no_second_hit(float dp, float c, float rem) {
+  return dp <= -0x1p-60f && c < 0.0f && rem <= (-2.0f * dp) * 0x1.ffffep-1f;
+}
+
+// ---------------------------------------------------------------------------
 // Cylinder, interior reflection (geometries.py:516-654).  Returns sum d_perp.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float sy, const WalkArgs& a,
@@ -217,7 +231,9 @@ __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float
   float rem = sl, dsum = 0.0f;
   for (int it = 0; it < 10; ++it) {
     const float dp = dx * x + dy * y;
-    const float disc = fmaxf(dp * dp - ((x * x + y * y) - RR), 0.0f);
+    const float c = (x * x + y * y) - RR;
+    if (it > 0 && no_second_hit(dp, c, rem)) break;
+    const float disc = fmaxf(dp * dp - c, 0.0f);
     const float sq = sqrt_rn(disc);
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem) && (sl > 0.0f))) break;
@@ -304,7 +320,9 @@ __device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float s
   float rem = sl, dsum = 0.0f;
   for (int it = 0; it < 10; ++it) {
     const float dp = (dx * x + dy * y) + dz * z;
-    const float disc = fmaxf(dp * dp - (((x * x + y * y) + z * z) - RR), 0.0f);
+    const float c = ((x * x + y * y) + z * z) - RR;
+    if (it > 0 && no_second_hit(dp, c, rem)) break;
+    const float disc = fmaxf(dp * dp - c, 0.0f);
     const float sq = sqrt_rn(disc);
     const float d = -dp + sq;
     if (!((d > EPS) && (d < rem))) break;

@@ -390,3 +390,35 @@ def test_recip_rn_matches_ieee_division():
     bad, first = C.c_ulonglong(0), C.c_uint32(0)
     assert lib.arith_check_recip(2_000_000, 3, 120, 1, C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} mismatches, first y bits {first.value:#x}"
+
+
+def test_bounce_shortcut_never_skips_a_hit():
+    """The walk's bounce-loop shortcut after a reflection (walk_kernel.h no_second_hit:
+    inside, heading inward, remaining path <= fl(2|dp| (1 - 2^-20))) only skips
+    iterations that find no hit: on 4e6 random float32 post-reflection states (radii
+    over two decades, positions nudged inside the wall or anywhere inside, remaining
+    paths from 1e-3 R to 3 R, some negative) the iteration's own float32 arithmetic
+    (d = -dp + sqrt(max(dp^2 - c, 0)), hit iff EPS < d < rem) never hits where the
+    shortcut is taken."""
+    rng = np.random.default_rng(17)
+    n = 4_000_000
+    f = np.float32
+    R = (10.0 ** rng.uniform(-6.5, -4.5, n)).astype(f)
+    RR = R * R
+    ang = rng.uniform(0, 2 * np.pi, n)
+    near = rng.random(n) < 0.5
+    rad = np.where(near, R * (1.0 - 1e-4 * (1 + rng.random(n))), R * np.sqrt(rng.random(n)))
+    x = (rad * np.cos(ang)).astype(f)
+    y = (rad * np.sin(ang)).astype(f)
+    th = rng.uniform(0, 2 * np.pi, n)
+    dx, dy = np.cos(th).astype(f), np.sin(th).astype(f)
+    rem = (R * 10.0 ** rng.uniform(-3.0, 0.5, n)).astype(f)
+    rem = np.where(rng.random(n) < 0.1, -rem, rem).astype(f)
+    dp = dx * x + dy * y
+    c = (x * x + y * y) - RR
+    take = (dp <= f(-(2.0 ** -60))) & (c < 0) & (rem <= (f(-2.0) * dp) * f(1 - 2.0 ** -20))
+    assert take.sum() > n // 4
+    disc = np.maximum(dp * dp - c, f(0))
+    d = -dp + np.sqrt(disc)
+    hit = (d > f(1e-7) * R) & (d < rem)
+    assert not np.any(take & hit)
