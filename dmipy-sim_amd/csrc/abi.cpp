@@ -265,6 +265,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   dmw::WalkArgs& a = P->args;
   a.size = geom->size;
   a.inv_size = geom->size > 0.0f ? dmw::recip_rn(geom->size) : 0.0f;
+  a.size_sq_cut = geom->size > 0.0f ? dmw::sqrt_cut(geom->size) : 0.0f;
   a.cell = geom->cell;
   a.inv_cell = geom->cell > 0.0f ? 1.0f / geom->cell : 0.0f;
   a.eps = geom->eps;

@@ -82,4 +82,14 @@ inline float recip_rn(float y) {
   return best;
 }
 
+// Host: the least float s >= 0 with sqrtf(s) >= r (r > 0 finite), so that for every
+// float s >= 0, sqrt_rn(s) >= r exactly when s >= sqrt_cut(r) (square root correctly
+// rounded and monotone; host sqrtf is the IEEE square root).
+inline float sqrt_cut(float r) {
+  float s = r * r;
+  while (s > 0.0f && std::sqrt(std::nextafter(s, 0.0f)) >= r) s = std::nextafter(s, 0.0f);
+  while (std::sqrt(s) < r) s = std::nextafter(s, INFINITY);
+  return s;
+}
+
 }  // namespace dmw

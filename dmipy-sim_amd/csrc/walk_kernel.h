@@ -72,6 +72,7 @@ struct WalkArgs {
   // geometry
   float size, cell, inv_cell, eps, nudge, rhoD, kappaD;
   float inv_size;            // RN(1 / size): reciprocal for the known-divisor quotient
+  float size_sq_cut;         // least float s with sqrtf(s) >= size: sqrt_rn(s) >= size <=> s >= cut
   int n_obj;
   const float4* objs;        // device (cx, cy, cz, R), native frame
   int grid_n;                // packed: grid cells per axis
@@ -252,8 +253,11 @@ __device__ __forceinline__ float cyl_reflect(float& x, float& y, float sx, float
   }
   const float rm = fmaxf(rem, 0.0f);
   float fx = x + dx * rm, fy = y + dy * rm;
-  const float fn = sqrt_rn(fx * fx + fy * fy);
-  if (fn >= R) {
+  // fn >= R decided on the squared norm (exact: sqrt_rn is monotone, size_sq_cut is the
+  // least square whose root reaches R); the root itself only inside the rare clamp
+  const float ss = fx * fx + fy * fy;
+  if (ss >= a.size_sq_cut) {
+    const float fn = sqrt_rn(ss);
     const float s = R - NUDGE;
     fx = div_rn(fx * s, fn);
     fy = div_rn(fy * s, fn);
@@ -341,8 +345,9 @@ __device__ __forceinline__ float sph_reflect(Pos& p, float sx, float sy, float s
   }
   const float rm = fmaxf(rem, 0.0f);
   float fx = x + dx * rm, fy = y + dy * rm, fz = z + dz * rm;
-  const float fn = sqrt_rn((fx * fx + fy * fy) + fz * fz);
-  if (fn >= R) {
+  const float ss = (fx * fx + fy * fy) + fz * fz;
+  if (ss >= a.size_sq_cut) {  // fn >= R on the squared norm, as in cyl_reflect
+    const float fn = sqrt_rn(ss);
     const float s = R - NUDGE;
     fx = div_rn(fx * s, fn); fy = div_rn(fy * s, fn); fz = div_rn(fz * s, fn);
   }

@@ -8,7 +8,9 @@
 //                     pair in 16 uses an all-ones mantissa divisor, one in 64 a zero
 //                     numerator.  known_divisor = 1 checks div_rn_r with the
 //                     reciprocal RN(1/y) taken from IEEE division;
-//   arith_check_recip (host only): recip_rn(y) against IEEE 1.0f / y.
+//   arith_check_recip (host only): recip_rn(y) against IEEE 1.0f / y;
+//   arith_check_sqrt_cut (host only): sqrt_cut(r) is the least float whose IEEE
+//                     square root reaches r.
 // Mismatches are counted; the first one found is reported.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -122,6 +124,27 @@ extern "C" int arith_check_recip(uint64_t n, uint32_t seed, int emax, int exhaus
   }
   if (exhaustive)
     for (uint32_t m = 0; m < (1u << 23); ++m) check(__builtin_bit_cast(float, 0x3F800000u | m));
+  *bad = nb;
+  return 0;
+}
+
+// Host: for n random r (exponents in [-emax, emax]), sqrtf(cut) >= r > sqrtf(cut-),
+// with cut = sqrt_cut(r) and cut- its predecessor.
+extern "C" int arith_check_sqrt_cut(uint64_t n, uint32_t seed, int emax, unsigned long long* bad,
+                                    uint32_t* first) {
+  unsigned long long nb = 0;
+  uint64_t v = 0xD1B54A32D192ED03ull ^ ((uint64_t)seed << 32);
+  for (uint64_t i = 0; i < n; ++i) {
+    v ^= v << 13; v ^= v >> 7; v ^= v << 17;
+    const uint32_t h = (uint32_t)v, g = (uint32_t)(v >> 32);
+    const int e = (int)(g % (uint32_t)(2 * emax + 1)) - emax;
+    const float r = __builtin_bit_cast(float, (h & 0x007FFFFFu) | ((uint32_t)(e + 127) << 23));
+    const float cut = dmw::sqrt_cut(r);
+    const float below = std::nextafter(cut, 0.0f);
+    if (!(std::sqrt(cut) >= r && std::sqrt(below) < r)) {
+      if (nb++ == 0) *first = __builtin_bit_cast(uint32_t, r);
+    }
+  }
   *bad = nb;
   return 0;
 }

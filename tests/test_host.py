@@ -422,3 +422,18 @@ def test_bounce_shortcut_never_skips_a_hit():
     d = -dp + np.sqrt(disc)
     hit = (d > f(1e-7) * R) & (d < rem)
     assert not np.any(take & hit)
+
+
+def test_sqrt_cut_is_the_least_square_reaching_r():
+    """The walk decides |f| >= R on the squared norm against sqrt_cut(R), the least float
+    whose square root reaches R (host code of tests/gpu_selftest/arith_check.hip)."""
+    import ctypes as C
+    lib_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_selftest", "libarith_check.so")
+    if not os.path.exists(lib_path):
+        pytest.fail("tests/gpu_selftest/libarith_check.so is not built (run build())")
+    lib = C.CDLL(lib_path)
+    lib.arith_check_sqrt_cut.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_ulonglong),
+                                         C.POINTER(C.c_uint32)]
+    bad, first = C.c_ulonglong(0), C.c_uint32(0)
+    assert lib.arith_check_sqrt_cut(2_000_000, 5, 40, C.byref(bad), C.byref(first)) == 0
+    assert bad.value == 0, f"{bad.value} failures, first r bits {first.value:#x}"
