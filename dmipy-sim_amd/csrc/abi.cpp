@@ -266,6 +266,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.size = geom->size;
   a.inv_size = geom->size > 0.0f ? dmw::recip_rn(geom->size) : 0.0f;
   a.size_sq_cut = geom->size > 0.0f ? dmw::sqrt_cut(geom->size) : 0.0f;
+  a.size_sq_cut_gt = geom->size > 0.0f ? dmw::sqrt_cut(std::nextafter(geom->size, INFINITY)) : 0.0f;
   a.cell = geom->cell;
   a.inv_cell = geom->cell > 0.0f ? 1.0f / geom->cell : 0.0f;
   a.eps = geom->eps;

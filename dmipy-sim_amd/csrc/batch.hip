@@ -32,6 +32,7 @@ struct InstDev {
   float size, eps, nudge, rhoD, kappaD, step_len, dt_sub, inv_T2, gdt;
   float inv_size;            // RN(1 / size)
   float size_sq_cut;         // least float s with sqrtf(s) >= size
+  float size_sq_cut_gt;      // least float s with sqrtf(s) > size
   float cell, inv_cell, grid_inv_h;
   float ax, ay, az, aux;
   int kind, wall, n_sub, has_t2;
@@ -78,7 +79,8 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   const bool active = widx < I.n_walkers;
   const uint32_t w = (uint32_t)(active ? widx : I.n_walkers - 1);
   WalkArgs a{};
-  a.size = I.size; a.inv_size = I.inv_size; a.size_sq_cut = I.size_sq_cut; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
+  a.size = I.size; a.inv_size = I.inv_size; a.size_sq_cut = I.size_sq_cut;
+  a.size_sq_cut_gt = I.size_sq_cut_gt; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
   a.cell = I.cell; a.inv_cell = I.inv_cell; a.grid_n = I.grid_n; a.grid_inv_h = I.grid_inv_h;
   a.exact_minimage = I.exact_minimage; a.n_obj = I.n_obj;
   a.ax = I.ax; a.ay = I.ay; a.az = I.az; a.aux = I.aux;
@@ -322,6 +324,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     d.size = I.geom.size; d.eps = I.geom.eps; d.nudge = I.geom.nudge;
     d.inv_size = I.geom.size > 0.0f ? dmw::recip_rn(I.geom.size) : 0.0f;
     d.size_sq_cut = I.geom.size > 0.0f ? dmw::sqrt_cut(I.geom.size) : 0.0f;
+    d.size_sq_cut_gt = I.geom.size > 0.0f ? dmw::sqrt_cut(std::nextafter(I.geom.size, INFINITY)) : 0.0f;
     d.rhoD = I.geom.rho_over_D; d.kappaD = I.geom.kappa_over_D;
     d.step_len = I.step_len; d.dt_sub = I.dt_sub; d.inv_T2 = I.inv_T2; d.gdt = I.gamma_dt_sub;
     d.kind = I.geom.kind;

@@ -73,6 +73,7 @@ struct WalkArgs {
   float size, cell, inv_cell, eps, nudge, rhoD, kappaD;
   float inv_size;            // RN(1 / size): reciprocal for the known-divisor quotient
   float size_sq_cut;         // least float s with sqrtf(s) >= size: sqrt_rn(s) >= size <=> s >= cut
+  float size_sq_cut_gt;      // least float s with sqrtf(s) > size
   int n_obj;
   const float4* objs;        // device (cx, cy, cz, R), native frame
   int grid_n;                // packed: grid cells per axis
@@ -302,11 +303,16 @@ __device__ __forceinline__ float cyl_permeate(float& x, float& y, float sx, floa
   const float qy = inside ? syh - NUDGE * ny : syh + NUDGE * ny;
   const float go = fmaxf(remaining - NUDGE, 0.0f);
   float fx = qx + rx * go, fy = qy + ry * go;
-  const float fnorm = sqrt_rn(fx * fx + fy * fy);
-  const float fsafe = fmaxf(fnorm, 1e-30f);
+  // fnorm >= R and fnorm <= R decided on the squared norm (as in cyl_reflect; fnorm > R
+  // exactly when the square reaches size_sq_cut_gt), the root only for the clamp
+  const float ss = fx * fx + fy * fy;
   const float target = inside ? R - NUDGE : R + NUDGE;
-  const bool wrong = inside ? (fnorm >= R) : (fnorm <= R);
-  if (wrong) { fx = div_rn(fx * target, fsafe); fy = div_rn(fy * target, fsafe); }
+  const bool wrong = inside ? (ss >= a.size_sq_cut) : (ss < a.size_sq_cut_gt);
+  if (wrong) {
+    const float fsafe = fmaxf(sqrt_rn(ss), 1e-30f);
+    fx = div_rn(fx * target, fsafe);
+    fy = div_rn(fy * target, fsafe);
+  }
   x = fx; y = fy;
   return (-2.0f * a.rhoD) * dperp;
 }
