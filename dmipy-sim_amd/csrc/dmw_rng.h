@@ -37,6 +37,15 @@ DMW_HD uint32_t mulhilo32(uint32_t a, uint32_t b, uint32_t* hi) {
   return (uint32_t)p;
 }
 
+// a ^ b ^ c: one v_bitop3_b32 (truth table 0x96) on the GPU instead of two XORs.
+DMW_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+#else
+  return a ^ b ^ c;
+#endif
+}
+
 // Philox4x32-10 block: 4 x 32 random bits for counter c under key (k0, k1).
 DMW_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
@@ -45,9 +54,9 @@ DMW_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
     uint32_t lo0 = mulhilo32(0xD2511F53u, c.x, &hi0);
     uint32_t lo1 = mulhilo32(0xCD9E8D57u, c.z, &hi1);
     u32x4 n;
-    n.x = hi1 ^ c.y ^ k0;
+    n.x = xor3(hi1, c.y, k0);
     n.y = lo1;
-    n.z = hi0 ^ c.w ^ k1;
+    n.z = xor3(hi0, c.w, k1);
     n.w = lo0;
     c = n;
     k0 += 0x9E3779B9u;
