@@ -1272,13 +1272,14 @@ __device__ __forceinline__ void phase_sincos(float x, float* s, float* c) {
   pc = fmaf(r2, pc, 4.16666667e-2f);
   pc = fmaf(r2, pc, -0.5f);
   const float cr = fmaf(r2, pc, 1.0f);
-  const int qi = ((int)q) & 3;
-  float cs = cr, sn = sr;
-  if (qi == 1) { cs = -sr; sn = cr; }
-  else if (qi == 2) { cs = -cr; sn = -sr; }
-  else if (qi == 3) { cs = sr; sn = -cr; }
-  *s = sn;
-  *c = cs;
+  // quadrant qi = q mod 4: (cos, sin) = (cr, sr), (-sr, cr), (-cr, -sr), (sr, -cr) --
+  // odd quadrants swap the pair, cos flips sign for qi = 1, 2 and sin for qi = 2, 3
+  // (exact sign flips, no branches)
+  const uint32_t qi = (uint32_t)(int)q;
+  const bool swap = (qi & 1u) != 0u;
+  const float c0 = swap ? sr : cr, s0 = swap ? cr : sr;
+  *c = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, c0) ^ (((qi + 1u) & 2u) << 30));
+  *s = __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, s0) ^ ((qi & 2u) << 30));
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
