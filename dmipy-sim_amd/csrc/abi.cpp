@@ -3,8 +3,8 @@
 // Uploads a walk (waveform, geometry tables, optional start positions and
 // injected increments), launches the walk kernel over walker tiles and
 // measurement blocks, reduces the per-wave partial sums in float64 and copies
-// the requested outputs back.  One HIP stream per calling thread; no state is
-// kept between calls except inside a dmw_plan.
+// the requested outputs back.  One HIP stream per calling thread and device; no
+// other state is kept between calls except inside a dmw_plan.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,8 +27,12 @@ namespace {
 
 thread_local std::string g_err;
 thread_local int g_device = 0;
-thread_local hipStream_t g_stream = nullptr;
-thread_local int g_stream_device = -1;
+
+// One non-blocking stream per (thread, device), created on first use and kept
+// for the life of the thread (at most one per device: switching devices back and
+// forth reuses them).  They are not destroyed at thread exit, where the HIP
+// runtime may already be shutting down.
+thread_local std::vector<hipStream_t> g_streams;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -51,18 +55,20 @@ int hip_fail(hipError_t e, const char* what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
-int ensure_stream(hipStream_t* out) {
+// Makes `device` current on this thread and returns its stream.
+int ensure_stream_on(int device, hipStream_t* out) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(DMW_ENODEV, "no HIP device visible");
-  if (g_device < 0 || g_device >= n) return fail(DMW_ENODEV, "device index out of range");
-  HIP_TRY(hipSetDevice(g_device), "hipSetDevice");
-  if (g_stream == nullptr || g_stream_device != g_device) {
-    HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking), "hipStreamCreate");
-    g_stream_device = g_device;
-  }
-  *out = g_stream;
+  if (device < 0 || device >= n) return fail(DMW_ENODEV, "device index out of range");
+  HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  std::vector<hipStream_t>& v = g_streams;
+  if ((int)v.size() < n) v.resize(n, nullptr);
+  if (v[device] == nullptr) HIP_TRY(hipStreamCreateWithFlags(&v[device], hipStreamNonBlocking), "hipStreamCreate");
+  *out = v[device];
   return DMW_OK;
 }
+
+int ensure_stream(hipStream_t* out) { return ensure_stream_on(g_device, out); }
 
 // Device allocation owned by a plan.
 struct DevBuf {
@@ -89,10 +95,15 @@ struct DevBuf {
   T* as() const { return reinterpret_cast<T*>(p); }
 };
 
+// Copies n bytes from host memory and waits for the copy: callers pass
+// temporaries that are freed on return.
 int upload(DevBuf& b, const void* src, size_t n, hipStream_t st) {
   int rc = b.alloc(n);
   if (rc) return rc;
-  if (n) HIP_TRY(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+  if (n) {
+    HIP_TRY(hipMemcpyAsync(b.p, src, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync H2D");
+    HIP_TRY(hipStreamSynchronize(st), "hipMemcpyAsync H2D");
+  }
   return DMW_OK;
 }
 
@@ -105,6 +116,7 @@ int batch_hip_fail(hipError_t e, const char* what) { return hip_fail(e, what); }
 
 // ---------------------------------------------------------------------------
 struct dmw_plan {
+  int device = 0;            // device holding the plan's buffers (current at dmw_plan_create)
   dmw_geometry geom{};
   dmw_myelin myelin{};
   double inv_norm = 1.0;     // 1 / (sum of spin weights): 1/N except for the myelinated cylinder
@@ -234,6 +246,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (rc) return rc;
 
   dmw_plan* P = new dmw_plan();
+  P->device = g_device;
   P->geom = *geom;
   P->geom.myelin = nullptr;
   P->geom.packed_myelin = nullptr;
@@ -512,7 +525,7 @@ static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid,
 int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   if (!P || !out || !out->signal) return fail(DMW_EINVAL, "null argument");
   hipStream_t st;
-  int rc = ensure_stream(&st);
+  int rc = ensure_stream_on(P->device, &st);
   if (rc) return rc;
   const long long N = P->run.n_walkers;
   dmw::WalkArgs a = P->args;
@@ -668,7 +681,7 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
   int rc = dmw_plan_create(&P, geom, &wave, &r);
   if (rc) return rc;
   hipStream_t st;
-  rc = ensure_stream(&st);
+  rc = ensure_stream_on(P->device, &st);
   if (rc) {
     dmw_plan_destroy(P);
     return rc;

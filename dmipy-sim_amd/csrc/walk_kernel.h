@@ -1445,8 +1445,10 @@ struct PhaseMFMA {
   // tiles 2p and 2p + 1 of one lane), copied straight into the wave's LDS slots at
   // the start of the sample and read back in update().
   __device__ __forceinline__ void prefetch(const WalkArgs& a, int t, int m0, int lane) {
-    const uint4* src = reinterpret_cast<const uint4*>(a.GsH) +
-                       (((size_t)t * 2 + ((lane >> 4) & 1)) * (a.n_meas_pad / 32) + m0 / 32) * 16 + (lane & 15);
+    // Uniform row base (scalar registers) plus this lane's 32-bit chunk offset.
+    const uint32_t row = (uint32_t)(a.n_meas_pad / 32) * 16;
+    const uint4* base = reinterpret_cast<const uint4*>(a.GsH) + ((size_t)t * 2 * row + (size_t)(m0 / 32) * 16);
+    const uint4* src = base + (((uint32_t)(lane >> 4) & 1u) * row + ((uint32_t)lane & 15u));
     uint4* dst = phase_b_slots<NP>() + (threadIdx.x >> 6) * (kWave * NP);
     static_for<NP>([&](auto pc) {
       constexpr int pj = decltype(pc)::value;

@@ -197,7 +197,8 @@ typedef struct dmw_plan dmw_plan;
 
 int dmw_plan_create(dmw_plan** plan, const dmw_geometry* geom,
                     const dmw_waveform* wave, const dmw_run* run);
-/* Run the planned walk with a new seed; inputs are already in HBM. */
+/* Run the planned walk with a new seed; inputs are already in HBM.  A plan runs
+ * on the device that was selected when it was created. */
 int dmw_plan_execute(dmw_plan* plan, uint64_t seed, dmw_result* out);
 int dmw_plan_destroy(dmw_plan* plan);
 
