@@ -47,6 +47,10 @@ DMW_HD uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
 }
 
 // Philox4x32-10 block: 4 x 32 random bits for counter c under key (k0, k1).
+// Rounds 0 and 1 use plain XORs: with a counter whose words 1-3 and key are the
+// same across a wavefront, round 0's first output word and round 1's first
+// product stay in scalar registers (the bit-op builtin would move them to
+// vector registers).
 DMW_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
@@ -54,9 +58,9 @@ DMW_HD u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
     uint32_t lo0 = mulhilo32(0xD2511F53u, c.x, &hi0);
     uint32_t lo1 = mulhilo32(0xCD9E8D57u, c.z, &hi1);
     u32x4 n;
-    n.x = xor3(hi1, c.y, k0);
+    n.x = r < 2 ? (hi1 ^ c.y ^ k0) : xor3(hi1, c.y, k0);
     n.y = lo1;
-    n.z = xor3(hi0, c.w, k1);
+    n.z = r < 2 ? (hi0 ^ c.w ^ k1) : xor3(hi0, c.w, k1);
     n.w = lo0;
     c = n;
     k0 += 0x9E3779B9u;
