@@ -40,7 +40,7 @@ PHASE_MFMA = "fp16 hi/lo split on matrix cores, fp32 accumulate"
 # device-to-device copy (FETCH x 2.000, WRITE x 1.000; profiles/walk_kernel_cylinder_pgse.md).
 # None where not measured.
 TRAFFIC_BYTES = {
-    "cylinder_pgse": 18_266_194 + 32_172_851,
+    "cylinder_pgse": 17_374_635 + 32_172_843,
     "free_pgse": None,
     "packed_cylinders_free_waveform": None,
 }
