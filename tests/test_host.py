@@ -437,3 +437,95 @@ def test_sqrt_cut_is_the_least_square_reaching_r():
     bad, first = C.c_ulonglong(0), C.c_uint32(0)
     assert lib.arith_check_sqrt_cut(2_000_000, 5, 40, C.byref(bad), C.byref(first)) == 0
     assert bad.value == 0, f"{bad.value} failures, first r bits {first.value:#x}"
+
+
+def _valid_plan_args():
+    """A valid Cylinder walk description for dmw_plan_create (5 samples, 64 walkers)."""
+    from dmipy_sim_amd import _native as nat
+    wf = ds.Waveform(G=np.zeros((1, 5, 3), np.float32), dt=1e-4, echo_idx=4)
+    plan = ds.make_walk_plan(ds.Cylinder(5e-6, (0, 0, 1)), D, wf.dt)
+    gd = plan.geometry_desc(n_walkers=64)
+    G = np.zeros((1, 5, 3), np.float32)
+    wd = nat.Waveform_t(n_meas=1, n_t=5, G=nat.fptr(G), chi=None, n_echo=0, echo_idx=None)
+    rd = nat.Run_t(n_walkers=64, seed=0, n_sub=plan.n_sub, step_len=plan.step_len,
+                   gamma_dt_sub=plan.gamma_dt_sub, dt_sub=plan.dt_sub, inv_T2=0.0, inv_T1=0.0,
+                   has_t2=0, has_t1=0, kahan=0, meas_block=0, r0=None, inj_normals=None,
+                   inj_uniforms=None)
+    return gd, wd, rd, G
+
+
+_EINVAL, _EUNSUP = _native.DMW_EINVAL, _native.DMW_EUNSUPPORTED
+_BAD_ECHO_SAME = np.array([3, 3], np.int32)
+_BAD_ECHO_RANGE = np.array([5], np.int32)
+_INJ = np.zeros(64 * 5 * 3 * 8, np.float32)
+
+
+@pytest.mark.parametrize("field,value,rc,text", [
+    ("run.n_walkers", 0, _EINVAL, "n_walkers"),
+    ("run.n_walkers", (1 << 31) + 1, _EINVAL, "n_walkers"),
+    ("wave.n_meas", 0, _EINVAL, "empty waveform"),
+    ("wave.G", None, _EINVAL, "empty waveform"),
+    ("run.n_sub", 0, _EINVAL, "n_sub"),
+    ("run.n_sub", 1 << 30, _EINVAL, "too large"),
+    ("geom.kind", 12, _EINVAL, "unknown geometry kind"),
+    ("geom.wall", 3, _EINVAL, "unknown wall mode"),
+    ("geom.kind+wall", (0, 1), _EUNSUP, "FreeDiffusion has no wall"),
+    ("geom.kind+wall", (1, 2), _EUNSUP, "Box1D has no permeable wall"),
+    ("geom.kind+wall", (8, 1), _EUNSUP, "PermeableShell relaxes only"),
+    ("geom.kind+wall", (9, 2), _EINVAL, "0 < r_inner < r_outer"),
+    ("geom.kind+wall", (10, 0), _EINVAL, "compartment constants"),
+    ("geom.kind+wall", (4, 0), _EINVAL, "packed substrate needs objects"),
+    ("wave.echo", _BAD_ECHO_SAME, _EINVAL, "ascending"),
+    ("wave.echo", _BAD_ECHO_RANGE, _EINVAL, "ascending"),
+    ("wave.n_echo", 1, _EINVAL, "bad echo list"),
+    ("inject_without_uniforms", None, _EINVAL, "injected uniforms"),
+])
+def test_plan_create_argument_checks_before_gpu(field, value, rc, text):
+    """dmw_plan_create rejects each malformed description with the status the
+    Python layer maps to ValueError / NotImplementedError (core.py:316-345) and a
+    message naming the problem, before it touches a GPU."""
+    import ctypes as C
+    from dmipy_sim_amd import _native as nat
+    gd, wd, rd, G = _valid_plan_args()
+    if field == "run.n_walkers":
+        rd.n_walkers = value
+    elif field == "run.n_sub":
+        rd.n_sub = value
+    elif field == "wave.n_meas":
+        wd.n_meas = value
+    elif field == "wave.G":
+        wd.G = None
+    elif field == "geom.kind":
+        gd.kind = value
+    elif field == "geom.wall":
+        gd.wall = value
+    elif field == "geom.kind+wall":
+        gd.kind, gd.wall = value
+        gd.n_obj = 0
+    elif field == "wave.echo":
+        wd.n_echo = len(value)
+        wd.echo_idx = nat.iptr(value)
+    elif field == "wave.n_echo":
+        wd.n_echo = value
+        wd.echo_idx = None
+    elif field == "inject_without_uniforms":
+        gd.wall = 2
+        rd.inj_normals = nat.fptr(_INJ)
+    h = C.c_void_p()
+    got = nat.lib().dmw_plan_create(C.byref(h), gd, wd, rd)
+    assert got == rc, nat.lib().dmw_last_error()
+    assert text in nat.lib().dmw_last_error().decode()
+    assert not h.value
+
+
+def test_plan_create_null_arguments():
+    import ctypes as C
+    from dmipy_sim_amd import _native as nat
+    gd, wd, rd, G = _valid_plan_args()
+    h = C.c_void_p()
+    assert nat.lib().dmw_plan_create(None, gd, wd, rd) == _EINVAL
+    assert nat.lib().dmw_plan_create(C.byref(h), None, wd, rd) == _EINVAL
+    assert nat.lib().dmw_plan_create(C.byref(h), gd, None, rd) == _EINVAL
+    assert nat.lib().dmw_plan_create(C.byref(h), gd, wd, None) == _EINVAL
+    assert b"null argument" in nat.lib().dmw_last_error()
+    assert nat.lib().dmw_walk(gd, wd, None, None) == _EINVAL
