@@ -149,11 +149,13 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   (void)hits; (void)ncross; (void)ncand;
 }
 
-// Three wavefronts per SIMD: the 64-measurement variant fits 168 VGPRs with 68 bytes of
-// scratch, and the persistent grid of three workgroups per CU is 18 % faster on the
-// 4096-instance Sphere/Cylinder batch than two at 187 VGPRs.
+// Four wavefronts per SIMD: the 64-measurement variant fits 128 VGPRs with 280 bytes of
+// scratch; with a persistent grid of four workgroups per CU the 4096-instance
+// Sphere/Cylinder batch runs in 1281 ms against 1354 ms at three (168 VGPRs, 68 bytes of
+// scratch) and 1652 ms at two (187 VGPRs): the per-unit instance loads and dispatch are
+// latency that more resident wavefronts hide.
 template <int MB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void batch_kernel(const BatchArgs b) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
   const unsigned int inc = (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u;
   for (;;) {
@@ -408,7 +410,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   int n_cu = 256;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) n_cu = prop.multiProcessorCount;
-  const int wps = waves_per_simd > 0 ? waves_per_simd : 3;  // the kernel's occupancy budget
+  const int wps = waves_per_simd > 0 ? waves_per_simd : 4;  // the kernel's occupancy budget
   B->grid = std::max(1, std::min(n_cu * wps, (B->n_units + 3) / 4));
 
   auto fail_free = [&](hipError_t err, const char* what) {

@@ -235,7 +235,7 @@ typedef struct dmw_batch_stats {
 } dmw_batch_stats;
 
 /* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored).
- * waves_per_simd: persistent wavefronts per SIMD (0 = 3, the kernel's occupancy budget). */
+ * waves_per_simd: persistent wavefronts per SIMD (0 = 4, the kernel's occupancy budget). */
 int dmw_batch_create(dmw_batch** batch, int32_t n_inst, const dmw_instance* inst,
                      int32_t n_sets, const dmw_waveform* sets, int32_t waves_per_simd,
                      int32_t use_graph);
