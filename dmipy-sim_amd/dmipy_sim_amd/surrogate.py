@@ -8,7 +8,7 @@ trained on the GPU by the library's own MFMA forward / backward / Adam kernels
 (csrc/mlp.hip, entry points dmw_mlp_* of include/dmipy_walk.h) on signals the
 batched persistent walk produces.  A 4-member ensemble gives a spread per
 prediction; predictions whose spread exceeds the accuracy bound are recomputed
-by the Monte-Carlo walk, which is also the judge of the surrogate's error.
+by the Monte-Carlo walk, which also measures the surrogate's error.
 
 Waveform embedding.  With q(t) = gamma * int_0^t G (accumulated in float64) and
 Q_k its discrete Fourier coefficients, the b-value splits over frequencies,
