@@ -153,7 +153,8 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
 // scratch; with a persistent grid of four workgroups per CU the 4096-instance
 // Sphere/Cylinder batch runs in 1281 ms against 1354 ms at three (168 VGPRs, 68 bytes of
 // scratch) and 1652 ms at two (187 VGPRs): the per-unit instance loads and dispatch are
-// latency that more resident wavefronts hide.
+// latency that more resident wavefronts hide.  Five (96 VGPRs, 372 bytes of scratch) is
+// slower again, 1314 ms.
 template <int MB>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
