@@ -178,6 +178,8 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (!out || !geom || !wave || !run) return fail(DMW_EINVAL, "null argument");
   *out = nullptr;
   if (run->n_walkers < 1 || run->n_walkers > (1LL << 31)) return fail(DMW_EINVAL, "n_walkers must be in [1, 2^31]");
+  if (run->first_walker < 0 || run->first_walker + run->n_walkers > (1LL << 32))
+    return fail(DMW_EINVAL, "first_walker must be >= 0 with first_walker + n_walkers <= 2^32");
   if (wave->n_meas < 1 || wave->n_t < 1 || !wave->G) return fail(DMW_EINVAL, "empty waveform");
   if (run->n_sub < 1) return fail(DMW_EINVAL, "n_sub must be >= 1");
   if ((long long)wave->n_t * run->n_sub >= (1LL << 32)) return fail(DMW_EINVAL, "n_t * n_sub too large");
@@ -203,7 +205,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     if (!(geom->aux > geom->size && geom->size > 0.0f))
       return fail(DMW_EINVAL, "outer_radius must be > inner_radius");
     const int64_t* ns = geom->myelin->n_start;
-    if (!run->r0 && !(0 <= ns[0] && ns[0] <= ns[1] && ns[1] <= run->n_walkers))
+    if (!run->r0 && !(0 <= ns[0] && ns[0] <= ns[1] && ns[1] <= run->first_walker + run->n_walkers))
       return fail(DMW_EINVAL, "MyelinatedCylinder start counts must satisfy 0 <= n0 <= n01 <= n_walkers");
   }
   const bool pmyel = geom->kind == DMW_PACKED_MYELINATED;
@@ -225,7 +227,8 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     if (st0[0] != 0) return fail(DMW_EINVAL, "start boundaries must begin at 0");
     for (int j = 0; j < 2 * geom->n_obj; ++j)
       if (st0[j + 1] < st0[j]) return fail(DMW_EINVAL, "start boundaries must be ascending");
-    if (st0[2 * geom->n_obj] > run->n_walkers) return fail(DMW_EINVAL, "start boundaries exceed n_walkers");
+    if (st0[2 * geom->n_obj] > run->first_walker + run->n_walkers)
+      return fail(DMW_EINVAL, "start boundaries exceed n_walkers");
   }
   const bool packed = geom->kind == DMW_PACKED_CYLINDERS || geom->kind == DMW_PACKED_SPHERES;
   if (packed && (geom->n_obj < 1 || !geom->centers || !geom->radii || !(geom->cell > 0.0f)))
@@ -313,9 +316,12 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
         wsum += (double)m.spin_weight[rr < R0 ? 0 : (rr < R1 ? 1 : 2)];
       }
     } else {
-      wsum = (double)m.n_start[0] * m.spin_weight[0] +
-             (double)(m.n_start[1] - m.n_start[0]) * m.spin_weight[1] +
-             (double)(N - m.n_start[1]) * m.spin_weight[2];
+      // start compartments by global walker index, counted over this call's range
+      const long long f = run->first_walker;
+      const long long c0 = std::min(N, std::max(0LL, (long long)m.n_start[0] - f));
+      const long long c01 = std::min(N, std::max(0LL, (long long)m.n_start[1] - f));
+      wsum = (double)c0 * m.spin_weight[0] + (double)(c01 - c0) * m.spin_weight[1] +
+             (double)(N - c01) * m.spin_weight[2];
     }
     P->inv_norm = 1.0 / wsum;
   }
@@ -374,7 +380,9 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     a.pm_rhoD = pm.rho_over_D;
     a.pm_wmy = pm.myelin_weight;
     a.pm_start = P->pm_start.as<long long>();
-    const long long n_my = pm.start[2 * n] - pm.start[n];
+    const long long f = run->first_walker;
+    const long long n_my = std::max(0LL, std::min(f + N, (long long)pm.start[2 * n]) -
+                                             std::max(f, (long long)pm.start[n]));
     P->inv_norm = 1.0 / ((double)(N - n_my) + (double)n_my * (double)pm.myelin_weight);
   }
   if (packed) {
@@ -456,6 +464,7 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.n_echo = wave->n_echo;
   a.echo_idx = wave->n_echo > 0 ? P->echo.as<int>() : nullptr;
   a.n_walkers = N;
+  a.first_walker = run->first_walker;
   a.instance = 0;
 
   if (run->r0) {

@@ -100,8 +100,9 @@ struct WalkArgs {
   const float* chi;          // device (n_t) or null
   int n_echo;
   const int* echo_idx;       // device (n_echo)
-  // walkers
-  long long n_walkers;
+  // walkers: this launch walks walkers [first_walker, first_walker + n_walkers) of
+  // a walk; the global index keys the Philox streams and the in-kernel start rules
+  long long n_walkers, first_walker;
   uint32_t seed_lo, seed_hi, instance;
   float ax, ay, az, aux;     // Ellipsoid 1/semi-axis^2 and clamp factor; shell R_out
   // MyelinatedCylinder, per compartment (intra, sheath, extra): step length, dt/T2,
@@ -1589,7 +1590,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   const long long wave_id = gid / kWave;
   const bool active = gid < a.n_walkers;
   const long long N = a.n_walkers;
-  const uint32_t w = (uint32_t)(active ? gid : N - 1);
+  const uint32_t w = (uint32_t)(active ? gid : N - 1);   // index into this launch's arrays
+  const uint32_t wg = w + (uint32_t)a.first_walker;       // global walker index
   const int m0 = blockIdx.y * MB;
   const bool primary = (blockIdx.y == 0);
   const bool inject = (a.inj_n != nullptr);
@@ -1600,7 +1602,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     if (PACKED) fold<DIM>(p, a);
     if (GEOM == G_MYEL) { p.cx = myel_classify(p.x, p.y, a); p.cy = p.cx; }
   } else {
-    p = sample_start<GEOM>(w, a, tb);
+    p = sample_start<GEOM>(wg, a, tb);
   }
   int comp = classify<GEOM>(p, a, tb);
   if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
@@ -1648,13 +1650,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
           uinj = a.inj_u[(size_t)gs * N + w];
       } else {
         if (!(gs & 1u) || (t == 0 && s == 0)) {
-          rnd = philox4x32_10({w, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
+          rnd = philox4x32_10({wg, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
         }
         const uint32_t ra = (gs & 1u) ? rnd.z : rnd.x;
         const uint32_t rb = (gs & 1u) ? rnd.w : rnd.y;
         iso_direction(ra, rb, &ux, &uy, &uz);
       }
-      const PowlesSrc ps{w, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
+      const PowlesSrc ps{wg, a.instance, gs, a.seed_lo, a.seed_hi, uinj, inject};
       float dlog;
       if (GEOM == G_PMYEL) {
         const float px0 = p.x, py0 = p.y, pz0 = p.z;

@@ -49,7 +49,8 @@ class Run_t(C.Structure):
                 ("step_len", C.c_float), ("gamma_dt_sub", C.c_float), ("dt_sub", C.c_float),
                 ("inv_T2", C.c_float), ("inv_T1", C.c_float), ("has_t2", C.c_int32),
                 ("has_t1", C.c_int32), ("kahan", C.c_int32), ("meas_block", C.c_int32),
-                ("r0", _fp), ("inj_normals", _fp), ("inj_uniforms", _fp)]
+                ("r0", _fp), ("inj_normals", _fp), ("inj_uniforms", _fp),
+                ("first_walker", C.c_int64)]
 
 
 class Result_t(C.Structure):

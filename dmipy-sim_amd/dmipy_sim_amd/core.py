@@ -59,7 +59,8 @@ def _kahan_auto(n_t, n_meas):
 
 def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None, r0=None,
          *, outputs=(), inject_normals=None, inject_uniforms=None, kahan=None,
-         meas_block=0, substep_rule="fused", echo_indices=None, counters=False, plan=None):
+         meas_block=0, substep_rule="fused", echo_indices=None, counters=False, plan=None,
+         first_walker=0, n_total=None):
     """One walk on the GPU; returns a dict of numpy arrays.
 
     ``outputs`` selects optional per-walker arrays: 'final_pos', 'comp_origin',
@@ -67,7 +68,11 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     Positions are returned in the lab frame.  ``inject_normals`` (N, n_t*n_sub, 3)
     and ``inject_uniforms`` (N, n_t*n_sub) replace the Philox stream with given
     step increments (raw normal triples, normalised in the kernel as the reference
-    does, physics.py:299-301).
+    does, physics.py:299-301).  ``first_walker`` makes this call walk walkers
+    [first_walker, first_walker + n_walkers) of a larger walk (their random streams
+    and start positions): a shard of a walk split over calls or devices;
+    ``n_total`` is then the whole walk's walker count (default first_walker +
+    n_walkers), which fixes the myelinated substrates' start allotment.
     """
     G = np.asarray(waveform.G, dtype=np.float32)
     if G.ndim != 3 or G.shape[2] != 3:
@@ -111,7 +116,12 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     if not meas_block:
         meas_block = autotune.cached_block(plan, n_meas, kahan)
 
-    gd = plan.geometry_desc(n_walkers=N)
+    if int(first_walker) < 0:
+        raise ValueError("first_walker must be >= 0")
+    total = int(first_walker) + N if n_total is None else int(n_total)
+    if total < int(first_walker) + N:
+        raise ValueError("n_total must cover walkers [first_walker, first_walker + n_walkers)")
+    gd = plan.geometry_desc(n_walkers=total)
     G = np.ascontiguousarray(G, dtype=np.float32)   # referenced by wd until the call returns
     wd = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
                         chi=nat.fptr(chi), n_echo=0 if echo is None else len(echo),
@@ -121,7 +131,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
                    inv_T2=plan.inv_T2, inv_T1=plan.inv_T1, has_t2=int(plan.has_t2),
                    has_t1=int(plan.has_t1), kahan=int(bool(kahan)), meas_block=int(meas_block),
                    r0=nat.fptr(r0), inj_normals=nat.fptr(inject_normals),
-                   inj_uniforms=nat.fptr(inject_uniforms))
+                   inj_uniforms=nat.fptr(inject_uniforms), first_walker=int(first_walker))
     rows = 1 if echo is None else len(echo)
     res = {"signal": np.zeros(rows * n_meas, np.float64)}
     want = set(outputs)

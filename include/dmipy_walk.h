@@ -157,6 +157,11 @@ typedef struct dmw_run {
   const float* r0;           /* (n_walkers, 3) start positions, or NULL = sample in-kernel */
   const float* inj_normals;  /* (n_walkers, n_t*n_sub, 3) raw normal triples, or NULL = Philox */
   const float* inj_uniforms; /* (n_walkers, n_t*n_sub) Powles uniforms (permeable walls) */
+  int64_t first_walker;      /* global index of this call's walker 0: the call walks walkers
+                                [first_walker, first_walker + n_walkers) of a larger walk, with
+                                their Philox streams and in-kernel start positions (a shard of a
+                                walk split over calls or devices); 0 for a whole walk.
+                                first_walker + n_walkers must not exceed 2^32. */
 } dmw_run;
 
 /* Outputs.  signal is required; every other array is optional (NULL). */

@@ -53,14 +53,16 @@ class OracleArgs(C.Structure):
                 ("m_step", C.c_float * 3), ("m_q2", C.c_float * 3), ("m_wf", C.c_float * 3),
                 ("m_k2", C.c_float * 4), ("m_n0", C.c_int64), ("m_n01", C.c_int64),
                 ("pm_inner", _fp), ("pm_step", _fp), ("pm_k2", _fp), ("pm_q2", C.c_float * 3),
-                ("pm_rhoD", C.c_float), ("pm_wmy", C.c_float), ("pm_start", C.POINTER(C.c_int64))]
+                ("pm_rhoD", C.c_float), ("pm_wmy", C.c_float), ("pm_start", C.POINTER(C.c_int64)),
+                ("first_walker", C.c_int64)]
 
 
 class OracleOut(C.Structure):
     _fields_ = [("signal", _dp), ("signal_im", _dp), ("final_pos", _fp),
                 ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
                 ("n_cross", _up), ("log_w", _fp), ("phi", _fp), ("n_hits", C.c_uint64),
-                ("pos_full", _fp), ("dlog_full", _fp), ("occ_full", _fp)]
+                ("pos_full", _fp), ("dlog_full", _fp), ("occ_full", _fp),
+                ("signal_x", _dp), ("signal_x_im", _dp), ("phi_x", _dp)]
 
 
 _lib = None
@@ -290,8 +292,15 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
 def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                inject_normals=None, inject_uniforms=None, cell_form=1, phase64=False,
                threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None,
-               instance=0, per_sample=False):
-    """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame)."""
+               instance=0, per_sample=False, first_walker=0, exact_phase=False):
+    """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame).
+
+    phase64: False = float32 phase (the reference's arithmetic), True = float32
+    increments summed in float64.  exact_phase: also return 'signal_x',
+    'signal_x_im' and 'phi_x', the same walk with every phase increment formed and
+    summed in float64 from the float32 positions (the exact phase of the float32
+    trajectory: the FP64 oracle the kernel's phase error is stated against).
+    first_walker: global index of walker 0 (a shard of a larger walk)."""
     p = plan if plan is not None else oracle_plan(geom, D, waveform.dt, T2, T1)
     G = np.ascontiguousarray(waveform.G if G_native is None else G_native, dtype=np.float32)
     n_meas, n_t, _ = G.shape
@@ -317,7 +326,8 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                    n_walkers=N, seed=int(seed) & (2**64 - 1), r0=fp(r0), inj_n=fp(inj_n),
                    inj_u=fp(inj_u), cell_form=int(cell_form), phase64=int(bool(phase64)),
                    threads=int(threads), walker_begin=int(w0), walker_end=int(w1),
-                   instance=int(instance), aux=float(p.get("aux", 0.0)))
+                   instance=int(instance), aux=float(p.get("aux", 0.0)),
+                   first_walker=int(first_walker))
     for k in range(3):
         A.axes[k] = float(p.get("axes", [0.0, 0.0, 0.0])[k])
     if p["kind"] == 10:
@@ -362,12 +372,20 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
         O.occ_full = fp(res["occ_full"])
         res["comp_full"] = np.zeros((N, n_t), np.int32)
         O.comp_full = res["comp_full"].ctypes.data_as(_ip)
+    if exact_phase:
+        res["signal_x"] = np.zeros(rows * n_meas)
+        res["signal_x_im"] = np.zeros(rows * n_meas)
+        res["phi_x"] = np.zeros((N, n_meas))
+        O.signal_x = res["signal_x"].ctypes.data_as(_dp)
+        O.signal_x_im = res["signal_x_im"].ctypes.data_as(_dp)
+        O.phi_x = res["phi_x"].ctypes.data_as(_dp)
     rc = lib().oracle_walk(C.byref(A), C.byref(O))
     if rc != 0:
         raise RuntimeError("oracle_walk failed")
     if echo is not None:
-        res["signal"] = res["signal"].reshape(rows, n_meas)
-        res["signal_im"] = res["signal_im"].reshape(rows, n_meas)
+        for k in ("signal", "signal_im", "signal_x", "signal_x_im"):
+            if k in res:
+                res[k] = res[k].reshape(rows, n_meas)
     res["n_hits"] = int(O.n_hits)
     res["n_sub"] = p["n_sub"]
     return res

@@ -86,6 +86,7 @@ typedef struct {
   float pm_rhoD;           /* wall relaxivity / D_ref */
   float pm_wmy;            /* myelin spin weight */
   const int64_t* pm_start; /* (2 n_obj + 1) start boundaries */
+  int64_t first_walker;    /* global index of walker 0 (keys the Philox streams and starts) */
 } oracle_args;
 
 typedef struct {
@@ -102,6 +103,12 @@ typedef struct {
   float* pos_full;         /* (N, n_t, 3) unfolded position after every sample, or NULL */
   float* dlog_full;        /* (N, n_t) log-weight summed over each sample's sub-steps, or NULL */
   float* occ_full;         /* (N, n_t) fraction of each sample's sub-steps outside, or NULL */
+  /* The exact phase of the walk's float32 trajectory, alongside (NULL = skip): each
+     increment gamma_dt_sub * (g . r) formed and summed in float64 from the float32
+     positions (unfolded in float64 for packed substrates). */
+  double* signal_x;        /* (rows, n_meas) Re <w e^{i phi_exact}> */
+  double* signal_x_im;
+  double* phi_x;           /* (N, n_meas) */
 } oracle_out;
 
 enum { K_FREE = 0, K_BOX = 1, K_CYL = 2, K_SPH = 3, K_PCYL = 4, K_PSPH = 5,
@@ -1012,9 +1019,11 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
   const float L = A->cell;
   const uint32_t key[2] = {(uint32_t)A->seed, (uint32_t)(A->seed >> 32)};
   double* acc = (double*)calloc((size_t)rows * M * 2, sizeof(double));
+  double* accx = (double*)calloc((size_t)rows * M * 2, sizeof(double));
+  const int want_x = O->signal_x != NULL;
   uint64_t hits_total = 0;
   double wsum = 0.0;      /* sum of spin weights (_ens denominator, core.py:426-429) */
-  if (!acc) return 1;
+  if (!acc || !accx) return 1;
 #ifdef _OPENMP
   if (A->threads > 0) omp_set_num_threads(A->threads);
 #endif
@@ -1023,17 +1032,20 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
     double* loc = (double*)calloc((size_t)rows * M * 2, sizeof(double));
     float* phi = (float*)malloc(sizeof(float) * (M > 0 ? M : 1));
     double* phi64 = (double*)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double* phix = (double*)malloc(sizeof(double) * (M > 0 ? M : 1));
+    double* locx = (double*)calloc((size_t)rows * M * 2, sizeof(double));
     uint64_t hits_loc = 0;
     double wsum_loc = 0.0;
-#pragma omp for schedule(dynamic, 64)
+#pragma omp for schedule(dynamic, 4)
     for (int64_t w = w0; w < w1; ++w) {
       float r[3];
       int cell[3] = {0, 0, 0};
       const int pmyel = A->kind == K_PMYEL;
       int pcid = 0;                                   /* packed myelin: compartment id */
-      if (pmyel) pcid = pm_start_oracle(A, (uint32_t)w, r);
+      const uint32_t wg = (uint32_t)(w + A->first_walker);   /* global walker index */
+      if (pmyel) pcid = pm_start_oracle(A, wg, r);
       else if (A->r0) { r[0] = A->r0[w * 3]; r[1] = A->r0[w * 3 + 1]; r[2] = A->r0[w * 3 + 2]; }
-      else sample_start(A, (uint32_t)w, r);
+      else sample_start(A, wg, r);
       const int pc0 = pm_to3(pcid, A->n_obj);
       float uw[3] = {r[0], r[1], r[2]};               /* packed myelin: unwrapped position */
       if (packed && A->cell_form) {
@@ -1047,7 +1059,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       const int c0 = mstate;
       int comp = myel ? mstate : (pmyel ? pc0 : classify(r, A));
       if (O->comp_origin) O->comp_origin[w] = comp;
-      for (int m = 0; m < M; ++m) { phi[m] = 0.0f; phi64[m] = 0.0; }
+      for (int m = 0; m < M; ++m) { phi[m] = 0.0f; phi64[m] = 0.0; phix[m] = 0.0; }
       float logw = 0.0f;
       uint32_t hits = 0, ncross = 0;
       uint32_t rnd[4] = {0, 0, 0, 0};
@@ -1065,12 +1077,12 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             if (A->inj_u) u = A->inj_u[(size_t)w * S + gs];
           } else {
             if ((gs & 1) == 0 || gs == 0) {
-              const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)(gs >> 1), 0u};
+              const uint32_t ctr[4] = {wg, A->instance, (uint32_t)(gs >> 1), 0u};
               philox10(ctr, key[0], key[1], rnd);
             }
             oracle_direction((gs & 1) ? rnd[2] : rnd[0], (gs & 1) ? rnd[3] : rnd[1], dir);
             if (A->wall == W_PERMEATE || myel || pmyel) {
-              const uint32_t ctr[4] = {(uint32_t)w, A->instance, (uint32_t)gs, 2u};
+              const uint32_t ctr[4] = {wg, A->instance, (uint32_t)gs, 2u};
               uint32_t o[4];
               philox10(ctr, key[0], key[1], o);
               u = unit_u(o[0]);
@@ -1113,6 +1125,17 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             if (A->phase64) phi64[m] += (double)A->gamma_dt_sub * (double)gr;
             else phi[m] = phi[m] + A->gamma_dt_sub * gr;
           }
+          if (want_x) {
+            double rd[3];
+            for (int d = 0; d < 3; ++d) rd[d] = (double)ru[d];
+            if (packed && A->cell_form)
+              for (int d = 0; d < 3; ++d) rd[d] = (double)r[d] + (double)L * (double)cell[d];
+            for (int m = 0; m < M; ++m) {
+              const float* g = A->G + ((size_t)m * T + t) * 3;
+              const double gr = ((double)g[0] * rd[0] + (double)g[1] * rd[1]) + (double)g[2] * rd[2];
+              phix[m] += (double)A->gamma_dt_sub * gr;
+            }
+          }
         }
         if (packed && A->cell_form) {
           for (int d = 0; d < dim; ++d) {
@@ -1135,6 +1158,10 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
             const double ph = A->phase64 ? phi64[m] : (double)phi[m];
             loc[((size_t)e_next * M + m) * 2] += wg * cos(ph);
             loc[((size_t)e_next * M + m) * 2 + 1] += wg * sin(ph);
+            if (want_x) {
+              locx[((size_t)e_next * M + m) * 2] += wg * cos(phix[m]);
+              locx[((size_t)e_next * M + m) * 2 + 1] += wg * sin(phix[m]);
+            }
           }
           ++e_next;
         }
@@ -1146,6 +1173,10 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
           const double ph = A->phase64 ? phi64[m] : (double)phi[m];
           loc[m * 2] += wg * cos(ph);
           loc[m * 2 + 1] += wg * sin(ph);
+          if (want_x) {
+            locx[m * 2] += wg * cos(phix[m]);
+            locx[m * 2 + 1] += wg * sin(phix[m]);
+          }
         }
       }
       if (O->final_pos)
@@ -1156,6 +1187,8 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
       if (O->log_w) O->log_w[w] = logw;
       if (O->phi)
         for (int m = 0; m < M; ++m) O->phi[w * M + m] = A->phase64 ? (float)phi64[m] : phi[m];
+      if (O->phi_x)
+        for (int m = 0; m < M; ++m) O->phi_x[w * M + m] = phix[m];
       hits_loc += hits;
       wsum_loc += A->kind == K_MYEL ? (double)A->m_wf[c0]
                                     : ((pmyel && pc0 == 1) ? (double)A->pm_wmy : 1.0);
@@ -1163,19 +1196,27 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
 #pragma omp critical
     {
       for (int i = 0; i < rows * M * 2; ++i) acc[i] += loc[i];
+      for (int i = 0; i < rows * M * 2; ++i) accx[i] += locx[i];
       hits_total += hits_loc;
       wsum += wsum_loc;
     }
     free(loc);
     free(phi);
     free(phi64);
+    free(phix);
+    free(locx);
   }
   const double inv = 1.0 / wsum;
   for (int i = 0; i < rows * M; ++i) {
     O->signal[i] = acc[2 * i] * inv;
     if (O->signal_im) O->signal_im[i] = acc[2 * i + 1] * inv;
+    if (want_x) {
+      O->signal_x[i] = accx[2 * i] * inv;
+      if (O->signal_x_im) O->signal_x_im[i] = accx[2 * i + 1] * inv;
+    }
   }
   O->n_hits = hits_total;
   free(acc);
+  free(accx);
   return 0;
 }

@@ -546,3 +546,42 @@ def test_packed_myelin_phase_uses_unwrapped_position(oracle):
     m = np.stack([ux_t - pos[:, 0], uy_t - pos[:, 1]], 1) / L
     assert np.all(np.abs(m - np.round(m)) < 1e-3)
     assert np.any(np.round(m) != 0)
+
+
+def test_exact_phase_is_the_float64_sum_over_the_trajectory(oracle):
+    """The oracle's exact phase (the FP64 reference the kernel's phase error is
+    stated against) equals sum_t gamma dt_sub G(t) . r(t) summed in float64 over
+    the walk's float32 positions: with FreeDiffusion n_sub = 1, so every sub-step
+    position is a per-sample output.  Its signal is mean exp(log_w) cos(phi_x), and
+    the float32 phase of the same walk stays within float32 accumulation error."""
+    rng = np.random.default_rng(3)
+    n_t, N, M = 60, 200, 5
+    G = rng.uniform(-0.5, 0.5, (M, n_t, 3)).astype(np.float32)
+    wf = ds.Waveform(G=G, dt=1e-4, echo_idx=n_t - 1)
+    geom = ds.FreeDiffusion()
+    p = oracle.oracle_plan(geom, D, wf.dt, T2=0.05)
+    assert p["n_sub"] == 1
+    res = oracle.run_oracle(geom, D, wf, N, seed=2, T2=0.05, per_sample=True, exact_phase=True)
+    pos = res["pos_full"].astype(np.float64)                       # (N, n_t, 3)
+    phi = p["gamma_dt_sub"] * np.einsum("mtc,wtc->wm", G.astype(np.float64), pos)
+    assert np.abs(phi).max() > 0.5                                 # a non-trivial phase
+    np.testing.assert_allclose(res["phi_x"], phi, rtol=1e-12, atol=1e-12)
+    w = np.exp(res["log_w"].astype(np.float64))
+    np.testing.assert_allclose(res["signal_x"], (w[:, None] * np.cos(phi)).mean(0), atol=1e-12)
+    np.testing.assert_allclose(res["signal_x_im"], (w[:, None] * np.sin(phi)).mean(0), atol=1e-12)
+    assert np.max(np.abs(res["phi"] - phi)) < 1e-5 * max(1.0, np.abs(phi).max())
+    assert np.max(np.abs(res["phi"] - phi)) > 0.0                  # float32 does round
+
+
+def test_first_walker_keys_the_streams(oracle):
+    """A walk of walkers [k, k + n) with first_walker = k is the slice of the
+    whole walk (the same Philox streams and start positions)."""
+    geom = packed_cyl(kappa=5e-5)
+    wf = ds.set_b(ds.pgse(0.002, 0.006, 0.05, [[1, 0, 0], [0, 1, 0]], 12, slew_rate=np.inf),
+                  [1e9, 2e9])
+    whole = oracle.run_oracle(geom, D, wf, 96, seed=4)
+    part = oracle.run_oracle(geom, D, wf, 40, seed=4, first_walker=50)
+    np.testing.assert_array_equal(part["final_pos"], whole["final_pos"][50:90])
+    np.testing.assert_array_equal(part["log_w"], whole["log_w"][50:90])
+    np.testing.assert_array_equal(part["n_cross"], whole["n_cross"][50:90])
+    np.testing.assert_array_equal(part["phi"], whole["phi"][50:90])
