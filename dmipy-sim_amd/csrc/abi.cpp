@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -70,14 +71,51 @@ int ensure_stream_on(int device, hipStream_t* out) {
 
 int ensure_stream(hipStream_t* out) { return ensure_stream_on(g_device, out); }
 
-// Device allocation owned by a plan.
+// ---------------------------------------------------------------------------
+// Allocation-free walks (dmw_walk_noalloc): every device buffer of the call is
+// carved from `scratch`, a static device array of kScratchBytes per GPU, instead
+// of hipMalloc.  Host code never names the array itself (that is a host-side
+// stand-in, not a device address): its device address comes from
+// hipGetSymbolAddress, once per device, with that device current.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_err[kMaxDevices];
+std::mutex g_scratch_mutex[kMaxDevices];
+
+// Bump allocator over one call's share of the scratch array.
+struct Arena {
+  char* base;
+  size_t cap, used;
+};
+thread_local Arena* g_arena = nullptr;
+
+// Device allocation owned by a plan (from the arena while one is installed).
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool in_arena = false;
   int alloc(size_t n) {
     if (n <= bytes && p) return DMW_OK;
     release();
     if (n == 0) return DMW_OK;
+    if (g_arena) {
+      const size_t off = (g_arena->used + 255) & ~(size_t)255;
+      if (off > g_arena->cap || n > g_arena->cap - off)
+        return fail(DMW_EINVAL, "dmw_walk_noalloc: the walk's device buffers need more than the " +
+                                    std::to_string(kScratchBytes >> 20) +
+                                    " MiB scratch array (fewer walkers, measurements or per-walker "
+                                    "outputs, or dmw_walk)");
+      p = g_arena->base + off;
+      g_arena->used = off + n;
+      bytes = n;
+      in_arena = true;
+      return DMW_OK;
+    }
     hipError_t e = hipMalloc(&p, n);
     if (e != hipSuccess) {
       p = nullptr;
@@ -87,9 +125,10 @@ struct DevBuf {
     return DMW_OK;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !in_arena) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    in_arena = false;
   }
   template <class T>
   T* as() const { return reinterpret_cast<T*>(p); }
@@ -496,8 +535,11 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   TRY(P->sig.alloc((size_t)std::max(1, wave->n_echo) * P->n_meas_pad * 2 * sizeof(double)));
   a.partials = P->partials.as<double>();
   a.echo_partials = wave->n_echo > 0 ? P->echo_partials.as<double>() : nullptr;
-  hipError_t e = hipEventCreate(&P->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&P->ev1);
+  hipError_t e = hipSuccess;
+  if (!g_arena) {   // allocation-free calls keep no timing events
+    e = hipEventCreate(&P->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&P->ev1);
+  }
   if (e != hipSuccess) {
     delete P;
     return hip_fail(e, "hipEventCreate");
@@ -572,9 +614,9 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   }
 
   const dim3 grid(P->grid_x, P->n_chunks);
-  HIP_TRY(hipEventRecord(P->ev0, st), "hipEventRecord");
+  if (P->ev0) HIP_TRY(hipEventRecord(P->ev0, st), "hipEventRecord");
   HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a), "walk kernel launch");
-  HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
+  if (P->ev1) HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
   // CPMG returns the echo rows; a plain walk returns the signal after the last sample.
   const double inv_n = P->inv_norm;
   double* sig = P->sig.as<double>();
@@ -592,7 +634,7 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   HIP_TRY(hipMemcpyAsync(hs.data(), sig, hs.size() * 8, hipMemcpyDeviceToHost, st), "hipMemcpyAsync D2H");
   HIP_TRY(hipStreamSynchronize(st), "walk");
   float ms = 0.0f;
-  HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
+  if (P->ev0) HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
   out->kernel_ms = ms;
   for (double v : hs)
     if (std::isnan(v))
@@ -658,6 +700,39 @@ int dmw_walk(const dmw_geometry* geom, const dmw_waveform* wave, const dmw_run* 
   if (rc) return rc;
   rc = dmw_plan_execute(P, run->seed, out);
   dmw_plan_destroy(P);
+  return rc;
+}
+
+// dmw_walk with every device buffer (staged inputs, partial sums, outputs) in the
+// device's scratch array: no hipMalloc / hipFree.  Set-up on the first call on a
+// device: its stream and the array's device address.  Calls on one device are
+// serialised here (they share the array, which keeps the last call's contents);
+// every buffer is fully written before it is read, except the counters, which are
+// cleared per call.
+int dmw_walk_noalloc(const dmw_geometry* geom, const dmw_waveform* wave, const dmw_run* run,
+                     dmw_result* out) {
+  if (!run || !out) return fail(DMW_EINVAL, "null argument");
+  const int dev = g_device;
+  if (dev < 0 || dev >= kMaxDevices) return fail(DMW_ENODEV, "device index out of range");
+  hipStream_t st;
+  int rc = ensure_stream_on(dev, &st);   // makes dev current
+  if (rc) return rc;
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* ptr = nullptr;
+    g_scratch_err[dev] = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[dev] = g_scratch_err[dev] == hipSuccess ? ptr : nullptr;
+  });
+  if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+    return hip_fail(g_scratch_err[dev] != hipSuccess ? g_scratch_err[dev] : hipErrorInvalidSymbol,
+                    "hipGetSymbolAddress(scratch)");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex[dev]);
+  Arena arena{static_cast<char*>(g_scratch_ptr[dev]), kScratchBytes, 0};
+  g_arena = &arena;
+  dmw_plan* P = nullptr;
+  rc = dmw_plan_create(&P, geom, wave, run);
+  if (!rc) rc = dmw_plan_execute(P, run->seed, out);
+  if (P) dmw_plan_destroy(P);
+  g_arena = nullptr;
   return rc;
 }
 

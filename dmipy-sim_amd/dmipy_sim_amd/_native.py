@@ -86,6 +86,8 @@ SIGNATURES = {
     "dmw_last_error": (C.c_char_p, []),
     "dmw_walk": (C.c_int, [C.POINTER(Geometry_t), C.POINTER(Waveform_t), C.POINTER(Run_t),
                            C.POINTER(Result_t)]),
+    "dmw_walk_noalloc": (C.c_int, [C.POINTER(Geometry_t), C.POINTER(Waveform_t),
+                                   C.POINTER(Run_t), C.POINTER(Result_t)]),
     "dmw_plan_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Geometry_t),
                                   C.POINTER(Waveform_t), C.POINTER(Run_t)]),
     "dmw_plan_execute": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Result_t)]),

@@ -60,7 +60,7 @@ def _kahan_auto(n_t, n_meas):
 def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None, r0=None,
          *, outputs=(), inject_normals=None, inject_uniforms=None, kahan=None,
          meas_block=0, substep_rule="fused", echo_indices=None, counters=False, plan=None,
-         first_walker=0, n_total=None):
+         first_walker=0, n_total=None, noalloc=False):
     """One walk on the GPU; returns a dict of numpy arrays.
 
     ``outputs`` selects optional per-walker arrays: 'final_pos', 'comp_origin',
@@ -73,6 +73,8 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     and start positions): a shard of a walk split over calls or devices;
     ``n_total`` is then the whole walk's walker count (default first_walker +
     n_walkers), which fixes the myelinated substrates' start allotment.
+    ``noalloc``: run through dmw_walk_noalloc (no device allocation; the call's
+    device buffers must fit the library's 64 MiB scratch array).
     """
     G = np.asarray(waveform.G, dtype=np.float32)
     if G.ndim != 3 or G.shape[2] != 3:
@@ -152,7 +154,8 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
                        pos_full=nat.fptr(res.get("pos_full")),
                        n_cross=nat.uptr(res.get("n_cross")), log_w=nat.fptr(res.get("log_w")),
                        phi=nat.fptr(res.get("phi")), want_counters=int(bool(counters)))
-    nat.check(nat.lib().dmw_walk(gd, wd, rd, out))
+    entry = nat.lib().dmw_walk_noalloc if noalloc else nat.lib().dmw_walk
+    nat.check(entry(gd, wd, rd, out))
     if echo is not None:
         res["signal"] = res["signal"].reshape(rows, n_meas)
         if "signal_im" in res:

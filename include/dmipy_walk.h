@@ -197,6 +197,19 @@ const char* dmw_last_error(void);
 int dmw_walk(const dmw_geometry* geom, const dmw_waveform* wave,
              const dmw_run* run, dmw_result* out);
 
+/* dmw_walk without device memory allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, results and kernels as
+ * dmw_walk; every device buffer of the call (staged waveform and geometry tables,
+ * start positions, injected increments, per-wave partial sums, per-walker outputs)
+ * comes from a static device array of 64 MiB per GPU, which the library holds from
+ * its load until the process ends.  A walk needing more fails with DMW_EINVAL
+ * before any launch.  The first call on a device creates the thread's stream and
+ * looks the array up; calls on one device share the array and are serialised by
+ * the library.  kernel_ms is not measured (0). */
+int dmw_walk_noalloc(const dmw_geometry* geom, const dmw_waveform* wave,
+                     const dmw_run* run, dmw_result* out);
+
 /* A walk whose inputs stay resident in device memory across executions. */
 typedef struct dmw_plan dmw_plan;
 
