@@ -163,17 +163,18 @@ struct dmw_plan {
   int n_meas = 0, n_t = 0, n_echo = 0;
   int mb = 4, n_chunks = 1, n_meas_pad = 4;
   bool kahan = false;
+  bool park = false;         // compensated phase parked in device memory between samples
   long long n_waves = 0;
   unsigned grid_x = 0;
   size_t lds = 0;
   dmw::WalkArgs args{};
   DevBuf Gs, GsH, Gsum, chi, echo, objs, cstart, citems, r0, inj_n, inj_u, pm_start;
-  DevBuf partials, echo_partials, sig;
+  DevBuf partials, echo_partials, sig, park_buf;
   DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~dmw_plan() {
     for (DevBuf* b : {&Gs, &GsH, &Gsum, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &pm_start, &partials,
-                      &echo_partials, &sig, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
+                      &echo_partials, &sig, &park_buf, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
                       &o_logw, &o_phi, &o_counters})
       b->release();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -196,6 +197,9 @@ int dmw_set_device(int device) {
 
 const char* dmw_last_error(void) { return g_err.c_str(); }
 
+// Sub-steps per sample from which the phase is parked in device memory.
+constexpr int kParkMinSub = 32;
+
 // Measurements per kernel pass.  Blocks of 32 or more without compensation use
 // the matrix-core phase update; the compensated (Kahan) form keeps phase and
 // compensation in VGPRs and is limited to 64 measurements per pass.
@@ -209,7 +213,8 @@ static int pick_meas_block(int n_meas, int requested, bool kahan) {
   if (n_meas <= 16) return 16;
   if (n_meas <= 32) return 32;
   if (n_meas <= 64) return 64;
-  return kahan ? 64 : 128;
+  if (kahan) return 64;
+  return n_meas <= 96 ? 96 : 128;
 }
 
 int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform* wave,
@@ -301,7 +306,15 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   P->mb = pick_meas_block(wave->n_meas, run->meas_block, P->kahan);
   if (P->mb < 0) {
     delete P;
-    return fail(DMW_EINVAL, "meas_block must be 0, 4, 16, 32, 64 or 128 (at most 64 with kahan)");
+    return fail(DMW_EINVAL, "meas_block must be 0, 4, 16, 32, 64, 96 or 128 (at most 64 with kahan)");
+  }
+  // Many sub-steps per sample: the phase is touched once per sample, so it is
+  // parked in device memory, compensated, and the sub-step loop runs without
+  // phase registers (four wavefronts per SIMD instead of two).
+  if (run->meas_block == 0 && run->n_sub >= kParkMinSub && wave->n_meas > 16) {
+    P->park = true;
+    P->kahan = true;
+    P->mb = dmw::kParkBlock;
   }
   P->n_chunks = (wave->n_meas + P->mb - 1) / P->mb;
   P->n_meas_pad = P->n_chunks * P->mb;
@@ -370,12 +383,12 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   // waveform operands.  The grid is coarsened to fit; only if even one cell does
   // not fit (and no meas_block was requested) is a narrower block taken.
   auto static_lds = [&](int mb) -> size_t { return (mb >= 32 && !P->kahan) ? 8192 + 128 * (size_t)mb : 0; };
-  auto fit_grid = [&](float step, int dim, size_t obj_bytes, Grid& G) -> bool {
+  auto fit_grid = [&](float step, int dim, size_t obj_bytes, Grid& G, double hyst = 0.0) -> bool {
     for (;;) {
       const size_t budget = 65536 - static_lds(P->mb);
-      G = build_grid(*geom, step, dim, budget, obj_bytes);
+      G = build_grid(*geom, step, dim, budget, obj_bytes, hyst);
       if ((size_t)geom->n_obj * obj_bytes + (G.start.size() + G.items.size()) * 4 <= budget) return true;
-      if (run->meas_block != 0 || P->mb <= 16) return false;
+      if (run->meas_block != 0 || P->mb <= 16 || P->park) return false;
       P->mb = P->mb > 32 ? P->mb / 2 : 16;
     }
   };
@@ -426,8 +439,12 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   }
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
+    // Permeable packings walk the screened path: cells expanded by half a side
+    // (a walker keeps its cell's list until it leaves that box).
+    const bool screened = geom->wall == DMW_WALL_PERMEATE;
+    const double hyst = screened ? 0.5 : 0.0;
     Grid G;
-    if (!fit_grid(run->step_len, dim, 16, G)) {
+    if (!fit_grid(run->step_len, dim, 16, G, hyst)) {
       delete P;
       return fail(DMW_EUNSUPPORTED, lds_msg);
     }
@@ -446,6 +463,23 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
     a.n_items = (int)G.items.size();
     a.exact_minimage = G.exact_minimage ? 1 : 0;
     P->lds = (size_t)geom->n_obj * 16 + (G.start.size() + G.items.size()) * 4;
+    if (screened) {
+      // The screen caches nearest images, which needs images unambiguous over a
+      // cell's box (no true-division minimum images); its margin covers the float32
+      // rounding of positions, images and the reference's roots (screen.h).
+      double rmax = 0.0;
+      for (int k = 0; k < geom->n_obj; ++k) rmax = std::max(rmax, (double)geom->radii[k]);
+      const double m = dmw::screen_margin(geom->cell, rmax, run->step_len);
+      const double smax = (double)run->step_len;
+      // cached lanes test at most 32 listed objects by bit mask
+      int max_len = 0;
+      for (size_t c = 0; c + 1 < G.start.size(); ++c) max_len = std::max(max_len, G.start[c + 1] - G.start[c]);
+      a.scr_cache = (G.exact_minimage || max_len > 32) ? 0 : 1;
+      a.scr_w = (float)(hyst * G.h);
+      a.scr_m2 = (float)(2.0 * m);
+      a.scr_cin = (float)(m * m);
+      a.scr_cout = (float)(m * m + 0.25 * smax * smax * (1.0 + 0x1p-20));
+    }
   }
   P->n_chunks = (wave->n_meas + P->mb - 1) / P->mb;
   P->n_meas_pad = P->n_chunks * P->mb;
@@ -530,6 +564,9 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.inj_u = (run->inj_normals && run->inj_uniforms) ? P->inj_u.as<float>() : nullptr;
 
   TRY(P->partials.alloc((size_t)P->n_waves * P->n_meas_pad * 2 * sizeof(double)));
+  if (P->park)   // per wavefront and pass: mb/4 groups x (phase, compensation) x 64 lanes of float4
+    TRY(P->park_buf.alloc((size_t)P->n_chunks * P->n_waves * (P->mb / 4) * 2 * dmw::kWave * 16));
+  a.park = P->park ? P->park_buf.as<float4>() : nullptr;
   if (wave->n_echo > 0)
     TRY(P->echo_partials.alloc((size_t)P->n_waves * wave->n_echo * P->n_meas_pad * 2 * sizeof(double)));
   TRY(P->sig.alloc((size_t)std::max(1, wave->n_echo) * P->n_meas_pad * 2 * sizeof(double)));
@@ -554,21 +591,21 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   return DMW_OK;
 }
 
-static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, dim3 grid, size_t lds,
+static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, bool park, dim3 grid, size_t lds,
                               hipStream_t st, const dmw::WalkArgs& a) {
   switch (kind) {
-    case DMW_FREE: return dmw::launch_walk_free(wall, mb, kahan, grid, lds, st, a);
-    case DMW_BOX1D: return dmw::launch_walk_box1d(wall, mb, kahan, grid, lds, st, a);
-    case DMW_CYLINDER: return dmw::launch_walk_cyl(wall, mb, kahan, grid, lds, st, a);
-    case DMW_SPHERE: return dmw::launch_walk_sph(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PACKED_CYLINDERS: return dmw::launch_walk_pcyl(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PACKED_SPHERES: return dmw::launch_walk_psph(wall, mb, kahan, grid, lds, st, a);
-    case DMW_ELLIPSOID: return dmw::launch_walk_ell(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PERMEABLE_SLAB: return dmw::launch_walk_slab(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PERMEABLE_SHELL: return dmw::launch_walk_shell(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PERMEABLE_SHELL_CYL: return dmw::launch_walk_shellc(wall, mb, kahan, grid, lds, st, a);
-    case DMW_MYELINATED_CYLINDER: return dmw::launch_walk_myel(wall, mb, kahan, grid, lds, st, a);
-    case DMW_PACKED_MYELINATED: return dmw::launch_walk_pmyel(wall, mb, kahan, grid, lds, st, a);
+    case DMW_FREE: return dmw::launch_walk_free(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_BOX1D: return dmw::launch_walk_box1d(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_CYLINDER: return dmw::launch_walk_cyl(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_SPHERE: return dmw::launch_walk_sph(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PACKED_CYLINDERS: return dmw::launch_walk_pcyl(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PACKED_SPHERES: return dmw::launch_walk_psph(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_ELLIPSOID: return dmw::launch_walk_ell(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PERMEABLE_SLAB: return dmw::launch_walk_slab(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PERMEABLE_SHELL: return dmw::launch_walk_shell(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PERMEABLE_SHELL_CYL: return dmw::launch_walk_shellc(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_MYELINATED_CYLINDER: return dmw::launch_walk_myel(wall, mb, kahan, park, grid, lds, st, a);
+    case DMW_PACKED_MYELINATED: return dmw::launch_walk_pmyel(wall, mb, kahan, park, grid, lds, st, a);
     default: return hipErrorInvalidValue;
   }
 }
@@ -615,7 +652,7 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
 
   const dim3 grid(P->grid_x, P->n_chunks);
   if (P->ev0) HIP_TRY(hipEventRecord(P->ev0, st), "hipEventRecord");
-  HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a), "walk kernel launch");
+  HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, P->park, grid, P->lds, st, a), "walk kernel launch");
   if (P->ev1) HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
   // CPMG returns the echo rows; a plain walk returns the signal after the last sample.
   const double inv_n = P->inv_norm;
@@ -636,6 +673,9 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   float ms = 0.0f;
   if (P->ev0) HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
   out->kernel_ms = ms;
+  out->phase_form = P->park ? DMW_PHASE_PARKED
+                            : (P->kahan ? DMW_PHASE_COMPENSATED
+                                        : (P->mb >= 32 ? DMW_PHASE_MATRIX_CORES : DMW_PHASE_REGISTERS));
   for (double v : hs)
     if (std::isnan(v))
       return fail(DMW_EHIP, "walk produced a NaN signal (non-finite input, or a phase operand beyond the float16 range)");
@@ -796,7 +836,7 @@ int dmw_traj_create(dmw_traj** out, const dmw_geometry* geom, const dmw_run* run
   for (int i = 0; i < 9; ++i) a.traj_R[i] = lab_from_native ? lab_from_native[i] : 0.0f;
   const dim3 grid(P->grid_x, P->n_chunks);
   e = hipEventRecord(P->ev0, st);
-  if (e == hipSuccess) e = launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, grid, P->lds, st, a);
+  if (e == hipSuccess) e = launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, P->park, grid, P->lds, st, a);
   if (e == hipSuccess) e = hipEventRecord(P->ev1, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   float ms = 0.0f;

@@ -11,8 +11,10 @@ namespace {
 
 // Periodic neighbour grid of a packed substrate: every cell lists, in ascending
 // object order, the objects whose disc/ball inflated by the largest single-step
-// reach intersects the cell.  The grid is coarsened until the object table
-// (obj_bytes per object) and the grid fit in max_bytes, or it is a single cell.
+// reach intersects the cell expanded by `hyst` cell sides on every side (the
+// walk's screened path keeps a walker on its cell's list until it leaves that
+// expanded box).  The grid is coarsened until the object table (obj_bytes per
+// object) and the grid fit in max_bytes, or it is a single cell.
 struct Grid {
   int n = 1;
   float h = 0.0f;
@@ -21,7 +23,7 @@ struct Grid {
 };
 
 Grid build_grid(const dmw_geometry& g, float step_len, int dim, size_t max_bytes = 60 * 1024,
-                size_t obj_bytes = 16) {
+                size_t obj_bytes = 16, double hyst = 0.0) {
   Grid G;
   const int n_obj = g.n_obj;
   const double L = g.cell;
@@ -41,6 +43,7 @@ Grid build_grid(const dmw_geometry& g, float step_len, int dim, size_t max_bytes
     const double h = L / n;
     const double tol = 1e-3 * h + 1e-12;
     const double inflate = 1.001 * step_len + 4.0 * g.nudge + tol;
+    const double grow = tol + hyst * h;   // the cell box expanded on every side
     const int n_cells = dim == 2 ? n * n : n * n * n;
     std::vector<std::vector<int>> lists(n_cells);
     for (int k = 0; k < n_obj; ++k) {
@@ -61,7 +64,7 @@ Grid build_grid(const dmw_geometry& g, float step_len, int dim, size_t max_bytes
                   const int sh[3] = {sx, sy, sz};
                   double d2 = 0.0;
                   for (int d = 0; d < dim; ++d) {
-                    const double lo = -0.5 * L + ci[d] * h - tol, hi = lo + h + 2 * tol;
+                    const double lo = -0.5 * L + ci[d] * h - grow, hi = lo + h + 2 * grow;
                     const double cc = c[d] + sh[d] * L;
                     const double e = cc < lo ? lo - cc : (cc > hi ? cc - hi : 0.0);
                     d2 += e * e;
@@ -81,7 +84,7 @@ Grid build_grid(const dmw_geometry& g, float step_len, int dim, size_t max_bytes
     }
     G.start[n_cells] = (int)G.items.size();
     const size_t lds = (size_t)n_obj * obj_bytes + (G.start.size() + G.items.size()) * 4;
-    G.exact_minimage = (rmax + inflate + h * std::sqrt((double)dim) + tol) >= 0.45 * L;
+    G.exact_minimage = (rmax + inflate + (h + 2 * grow) * std::sqrt((double)dim)) >= 0.45 * L;
     if (lds <= max_bytes || n == 1) return G;
     n = std::max(1, n * 3 / 4);
   }

@@ -47,6 +47,7 @@
 
 #include "dmw_rng.h"
 #include "rn_arith.h"
+#include "screen.h"
 
 #pragma clang fp contract(off)
 
@@ -65,6 +66,18 @@ enum WallKind { W_REFLECT = 0, W_SURFACE = 1, W_PERMEATE = 2 };
 
 typedef __attribute__((address_space(4))) const float cfloat;
 
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [0, N), so array
+// indices stay constants and per-walker arrays stay in registers.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+
 constexpr int kBlock = 256;
 constexpr int kWave = 64;
 
@@ -82,6 +95,11 @@ struct WalkArgs {
   const int* cell_items;     // device
   int n_items;
   int exact_minimage;        // 1: min-image folds by true division
+  // screened packed permeation: cells expanded by scr_w on every side; scr_cache =
+  // 1 caches the cell's objects for the wall screen (screen.h) with margin m:
+  // scr_m2 = 2 m, scr_cout = m^2 + step^2 / 4, scr_cin = m^2
+  int scr_cache;
+  float scr_w, scr_m2, scr_cout, scr_cin;
   // step
   int n_sub;
   float step_len, dt_sub, inv_T2, inv_T1;
@@ -118,6 +136,7 @@ struct WalkArgs {
   const float* inj_n;        // device (S, 3, N) or null, S = n_t * n_sub
   const float* inj_u;        // device (S, N) or null
   // outputs
+  float4* park;              // parked phase: (grid.y, waves, MB/4, 2, 64) float4, or null
   double* partials;          // (n_waves, n_meas_pad, 2)
   double* echo_partials;     // (n_waves, n_echo, n_meas_pad, 2)
   float* final_pos;          // (3, N)
@@ -800,20 +819,25 @@ __device__ __forceinline__ float packed_reflect(Pos& p, float sx, float sy, floa
 
 // Powles permeation against the nearest object wall, from inside or outside
 // (PackedCylinders.permeate geometries.py:2018-2183, PackedSpheres 2633-2756).
-template <int DIM>
-__device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
-                                                 const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
-                                                 uint32_t& ncross, uint32_t& ncand) {
+// The candidate objects are the list [beg, end) of a grid cell whose listed
+// objects include every object within one step of the walker; with MASKED, only
+// the entries beg + j with bit j of `mask` set (the others are known to have no
+// valid root), in list order.
+template <int DIM, bool MASKED = false>
+__device__ __forceinline__ float packed_permeate_list(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                                      const Tables& tb, int beg, int end, const PowlesSrc& ps,
+                                                      uint32_t& hits, uint32_t& ncross, uint32_t& ncand,
+                                                      uint32_t mask = 0u) {
   const float EPS = a.eps, NUDGE = a.nudge;
   const float sl = (DIM == 2) ? sqrt_rn(sx * sx + sy * sy) : sqrt_rn((sx * sx + sy * sy) + sz * sz);
   float dx = 0.0f, dy = 0.0f, dz = 0.0f;
   if (sl > 0.0f) { dx = div_rn(sx, sl); dy = div_rn(sy, sl); if (DIM == 3) dz = div_rn(sz, sl); }
-  const int gc = cell_of<DIM>(p, a);
-  const int beg = tb.start[gc], end = tb.start[gc + 1];
-  ncand += (uint32_t)(end - beg);
+  ncand += MASKED ? (uint32_t)__builtin_popcount(mask) : (uint32_t)(end - beg);
   float best_t = __builtin_inff(), best_disc = 0.0f, bqx = 0.0f, bqy = 0.0f, bqz = 0.0f, bR = 1.0f;
   bool hit = false, hit_inside = false;
-  for (int i = beg; i < end; ++i) {
+  for (int i = MASKED ? beg + __builtin_ctz(mask | 0x80000000u) : beg; MASKED ? mask != 0u : i < end;
+       i = MASKED ? beg + __builtin_ctz(mask | 0x80000000u) : i + 1) {
+    if (MASKED) mask &= mask - 1u;
     const float4 o = tb.objs[tb.items[i]];
     float qx, qy, qz;
     min_image<DIM>(p, o, a, qx, qy, qz);
@@ -871,6 +895,111 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
   p.z = (DIM == 3) ? p.z + (fz - bqz) : p.z + sz;
   return (-2.0f * a.rhoD) * dperp;
 }
+
+template <int DIM>
+__device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                                 const Tables& tb, const PowlesSrc& ps, uint32_t& hits,
+                                                 uint32_t& ncross, uint32_t& ncand) {
+  const int gc = cell_of<DIM>(p, a);
+  return packed_permeate_list<DIM>(p, sx, sy, sz, a, tb, tb.start[gc], tb.start[gc + 1], ps, hits, ncross, ncand);
+}
+
+// ---------------------------------------------------------------------------
+// Screened packed permeation.  Each lane keeps the grid cell it walks in, as a
+// box expanded by a.scr_w on every side (the host lists, for that box, every
+// object within one step of it), and, when the list holds at most SCR_MAXC
+// objects, each object's image centre, radius and f = |x - c|^2 - R^2 at the
+// walker.  A sub-step that ends inside the box and that the wall screen
+// (screen.h) clears for every cached object has no valid root in the reference's
+// test (geometries.py:2067-2093, 2667-2691): its result is r + s, computed here
+// exactly as the reference's no-hit branch does (geometries.py:2099-2102).  Only
+// the other lanes run the reference's root test, over the cached cell's list.
+// ---------------------------------------------------------------------------
+constexpr int SCR_MAXC = 3;
+
+template <int DIM>
+struct PackedScreen {
+  int cnt;                 // cached objects; SCR_MAXC + 1: not cached (longer list, or no cache)
+  int beg, len;            // the cell's list in LDS
+  float blo[DIM], bhi[DIM];
+  float cx[SCR_MAXC], cy[SCR_MAXC], cz[SCR_MAXC], R[SCR_MAXC], f[SCR_MAXC];
+
+  __device__ __forceinline__ int cached() const { return cnt > SCR_MAXC ? 0 : cnt; }
+
+  __device__ __forceinline__ bool in_box(float x, float y, float z) const {
+    bool in = (x > blo[0]) & (x < bhi[0]) & (y > blo[1]) & (y < bhi[1]);
+    if (DIM == 3) in = in & (z > blo[DIM - 1]) & (z < bhi[DIM - 1]);
+    return in;
+  }
+
+  __device__ __forceinline__ void set_f(const Pos& p) {
+    static_for<SCR_MAXC>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      f[j] = screen_f(p.x, p.y, p.z, cx[j], cy[j], cz[j], R[j], DIM);
+    });
+  }
+
+  // The walker's cell (as cell_of computes it), its box in the walker's frame, the
+  // list, and the cached objects' nearest images.
+  __device__ __forceinline__ void refresh(const Pos& p, const WalkArgs& a, const Tables& tb) {
+    const float L = a.cell, h = a.grid_h, w = a.scr_w;
+    const float v[3] = {p.x, p.y, p.z};
+    int ic[3] = {0, 0, 0};
+    static_for<DIM>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      const float k = floorf(v[d] * a.inv_cell + 0.5f);
+      const float fv = v[d] - a.cell * k;
+      int i = (int)floorf((fv + 0.5f * a.cell) * a.grid_inv_h);
+      i = i < 0 ? 0 : (i >= a.grid_n ? a.grid_n - 1 : i);
+      ic[d] = i;
+      const float lo = (-0.5f * L + (float)i * h) + L * k;
+      blo[d] = lo - w;
+      bhi[d] = (lo + h) + w;
+    });
+    const int gc = (DIM == 2) ? ic[1] * a.grid_n + ic[0] : (ic[2] * a.grid_n + ic[1]) * a.grid_n + ic[0];
+    beg = tb.start[gc];
+    len = tb.start[gc + 1] - beg;
+    cnt = (a.scr_cache && len <= SCR_MAXC) ? len : SCR_MAXC + 1;
+    // every slot is written (unused ones with zeros), so no earlier value stays live
+    static_for<SCR_MAXC>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const bool use = j < cached();
+      const float4 o = use ? tb.objs[tb.items[beg + j]] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+      cx[j] = use ? o.x + L * floorf((v[0] - o.x) * a.inv_cell + 0.5f) : 0.0f;
+      cy[j] = use ? o.y + L * floorf((v[1] - o.y) * a.inv_cell + 0.5f) : 0.0f;
+      cz[j] = (DIM == 3 && use) ? o.z + L * floorf((v[2] - o.z) * a.inv_cell + 0.5f) : 0.0f;
+      R[j] = o.w;
+      f[j] = screen_f(p.x, p.y, p.z, cx[j], cy[j], cz[j], R[j], DIM);
+    });
+  }
+
+  // The sub-step to (nx, ny, nz) = p + s: returns the cached objects the screen
+  // cannot clear (bit j for slot j; bit SCR_MAXC when the list is not cached: all
+  // of it needs the reference's test) and sets `leave` when the end point is
+  // outside the cell box; fn receives f at the end point.  Branch-free.
+  __device__ __forceinline__ uint32_t test(float nx, float ny, float nz, const WalkArgs& a,
+                                           float (&fn)[SCR_MAXC], bool& leave) const {
+    const int nc = cached();
+    uint32_t flags = (uint32_t)(cnt > SCR_MAXC) << SCR_MAXC;
+    static_for<SCR_MAXC>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      const float f1 = screen_f(nx, ny, nz, cx[j], cy[j], cz[j], R[j], DIM);
+      fn[j] = f1;
+      const uint32_t unclear = (uint32_t)!screen_clear(f[j], f1, R[j], a.scr_m2, a.scr_cout, a.scr_cin);
+      flags |= (unclear & (uint32_t)(j < nc)) << j;
+    });
+    leave = !in_box(nx, ny, nz);
+    return flags;
+  }
+
+  __device__ __forceinline__ void commit(const float (&fn)[SCR_MAXC]) {
+    static_for<SCR_MAXC>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      f[j] = fn[j];
+    });
+  }
+
+};
 
 // ---------------------------------------------------------------------------
 // PackedMyelinatedCylinders (make_packed_myelin_traj_step_fn, physics.py:682-978,
@@ -1244,17 +1373,6 @@ __device__ __forceinline__ float outside_norm(const Pos& p, const WalkArgs& a) {
   return 0.0f;
 }
 
-// Compile-time loop: f(std::integral_constant<int, i>) for i in [0, N), so array
-// indices stay constants and per-walker arrays stay in registers.
-template <class F, int... I>
-__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for(F&& f) {
-  static_for_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 // sin and cos of a phase: quarter-turn reduction r = x - q*pi/2 with pi/2 split in
 // two float32 parts (Cody-Waite, fused multiply-adds), then the Taylor polynomials
 // of sin (to r^9) and cos (to r^10) on |r| <= pi/4.  Absolute error ~1e-7 for
@@ -1356,6 +1474,90 @@ struct PhaseVALU {
       constexpr int m = decltype(mc)::value;
       if (m0 + m < a.n_meas) a.phi[(size_t)(m0 + m) * N + wave_base + lane] = phi[m] - kc[m];
     });
+  }
+};
+
+// Parked compensated form, for walks with many sub-steps per waveform sample:
+// the phase of up to MB measurements and its Kahan compensation live in device
+// memory between samples (per wavefront, groups of four measurements as
+// lane-contiguous float4 rows: (group, phi | compensation, lane)), so the
+// sub-step loop keeps no phase registers.  Per sample the update streams them
+// through registers sixteen measurements at a time -- 2 x 16 x 4 bytes per lane
+// each way, against hundreds of sub-steps of wall arithmetic -- with exactly
+// PhaseVALU<MB, true>'s arithmetic per measurement, so its phases equal those of
+// the register-resident compensated form bit for bit.
+template <int MB>
+struct PhaseParked {
+  static constexpr int CH = 16;        // measurements per chunk
+  float4* base;                        // this wavefront's rows
+  int nch;                             // chunks holding measurements < n_meas
+  __device__ __forceinline__ void init() {}
+  __device__ __forceinline__ float4* row(int group, int part, int lane) const {
+    return base + ((size_t)(group * 2 + part) * kWave + lane);
+  }
+  __device__ __forceinline__ void set_ref(const WalkArgs& a, int m0, int lane, float, float, float) {
+    const size_t wave = (size_t)blockIdx.y * gridDim.x * (kBlock / kWave) +
+                        (size_t)blockIdx.x * (kBlock / kWave) + (threadIdx.x / kWave);
+    base = a.park + wave * (size_t)(MB / 4) * 2 * kWave;
+    const int rem = a.n_meas - m0;
+    nch = ((rem < MB ? rem : MB) + CH - 1) / CH;
+    for (int q = 0; q < nch * (CH / 4); ++q) {
+      *row(q, 0, lane) = float4{0.0f, 0.0f, 0.0f, 0.0f};
+      *row(q, 1, lane) = float4{0.0f, 0.0f, 0.0f, 0.0f};
+    }
+  }
+  __device__ __forceinline__ void prefetch(const WalkArgs&, int, int, int) {}
+  __device__ __forceinline__ void add_segment(const WalkArgs&, int, int, int) {}
+  __device__ __forceinline__ void load(int c, int lane, float (&phi)[CH], float (&kc)[CH]) const {
+    static_for<CH / 4>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const float4 v = *row(c * (CH / 4) + i, 0, lane), w = *row(c * (CH / 4) + i, 1, lane);
+      phi[4 * i] = v.x; phi[4 * i + 1] = v.y; phi[4 * i + 2] = v.z; phi[4 * i + 3] = v.w;
+      kc[4 * i] = w.x; kc[4 * i + 1] = w.y; kc[4 * i + 2] = w.z; kc[4 * i + 3] = w.w;
+    });
+  }
+  __device__ __forceinline__ void update(const WalkArgs& a, int t, int m0, int lane, float Px, float Py, float Pz) {
+#pragma unroll 1
+    for (int c = 0; c < nch; ++c) {
+      const cfloat* g = (const cfloat*)(a.Gs + ((size_t)t * a.n_meas_pad + m0 + c * CH) * 3);
+      float phi[CH], kc[CH];
+      load(c, lane, phi, kc);
+      static_for<CH>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const float dphi = fmaf(g[3 * m + 2], Pz, fmaf(g[3 * m + 1], Py, g[3 * m] * Px));
+        const float yk = dphi - kc[m];
+        const float tk = phi[m] + yk;
+        kc[m] = (tk - phi[m]) - yk;
+        phi[m] = tk;
+      });
+      static_for<CH / 4>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        *row(c * (CH / 4) + i, 0, lane) = float4{phi[4 * i], phi[4 * i + 1], phi[4 * i + 2], phi[4 * i + 3]};
+        *row(c * (CH / 4) + i, 1, lane) = float4{kc[4 * i], kc[4 * i + 1], kc[4 * i + 2], kc[4 * i + 3]};
+      });
+    }
+  }
+  __device__ __forceinline__ void emit(const WalkArgs&, int, float wgt, bool active, int lane, double* dst) const {
+#pragma unroll 1
+    for (int c = 0; c < nch; ++c) {
+      float phi[CH], kc[CH];
+      load(c, lane, phi, kc);
+      emit_partials<CH, true>(phi, kc, wgt, active, lane, dst + 2 * CH * c);
+    }
+  }
+  __device__ __forceinline__ void store_phi(const WalkArgs& a, int m0, long long wave_base, bool active,
+                                            int lane) const {
+    if (!active) return;
+    const long long N = a.n_walkers;
+    for (int c = 0; c < nch; ++c) {
+      float phi[CH], kc[CH];
+      load(c, lane, phi, kc);
+      static_for<CH>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int mm = m0 + c * CH + m;
+        if (mm < a.n_meas) a.phi[(size_t)mm * N + wave_base + lane] = phi[m] - kc[m];
+      });
+    }
   }
 };
 
@@ -1550,9 +1752,12 @@ struct PhaseMFMA {
 };
 
 // Kahan compensation and blocks narrower than one 16-wide matrix tile run on the
-// VALU; every other block uses the matrix cores.
-template <int MB, bool KAHAN>
-using PhaseAcc = typename std::conditional<(KAHAN || MB < 32), PhaseVALU<MB, KAHAN>, PhaseMFMA<MB>>::type;
+// VALU; every other block uses the matrix cores; PARK keeps the compensated phase
+// in device memory between samples.
+template <int MB, bool KAHAN, bool PARK = false>
+using PhaseAcc = typename std::conditional<
+    PARK, PhaseParked<MB>,
+    typename std::conditional<(KAHAN || MB < 32), PhaseVALU<MB, KAHAN>, PhaseMFMA<MB>>::type>::type;
 
 template <int MB, bool KAHAN>
 constexpr bool uses_mfma() { return !(KAHAN || MB < 32); }
@@ -1560,8 +1765,9 @@ constexpr bool uses_mfma() { return !(KAHAN || MB < 32); }
 // ---------------------------------------------------------------------------
 // The walk kernel.  Grid: x = walker tiles of 256, y = measurement blocks.
 // ---------------------------------------------------------------------------
-template <int GEOM, int WALL, int MB, bool KAHAN>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void walk_kernel(const WalkArgs a) {
+template <int GEOM, int WALL, int MB, bool KAHAN, bool PARK = false>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4 : 2))) void walk_kernel(
+    const WalkArgs a) {
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
   constexpr bool TABLES = PACKED || GEOM == G_PMYEL;
   constexpr int OBJ_ROWS = (GEOM == G_PMYEL) ? 3 : 1;
@@ -1607,7 +1813,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   int comp = classify<GEOM>(p, a, tb);
   if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
 
-  PhaseAcc<MB, KAHAN> ph;
+  PhaseAcc<MB, KAHAN, PARK> ph;
   ph.init();
   const long long wave_base = wave_id * kWave;
   float logw = 0.0f;
@@ -1626,6 +1832,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
                fsub * (p.z + L * (float)p.cz));
   else
     ph.set_ref(a, m0, lane, fsub * p.x, fsub * p.y, fsub * p.z);
+
+  // Screened packed permeation (see PackedScreen).
+  constexpr bool SCREEN = PACKED && WALL == W_PERMEATE;
+  PackedScreen<DIM> scr;
+  if (SCREEN) scr.refresh(p, a, tb);
 
   for (int t = 0; t < a.n_t; ++t) {
     ph.prefetch(a, t, m0, lane);
@@ -1677,7 +1888,41 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
       } else {
         const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
         const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
-        dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+        if (SCREEN) {
+          // r + s, the reference's no-hit result, unless the screen cannot clear it
+          const float nx = p.x + sx, ny = p.y + sy, nz = p.z + sz;
+          float fn[SCR_MAXC];
+          bool leave;
+          const uint32_t flags = scr.test(nx, ny, nz, a, fn, leave);
+          const bool exact = flags != 0u;
+          dlog = 0.0f;
+          if (__ballot(exact | leave) == 0ull) {
+            p.x = nx; p.y = ny; p.z = nz;
+            scr.commit(fn);
+          } else if (!a.scr_cache) {
+            // no cached objects (lists too long to mask): the reference's test, every lane
+            dlog = packed_permeate_list<DIM>(p, sx, sy, sz, a, tb, scr.beg, scr.beg + scr.len, ps, hits,
+                                             ncross, ncand);
+            if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);
+          } else {
+            bool moved = false;
+            if (exact) {
+              // the reference's root test over the objects the screen did not clear
+              // (the whole list when it is not cached)
+              const uint32_t m = (flags >> SCR_MAXC) ? (uint32_t)((1ull << scr.len) - 1ull) : flags;
+              dlog = packed_permeate_list<DIM, true>(p, sx, sy, sz, a, tb, scr.beg, 0, ps, hits, ncross,
+                                                     ncand, m);
+              moved = !((p.x == nx) & (p.y == ny) & (p.z == nz));
+            } else {
+              p.x = nx; p.y = ny; p.z = nz;
+            }
+            if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);   // a new cell
+            else if (moved) scr.set_f(p);                            // reflected
+            else scr.commit(fn);
+          }
+        } else {
+          dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+        }
         // Relaxation (physics.py:307-311): walls and T2 while transverse, T1 while stored.
         if (WALL != W_REFLECT) dlog = dlog * chi;
         if (GEOM == G_MYEL) {
@@ -1717,6 +1962,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
     }
     ph.update(a, t, m0, lane, Px, Py, Pz);
     if (PACKED) fold<DIM>(p, a);
+    if (SCREEN) scr.refresh(p, a, tb);   // the folded frame: cell, box and images anew
     if (a.comp_full) comp = classify<GEOM>(p, a, tb);
     if (primary && active) {
       if (a.comp_full) a.comp_full[(size_t)t * N + w] = comp;

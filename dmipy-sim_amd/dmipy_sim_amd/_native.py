@@ -12,6 +12,10 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libdmw.so"
 _lib = None
 
 DMW_OK, DMW_EINVAL, DMW_EUNSUPPORTED, DMW_ENOMEM, DMW_ENODEV, DMW_EHIP = range(6)
+# dmw_phase_form: how a walk accumulated the phase
+PHASE_FORMS = ("float32 registers", "float32 Kahan-compensated, registers",
+               "fp16 hi/lo split on matrix cores, fp32 accumulate",
+               "float32 Kahan-compensated, parked in HBM between samples")
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -58,7 +62,7 @@ class Result_t(C.Structure):
                 ("comp_origin", _ip), ("comp_final", _ip), ("comp_full", _ip),
                 ("pos_full", _fp), ("n_cross", _up), ("log_w", _fp), ("phi", _fp),
                 ("kernel_ms", C.c_double), ("n_hits", C.c_uint64), ("n_cand", C.c_uint64),
-                ("want_counters", C.c_int32)]
+                ("want_counters", C.c_int32), ("phase_form", C.c_int32)]
 
 
 class Instance_t(C.Structure):

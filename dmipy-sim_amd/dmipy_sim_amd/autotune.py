@@ -21,7 +21,7 @@ import numpy as np
 
 from .physics import make_walk_plan
 
-MEAS_BLOCKS = (4, 16, 32, 64, 128)
+MEAS_BLOCKS = (4, 16, 32, 64, 96, 128)
 _cache = {}
 
 
@@ -35,7 +35,7 @@ def plan_key(plan, n_meas, kahan):
 
 def candidate_blocks(n_meas, kahan):
     """Widths worth timing: every block up to the first one covering n_meas
-    (128 needs the uncompensated matrix-core form)."""
+    (96 and 128 need the uncompensated matrix-core form)."""
     out = []
     for mb in MEAS_BLOCKS:
         if kahan and mb > 64:

@@ -166,6 +166,7 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
             if k in res:
                 res[k] = np.einsum('ij,...j->...i', rinv, res[k].astype(np.float64)).astype(np.float32)
     res["kernel_ms"] = out.kernel_ms
+    res["phase_form"] = nat.PHASE_FORMS[out.phase_form]
     res["n_sub"] = plan.n_sub
     res["plan"] = plan
     if counters:
@@ -482,7 +483,7 @@ class PlannedWalk:
         sig = np.zeros(self.n_meas, np.float64)
         out = nat.Result_t(signal=nat.dptr(sig), want_counters=int(bool(counters)))
         nat.check(nat.lib().dmw_plan_execute(self._h, int(seed) & (2**64 - 1), out))
-        res = {"signal": sig, "kernel_ms": out.kernel_ms}
+        res = {"signal": sig, "kernel_ms": out.kernel_ms, "phase_form": nat.PHASE_FORMS[out.phase_form]}
         if counters:
             res["n_hits"] = int(out.n_hits)
             res["n_cand"] = int(out.n_cand)

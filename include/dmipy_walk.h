@@ -164,6 +164,19 @@ typedef struct dmw_run {
                                 first_walker + n_walkers must not exceed 2^32. */
 } dmw_run;
 
+/* How a walk accumulated the phase (dmw_result.phase_form).  Automatic
+ * measurement blocks (run->meas_block = 0) with at least 32 sub-steps per sample
+ * and more than 16 measurements park a Kahan-compensated float32 phase in device
+ * memory between samples; otherwise the phase stays in registers: compensated on
+ * request (run->kahan), on the matrix cores (float16 hi/lo operands, float32
+ * accumulation) for blocks of 32 or more, plain float32 below. */
+enum dmw_phase_form {
+  DMW_PHASE_REGISTERS = 0,
+  DMW_PHASE_COMPENSATED = 1,
+  DMW_PHASE_MATRIX_CORES = 2,
+  DMW_PHASE_PARKED = 3
+};
+
 /* Outputs.  signal is required; every other array is optional (NULL). */
 typedef struct dmw_result {
   double* signal;            /* (n_meas) or (n_echo, n_meas): Re <w e^{i phi}> */
@@ -180,6 +193,7 @@ typedef struct dmw_result {
   uint64_t n_hits;           /* out: wall events (reflections + transmissions) */
   uint64_t n_cand;           /* out: candidate objects tested (packed substrates) */
   int32_t want_counters;     /* in: 1 = fill n_hits / n_cand */
+  int32_t phase_form;        /* out: enum dmw_phase_form of the walk */
 } dmw_result;
 
 /* Number of visible HIP devices (0 when none). */

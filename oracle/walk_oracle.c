@@ -431,6 +431,42 @@ static float packed_reflect(float r[3], const float st[3], const oracle_args* A,
   return any_hit ? remaining * (sqrtf(fmaxf(disc_min, 0.0f)) / Rh) : 0.0f;
 }
 
+/* The per-object root test of PackedCylinders/PackedSpheres.permeate
+   (geometries.py:2067-2093, 2667-2691): minimum-image offset, inside flag, the
+   exit root from inside or the entry root from outside; valid in (EPS, |s|). */
+static int permeate_root(const float r[3], const float c[3], float Rk, const float dh[3], float sl,
+                         float L, float EPS, int dim, float* t_out, float* disc_out, int* inside_out) {
+  float q[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) q[d] = mi(r[d] - c[d], L);
+  const float dist2 = dotd(q, q, dim);
+  const int inside = dist2 < Rk * Rk;
+  const float dp = dotd(dh, q, dim);
+  const float disc = dp * dp - (dist2 - Rk * Rk);
+  const float sq = sqrtf(fmaxf(disc, 0.0f));
+  const float t = inside ? (-dp + sq) : (-dp - sq);
+  *t_out = t;
+  *disc_out = disc;
+  *inside_out = inside;
+  return (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+}
+
+/* Batch of single-object root tests (for checking the GPU walk's wall screen):
+   walker r[i], step st[i] (its first dim components form the geometry's step, as in
+   packed_permeate), object centre c[i] and radius R[i]; valid[i] = 1 when the
+   reference finds a valid root. */
+void oracle_permeate_roots(int64_t n, const float* r, const float* st, const float* c,
+                           const float* R, float L, float eps, int32_t dim, int32_t* valid) {
+  for (int64_t i = 0; i < n; ++i) {
+    const float s[3] = {st[3 * i], st[3 * i + 1], dim == 3 ? st[3 * i + 2] : 0.0f};
+    const float sl = sqrtf(dotd(s, s, dim));
+    float dh[3] = {0.0f, 0.0f, 0.0f};
+    if (sl > 0.0f) for (int d = 0; d < dim; ++d) dh[d] = s[d] / sl;
+    float t, disc;
+    int inside;
+    valid[i] = permeate_root(r + 3 * i, c + 3 * i, R[i], dh, sl, L, eps, dim, &t, &disc, &inside);
+  }
+}
+
 static float packed_permeate(float r[3], const float st[3], float u, const oracle_args* A, int dim,
                              uint32_t* hits, uint32_t* ncross) {
   const float L = A->cell, EPS = A->eps, NUDGE = A->nudge;
@@ -441,17 +477,10 @@ static float packed_permeate(float r[3], const float st[3], float u, const oracl
   int imin = 0, in_min = 0;
   float tmin = INFINITY, disc_min = 0.0f;
   for (int k = 0; k < A->n_obj; ++k) {
-    float c[3], q[3] = {0, 0, 0};
+    float c[3], t, disc;
+    int inside;
     obj_center(A, k, dim, c);
-    for (int d = 0; d < dim; ++d) q[d] = mi(r[d] - c[d], L);
-    const float Rk = A->radii[k];
-    const float dist2 = dotd(q, q, dim);
-    const int inside = dist2 < Rk * Rk;
-    const float dp = dotd(dh, q, dim);
-    const float disc = dp * dp - (dist2 - Rk * Rk);
-    const float sq = sqrtf(fmaxf(disc, 0.0f));
-    const float t = inside ? (-dp + sq) : (-dp - sq);
-    const int valid = (disc > 0.0f) && (t > EPS) && (t < sl) && (sl > 0.0f);
+    const int valid = permeate_root(r, c, A->radii[k], dh, sl, L, EPS, dim, &t, &disc, &inside);
     const float tv = valid ? t : INFINITY;
     if (tv < tmin) { tmin = tv; imin = k; disc_min = disc; in_min = inside; }
   }

@@ -52,29 +52,36 @@ def test_packed_cylinders_slice_bit_exact_with_fp64_error(gpu, oracle):
     plan = ds.make_walk_plan(geom, Dw, wf.dt)
     assert plan.n_sub == 558 and wf.G.shape == (90, 2000, 3)
     print(f"packed cylinders: walkers [{first}, {first + N}) of 4e6, n_t 2000 x n_sub 558, "
-          f"90 measurements, seed {seed}; GPU matrix-core phase", flush=True)
+          f"90 measurements, seed {seed}; GPU default phase", flush=True)
     g = gpu.walk(geom, Dw, wf, N, seed=seed, outputs=OUTS, first_walker=first)
-    print("packed cylinders: GPU compensated phase (two 64-measurement passes)", flush=True)
-    gk = gpu.walk(geom, Dw, wf, N, seed=seed, outputs=OUTS, first_walker=first, kahan=True)
+    assert g["phase_form"].endswith("parked in HBM between samples"), g["phase_form"]
+    print("packed cylinders: GPU matrix-core phase (one 96-measurement pass)", flush=True)
+    gm = gpu.walk(geom, Dw, wf, N, seed=seed, outputs=OUTS, first_walker=first, meas_block=96)
+    print("packed cylinders: GPU register compensated phase (two 64-measurement passes)", flush=True)
+    gk = gpu.walk(geom, Dw, wf, N, seed=seed, outputs=OUTS, first_walker=first, kahan=True,
+                  meas_block=64)
     print("packed cylinders: oracle (float32 phase + exact float64 phase)", flush=True)
     o = oracle.run_oracle(geom, Dw, wf, N, seed=seed, first_walker=first, exact_phase=True)
-    _same_walk(g, o)
-    _same_walk(gk, o)
+    for r in (g, gm, gk):
+        _same_walk(r, o)
+    # the parked compensated phase is the register-resident compensated one, bit for bit
+    np.testing.assert_array_equal(g["phi"], gk["phi"])
     assert o["n_cross"].sum() > 0 and np.all(np.isfinite(g["signal"]))
     err32 = float(np.max(np.abs(g["signal"] - o["signal"])))
-    dphi_m, dS_m = _phase_error(g, o)
+    dphi_p, dS_p = _phase_error(g, o)
+    dphi_m, dS_m = _phase_error(gm, o)
     dphi_k, dS_k = _phase_error(gk, o)
     dphi_o = float(np.max(np.abs(o["phi"].astype(np.float64) - o["phi_x"])))
     dS_o = float(np.max(np.abs(o["signal"] - o["signal_x"])))
-    print(f"packed cylinders FP64 error: matrix-core max|dphi| {dphi_m:.3e} max|dS| {dS_m:.3e}; "
-          f"compensated {dphi_k:.3e} / {dS_k:.3e}; reference-order float32 {dphi_o:.3e} / "
-          f"{dS_o:.3e}; |S_gpu - S_oracle32| {err32:.3e}; max|phi| {np.abs(o['phi_x']).max():.1f}",
-          flush=True)
-    # Against the exact phase both GPU paths stay below 1e-5 (matrix-core ~5e-7,
+    print(f"packed cylinders FP64 error: parked compensated max|dphi| {dphi_p:.3e} max|dS| {dS_p:.3e}; "
+          f"matrix-core {dphi_m:.3e} / {dS_m:.3e}; register compensated {dphi_k:.3e} / {dS_k:.3e}; "
+          f"reference-order float32 {dphi_o:.3e} / {dS_o:.3e}; |S_gpu - S_oracle32| {err32:.3e}; "
+          f"max|phi| {np.abs(o['phi_x']).max():.1f}", flush=True)
+    # Against the exact phase every GPU path stays below 1e-5 (matrix-core ~5e-7,
     # compensated ~5e-9); the oracle's float32 phase in the reference's order (one
     # rounding per sub-step, 1.1e6 of them) is itself ~6e-5 from exact here, so the
     # GPU and that oracle agree to within the oracle's own rounding error.
-    assert dS_m <= 1e-5 and dS_k <= 1e-5
+    assert dS_p <= 1e-5 and dS_m <= 1e-5 and dS_k <= 1e-5
     assert err32 <= dS_o + 1e-5
 
 
@@ -87,7 +94,7 @@ def test_packed_cylinders_full_launch(gpu):
     pw.close()
     S = r["signal"]
     ws = N * 2000 * 558
-    print(f"packed cylinders full: kernel {r['kernel_ms']:.0f} ms, "
+    print(f"packed cylinders full ({r['phase_form']}): kernel {r['kernel_ms']:.0f} ms, "
           f"{ws / (r['kernel_ms'] * 1e-3):.3e} walker-steps/s, hits/WS {r['n_hits'] / ws:.5f}, "
           f"S range [{S.min():.4f}, {S.max():.4f}]", flush=True)
     assert S.shape == (90,) and np.all(np.isfinite(S))

@@ -272,7 +272,7 @@ def test_autotune_candidates_and_cache(tmp_path):
     from dmipy_sim_amd import autotune
     assert autotune.candidate_blocks(3, False) == [4]
     assert autotune.candidate_blocks(16, False) == [4, 16]
-    assert autotune.candidate_blocks(90, False) == [4, 16, 32, 64, 128]
+    assert autotune.candidate_blocks(90, False) == [4, 16, 32, 64, 96]
     assert autotune.candidate_blocks(128, True) == [4, 16, 32, 64]
     wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 50)
     plan = ds.make_walk_plan(ds.Cylinder(5e-6, (0, 0, 1)), D, wf.dt)
