@@ -1,22 +1,37 @@
 """Throughput of the Monte-Carlo walk on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload cylinder|free|packed|batch|surrogate]
+                    [--workload packed|cylinder|free|batch|surrogate|replay]
 
-A step is one complete walk of the workload (all walkers, all n_t x n_sub
-sub-steps, all measurements, float64 ensemble reduction) with the waveform and
-geometry tables already resident in HBM.  The default workload is the
-Cylinder PGSE configuration (R = 5 um, rho = 1e-6 m/s, 1e6 walkers, n_t = 1000,
-128 measurements).  ``batch`` is the 4096-instance Sphere/Cylinder batch in one
-persistent graph-captured launch (a step = the whole batch); ``surrogate``
-trains and measures the learned surrogate against the batched walk.  Prints one
-JSON line on stdout; progress goes to stderr.
+The default workload is the packed-cylinder job the walker-steps/s target is
+stated on: 64 parallel cylinders (Gamma radii, mean 3 um, VF 0.6, permeability
+1e-5 m/s), 4e6 walkers, 90 free waveforms of n_t = 2000 samples, 558 sub-steps
+per sample (the reference's R/25 rule).  The job is one walk of all 4e6 walkers
+(the reference splits it into walker batches of 5e5 only to bound its (N, n_meas)
+phase carry; here each walker's Philox stream is keyed by its global index), timed
+in K sample-slices: a step is one dmw_plan_advance launch that walks every walker
+through n_t / K samples (n_t / K x 558 sub-steps) from the state the previous step
+left in HBM, so the K timed steps are exactly one whole job.  The W warmup steps
+walk slices of another walk (seed + 1), which is then abandoned.  The geometry
+tables and waveform stay resident in HBM.  ``cylinder`` is the Cylinder PGSE configuration (1e6 walkers, 128
+measurements, a step = the whole walk), ``batch`` the 4096-instance
+Sphere/Cylinder batch in one graph launch, ``surrogate`` the learned surrogate
+against the batched walk.  Prints one JSON line on stdout; progress goes to
+stderr.
 
 Roofline: the walk keeps every walker in registers, so it is bound by float32
 vector (VALU) issue, not by HBM.  ``achieved`` counts algorithmic flops per
 walker-step (frozen model ``falg`` below: FMA = 2, other arithmetic = 1, RNG
 integer work excluded) times walker-steps per launch, divided by the walk
 kernel's average duration from HIP events on the kernel's stream.
+
+cpu_baseline: the oracle (the reference's float32 step restated operation by
+operation, with its cost structure: every object tested every sub-step, the
+phase of every measurement advanced every sub-step) timed on the host cores in
+three builds -- the checker build, an -O3 -march=native build, and, for the
+packed workload, a NumPy array-at-a-time restatement with the reference's
+(walkers x objects) intermediates -- on all cores of the affinity mask, plus a
+one-core figure; ``value`` is the fastest all-core variant.
 """
 import argparse
 import json
@@ -24,6 +39,8 @@ import os
 import statistics
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "dmipy-sim_amd"))
@@ -42,7 +59,8 @@ PHASE_MFMA = "fp16 hi/lo split on matrix cores, fp32 accumulate"
 TRAFFIC_BYTES = {
     "cylinder_pgse": 17_374_635 + 32_172_843,
     "free_pgse": None,
-    "packed_cylinders_free_waveform": None,
+    # one 100-sample launch of the 4e6-walker job (profiles/walk_kernel_packed_cylinders.md)
+    "packed_cylinders_free_waveform": 312_439_398_400 + 307_147_776_000,
 }
 
 
@@ -144,7 +162,7 @@ def cpu_baseline_batch(inst, wf, plans, target_s):
     """The oracle on a sample of the batch: every 256th instance, walkers cut so
     the sample takes about target_s seconds on the host cores."""
     from oracle import oracle as orc
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = _affinity_cores()
     pick = list(range(0, len(inst), 256))
     ws_total = sum(wf.G.shape[1] * plans[i].n_sub for i in pick)   # per walker, all picked
     t0 = time.perf_counter()
@@ -265,7 +283,7 @@ def cpu_baseline_surrogate(n_walkers, target_s):
     import numpy as np
     from oracle import oracle as orc
     from dmipy_sim_amd import surrogate as sg
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    threads = _affinity_cores()
     rng = np.random.default_rng(11)
     p = sg.sample_params(rng)
     geom = sg.build_substrate(p, seed=5)
@@ -300,38 +318,175 @@ def build_workload(name):
     raise SystemExit(f"unknown workload {name!r}")
 
 
-def slice_waveform(wf, n_t):
-    from dmipy_sim_amd import Waveform
-    return Waveform(G=wf.G[:, :n_t].copy(), dt=wf.dt, echo_idx=n_t - 1)
+def _affinity_cores():
+    """Host threads for the CPU baseline: every core of the affinity mask, or the
+    share the job is given when OMP_NUM_THREADS sets one."""
+    avail = len(os.sched_getaffinity(0))
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", avail))
+    except ValueError:
+        share = avail
+    return max(1, min(avail, share))
+
+
+def _oracle_rate(orc, cfg, wf, plan, n, threads, seed, library=None):
+    t0 = time.perf_counter()
+    orc.run_oracle(cfg["geometry"], cfg["diffusivity"], wf, n, seed=seed, threads=threads, plan=plan,
+                   library=library)
+    el = time.perf_counter() - t0
+    return n * wf.G.shape[1] * plan["n_sub"] / el, el
+
+
+def _sized_rate(orc, cfg, wf, plan, threads, target_s, seed, library=None, reps=1):
+    """Walkers sized so one run takes about target_s (every walker walks all
+    samples and sub-steps); median of reps runs."""
+    probe = threads
+    _, el = _oracle_rate(orc, cfg, wf, plan, probe, threads, seed, library)
+    rounds = max(1, int(target_s / max(el, 1e-3)))
+    n = min(cfg["n_walkers"], probe * rounds)
+    rates = [_oracle_rate(orc, cfg, wf, plan, n, threads, seed + 1 + r, library)[0] for r in range(reps)]
+    return statistics.median(rates), n
 
 
 def cpu_baseline(cfg, target_s, reps):
-    """The oracle (reference-faithful float32 restatement: fixed bounce loops,
-    all objects tested, per-sub-step phase of every measurement) on the host cores,
-    over a bounded walker sample of the same workload."""
+    """The oracle on the host cores over a bounded walker sample of the same
+    workload (every sampled walker walks all samples and sub-steps): the checker
+    build and an -O3 -march=native build on all cores of the affinity mask, the
+    native build on one core, and for the packed workload the NumPy
+    array-at-a-time restatement.  value = the fastest all-core variant."""
+    import tempfile
     from oracle import oracle as orc
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    geom, wf, D = cfg["geometry"], cfg["waveform"], cfg["diffusivity"]
-    plan = orc.oracle_plan(geom, D, wf.dt)
+    cores = _affinity_cores()
+    wf = cfg["waveform"]
+    plan = orc.oracle_plan(cfg["geometry"], cfg["diffusivity"], wf.dt)
     n_t = wf.G.shape[1]
-    probe = 64 * threads
-    t0 = time.perf_counter()
-    orc.run_oracle(geom, D, wf, probe, seed=1, threads=threads, plan=plan)
-    dt = time.perf_counter() - t0
-    per_walker = dt / probe
-    n = int(max(probe, min(cfg["n_walkers"], target_s / max(per_walker, 1e-9))))
-    n = (n // threads) * threads or threads
-    rates = []
-    for r in range(reps):
+    out = {"unit": "walker-steps/s", "cores": cores, "cores_available": len(os.sched_getaffinity(0)),
+           "kind": "port"}
+    r_chk, n_chk = _sized_rate(orc, cfg, wf, plan, cores, target_s, 10, reps=reps)
+    log(f"cpu_baseline checker build, {cores} threads: {n_chk} walkers -> {r_chk:.3e} WS/s")
+    variants = {"checker_build_all_cores": r_chk}
+    try:
+        with tempfile.TemporaryDirectory() as d:
+            nat = orc.native_lib(d)
+            r_nat, n_nat = _sized_rate(orc, cfg, wf, plan, cores, target_s, 20, nat, reps=reps)
+            r_one, n_one = _sized_rate(orc, cfg, wf, plan, 1, target_s / 2, 30, nat)
+        log(f"cpu_baseline -O3 -march=native, {cores} threads: {n_nat} walkers -> {r_nat:.3e} WS/s; "
+            f"1 thread: {n_one} walkers -> {r_one:.3e} WS/s")
+        variants["native_O3_all_cores"] = r_nat
+        out["one_core"] = r_one
+        out["one_core_build"] = "gcc -O3 -march=native"
+    except Exception as exc:   # no compiler on the host: the checker build stands alone
+        log(f"cpu_baseline: native build unavailable ({exc})")
+    if cfg["name"] == "packed_cylinders_free_waveform":
+        from oracle import numpy_walk
+        geom = cfg["geometry"]
+        n_np, samples = 1024, 2
+        r0 = np.asarray(geom.init_positions(n_np, 3), np.float32)
         t0 = time.perf_counter()
-        orc.run_oracle(geom, D, wf, n, seed=2 + r, threads=threads, plan=plan)
+        numpy_walk.packed_cylinder_walk(geom._centers_np, geom._radii_np, plan["cell"], plan["kappa_over_D"],
+                                        plan["eps"], plan["nudge"], plan["step_len"], plan["gamma_dt_sub"],
+                                        np.asarray(wf.G, np.float32), plan["n_sub"], r0, seed=1,
+                                        samples=samples)
         el = time.perf_counter() - t0
-        rates.append(n * n_t * plan["n_sub"] / el)
-        log(f"cpu_baseline rep {r}: {n} walkers in {el:.2f} s -> {rates[-1]:.3e} WS/s")
-    return {"value": statistics.median(rates), "unit": "walker-steps/s", "cores": threads,
-            "kind": "port",
-            "sample": (f"{n} of {cfg['n_walkers']} walkers x all {n_t} samples x n_sub "
-                       f"{plan['n_sub']}, median of {reps} runs, OpenMP over walkers")}
+        r_np = n_np * samples * plan["n_sub"] / el
+        log(f"cpu_baseline NumPy array-at-a-time: {n_np} walkers x {samples} samples in {el:.1f} s -> "
+            f"{r_np:.3e} WS/s")
+        variants["numpy_array_at_a_time"] = r_np
+    best = max(variants, key=variants.get)
+    out["value"] = variants[best]
+    out["fastest"] = best
+    out["variants"] = {k: float(f"{v:.4e}") for k, v in variants.items()}
+    out["sample"] = (f"{n_chk} of {cfg['n_walkers']} walkers x all {n_t} samples x n_sub {plan['n_sub']} "
+                     f"(checker build; OpenMP over walkers, {cores} threads), median of {reps}")
+    return out
+
+
+def run_walk(args):
+    """One walk workload with its inputs resident in HBM.  The packed job is timed
+    in sample-slices of one whole walk (a step = one resumed launch); the other
+    walk workloads time a whole planned walk per step."""
+    import dmipy_sim_amd as ds
+    cfg = build_workload(args.workload)
+    wf = cfg["waveform"]
+    n_job = cfg["n_walkers"]
+    n_meas, n_t = int(wf.G.shape[0]), int(wf.G.shape[1])
+    sliced = args.workload == "packed"
+    t0 = time.perf_counter()
+    pw = ds.PlannedWalk(cfg["geometry"], cfg["diffusivity"], wf, n_job, meas_block=args.meas_block)
+    plan = pw.plan
+    ws_job = pw.walker_steps
+    if sliced:
+        if not 1 <= args.steps <= n_t:
+            raise SystemExit(f"bench.py: --steps must be in [1, {n_t}] (a step walks n_t / steps samples)")
+        bounds = [round(k * n_t / args.steps) for k in range(args.steps + 1)]
+        slices = [b - a for a, b in zip(bounds, bounds[1:])]
+        log(f"workload {cfg['name']}: {n_job} walkers, n_meas={n_meas} n_t={n_t} n_sub={plan.n_sub}; "
+            f"one walk in {args.steps} steps of {slices[0]} samples (plan {time.perf_counter() - t0:.1f} s)")
+        wseed = cfg["seed"] + 1
+        for i in range(args.warmup):
+            r = pw.advance(slices[0], seed=wseed)
+            log(f"  warmup {i + 1}/{args.warmup}: {r['t_done']} samples, kernel {r['kernel_ms']:.1f} ms")
+        pw.rewind()
+        kms, ws = [], []
+        t0 = time.perf_counter()
+        for i, k in enumerate(slices):
+            r = pw.advance(k, seed=cfg["seed"], counters=True)
+            kms.append(r["kernel_ms"])
+            ws.append(n_job * k * plan.n_sub)
+            if (i + 1) % max(1, args.steps // 5) == 0 or i + 1 == args.steps:
+                log(f"  step {i + 1}/{args.steps}: samples [{bounds[i]}, {bounds[i + 1]}), "
+                    f"kernel {kms[-1]:.1f} ms")
+        wall = time.perf_counter() - t0
+        assert r["t_done"] == n_t, r["t_done"]
+        res = r
+        value = ws_job / wall
+    else:
+        log(f"workload {cfg['name']}: {n_job} walkers, n_meas={n_meas} n_t={n_t} n_sub={plan.n_sub}; "
+            f"a step is the whole walk (plan {time.perf_counter() - t0:.1f} s)")
+        res = pw.run(seed=cfg["seed"], counters=True)
+        for i in range(args.warmup):
+            pw.run(seed=cfg["seed"] + 1 + i)
+        kms, ws = [], []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            kms.append(pw.run(seed=cfg["seed"] + 1 + i)["kernel_ms"])
+            ws.append(ws_job)
+        wall = time.perf_counter() - t0
+        value = ws_job * args.steps / wall
+    ms_step = 1e3 * wall / args.steps
+    hbar = res["n_hits"] / ws_job
+    kbar = res["n_cand"] / ws_job
+    log(f"phase: {res['phase_form']}; wall events/WS={hbar:.5f} exact-test candidates/WS={kbar:.4f}; "
+        f"signal[:4]={res['signal'][:4]}")
+    kmean = statistics.mean(kms)
+    f = falg(cfg, plan, hbar, kbar)
+    # algorithmic flops of every timed launch over the launches' summed duration
+    achieved = f * sum(ws) / (sum(kms) * 1e-3) / 1e12
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, kernel {kmean:.3f} ms (min {min(kms):.3f}), "
+        f"{value:.4e} WS/s, F_alg={f:.1f} -> {achieved:.2f} TFLOP/s")
+    traffic = TRAFFIC_BYTES.get(cfg["name"])
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                "traffic": traffic, "kernel": "walk_kernel", "kernel_ms": round(kmean, 4),
+                "kernel_ms_min": round(min(kms), 4), "walker_steps_per_launch": int(statistics.mean(ws)),
+                "flops_per_walker_step": round(f, 2), "wall_events_per_walker_step": round(hbar, 5),
+                "exact_test_candidates_per_walker_step": round(kbar, 5)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, args.cpu_seconds, args.cpu_reps)
+        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        if "one_core" in cpu:
+            cpu["gpu_over_one_core"] = round(value / cpu["one_core"], 1)
+    config = {"workload": cfg["name"], "walkers": n_job, "n_t": n_t, "n_meas": n_meas,
+              "n_sub": plan.n_sub, "geometry": type(cfg["geometry"]).__name__,
+              "diffusivity": cfg["diffusivity"], "phase": res["phase_form"], "ensemble_sum": "fp64"}
+    if sliced:
+        config.update({"samples_per_step": slices[0], "steps_per_job": args.steps})
+    line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
 
 
 def main():
@@ -339,7 +494,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cylinder",
+    ap.add_argument("--workload", default="packed",
                     choices=["cylinder", "free", "packed", "batch", "surrogate", "replay"])
     ap.add_argument("--n-inst", type=int, default=4096, help="batch workload: instances")
     ap.add_argument("--walker-scale", type=float, default=1.0,
@@ -352,11 +507,9 @@ def main():
     ap.add_argument("--surr-epochs", type=int, default=60)
     ap.add_argument("--meas-tiles", type=int, default=0,
                     help="replay workload: 16-measurement tiles per pass (0 = automatic)")
-    ap.add_argument("--slice-t", type=int, default=0,
-                    help="packed workload: walk only the first N waveform samples")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=6.0)
-    ap.add_argument("--cpu-reps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-reps", type=int, default=1)
     ap.add_argument("--meas-block", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="run only a device-to-device copy kernel of BYTES bytes (HBM "
@@ -384,62 +537,7 @@ def main():
     if args.workload == "replay":
         return run_replay(args)
 
-    import dmipy_sim_amd as ds
-    cfg = build_workload(args.workload)
-    wf = cfg["waveform"]
-    sliced = None
-    if args.workload == "packed":
-        n_slice = args.slice_t or 100
-        if n_slice < wf.G.shape[1]:
-            wf = slice_waveform(wf, n_slice)
-            sliced = n_slice
-    log(f"workload {cfg['name']}: N={cfg['n_walkers']} n_meas={wf.G.shape[0]} n_t={wf.G.shape[1]}")
-    pw = ds.PlannedWalk(cfg["geometry"], cfg["diffusivity"], wf, cfg["n_walkers"],
-                        meas_block=args.meas_block)
-    plan = pw.plan
-    log(f"plan: wall={plan.wall} n_sub={plan.n_sub} kahan={pw.kahan} "
-        f"walker-steps/launch={pw.walker_steps:.3e}")
-    probe = pw.run(seed=cfg["seed"], counters=True)
-    hbar = probe["n_hits"] / pw.walker_steps
-    kbar = probe["n_cand"] / pw.walker_steps
-    log(f"counters: hits/WS={hbar:.4f} candidates/WS={kbar:.3f}; signal[:4]={probe['signal'][:4]}")
-    for i in range(args.warmup):
-        pw.run(seed=cfg["seed"] + 1 + i)
-    kms = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        kms.append(pw.run(seed=cfg["seed"] + 100 + i)["kernel_ms"])
-    wall = time.perf_counter() - t0
-    ms_step = 1e3 * wall / args.steps
-    value = pw.walker_steps * args.steps / wall
-    kmean = statistics.mean(kms)
-    f = falg(cfg, plan, hbar, kbar)
-    achieved = f * pw.walker_steps / (kmean * 1e-3) / 1e12
-    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, kernel {kmean:.3f} ms, "
-        f"{value:.4e} WS/s, F_alg={f:.1f} -> {achieved:.2f} TFLOP/s")
-    traffic = TRAFFIC_BYTES.get(cfg["name"])
-    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": traffic, "kernel": "walk_kernel", "kernel_ms": round(kmean, 4),
-                "flops_per_walker_step": round(f, 2), "hits_per_walker_step": round(hbar, 5),
-                "candidates_per_walker_step": round(kbar, 4)}
-    cpu = None
-    if not args.no_cpu:
-        ccfg = dict(cfg)
-        ccfg["waveform"] = wf
-        cpu = cpu_baseline(ccfg, args.cpu_seconds, args.cpu_reps)
-        cpu["gpu_over_cpu"] = round(value / cpu["value"], 1)
-    config = {"workload": cfg["name"], "walkers": cfg["n_walkers"], "n_t": int(wf.G.shape[1]),
-              "n_meas": int(wf.G.shape[0]), "n_sub": plan.n_sub,
-              "geometry": type(cfg["geometry"]).__name__, "diffusivity": cfg["diffusivity"],
-              "phase": "kahan fp32 (VALU)" if pw.kahan else PHASE_MFMA, "ensemble_sum": "fp64"}
-    if sliced:
-        config["time_slice"] = f"first {sliced} of 2000 samples"
-    line = {"metric": METRIC, "value": value, "unit": "walker-steps/s", "n_gpus": 1,
-            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-            "data": "synthetic", "config": config, "roofline": roofline, "cpu_baseline": cpu}
-    print(json.dumps(line), flush=True)
+    return run_walk(args)
 
 
 if __name__ == "__main__":

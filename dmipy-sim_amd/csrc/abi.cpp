@@ -169,12 +169,18 @@ struct dmw_plan {
   size_t lds = 0;
   dmw::WalkArgs args{};
   DevBuf Gs, GsH, Gsum, chi, echo, objs, cstart, citems, r0, inj_n, inj_u, pm_start;
-  DevBuf partials, echo_partials, sig, park_buf;
+  DevBuf partials, echo_partials, sig, park_buf, wstate;
+  // walk in progress (dmw_plan_advance): next sample, its seed, the outputs it
+  // was started with, which half of wstate holds the walkers
+  int t_next = 0;
+  uint64_t walk_seed = 0;
+  unsigned out_mask = 0;
+  int state_half = 0;
   DevBuf o_pos, o_c0, o_cf, o_cfull, o_pfull, o_ncross, o_logw, o_phi, o_counters;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   ~dmw_plan() {
     for (DevBuf* b : {&Gs, &GsH, &Gsum, &chi, &echo, &objs, &cstart, &citems, &r0, &inj_n, &inj_u, &pm_start, &partials,
-                      &echo_partials, &sig, &park_buf, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
+                      &echo_partials, &sig, &park_buf, &wstate, &o_pos, &o_c0, &o_cf, &o_cfull, &o_pfull, &o_ncross,
                       &o_logw, &o_phi, &o_counters})
       b->release();
     if (ev0) (void)hipEventDestroy(ev0);
@@ -517,6 +523,8 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   a.has_t2 = run->has_t2;
   a.has_t1 = run->has_t1;
   a.n_t = wave->n_t;
+  a.t_begin = 0;
+  a.t_end = wave->n_t;
   a.n_meas = wave->n_meas;
   a.n_meas_pad = P->n_meas_pad;
   a.Gs = P->Gs.as<float>();
@@ -610,8 +618,30 @@ static hipError_t launch_geom(int kind, int wall, int mb, bool kahan, bool park,
   }
 }
 
-int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
-  if (!P || !out || !out->signal) return fail(DMW_EINVAL, "null argument");
+// Outputs a result asks for, as a bit mask (a walk split over several calls
+// keeps the set it started with).
+static unsigned out_mask_of(const dmw_result* out) {
+  const void* f[] = {out->signal_im, out->final_pos, out->comp_origin, out->comp_final, out->comp_full,
+                     out->pos_full,  out->n_cross,   out->log_w,       out->phi};
+  unsigned m = out->want_counters ? 1u : 0u;
+  for (int i = 0; i < 9; ++i)
+    if (f[i]) m |= 2u << i;
+  return m;
+}
+
+static int phase_form_of(const dmw_plan* P) {
+  return P->park ? DMW_PHASE_PARKED
+                 : (P->kahan ? DMW_PHASE_COMPENSATED
+                             : (P->mb >= 32 ? DMW_PHASE_MATRIX_CORES : DMW_PHASE_REGISTERS));
+}
+
+// Walks samples [t_begin, t_end) of the plan's walk.  The launch that starts at
+// sample 0 clears the counters; one that stops before n_t keeps the walkers in
+// P->wstate (two halves: the launch reads one and writes the other, since the
+// measurement passes of one launch read the state while its primary pass
+// writes) and returns kernel_ms only; the launch that reaches n_t reduces the
+// ensemble and fills *out.
+static int plan_walk(dmw_plan* P, uint64_t seed, int t_begin, int t_end, dmw_result* out) {
   hipStream_t st;
   int rc = ensure_stream_on(P->device, &st);
   if (rc) return rc;
@@ -619,6 +649,15 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   dmw::WalkArgs a = P->args;
   a.seed_lo = (uint32_t)(seed & 0xFFFFFFFFu);
   a.seed_hi = (uint32_t)(seed >> 32);
+  a.t_begin = t_begin;
+  a.t_end = t_end;
+  if (t_begin > 0 || t_end < P->n_t) {
+    rc = P->wstate.alloc((size_t)2 * 3 * N * sizeof(float4));
+    if (rc) return rc;
+    float4* half = P->wstate.as<float4>();
+    a.state_in = half + (size_t)P->state_half * 3 * N;
+    a.state_out = half + (size_t)(1 - P->state_half) * 3 * N;
+  }
 
 #define OUTBUF(field, buf, bytes)                   \
   if (out->field) {                                 \
@@ -646,7 +685,7 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   if (out->want_counters) {
     rc = P->o_counters.alloc(16);
     if (rc) return rc;
-    HIP_TRY(hipMemsetAsync(P->o_counters.p, 0, 16, st), "hipMemsetAsync");
+    if (t_begin == 0) HIP_TRY(hipMemsetAsync(P->o_counters.p, 0, 16, st), "hipMemsetAsync");
     a.counters = P->o_counters.as<unsigned long long>();
   }
 
@@ -654,6 +693,15 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   if (P->ev0) HIP_TRY(hipEventRecord(P->ev0, st), "hipEventRecord");
   HIP_TRY(launch_geom(P->geom.kind, P->geom.wall, P->mb, P->kahan, P->park, grid, P->lds, st, a), "walk kernel launch");
   if (P->ev1) HIP_TRY(hipEventRecord(P->ev1, st), "hipEventRecord");
+  out->phase_form = phase_form_of(P);
+  if (t_end < P->n_t) {
+    HIP_TRY(hipStreamSynchronize(st), "walk");
+    float ms = 0.0f;
+    if (P->ev0) HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
+    out->kernel_ms = ms;
+    P->state_half = 1 - P->state_half;
+    return DMW_OK;
+  }
   // CPMG returns the echo rows; a plain walk returns the signal after the last sample.
   const double inv_n = P->inv_norm;
   double* sig = P->sig.as<double>();
@@ -673,9 +721,6 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
   float ms = 0.0f;
   if (P->ev0) HIP_TRY(hipEventElapsedTime(&ms, P->ev0, P->ev1), "hipEventElapsedTime");
   out->kernel_ms = ms;
-  out->phase_form = P->park ? DMW_PHASE_PARKED
-                            : (P->kahan ? DMW_PHASE_COMPENSATED
-                                        : (P->mb >= 32 ? DMW_PHASE_MATRIX_CORES : DMW_PHASE_REGISTERS));
   for (double v : hs)
     if (std::isnan(v))
       return fail(DMW_EHIP, "walk produced a NaN signal (non-finite input, or a phase operand beyond the float16 range)");
@@ -725,6 +770,44 @@ int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
     out->n_hits = c[0];
     out->n_cand = c[1];
   }
+  return DMW_OK;
+}
+
+int dmw_plan_execute(dmw_plan* P, uint64_t seed, dmw_result* out) {
+  if (!P || !out || !out->signal) return fail(DMW_EINVAL, "null argument");
+  P->t_next = 0;   // abandons a walk in progress
+  return plan_walk(P, seed, 0, P->n_t, out);
+}
+
+int dmw_plan_advance(dmw_plan* P, uint64_t seed, int32_t n_samples, dmw_result* out, int32_t* t_done) {
+  if (!P || !out || !out->signal || !t_done) return fail(DMW_EINVAL, "null argument");
+  if (n_samples < 1) return fail(DMW_EINVAL, "n_samples must be >= 1");
+  const unsigned mask = out_mask_of(out);
+  if (P->t_next == 0) {
+    P->walk_seed = seed;
+    P->out_mask = mask;
+    P->state_half = 0;
+  } else {
+    if (seed != P->walk_seed)
+      return fail(DMW_EINVAL, "dmw_plan_advance: the seed differs from the one the walk in progress started with");
+    if (mask != P->out_mask)
+      return fail(DMW_EINVAL, "dmw_plan_advance: the outputs asked for differ from those the walk in progress started with");
+  }
+  const int t0 = P->t_next;
+  const int t1 = (int)std::min<long long>((long long)P->n_t, (long long)t0 + n_samples);
+  if ((t0 > 0 || t1 < P->n_t) && !P->park)
+    return fail(DMW_EUNSUPPORTED,
+                "a walk split over several calls needs the parked phase form (meas_block 0, n_sub >= " +
+                    std::to_string(kParkMinSub) + ", n_meas > 16); walk every sample in one call");
+  const int rc = plan_walk(P, seed, t0, t1, out);
+  P->t_next = (rc == DMW_OK && t1 < P->n_t) ? t1 : 0;
+  *t_done = rc == DMW_OK ? t1 : 0;
+  return rc;
+}
+
+int dmw_plan_rewind(dmw_plan* P) {
+  if (!P) return fail(DMW_EINVAL, "null argument");
+  P->t_next = 0;
   return DMW_OK;
 }
 

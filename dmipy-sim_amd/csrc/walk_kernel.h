@@ -137,6 +137,13 @@ struct WalkArgs {
   const float* inj_u;        // device (S, N) or null
   // outputs
   float4* park;              // parked phase: (grid.y, waves, MB/4, 2, 64) float4, or null
+  // Samples [t_begin, t_end) of the walk.  A launch with t_begin > 0 resumes the
+  // walkers from state_in; one with t_end < n_t writes their state to state_out
+  // instead of the ensemble.  State: three float4 rows of N, (x, y, z, log w),
+  // (cell x, cell y, cell z, crossings) as int bits, (unwrapped x, y, z, 0).
+  int t_begin, t_end;
+  const float4* state_in;
+  float4* state_out;
   double* partials;          // (n_waves, n_meas_pad, 2)
   double* echo_partials;     // (n_waves, n_echo, n_meas_pad, 2)
   float* final_pos;          // (3, N)
@@ -1501,6 +1508,7 @@ struct PhaseParked {
     base = a.park + wave * (size_t)(MB / 4) * 2 * kWave;
     const int rem = a.n_meas - m0;
     nch = ((rem < MB ? rem : MB) + CH - 1) / CH;
+    if (a.t_begin > 0) return;   // resumed: the rows hold the phase so far
     for (int q = 0; q < nch * (CH / 4); ++q) {
       *row(q, 0, lane) = float4{0.0f, 0.0f, 0.0f, 0.0f};
       *row(q, 1, lane) = float4{0.0f, 0.0f, 0.0f, 0.0f};
@@ -1802,30 +1810,43 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
   const bool primary = (blockIdx.y == 0);
   const bool inject = (a.inj_n != nullptr);
 
+  const bool resume = a.t_begin > 0;
   Pos p;
-  if (a.r0) {
-    p = Pos{a.r0[w], a.r0[N + w], a.r0[2 * N + w], 0, 0, 0};
-    if (PACKED) fold<DIM>(p, a);
-    if (GEOM == G_MYEL) { p.cx = myel_classify(p.x, p.y, a); p.cy = p.cx; }
+  float logw = 0.0f;
+  uint32_t hits = 0, ncross = 0, ncand = 0;
+  // PackedMyelinatedCylinders: the phase uses the position unwrapped by the
+  // minimum-image displacement of every sub-step (physics.py:1040-1047).
+  float uwx, uwy, uwz;
+  if (resume) {
+    const float4 s0 = a.state_in[w], s1 = a.state_in[N + w];
+    const float4 s2 = (GEOM == G_PMYEL) ? a.state_in[2 * N + w] : float4{0.0f, 0.0f, 0.0f, 0.0f};
+    p = Pos{s0.x, s0.y, s0.z, __float_as_int(s1.x), __float_as_int(s1.y), __float_as_int(s1.z)};
+    logw = s0.w;
+    ncross = __float_as_uint(s1.w);
+    uwx = s2.x; uwy = s2.y; uwz = s2.z;
+    if (GEOM != G_PMYEL) { uwx = p.x; uwy = p.y; uwz = p.z; }   // unused: not kept live
   } else {
-    p = sample_start<GEOM>(wg, a, tb);
+    if (a.r0) {
+      p = Pos{a.r0[w], a.r0[N + w], a.r0[2 * N + w], 0, 0, 0};
+      if (PACKED) fold<DIM>(p, a);
+      if (GEOM == G_MYEL) { p.cx = myel_classify(p.x, p.y, a); p.cy = p.cx; }
+    } else {
+      p = sample_start<GEOM>(wg, a, tb);
+    }
+    uwx = p.x; uwy = p.y; uwz = p.z;
   }
   int comp = classify<GEOM>(p, a, tb);
-  if (primary && active && a.comp_origin) a.comp_origin[w] = comp;
+  if (primary && active && a.comp_origin && !resume) a.comp_origin[w] = comp;
 
   PhaseAcc<MB, KAHAN, PARK> ph;
   ph.init();
   const long long wave_base = wave_id * kWave;
-  float logw = 0.0f;
-  uint32_t hits = 0, ncross = 0, ncand = 0;
   const float L = a.cell;
   const int n_sub = a.n_sub;
   const float fsub = (float)n_sub;
   u32x4 rnd{0u, 0u, 0u, 0u};
   int next_echo = 0;
-  // PackedMyelinatedCylinders: the phase uses the position unwrapped by the
-  // minimum-image displacement of every sub-step (physics.py:1040-1047).
-  float uwx = p.x, uwy = p.y, uwz = p.z;
+  while (next_echo < a.n_echo && a.echo_idx[next_echo] < a.t_begin) ++next_echo;
   // Start point of the matrix-core phase: n_sub times the unfolded start position.
   if (PACKED)
     ph.set_ref(a, m0, lane, fsub * (p.x + L * (float)p.cx), fsub * (p.y + L * (float)p.cy),
@@ -1838,7 +1859,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
   PackedScreen<DIM> scr;
   if (SCREEN) scr.refresh(p, a, tb);
 
-  for (int t = 0; t < a.n_t; ++t) {
+  for (int t = a.t_begin; t < a.t_end; ++t) {
     ph.prefetch(a, t, m0, lane);
     const float chi = a.chi ? a.chi[t] : 1.0f;
     // relaxation per sub-step: T2 while transverse, T1 while stored (physics.py:307-311)
@@ -1860,7 +1881,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
         if ((WALL == W_PERMEATE || GEOM == G_MYEL || GEOM == G_PMYEL) && a.inj_u)
           uinj = a.inj_u[(size_t)gs * N + w];
       } else {
-        if (!(gs & 1u) || (t == 0 && s == 0)) {
+        if (!(gs & 1u) || (t == a.t_begin && s == 0)) {
           rnd = philox4x32_10({wg, a.instance, gs >> 1, TAG_STEP}, a.seed_lo, a.seed_hi);
         }
         const uint32_t ra = (gs & 1u) ? rnd.z : rnd.x;
@@ -2000,16 +2021,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
     }
   }
 
-  if (a.comp_final && !a.comp_full) comp = classify<GEOM>(p, a, tb);
-  float wgt = expf(logw);
-  // MyelinatedCylinder: water-fraction spin weight of the start compartment (_ens, core.py:426-429, 489)
-  if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
-  // PackedMyelinatedCylinders: myelin water carries its proton density (core.py:552-553)
-  if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
-  ph.emit(a, m0, wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
-  if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
-  if (active) {
-    if (primary) {
+  if (a.t_end < a.n_t) {
+    // more samples follow in another launch: keep the walker, not the ensemble
+    if (primary && active) {
+      a.state_out[w] = float4{p.x, p.y, p.z, logw};
+      a.state_out[N + w] = float4{__int_as_float(p.cx), __int_as_float(p.cy), __int_as_float(p.cz),
+                                  __uint_as_float(ncross)};
+      if (GEOM == G_PMYEL) a.state_out[2 * N + w] = float4{uwx, uwy, uwz, 0.0f};
+    }
+  } else {
+    if (a.comp_final && !a.comp_full) comp = classify<GEOM>(p, a, tb);
+    float wgt = expf(logw);
+    // MyelinatedCylinder: water-fraction spin weight of the start compartment (_ens, core.py:426-429, 489)
+    if (GEOM == G_MYEL) wgt = wgt * pick3(p.cy, a.m_wf[0], a.m_wf[1], a.m_wf[2]);
+    // PackedMyelinatedCylinders: myelin water carries its proton density (core.py:552-553)
+    if (GEOM == G_PMYEL && p.cz == 1) wgt = wgt * a.pm_wmy;
+    ph.emit(a, m0, wgt, active, lane, a.partials + ((size_t)wave_id * a.n_meas_pad + m0) * 2);
+    if (a.phi) ph.store_phi(a, m0, wave_base, active, lane);
+    if (active && primary) {
       if (a.final_pos) {
         a.final_pos[w] = PACKED ? p.x + L * (float)p.cx : p.x;
         a.final_pos[N + w] = PACKED ? p.y + L * (float)p.cy : p.y;

@@ -96,6 +96,9 @@ SIGNATURES = {
                                   C.POINTER(Waveform_t), C.POINTER(Run_t)]),
     "dmw_plan_execute": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(Result_t)]),
     "dmw_plan_destroy": (C.c_int, [C.c_void_p]),
+    "dmw_plan_advance": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int32, C.POINTER(Result_t),
+                                   C.POINTER(C.c_int32)]),
+    "dmw_plan_rewind": (C.c_int, [C.c_void_p]),
     "dmw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Instance_t),
                                    C.c_int32, C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
     "dmw_batch_execute": (C.c_int, [C.c_void_p, C.c_uint64, _dp, C.POINTER(BatchStats_t)]),

@@ -60,7 +60,7 @@ def _kahan_auto(n_t, n_meas):
 def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None, r0=None,
          *, outputs=(), inject_normals=None, inject_uniforms=None, kahan=None,
          meas_block=0, substep_rule="fused", echo_indices=None, counters=False, plan=None,
-         first_walker=0, n_total=None, noalloc=False):
+         first_walker=0, n_total=None, noalloc=False, slices=None):
     """One walk on the GPU; returns a dict of numpy arrays.
 
     ``outputs`` selects optional per-walker arrays: 'final_pos', 'comp_origin',
@@ -75,6 +75,9 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
     n_walkers), which fixes the myelinated substrates' start allotment.
     ``noalloc``: run through dmw_walk_noalloc (no device allocation; the call's
     device buffers must fit the library's 64 MiB scratch array).
+    ``slices``: sample counts of successive dmw_plan_advance calls that walk the
+    waveform in pieces, the walkers kept in device memory between them (the last
+    piece is extended to n_t); the result equals the one-call walk's.
     """
     G = np.asarray(waveform.G, dtype=np.float32)
     if G.ndim != 3 or G.shape[2] != 3:
@@ -154,8 +157,13 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
                        pos_full=nat.fptr(res.get("pos_full")),
                        n_cross=nat.uptr(res.get("n_cross")), log_w=nat.fptr(res.get("log_w")),
                        phi=nat.fptr(res.get("phi")), want_counters=int(bool(counters)))
-    entry = nat.lib().dmw_walk_noalloc if noalloc else nat.lib().dmw_walk
-    nat.check(entry(gd, wd, rd, out))
+    if slices is not None:
+        if noalloc:
+            raise ValueError("slices and noalloc exclude each other")
+        _walk_in_slices(gd, wd, rd, out, [int(k) for k in slices], n_t)
+    else:
+        entry = nat.lib().dmw_walk_noalloc if noalloc else nat.lib().dmw_walk
+        nat.check(entry(gd, wd, rd, out))
     if echo is not None:
         res["signal"] = res["signal"].reshape(rows, n_meas)
         if "signal_im" in res:
@@ -173,6 +181,31 @@ def walk(geometry, diffusivity, waveform, n_walkers, seed=123, T2=None, T1=None,
         res["n_hits"] = int(out.n_hits)
         res["n_cand"] = int(out.n_cand)
     return res
+
+
+def _walk_in_slices(gd, wd, rd, out, slices, n_t):
+    """One walk through dmw_plan_advance, ``slices`` samples per call; the last
+    call walks every remaining sample.  out.kernel_ms is summed over the calls."""
+    import ctypes as C
+    lib = nat.lib()
+    h = C.c_void_p()
+    nat.check(lib.dmw_plan_create(C.byref(h), gd, wd, rd))
+    try:
+        t_done = C.c_int32(0)
+        total_ms = 0.0
+        for k in slices:
+            if k < 1:
+                raise ValueError("slices must be positive sample counts")
+            nat.check(lib.dmw_plan_advance(h, rd.seed, k, out, C.byref(t_done)))
+            total_ms += out.kernel_ms
+            if t_done.value == n_t:
+                break
+        if t_done.value < n_t:
+            nat.check(lib.dmw_plan_advance(h, rd.seed, n_t - t_done.value, out, C.byref(t_done)))
+            total_ms += out.kernel_ms
+        out.kernel_ms = total_ms
+    finally:
+        lib.dmw_plan_destroy(h)
 
 
 def simulate(
@@ -488,6 +521,31 @@ class PlannedWalk:
             res["n_hits"] = int(out.n_hits)
             res["n_cand"] = int(out.n_cand)
         return res
+
+    def advance(self, n_samples, seed=123, counters=False):
+        """Walk the next ``n_samples`` samples of the walk in progress (a new walk
+        with ``seed`` when none is); the walkers stay in device memory between
+        calls.  Returns ``t_done`` and ``kernel_ms`` of this call, and once
+        ``t_done == n_t`` the same entries as ``run`` (counters summed over the
+        calls).  Needs the parked phase form unless one call walks every sample."""
+        import ctypes as C
+        sig = np.zeros(self.n_meas, np.float64)
+        out = nat.Result_t(signal=nat.dptr(sig), want_counters=int(bool(counters)))
+        t_done = C.c_int32(0)
+        nat.check(nat.lib().dmw_plan_advance(self._h, int(seed) & (2**64 - 1), int(n_samples), out,
+                                              C.byref(t_done)))
+        res = {"t_done": int(t_done.value), "kernel_ms": out.kernel_ms,
+               "phase_form": nat.PHASE_FORMS[out.phase_form]}
+        if res["t_done"] == self.n_t:
+            res["signal"] = sig
+            if counters:
+                res["n_hits"] = int(out.n_hits)
+                res["n_cand"] = int(out.n_cand)
+        return res
+
+    def rewind(self):
+        """Abandon the walk in progress."""
+        nat.check(nat.lib().dmw_plan_rewind(self._h))
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

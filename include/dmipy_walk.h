@@ -234,6 +234,28 @@ int dmw_plan_create(dmw_plan** plan, const dmw_geometry* geom,
 int dmw_plan_execute(dmw_plan* plan, uint64_t seed, dmw_result* out);
 int dmw_plan_destroy(dmw_plan* plan);
 
+/* Resumable walk.  No counterpart in the reference, whose simulate() walks every
+ * sample in one call: this is for walks checkpointed between launches (long walks
+ * under a launch time limit, schedulers that interleave work) and for timing one
+ * walk in slices.  Walks samples [t, t + n_samples) of the plan's waveform,
+ * clipped at n_t, from the walkers the previous call left in device memory
+ * (position and periodic cell, log-weight, crossings, parked phase: 48 bytes per
+ * walker, twice, besides the phase); *t_done receives the samples walked so far.
+ * The first call (after dmw_plan_create, a finished walk, dmw_plan_execute or
+ * dmw_plan_rewind) starts a walk with `seed`; the following calls must pass the
+ * same seed and ask for the same outputs (DMW_EINVAL otherwise).  Calls before
+ * the last fill out->kernel_ms (that call's launch) and out->phase_form only; the
+ * call that reaches n_t fills *out as dmw_plan_execute does (counters summed over
+ * the calls) and leaves the plan ready for a new walk.  The result equals
+ * dmw_plan_execute's bit for bit.  A walk split over several calls needs the
+ * parked phase form, which a plan takes with meas_block 0, n_sub >= 32 and
+ * n_meas > 16: other plans return DMW_EUNSUPPORTED unless one call walks every
+ * sample.  Calls on one plan must not overlap. */
+int dmw_plan_advance(dmw_plan* plan, uint64_t seed, int32_t n_samples,
+                     dmw_result* out, int32_t* t_done);
+/* Abandons the walk in progress, if any. */
+int dmw_plan_rewind(dmw_plan* plan);
+
 /* ---- Batched persistent launch (many substrate instances, one launch) ----
  * Has no single counterpart in the reference, where each instance is a separate
  * simulate() call (core.py:193) -- e.g. the serial loop of simulate_mixture

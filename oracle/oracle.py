@@ -73,18 +73,32 @@ def build():
     subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
 
 
+def _bind(path):
+    h = C.CDLL(path)
+    h.oracle_walk.restype = C.c_int
+    h.oracle_walk.argtypes = [C.POINTER(OracleArgs), C.POINTER(OracleOut)]
+    h.oracle_philox4x32_10.restype = None
+    h.oracle_philox4x32_10.argtypes = [_up, _up, _up]
+    return h
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        h = C.CDLL(LIB)
-        h.oracle_walk.restype = C.c_int
-        h.oracle_walk.argtypes = [C.POINTER(OracleArgs), C.POINTER(OracleOut)]
-        h.oracle_philox4x32_10.restype = None
-        h.oracle_philox4x32_10.argtypes = [_up, _up, _up]
-        _lib = h
+        _lib = _bind(LIB)
     return _lib
+
+
+def native_lib(out_dir):
+    """The same restatement built for speed on the host it runs on (a CPU timing
+    baseline, not the checker): -O3 -march=native with FMA contraction allowed,
+    OpenMP over walkers.  Compiled into out_dir; returns a bound library."""
+    path = os.path.join(out_dir, "liboracle_walk_native.so")
+    subprocess.run(["gcc", "-O3", "-march=native", "-fPIC", "-std=c11", "-fopenmp", "-shared",
+                    "-o", path, os.path.join(HERE, "walk_oracle.c"), "-lm"], check=True)
+    return _bind(path)
 
 
 def philox4x32_10(ctr, key):
@@ -292,7 +306,7 @@ def oracle_plan(geom, D, dt, T2=None, T1=None):
 def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
                inject_normals=None, inject_uniforms=None, cell_form=1, phase64=False,
                threads=0, echo_indices=None, walker_range=None, plan=None, G_native=None,
-               instance=0, per_sample=False, first_walker=0, exact_phase=False):
+               instance=0, per_sample=False, first_walker=0, exact_phase=False, library=None):
     """Walk on the CPU; returns a dict like dmipy_sim_amd.walk (native frame).
 
     phase64: False = float32 phase (the reference's arithmetic), True = float32
@@ -300,7 +314,8 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
     'signal_x_im' and 'phi_x', the same walk with every phase increment formed and
     summed in float64 from the float32 positions (the exact phase of the float32
     trajectory: the FP64 oracle the kernel's phase error is stated against).
-    first_walker: global index of walker 0 (a shard of a larger walk)."""
+    first_walker: global index of walker 0 (a shard of a larger walk).  library: a
+    bound library from native_lib (default: the checker build)."""
     p = plan if plan is not None else oracle_plan(geom, D, waveform.dt, T2, T1)
     G = np.ascontiguousarray(waveform.G if G_native is None else G_native, dtype=np.float32)
     n_meas, n_t, _ = G.shape
@@ -379,7 +394,7 @@ def run_oracle(geom, D, waveform, n_walkers, seed=0, T2=None, T1=None, r0=None,
         O.signal_x = res["signal_x"].ctypes.data_as(_dp)
         O.signal_x_im = res["signal_x_im"].ctypes.data_as(_dp)
         O.phi_x = res["phi_x"].ctypes.data_as(_dp)
-    rc = lib().oracle_walk(C.byref(A), C.byref(O))
+    rc = (library or lib()).oracle_walk(C.byref(A), C.byref(O))
     if rc != 0:
         raise RuntimeError("oracle_walk failed")
     if echo is not None:

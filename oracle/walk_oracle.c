@@ -1065,7 +1065,7 @@ int oracle_walk(const oracle_args* A, oracle_out* O) {
     double* locx = (double*)calloc((size_t)rows * M * 2, sizeof(double));
     uint64_t hits_loc = 0;
     double wsum_loc = 0.0;
-#pragma omp for schedule(dynamic, 4)
+#pragma omp for schedule(dynamic, 1)
     for (int64_t w = w0; w < w1; ++w) {
       float r[3];
       int cell[3] = {0, 0, 0};
