@@ -406,12 +406,15 @@ def simulate_mixture(compartments, waveform, seed=123):
     All compartments run in one batched persistent launch (each as instance
     word 0 with seed offset i, i.e. the walk ``simulate(..., seed=seed + i)``
     performs) when the waveform has at most 64 measurements and needs no
-    compensated phase; otherwise one ``simulate`` call per compartment."""
+    compensated phase and no compartment is myelinated (those walk on the
+    single-walk path only); otherwise one ``simulate`` call per compartment."""
     total = sum(c['fraction'] for c in compartments)
     if abs(total - 1.0) > 1e-6:
         raise ValueError(f"Compartment fractions must sum to 1, got {total}")
     n_meas, n_t = np.asarray(waveform.G).shape[:2]
-    if compartments and n_meas <= 64 and not _kahan_auto(n_t, n_meas):
+    batchable = all(getattr(c['geometry'], '_walk_kind', None)
+                    not in (geo.KIND_MYELIN, geo.KIND_PACKED_MYELIN) for c in compartments)
+    if compartments and batchable and n_meas <= 64 and not _kahan_auto(n_t, n_meas):
         from .batch import BatchedWalk
         check_gpu(None)
         specs = [dict(geometry=c['geometry'], diffusivity=c['diffusivity'],

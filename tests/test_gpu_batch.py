@@ -121,3 +121,24 @@ def test_simulate_mixture_equals_serial_calls(gpu):
               for i, c in enumerate(comps))
     assert mix.dtype == np.float32 and mix.shape == (wf.G.shape[0],)
     assert np.max(np.abs(mix - ref)) <= 1e-5
+
+
+def test_simulate_mixture_with_myelinated_compartment(gpu):
+    """A MyelinatedCylinder compartment (diffusivity None, its own per-compartment
+    diffusivities) cannot ride the batched launch: the mixture walks every
+    compartment with simulate(..., seed=seed+i) instead (core.py:846-856) and
+    equals those serial calls exactly."""
+    wf = small_pgse()
+    myel = ds.MyelinatedCylinder(3e-6, 5e-6, (0, 0, 1), D_intra=2e-9, D_extra=1.5e-9,
+                                 D_myelin=4e-10, T2_myelin=0.015)
+    comps = [dict(fraction=0.6, n_walkers=900, diffusivity=None, geometry=myel),
+             dict(fraction=0.4, n_walkers=500, diffusivity=D, geometry=ds.Cylinder(3e-6, (0, 0, 1)))]
+    print("mixture: MyelinatedCylinder + Cylinder, 16 measurements", flush=True)
+    mix = gpu.simulate_mixture(comps, wf, seed=40)
+    ref = None
+    for i, c in enumerate(comps):
+        s = c["fraction"] * gpu.simulate(c["n_walkers"], c["diffusivity"], wf, c["geometry"], seed=40 + i)
+        ref = s if ref is None else ref + s
+    assert mix.dtype == np.float32 and mix.shape == (wf.G.shape[0],)
+    np.testing.assert_array_equal(mix, ref)
+    assert np.all(mix[wf.G.reshape(wf.G.shape[0], -1).any(1)] < 1.0)
