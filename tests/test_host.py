@@ -529,3 +529,24 @@ def test_plan_create_null_arguments():
     assert nat.lib().dmw_plan_create(C.byref(h), gd, wd, None) == _EINVAL
     assert b"null argument" in nat.lib().dmw_last_error()
     assert nat.lib().dmw_walk(gd, wd, None, None) == _EINVAL
+
+
+def test_integration_stub_structs_match_header():
+    """The ctypes structures of the reference-side binding shown in INTEGRATION.md
+    (Option B) have the header's sizes and offsets: a stub missing an appended
+    field would let the library write past the caller's structure."""
+    import ctypes as C
+    txt = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = txt.split("```python\n# dmipy_sim/_hip.py", 1)[1].split("\n", 1)[1].split("```", 1)[0]
+    defs = block.split("_lib.dmw_walk.argtypes")[0]
+    defs = "\n".join(l for l in defs.splitlines() if not l.startswith("_lib = "))
+    ns = {}
+    exec(defs, ns)
+    pairs = {"Geometry": _native.Geometry_t, "Waveform": _native.Waveform_t,
+             "Run": _native.Run_t, "Result": _native.Result_t}
+    for stub, mirror in pairs.items():
+        st = ns[stub]
+        assert C.sizeof(st) == C.sizeof(mirror), stub
+        assert [f for f, _ in st._fields_] == [f for f, _ in mirror._fields_], stub
+        for f, _ in st._fields_:
+            assert getattr(st, f).offset == getattr(mirror, f).offset, (stub, f)
