@@ -263,7 +263,9 @@ def run_surrogate(args):
               "members": sur["members"], "inference_batch": sur["rows"],
               "training_signals": r["train"]["signals"], "training_walkers": r["train"]["walkers"],
               "epochs": r["fit"]["epochs"], "held_out_signals": r["held_out"]["signals"],
-              "held_out_walkers": r["held_out"]["walkers"], "n_t": 250, "n_meas": 64}
+              "held_out_walkers": r["held_out"]["walkers"], "n_t": sg.N_T, "n_meas": sg.N_MEAS,
+              "substrate": f"{sg.N_CYL} packed permeable cylinders, Gamma radii clipped at "
+                           f"{sg.R_MIN * 1e6:g} um, volume fraction {sg.PRIOR['vf']}"}
     line = {"metric": METRIC, "value": sur["signals_per_s"], "unit": "signals/s", "n_gpus": 1,
             "steps": 1, "warmup": 1, "ms_per_step": sur["ms"], "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
@@ -304,7 +306,7 @@ def cpu_baseline_surrogate(n_walkers, target_s):
     return {"value": sps, "unit": "signals/s", "cores": threads, "kind": "port",
             "walker_steps_per_s": ws_rate,
             "sample": f"one generator substrate (n_sub {plan['n_sub']}), {n} walkers x "
-                      f"{n_t} samples x 64 measurements; signals/s at {n_walkers} walkers"}
+                      f"{n_t} samples x {n_meas} measurements; signals/s at {n_walkers} walkers"}
 
 
 def build_workload(name):
@@ -500,8 +502,8 @@ def main():
     ap.add_argument("--walker-scale", type=float, default=1.0,
                     help="batch workload: scale every instance's walker count")
     ap.add_argument("--waves-per-simd", type=int, default=0, help="batch workload: persistent waves")
-    ap.add_argument("--surr-instances", type=int, default=3125,
-                    help="surrogate workload: training substrates (x 64 signals)")
+    ap.add_argument("--surr-instances", type=int, default=2223,
+                    help="surrogate workload: training substrates (x 90 signals)")
     ap.add_argument("--surr-walkers", type=int, default=10_000,
                     help="surrogate workload: walkers per training substrate")
     ap.add_argument("--surr-epochs", type=int, default=60)

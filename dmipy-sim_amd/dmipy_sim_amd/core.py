@@ -248,6 +248,14 @@ def simulate(
         if gap is not None:
             raise NotImplementedError(f"engine='replay' cannot serve this run: {gap}. "
                                       "Use engine='fused' (or 'auto').")
+        # Routed before the fused engine's OOM back-off and walker batching, as the
+        # reference does (core.py:334-349): one producer walk under one seed.  The
+        # trajectories stay resident in HBM, so walker_batch_size (the reference's
+        # host-side producer batch) bounds nothing here and is accepted as is.
+        check_gpu(n_walkers, require_gpu, what="simulate")
+        wf_r = waveform.waveform if hasattr(waveform, 'waveform') else waveform
+        return _simulate_via_replay(n_walkers, diffusivity, wf_r, geometry, seed=seed,
+                                    T2=T2, T1=T1, r0=r0)
     _myelinated = (getattr(geometry, '_is_myelinated', False)
                    or getattr(geometry, '_is_packed_myelinated', False))
     if return_positions == 'full' and _myelinated:
@@ -293,9 +301,6 @@ def simulate(
 
     if hasattr(waveform, 'waveform'):
         waveform = waveform.waveform
-    if engine == "replay":
-        return _simulate_via_replay(n_walkers, diffusivity, waveform, geometry, seed=seed,
-                                    T2=T2, T1=T1, r0=r0)
     track = return_compartments is not False
     outs = []
     if return_positions == 'full':

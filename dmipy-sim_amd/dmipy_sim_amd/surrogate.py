@@ -34,11 +34,19 @@ N_SUBSTRATE = 6
 N_WAVE = 64
 N_IN = N_SUBSTRATE + N_WAVE
 
-# Prior over substrate instances of the training generator.
+# Prior over substrate instances of the training generator: the packed-cylinder
+# family (64 cylinders with Gamma-distributed radii, densely packed, permeable)
+# around 64 cylinders of mean radius 3 um, shape 4, volume fraction 0.6.
 PRIOR = dict(mean_r=(1.5e-6, 3.5e-6), shape=(2.0, 8.0), vf=(0.3, 0.65),
              kappa=(1e-6, 1e-4), rho=(1e-7, 1e-5), D=(1.0e-9, 2.5e-9))
-R_MIN = 1.2e-6      # radii are clipped here (bounds the permeable sub-step count)
-N_CYL = 32
+# Radii are clipped here.  The permeable sub-step rule (R_min / 25) makes a walk's
+# cost grow as 1 / R_min^2; at 1 um the mean generator substrate costs 3.2e5
+# sub-steps per walker over 50 ms, unclipped draws of shape 2 reach 1e7.
+R_MIN = 1.0e-6
+N_CYL = 64
+# Waveform family: 90 free waveforms per substrate, n_t = 2000 over 50 ms.
+N_MEAS, N_T, T_WAVE = 90, 2000, 0.05
+BATCH_MEAS = 64     # measurements per waveform set of one batched instance
 
 
 # ---------------------------------------------------------------------------
@@ -256,10 +264,10 @@ def build_substrate(p, seed, n_cyl=N_CYL):
                                permeability=p["kappa"])
 
 
-def random_waveforms(rng, n_meas=64, n_t=250, T=0.025, gmax=(0.02, 0.3)):
-    """Free waveforms as in the config-3 generator: per axis 4 sines with
-    f ~ U(10, 500) Hz, phase ~ U(0, 2 pi), amplitude ~ U(0, 1); zero time mean;
-    max |G| ~ U(gmax); the first row is b = 0."""
+def random_waveforms(rng, n_meas=N_MEAS, n_t=N_T, T=T_WAVE, gmax=(0.05, 0.3)):
+    """Free waveforms of the packed-cylinder workload's family: per axis 4 sines
+    with f ~ U(10, 500) Hz, phase ~ U(0, 2 pi), amplitude ~ U(0, 1); zero time
+    mean; max |G| ~ U(gmax); the first row is b = 0."""
     dt = T / (n_t - 1)
     t = np.arange(n_t) * dt
     f = rng.uniform(10, 500, (n_meas, 3, 4))
@@ -273,7 +281,43 @@ def random_waveforms(rng, n_meas=64, n_t=250, T=0.025, gmax=(0.02, 0.3)):
     return wfm.Waveform(G=G.astype(np.float32), dt=float(dt), echo_idx=n_t - 1)
 
 
-def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=512,
+def split_measurements(waveform, max_meas=BATCH_MEAS):
+    """A Waveform as consecutive measurement sets of one size k <= max_meas
+    (the batched launch's limit), the last one padded with zero-gradient rows;
+    returns the list of Waveforms."""
+    G = np.asarray(waveform.G, np.float32)
+    n_meas = G.shape[0]
+    parts = -(-n_meas // max_meas)
+    k = -(-n_meas // parts)
+    Gp = np.zeros((parts * k,) + G.shape[1:], np.float32)
+    Gp[:n_meas] = G
+    return [wfm.Waveform(G=np.ascontiguousarray(Gp[j * k:(j + 1) * k]), dt=waveform.dt,
+                         echo_idx=waveform.echo_idx) for j in range(parts)]
+
+
+def walk_signals(instances, waveforms, n_walkers, seed):
+    """Signals (n_inst, n_meas) of every instance under its own waveform, by
+    the batched persistent walk.  A waveform of more than BATCH_MEAS
+    measurements is walked as measurement sets of at most BATCH_MEAS, one
+    batched instance each (its own Philox instance word, hence its own
+    walkers)."""
+    from .batch import BatchedWalk
+    n_meas = waveforms[0].G.shape[0]
+    specs, waves = [], []
+    for inst, wf in zip(instances, waveforms):
+        for part in split_measurements(wf):
+            specs.append(dict(geometry=inst["geometry"], diffusivity=inst["diffusivity"],
+                              n_walkers=int(n_walkers), waveform=len(waves)))
+            waves.append(part)
+    bw = BatchedWalk(specs, waves)
+    try:
+        sig, st = bw.run(seed=seed)
+    finally:
+        bw.close()
+    return sig.reshape(len(instances), -1)[:, :n_meas], st
+
+
+def generate(n_instances, n_walkers, seed, n_meas=N_MEAS, n_t=N_T, T=T_WAVE, chunk=256,
              walk_seed=None, log=None):
     """Batched-MC dataset: (X (n*n_meas, 70) float32, S (n*n_meas) float64, info).
 
@@ -281,7 +325,6 @@ def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=51
     n_meas random waveforms; instances run ``chunk`` at a time in one batched
     persistent launch each.
     """
-    from .batch import BatchedWalk
     rng = np.random.default_rng(seed)
     X, S = [], []
     gpu_ms = 0.0
@@ -289,21 +332,19 @@ def generate(n_instances, n_walkers, seed, n_meas=64, n_t=250, T=0.025, chunk=51
     params = []
     instances = []
     for c0 in range(0, n_instances, chunk):
-        specs, waves = [], []
+        insts, waves = [], []
         for i in range(c0, min(n_instances, c0 + chunk)):
             p = sample_params(rng)
             geom = build_substrate(p, seed=int(rng.integers(2**31)))
             p = realized_params(p, geom)
             wf = random_waveforms(rng, n_meas=n_meas, n_t=n_t, T=T)
-            specs.append(dict(geometry=geom, diffusivity=p["D"], n_walkers=int(n_walkers),
-                              waveform=len(waves)))
+            insts.append(dict(geometry=geom, diffusivity=p["D"]))
             waves.append(wf)
             params.append(p)
-            instances.append((dict(geometry=geom, diffusivity=p["D"]), wf))
+            instances.append((insts[-1], wf))
             X.append(features(p, wf))
-        bw = BatchedWalk(specs, waves)
-        sig, st = bw.run(seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
-        bw.close()
+        sig, st = walk_signals(insts, waves, n_walkers,
+                               seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
         S.append(sig.reshape(-1))
         gpu_ms += st["total_ms"]
         walker_steps += st["walker_steps"]
@@ -406,7 +447,6 @@ def predict_with_fallback(ensemble, instances, waveforms, threshold=5e-3, n_walk
     with any flagged measurement is walked with ``n_walkers`` walkers and its
     flagged measurements are replaced.  Returns (signals (n_inst, n_meas),
     fallback mask, timing dict)."""
-    from .batch import BatchedWalk
     X = np.concatenate([features(instance_params(s["geometry"], s["diffusivity"]), w)
                         for s, w in zip(instances, waveforms)])
     n_meas = waveforms[0].G.shape[0]
@@ -417,11 +457,8 @@ def predict_with_fallback(ensemble, instances, waveforms, threshold=5e-3, n_walk
     rows = np.nonzero(flag.any(axis=1))[0]
     mc_ms = 0.0
     if rows.size:
-        specs = [dict(geometry=instances[i]["geometry"], diffusivity=instances[i]["diffusivity"],
-                      n_walkers=int(n_walkers), waveform=j) for j, i in enumerate(rows)]
-        bw = BatchedWalk(specs, [waveforms[i] for i in rows])
-        sig, st = bw.run(seed)
-        bw.close()
+        sig, st = walk_signals([instances[i] for i in rows], [waveforms[i] for i in rows],
+                               n_walkers, seed)
         mc_ms = st["total_ms"]
         for j, i in enumerate(rows):
             out[i, flag[i]] = sig[j, flag[i]]
@@ -478,9 +515,11 @@ def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_0
     res["spread_mean"] = float(spread.mean())
     insts = [a for a, _ in ih["instances"]]
     waves = [w for _, w in ih["instances"]]
-    hybrid, flag, _ = predict_with_fallback(ens, insts, waves, threshold=threshold,
-                                            n_walkers=n_walkers, seed=777)
+    hybrid, flag, fb = predict_with_fallback(ens, insts, waves, threshold=threshold,
+                                             n_walkers=n_walkers, seed=777)
     res["fallback_fraction"] = float(flag.mean())
+    res["fallback_instance_fraction"] = float(flag.any(axis=1).mean())
+    res["fallback_mc_ms"] = fb["mc_ms"]
     res["rmse_hybrid"] = float(np.sqrt(np.mean((hybrid.reshape(-1) - Sh) ** 2)))
     log(f"held-out rmse: surrogate {res['rmse_surrogate']:.4f}, hybrid {res['rmse_hybrid']:.4f}, "
         f"fallback {res['fallback_fraction']:.3%}")
@@ -505,7 +544,11 @@ def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_0
     mc_sps = Sr.size / (ir["gpu_ms"] * 1e-3)
     res["mc"] = dict(walkers=nw, rmse=rmse_mc, ladder=ladder, signals_per_s=mc_sps,
                      instances_timed=n_rate)
-    eff = infer_rows / (ms_total * 1e-3 + res["fallback_fraction"] * infer_rows / mc_sps)
+    # The hybrid's cost: inference, plus the walk of every instance with a
+    # flagged measurement -- all of its measurements, at the fallback's
+    # n_walkers (the walk's cost is linear in walkers; mc_sps is measured at nw).
+    fallback_s_per_row = res["fallback_instance_fraction"] * (n_walkers / nw) / mc_sps
+    eff = infer_rows / (ms_total * 1e-3 + infer_rows * fallback_s_per_row)
     res["hybrid_signals_per_s"] = eff
     res["gain"] = sur_sps / mc_sps
     res["gain_hybrid"] = eff / mc_sps

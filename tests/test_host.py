@@ -228,6 +228,35 @@ def test_oom_backoff_halves_walker_batches(monkeypatch):
     assert s[0] == pytest.approx(np.mean([11, 12, 13, 14]))
 
 
+def test_replay_routes_before_walker_batches(monkeypatch):
+    """engine='replay' is routed before the fused engine's OOM back-off and
+    walker batching (core.py:334-349): one producer walk under the caller's
+    seed, whatever walker_batch_size says; AcquisitionScheme-style wrappers are
+    unwrapped.  The replay and the fused walk are host stubs."""
+    from dmipy_sim_amd import core
+    calls = []
+
+    def fake_replay(n_walkers, diffusivity, waveform, geometry, *, seed, T2, T1, r0):
+        calls.append((n_walkers, seed, waveform))
+        return np.full(waveform.G.shape[0], 0.5, np.float32)
+
+    def no_walk(*a, **k):
+        raise AssertionError("the fused walk must not run for engine='replay'")
+
+    monkeypatch.setattr(core, "_simulate_via_replay", fake_replay)
+    monkeypatch.setattr(core, "walk", no_walk)
+    monkeypatch.setattr(core, "check_gpu", lambda *a, **k: True)
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 20)
+
+    class Scheme:
+        waveform = wf
+
+    s = ds.simulate(1000, D, Scheme(), ds.Cylinder(5e-6, (0, 0, 1)), seed=10, walker_batch_size=100,
+                    engine="replay")
+    assert calls == [(1000, 10, wf)]
+    assert s.dtype == np.float32 and s.shape == (1,)
+
+
 def test_batch_argument_errors_before_gpu():
     """The batched launch validates its descriptors before touching a device."""
     import ctypes as C

@@ -127,3 +127,37 @@ def test_features_shape_and_prior():
     assert np.all(np.abs(x[:, :6]) <= 1.5)
     geom = sg.build_substrate(p, seed=4)
     assert np.min(geom._radii_np) >= sg.R_MIN * (1 - 1e-6)
+
+
+def test_generator_matches_the_packed_workload_family():
+    """The training generator draws the packed-cylinder workload's family: 64
+    cylinders, 90 free waveforms of n_t = 2000 over 50 ms with zero net moment
+    and max |G| in [0.05, 0.3] T/m; a volume-fraction prior containing 0.6."""
+    rng = np.random.default_rng(8)
+    wf = sg.random_waveforms(rng)
+    G = np.asarray(wf.G, np.float64)
+    assert G.shape == (90, 2000, 3)
+    assert wf.dt == pytest.approx(0.05 / 1999)
+    np.testing.assert_allclose(G.sum(axis=1), 0.0, atol=1e-4)
+    gmax = np.linalg.norm(G, axis=2).max(axis=1)
+    assert gmax[0] == 0.0 and np.all((gmax[1:] >= 0.05 * 0.999) & (gmax[1:] <= 0.3 * 1.001))
+    assert sg.PRIOR["vf"][0] <= 0.6 <= sg.PRIOR["vf"][1]
+    geom = sg.build_substrate(sg.sample_params(rng), seed=2)
+    assert len(geom._radii_np) == 64
+
+
+@pytest.mark.parametrize("n_meas", [1, 45, 64, 65, 90, 130])
+def test_split_measurements_covers_every_measurement(n_meas):
+    """Batched instances take at most 64 measurements: a waveform is walked as
+    equal-size sets, the last padded with zero-gradient rows, which the
+    concatenation drops again."""
+    rng = np.random.default_rng(n_meas)
+    wf = sg.random_waveforms(rng, n_meas=n_meas, n_t=40)
+    parts = sg.split_measurements(wf)
+    k = parts[0].G.shape[0]
+    assert k <= 64 and all(p.G.shape == (k, 40, 3) for p in parts)
+    assert len(parts) == -(-n_meas // 64)
+    cat = np.concatenate([p.G for p in parts])
+    np.testing.assert_array_equal(cat[:n_meas], wf.G)
+    assert not np.any(cat[n_meas:])
+    assert all(p.dt == wf.dt for p in parts)
