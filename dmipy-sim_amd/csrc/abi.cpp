@@ -201,6 +201,16 @@ int dmw_set_device(int device) {
   return DMW_OK;
 }
 
+int dmw_device_arch(int32_t device, char* buf, int32_t len) {
+  if (!buf || len < 1) return fail(DMW_EINVAL, "null or empty buffer");
+  buf[0] = '\0';
+  hipDeviceProp_t prop;
+  const hipError_t e = hipGetDeviceProperties(&prop, device);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+  std::snprintf(buf, (size_t)len, "%s", prop.gcnArchName);
+  return DMW_OK;
+}
+
 const char* dmw_last_error(void) { return g_err.c_str(); }
 
 // Sub-steps per sample from which the phase is parked in device memory.

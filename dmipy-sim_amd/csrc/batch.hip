@@ -3,10 +3,13 @@
 // The reference runs one simulate() call per substrate (core.py:193), e.g. the
 // serial loop of simulate_mixture (core.py:821-858).  Here every instance (any
 // analytic geometry with its own wall, D, sizes, rho, kappa, walker count,
-// sub-step count and waveform set) is cut into 64-walker work units; a
-// persistent grid of wavefronts pops units from a queue ordered by decreasing
-// cost (n_sub), so the long instances start first and the short ones fill the
-// tail.  Packed instances keep their object tables and periodic neighbour grids
+// sub-step count and waveform set) is cut into 64-walker tiles; a persistent
+// grid of wavefronts pops runs of consecutive tiles of one instance (the
+// walkers a wavefront takes per pop: 64, 128, 256 or 512) from a queue ordered
+// by decreasing cost (n_sub), so the long instances start first and the short
+// ones fill the tail.  The sub-step loop is compiled plain and unrolled by two;
+// both knobs, and the number of resident wavefronts, are chosen by measurement
+// (autotune.py).  Packed instances keep their object tables and periodic neighbour grids
 // in one concatenated device array (read through the L1/L2 caches).  Each unit
 // writes float64 partial sums; a second kernel reduces every instance's units
 // in a fixed order.  The three launches (queue reset, walk, reduce) are
@@ -42,7 +45,7 @@ struct InstDev {
   long long g_off;          // offset (floats) of the instance's waveform set
   long long gh_off, gsum_off;  // offsets of its float16 phase operands (uint2) and column sums (floats)
   float p_scale, inv_pg_scale;  // matrix-core phase scales (powers of two)
-  int unit_begin, unit_end;
+  int unit_begin, unit_end;  // the instance's tiles: partial rows [unit_begin, unit_end)
   unsigned long long seed_offset;
   unsigned int stream;       // Philox instance word
   int pad_;
@@ -50,8 +53,9 @@ struct InstDev {
 
 struct BatchArgs {
   const InstDev* inst;
-  const int2* units;        // (instance, tile) in queue order
-  int n_units;
+  const int2* units;        // queue entries (instance, first tile) in queue order
+  int n_units;              // queue entries
+  int tiles_per_pop;        // tiles an entry covers (fewer at an instance's end)
   const float* G;           // sets x (n_t, n_meas_pad, 3) T/m, not scaled
   const uint2* GH;          // sets x (n_t, 2, n_meas_pad) float16 x4 hi/lo parts of G * g_scale(set)
   const float* Gsum;        // sets x (3, n_meas_pad): sum over t of G
@@ -62,11 +66,10 @@ struct BatchArgs {
   const int* items;         // concatenated grid cell lists
   const unsigned long long* seed;  // device, read at kernel start
   unsigned int* queue;      // work counter
-  double* partials;         // (n_units, n_meas_pad, 2) indexed by unit id
-  const int* unit_id;       // queue position -> unit id
+  double* partials;         // (tiles, n_meas_pad, 2), row unit_begin + tile of the instance
 };
 
-template <int GEOM, int WALL, int MB>
+template <int GEOM, int WALL, int MB, int UNROLL>
 __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, int tile, int unit,
                                           unsigned long long base_seed) {
   const unsigned long long seed = base_seed + I.seed_offset;
@@ -114,6 +117,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
     const float q2chi = I.has_t2 ? (I.dt_sub * I.inv_T2) * chi : 0.0f;  // zero when off (x - 0 == x)
     const float x0 = p.x, y0 = p.y, z0 = p.z;
     float ax = 0.0f, ay = 0.0f, az = 0.0f;
+#pragma unroll UNROLL
     for (int s = 0; s < n_sub; ++s) {
       const uint32_t gs = (uint32_t)t * (uint32_t)n_sub + (uint32_t)s;
       if (!(gs & 1u) || (t == 0 && s == 0))
@@ -155,7 +159,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
 // scratch) and 1652 ms at two (187 VGPRs): the per-unit instance loads and dispatch are
 // latency that more resident wavefronts hide.  Five (96 VGPRs, 372 bytes of scratch) is
 // slower again, 1314 ms.
-template <int MB>
+template <int MB, int UNROLL>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
   const unsigned int inc = (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u;
@@ -171,11 +175,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     const int pos = (int)__builtin_amdgcn_readfirstlane(old);
     if (pos >= b.n_units) break;  // uniform: every wave reaches this exit
     const int2 u = b.units[pos];
-    const int unit = b.unit_id[pos];
+    const int tile_end = min(u.y + b.tiles_per_pop, b.inst[u.x].unit_end - b.inst[u.x].unit_begin);
+    for (int tile = u.y; tile < tile_end; ++tile) {
+    // re-read per tile (scalar loads) rather than held across the walk
     const InstDev I = b.inst[u.x];
     switch (I.kind * 3 + I.wall) {
 #define DMW_UNIT(G, W) \
-      case G * 3 + W: walk_unit<G, W, MB>(b, I, u.y, unit, seed); break;
+      case G * 3 + W: walk_unit<G, W, MB, UNROLL>(b, I, tile, I.unit_begin + tile, seed); break;
       DMW_UNIT(G_FREE, W_REFLECT)
       DMW_UNIT(G_BOX1D, W_REFLECT)
       DMW_UNIT(G_BOX1D, W_SURFACE)
@@ -204,6 +210,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 #undef DMW_UNIT
       default: break;
     }
+    }
   }
 }
 
@@ -228,7 +235,7 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const InstDev* __rest
 // Host side.
 // ---------------------------------------------------------------------------
 struct dmw_batch {
-  int n_inst = 0, n_units = 0, n_meas = 0, n_meas_pad = 0, mb = 64, grid = 0;
+  int n_inst = 0, n_units = 0, n_meas = 0, n_meas_pad = 0, mb = 64, grid = 0, unroll = 1;
   uint64_t walker_steps = 0;
   std::vector<void*> allocs;
   void *d_inst = nullptr, *d_seed = nullptr, *d_queue = nullptr, *d_part = nullptr, *d_out = nullptr;
@@ -265,10 +272,15 @@ hipError_t enqueue_batch(dmw_batch* B) {
   hipLaunchKernelGGL(dmw::batch_reset_kernel, dim3(1), dim3(1), 0, B->st, (unsigned int*)B->d_queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if (B->mb == 16)
-    hipLaunchKernelGGL((dmw::batch_kernel<16>), dim3(B->grid), dim3(dmw::kBlock), 0, B->st, B->args);
+  const dim3 g(B->grid), blk(dmw::kBlock);
+  if (B->mb == 16 && B->unroll == 2)
+    hipLaunchKernelGGL((dmw::batch_kernel<16, 2>), g, blk, 0, B->st, B->args);
+  else if (B->mb == 16)
+    hipLaunchKernelGGL((dmw::batch_kernel<16, 1>), g, blk, 0, B->st, B->args);
+  else if (B->unroll == 2)
+    hipLaunchKernelGGL((dmw::batch_kernel<64, 2>), g, blk, 0, B->st, B->args);
   else
-    hipLaunchKernelGGL((dmw::batch_kernel<64>), dim3(B->grid), dim3(dmw::kBlock), 0, B->st, B->args);
+    hipLaunchKernelGGL((dmw::batch_kernel<64, 1>), g, blk, 0, B->st, B->args);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(dmw::batch_reduce_kernel, dim3(B->n_inst), dim3(256), 0, B->st,
@@ -282,9 +294,18 @@ hipError_t enqueue_batch(dmw_batch* B) {
 extern "C" {
 
 int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, int32_t n_sets,
-                     const dmw_waveform* sets, int32_t waves_per_simd, int32_t use_graph) {
+                     const dmw_waveform* sets, const dmw_batch_tuning* tuning, int32_t use_graph) {
   if (!out || !inst || !sets || n_inst < 1 || n_sets < 1) return dmw::batch_fail(DMW_EINVAL, "null or empty batch");
   *out = nullptr;
+  const dmw_batch_tuning tn = tuning ? *tuning : dmw_batch_tuning{0, 0, 0};
+  if (tn.waves_per_simd < 0 || tn.waves_per_simd > 8)
+    return dmw::batch_fail(DMW_EINVAL, "waves_per_simd must be 0 (automatic) or 1..8");
+  const int wpw = tn.walkers_per_wave == 0 ? dmw::kWave : tn.walkers_per_wave;
+  if (wpw != 64 && wpw != 128 && wpw != 256 && wpw != 512)
+    return dmw::batch_fail(DMW_EINVAL, "walkers_per_wave must be 0 (64), 64, 128, 256 or 512");
+  const int unroll = tn.unroll == 0 ? 1 : tn.unroll;
+  if (unroll != 1 && unroll != 2) return dmw::batch_fail(DMW_EINVAL, "unroll must be 0 (1), 1 or 2");
+  const int tpp = wpw / dmw::kWave;
   const int n_meas = sets[0].n_meas, n_t = sets[0].n_t;
   if (n_meas < 1 || n_meas > 64 || n_t < 1)
     return dmw::batch_fail(DMW_EINVAL, "batched launch needs 1..64 measurements");
@@ -315,6 +336,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   B->n_inst = n_inst;
   B->n_meas = n_meas;
   B->mb = n_meas <= 16 ? 16 : 64;
+  B->unroll = unroll;
   B->n_meas_pad = B->mb;
   B->use_graph = use_graph != 0;
   const int Mp = B->n_meas_pad;
@@ -324,6 +346,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   std::vector<double> cost;
   std::vector<float4> objs;
   std::vector<int> starts, items;
+  int n_tiles = 0;
   for (int i = 0; i < n_inst; ++i) {
     const dmw_instance& I = inst[i];
     dmw::InstDev& d = hi[i];
@@ -368,22 +391,25 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
       d.exact_minimage = G.exact_minimage ? 1 : 0;
       d.n_obj = I.geom.n_obj;
     }
-    d.unit_begin = (int)units.size();
+    d.unit_begin = n_tiles;
     const int tiles = (int)((I.n_walkers + dmw::kWave - 1) / dmw::kWave);
     const double packed_cost = (I.geom.kind >= DMW_PACKED_CYLINDERS) ? 2.0 : 1.0;
-    for (int j = 0; j < tiles; ++j) {
+    for (int j = 0; j < tiles; j += tpp) {
       units.push_back(make_int2(i, j));
-      cost.push_back((double)I.n_sub * packed_cost * (I.geom.wall == DMW_WALL_PERMEATE ? 1.3 : 1.0));
+      cost.push_back((double)std::min(tpp, tiles - j) * I.n_sub * packed_cost *
+                     (I.geom.wall == DMW_WALL_PERMEATE ? 1.3 : 1.0));
     }
-    d.unit_end = (int)units.size();
+    n_tiles += tiles;
+    d.unit_end = n_tiles;
     B->walker_steps += (uint64_t)I.n_walkers * (uint64_t)n_t * (uint64_t)I.n_sub;
   }
-  B->n_units = (int)units.size();
-  std::vector<int> order(B->n_units);
+  B->n_units = n_tiles;
+  const int n_entries = (int)units.size();
+  std::vector<int> order(n_entries);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-  std::vector<int2> q_units(B->n_units);
-  for (int k = 0; k < B->n_units; ++k) q_units[k] = units[order[k]];
+  std::vector<int2> q_units(n_entries);
+  for (int k = 0; k < n_entries; ++k) q_units[k] = units[order[k]];
 
   std::vector<float> G((size_t)n_sets * n_t * Mp * 3, 0.0f);
   for (int s = 0; s < n_sets; ++s)
@@ -411,14 +437,14 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   int n_cu = 256;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) n_cu = prop.multiProcessorCount;
-  const int wps = waves_per_simd > 0 ? waves_per_simd : 4;  // the kernel's occupancy budget
-  B->grid = std::max(1, std::min(n_cu * wps, (B->n_units + 3) / 4));
+  const int wps = tn.waves_per_simd > 0 ? tn.waves_per_simd : 4;  // the kernel's occupancy budget
+  B->grid = std::max(1, std::min(n_cu * wps, (n_entries + 3) / 4));
 
   auto fail_free = [&](hipError_t err, const char* what) {
     delete B;
     return dmw::batch_hip_fail(err, what);
   };
-  void *d_units, *d_uid, *d_G, *d_GH, *d_Gsum, *d_chi = nullptr, *d_objs, *d_starts, *d_items;
+  void *d_units, *d_G, *d_GH, *d_Gsum, *d_chi = nullptr, *d_objs, *d_starts, *d_items;
 #define BT(x, what)                                       \
   do {                                                    \
     hipError_t err_ = (x);                                \
@@ -427,7 +453,6 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   BT(hipStreamCreateWithFlags(&B->st, hipStreamNonBlocking), "hipStreamCreate");
   BT(B->upload(&B->d_inst, hi.data(), hi.size() * sizeof(dmw::InstDev)), "batch upload");
   BT(B->upload(&d_units, q_units.data(), q_units.size() * sizeof(int2)), "batch upload");
-  BT(B->upload(&d_uid, order.data(), order.size() * sizeof(int)), "batch upload");
   BT(B->upload(&d_G, G.data(), G.size() * 4), "batch upload");
   BT(B->upload(&d_GH, GH.data(), GH.size() * sizeof(uint2)), "batch upload");
   BT(B->upload(&d_Gsum, Gsum.data(), Gsum.size() * 4), "batch upload");
@@ -446,8 +471,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   dmw::BatchArgs& a = B->args;
   a.inst = (const dmw::InstDev*)B->d_inst;
   a.units = (const int2*)d_units;
-  a.unit_id = (const int*)d_uid;
-  a.n_units = B->n_units;
+  a.n_units = n_entries;
+  a.tiles_per_pop = tpp;
   a.G = (const float*)d_G;
   a.GH = (const uint2*)d_GH;
   a.Gsum = (const float*)d_Gsum;

@@ -216,6 +216,110 @@ __global__ __launch_bounds__(256) void gemm_fwd_big(int M, int N, int K, const _
   }
 }
 
+// Layer-fused inference for width 512: one workgroup (8 wavefronts) per strip of
+// FR = 128 rows keeps the strip's activations in LDS (128 x 512 bf16, 130 KB)
+// through every hidden layer and the output layer, so activations never touch
+// HBM; only the input strip is read and one float per row written.  Per layer,
+// wavefront w computes features [64w, 64w + 64) of all 128 rows as the
+// transposed product Y^T = W^T X^T: A operands are rows of the transposed
+// weights W^T (streamed from L2, each wavefront's next K step prefetched into
+// registers), B operands are activation rows read from LDS, and each lane's
+// accumulator holds four consecutive features of one row, written back as one
+// 8-byte LDS store after a barrier (every wavefront has then read the layer's
+// input).  Per output the K steps of 32 run in ascending order and the epilogue
+// bf16(GELU(acc + b)) and the output-layer dot product (per lane strided, then a
+// butterfly) are those of gemm_fwd_big / out_fwd_kernel.
+constexpr int FR = 128, FH = 512, FLD = FH + 8, FMAXL = 16;
+
+struct FusedArgs {
+  int depth, k0;
+  const __bf16* X0;              // (B, k0) input rows
+  const __bf16* Wt;              // transposed bf16 weights: layer l at Wt + w_off[l], (FH, K_l) row-major
+  const float* P;                // fp32 parameters (biases, output layer)
+  long long w_off[FMAXL], b_off[FMAXL];
+  long long wout_off, bout_off;
+  float* y;
+};
+
+__global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
+  __shared__ __attribute__((aligned(16))) __bf16 X[FR][FLD];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const long long r0 = (long long)blockIdx.x * FR;
+  {
+    const int cpr = a.k0 / 8;    // 16-byte chunks per input row
+    for (int c = tid; c < FR * cpr; c += 512) {
+      const int r = c / cpr, q = (c - r * cpr) * 8;
+      *reinterpret_cast<bf16x8*>(&X[r][q]) = *reinterpret_cast<const bf16x8*>(a.X0 + (r0 + r) * a.k0 + q);
+    }
+  }
+  __syncthreads();
+  const int rq = lane & 15, kq = (lane >> 4) * 8;
+  const int f0 = wv * 64;
+  for (int l = 0; l < a.depth; ++l) {
+    const int K = l == 0 ? a.k0 : FH;
+    const __bf16* W = a.Wt + a.w_off[l];
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const __bf16* wp[4];
+    bf16x8 wf[4], wn[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      wp[i] = W + (size_t)(f0 + 16 * i + rq) * K + kq;
+      wf[i] = *reinterpret_cast<const bf16x8*>(wp[i]);
+    }
+    for (int k = 0; k < K; k += 32) {
+      const bool more = k + 32 < K;
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wn[i] = *reinterpret_cast<const bf16x8*>(wp[i] + k + 32);
+      }
+      bf16x8 xf[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) xf[j] = *reinterpret_cast<const bf16x8*>(&X[16 * j + rq][k + kq]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[i], xf[j], acc[i][j], 0, 0, 0);
+      if (more) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) wf[i] = wn[i];
+      }
+    }
+    __syncthreads();   // every wavefront has read this layer's input
+    const float* bias = a.P + a.b_off[l];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int f = f0 + 16 * i + 4 * (lane >> 4);
+      const float b0 = bias[f], b1 = bias[f + 1], b2 = bias[f + 2], b3 = bias[f + 3];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        const bf16x4 v = {f2bf(gelu(acc[i][j][0] + b0)), f2bf(gelu(acc[i][j][1] + b1)),
+                          f2bf(gelu(acc[i][j][2] + b2)), f2bf(gelu(acc[i][j][3] + b3))};
+        *reinterpret_cast<bf16x4*>(&X[16 * j + rq][f]) = v;
+      }
+    }
+    __syncthreads();
+  }
+  // output layer: wavefront w takes rows 16w .. 16w + 15
+  float w8[FH / 64];
+#pragma unroll
+  for (int q = 0; q < FH / 64; ++q) w8[q] = a.P[a.wout_off + lane + 64 * q];
+  const float bo = a.P[a.bout_off];
+  for (int rr = 0; rr < FR / 8; ++rr) {
+    const int row = wv * (FR / 8) + rr;
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < FH / 64; ++q) s += bf2f(X[row][lane + 64 * q]) * w8[q];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) a.y[r0 + row] = s + bo;
+  }
+}
+
 // y[b] = Y[b] . w + b0; with targets also the squared error and dy = 2 (y - t) / B.
 __global__ void out_fwd_kernel(int B, int H, const __bf16* __restrict__ Y, const float* __restrict__ w,
                                const float* __restrict__ b0, const float* __restrict__ t,
@@ -349,6 +453,7 @@ __global__ void f32_to_bf16_kernel(long long n, const float* __restrict__ a, __b
 // ---------------------------------------------------------------------------
 struct dmw_mlp {
   int n_in = 0, k0 = 0, H = 0, depth = 0, max_batch = 0;
+  bool fused = true;               // inference through the layer-fused kernel where it applies
   long long n_params = 0;          // internal (padded) parameter count
   std::vector<long long> w_off, b_off;  // offsets of W_l, b_l in the flat fp32 vector
   long long wout_off = 0, bout_off = 0;
@@ -416,6 +521,25 @@ unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 // are not kept, and batches of 128-row tiles take the large-tile GEMM.
 int forward(dmw_mlp* m, int B, const float* targets_dev) {
   hipStream_t st = m->st;
+  if (targets_dev == nullptr && m->fused && B % dmw::FR == 0 && m->H == dmw::FH && m->depth <= dmw::FMAXL &&
+      m->k0 <= dmw::FH) {
+    dmw::FusedArgs fa{};
+    fa.depth = m->depth;
+    fa.k0 = m->k0;
+    fa.X0 = m->Y[0];
+    fa.Wt = m->pbfT;
+    fa.P = m->p;
+    for (int l = 0; l < m->depth; ++l) {
+      fa.w_off[l] = m->wbf_off[l];
+      fa.b_off[l] = m->b_off[l];
+    }
+    fa.wout_off = m->wout_off;
+    fa.bout_off = m->bout_off;
+    fa.y = m->out;
+    hipLaunchKernelGGL(dmw::mlp_infer_fused, dim3(B / dmw::FR), dim3(512), 0, st, fa);
+    MLP_TRY(hipGetLastError(), "mlp fused inference");
+    return DMW_OK;
+  }
   const bool big = targets_dev == nullptr && B % dmw::BM == 0 && m->H % dmw::BN == 0;
   for (int l = 0; l < m->depth; ++l) {
     const int K = l == 0 ? m->k0 : m->H;

@@ -83,10 +83,16 @@ class BatchStats_t(C.Structure):
                 ("walker_steps", C.c_uint64), ("n_units", C.c_int32), ("grid", C.c_int32)]
 
 
+class BatchTuning_t(C.Structure):
+    _fields_ = [("waves_per_simd", C.c_int32), ("walkers_per_wave", C.c_int32),
+                ("unroll", C.c_int32)]
+
+
 #: Every symbol include/dmipy_walk.h declares, with (restype, argtypes).
 SIGNATURES = {
     "dmw_device_count": (C.c_int, []),
     "dmw_set_device": (C.c_int, [C.c_int]),
+    "dmw_device_arch": (C.c_int, [C.c_int32, C.c_char_p, C.c_int32]),
     "dmw_last_error": (C.c_char_p, []),
     "dmw_walk": (C.c_int, [C.POINTER(Geometry_t), C.POINTER(Waveform_t), C.POINTER(Run_t),
                            C.POINTER(Result_t)]),
@@ -100,7 +106,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_int32)]),
     "dmw_plan_rewind": (C.c_int, [C.c_void_p]),
     "dmw_batch_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(Instance_t),
-                                   C.c_int32, C.POINTER(Waveform_t), C.c_int32, C.c_int32]),
+                                   C.c_int32, C.POINTER(Waveform_t), C.POINTER(BatchTuning_t),
+                                   C.c_int32]),
     "dmw_batch_execute": (C.c_int, [C.c_void_p, C.c_uint64, _dp, C.POINTER(BatchStats_t)]),
     "dmw_batch_destroy": (C.c_int, [C.c_void_p]),
     "dmw_traj_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(Geometry_t), C.POINTER(Run_t),
@@ -151,6 +158,16 @@ def lib():
 
 def device_count():
     return int(lib().dmw_device_count())
+
+
+def device_arch(device=0):
+    """The device's architecture name with its feature flags sanitised for a
+    file name ("gfx950_sramecc+_xnack-"), or None without a device."""
+    if device_count() <= device:
+        return None
+    buf = C.create_string_buffer(128)
+    check(lib().dmw_device_arch(int(device), buf, len(buf)))
+    return buf.value.decode().replace(":", "_") or None
 
 
 def check(rc):

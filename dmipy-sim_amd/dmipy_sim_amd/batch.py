@@ -27,9 +27,16 @@ class BatchedWalk:
     default 0), 'stream' (Philox instance word, default the instance index) and
     'seed_offset' (added to the run seed, default 0).  ``waveforms`` is one
     Waveform or a list of them.
+
+    Launch shape (results do not depend on it): ``waves_per_simd`` persistent
+    wavefronts per SIMD (0: 4), ``walkers_per_wave`` walkers a wavefront takes
+    per queue pop (64, 128, 256, 512; 0: 64), ``unroll`` of the sub-step loop
+    (1, 2; 0: 1).  ``tuning="auto"`` takes the autotuner's cached choice for
+    this batch's key (see autotune.tune_batch), or the defaults if none.
     """
 
-    def __init__(self, instances, waveforms, waves_per_simd=0, use_graph=True):
+    def __init__(self, instances, waveforms, waves_per_simd=0, walkers_per_wave=0, unroll=0,
+                 use_graph=True, tuning=None):
         if not isinstance(waveforms, (list, tuple)):
             waveforms = [waveforms]
         if len(instances) < 1 or len(waveforms) < 1:
@@ -57,6 +64,7 @@ class BatchedWalk:
         self._half = np.ones(len(instances), np.float64)
         set_key = {}
         sets_G = []
+        kind_walls = set()
         self._keep = []          # host arrays referenced by the descriptors
         for i, spec in enumerate(instances):
             k = int(spec.get('waveform', 0))
@@ -84,6 +92,7 @@ class BatchedWalk:
                                     seed_offset=int(spec.get('seed_offset', 0)) & (2**64 - 1))
             self.n_sub[i] = plan.n_sub
             self.n_walkers[i] = int(spec['n_walkers'])
+            kind_walls.add((gd.kind, gd.wall))
             if getattr(wf, 'stimulated_echo', False):
                 self._half[i] = 0.5
         self._chi = None if chi0 is None else np.ascontiguousarray(chi0)
@@ -92,9 +101,18 @@ class BatchedWalk:
         for s, G in enumerate(sets_G):
             wds[s] = nat.Waveform_t(n_meas=n_meas, n_t=n_t, G=nat.fptr(G),
                                     chi=nat.fptr(self._chi), n_echo=0, echo_idx=None)
+        from . import autotune
+        self.key = autotune.batch_key(kind_walls, self.n_sub, n_meas)
+        if tuning == "auto":
+            waves_per_simd, walkers_per_wave, unroll = autotune.cached_batch_shape(self.key)
+        elif tuning is not None:
+            raise ValueError("tuning must be None or 'auto'")
+        self.shape = dict(waves_per_simd=int(waves_per_simd), walkers_per_wave=int(walkers_per_wave),
+                          unroll=int(unroll))
+        tn = nat.BatchTuning_t(**self.shape)
         h = C.c_void_p()
         nat.check(nat.lib().dmw_batch_create(C.byref(h), len(instances), arr, len(sets_G), wds,
-                                             int(waves_per_simd), int(bool(use_graph))))
+                                             C.byref(tn), int(bool(use_graph))))
         self._h = h
         self._keep = None
         self.n_inst = len(instances)

@@ -202,6 +202,11 @@ int dmw_device_count(void);
 /* Select the device used by this thread's calls (default 0). */
 int dmw_set_device(int device);
 
+/* Architecture name of a device ("gfx950:sramecc+:xnack-", ...) into buf (len
+ * bytes, NUL-terminated, truncated if short).  No counterpart in the reference:
+ * keys the autotuner's per-architecture cache. */
+int dmw_device_arch(int32_t device, char* buf, int32_t len);
+
 /* Message of the last failure on the calling thread. */
 const char* dmw_last_error(void);
 
@@ -288,10 +293,19 @@ typedef struct dmw_batch_stats {
   int32_t grid;              /* persistent workgroups */
 } dmw_batch_stats;
 
-/* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored).
- * waves_per_simd: persistent wavefronts per SIMD (0 = 4, the kernel's occupancy budget). */
+/* Launch shape of the batched walk; zeros (or a NULL pointer) take the defaults.
+ * The results do not depend on it: every tile of 64 walkers walks its own
+ * Philox streams and writes its own partial row, reduced in a fixed order. */
+typedef struct dmw_batch_tuning {
+  int32_t waves_per_simd;    /* persistent wavefronts per SIMD, 1..8 (0 = 4, the kernel's occupancy budget) */
+  int32_t walkers_per_wave;  /* walkers a wavefront takes per queue pop, walked as consecutive
+                                64-walker tiles: 64, 128, 256 or 512 (0 = 64) */
+  int32_t unroll;            /* sub-step loop unrolling: 1 or 2 (0 = 1) */
+} dmw_batch_tuning;
+
+/* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored). */
 int dmw_batch_create(dmw_batch** batch, int32_t n_inst, const dmw_instance* inst,
-                     int32_t n_sets, const dmw_waveform* sets, int32_t waves_per_simd,
+                     int32_t n_sets, const dmw_waveform* sets, const dmw_batch_tuning* tuning,
                      int32_t use_graph);
 /* signals: (n_inst, n_meas) Re <w e^{i phi}> per instance. */
 int dmw_batch_execute(dmw_batch* batch, uint64_t seed, double* signals,

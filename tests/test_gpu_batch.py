@@ -142,3 +142,47 @@ def test_simulate_mixture_with_myelinated_compartment(gpu):
     assert mix.dtype == np.float32 and mix.shape == (wf.G.shape[0],)
     np.testing.assert_array_equal(mix, ref)
     assert np.all(mix[wf.G.reshape(wf.G.shape[0], -1).any(1)] < 1.0)
+
+
+def test_launch_shapes_give_identical_signals(gpu, oracle):
+    """Every launch shape (persistent waves per SIMD, walkers per wavefront pop,
+    sub-step unrolling) walks the same tiles with the same Philox streams and
+    writes the same partial rows: the signals are bit-identical, and the
+    unrolled kernel still reproduces the oracle."""
+    wfs, specs = _instances()
+    ref = None
+    for shape in ((0, 0, 0), (4, 512, 2), (2, 128, 1), (1, 256, 2), (3, 64, 1)):
+        print(f"batch launch shape {shape}", flush=True)
+        bw = gpu.BatchedWalk(specs, wfs, waves_per_simd=shape[0], walkers_per_wave=shape[1],
+                             unroll=shape[2])
+        sig, _ = bw.run(31)
+        bw.close()
+        if ref is None:
+            ref = sig
+        assert np.array_equal(sig, ref), shape
+    s = specs[3]
+    o = oracle.run_oracle(s["geometry"], s["diffusivity"], wfs[s["waveform"]], s["n_walkers"],
+                          seed=31, T2=s["T2"], instance=3)
+    assert np.max(np.abs(ref[3] - o["signal"])) <= 1e-5
+
+
+def test_autotuner_times_every_knob_and_caches(gpu, tmp_path, monkeypatch):
+    """tune_batch times the three knobs by coordinate descent on the batch and
+    keeps the fastest under the batch's key; tuning='auto' then takes it."""
+    from dmipy_sim_amd import autotune
+    monkeypatch.setenv("DMIPY_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    autotune.clear()
+    autotune._loaded["done"] = False
+    wfs, specs = _instances()
+    best, times = autotune.tune_batch(specs, wfs, walker_scale=1.0, reps=1,
+                                      log=lambda s: print(s, flush=True))
+    assert times[best] == min(times.values())
+    seen = {k[0] for k in times}, {k[1] for k in times}, {k[2] for k in times}
+    assert seen[0] == set(autotune.BATCH_WAVES_PER_SIMD)
+    assert seen[1] == set(autotune.BATCH_WALKERS_PER_WAVE)
+    assert seen[2] == set(autotune.BATCH_UNROLL)
+    assert (tmp_path / "tune.json").exists()
+    bw = gpu.BatchedWalk(specs, wfs, tuning="auto")
+    assert (bw.shape["waves_per_simd"], bw.shape["walkers_per_wave"], bw.shape["unroll"]) == best
+    bw.close()
+    autotune.clear()

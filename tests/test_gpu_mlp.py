@@ -115,24 +115,42 @@ def test_batch_validation(gpu):
     m.close()
 
 
-def test_large_tile_inference_gemm(gpu):
-    """Inference batches of 128-row tiles take the 128 x 128-tile GEMM (transposed bf16
-    weights, XCD-ordered tiles); it runs the same MFMA sequence and epilogue per
-    output as the 64 x 64 kernel, so rows agree bit for bit with a batch that takes
-    the small-tile path (4032 rows is not a multiple of 128), and the network still
-    matches the oracle at the width and depth of the surrogate."""
-    print("mlp large-tile inference 70->512x8, batch 4096", flush=True)
-    m = _net(gpu, seed=3, max_batch=4096, width=512, depth=8)
+@pytest.mark.parametrize("width", [512, 256])
+def test_large_tile_inference_gemm(gpu, width):
+    """Inference batches of 128-row strips take the layer-fused kernel at width 512
+    (activations kept in LDS through all layers) and the 128 x 128-tile GEMM at
+    other widths (transposed bf16 weights, XCD-ordered tiles); both run the same
+    MFMA sequence and epilogue per output as the 64 x 64 kernel, so rows agree bit
+    for bit with a batch that takes the small-tile path (4032 rows is not a
+    multiple of 128), and the network still matches the oracle at the depth of the
+    surrogate."""
+    print(f"mlp large-tile inference 70->{width}x8, batch 4096", flush=True)
+    m = _net(gpu, seed=3, max_batch=4096, width=width, depth=8)
     p = m.get_params()
     x, _ = _data(4096, 11)
     y_big = m.forward(x)
     y_small = m.forward(x[:4032])
     np.testing.assert_array_equal(y_big[:4032], y_small)
-    # eight bf16 layers of 512: an accumulation-order difference that flips one bf16
+    # eight bf16 layers: an accumulation-order difference that flips one bf16
     # rounding propagates, so the per-row bar is wider than for three layers of 128
     # (a wrong transpose or activation is still off by O(1))
-    ref = mo.forward(p, x[:512], N_IN, 512, 8, bf16=True)
+    ref = mo.forward(p, x[:512], N_IN, width, 8, bf16=True)
     scale = max(1.0, float(np.max(np.abs(ref))))
     assert np.max(np.abs(y_big[:512] - ref)) <= 1.5e-2 * scale
     assert np.sqrt(np.mean((y_big[:512] - ref) ** 2)) <= 1.5e-3 * scale
+    m.close()
+
+
+def test_fused_inference_on_resident_rows(gpu):
+    """predict() over resident rows in strips of 128 (the fused kernel, the last
+    strip padded by repetition) equals forward() in 64-row batches (the
+    per-layer path) row for row."""
+    print("mlp fused predict 70->512x8, 1000 rows", flush=True)
+    m = _net(gpu, seed=5, max_batch=512, width=512, depth=8)
+    x, t = _data(1000, 13)
+    m.set_data(x, t)
+    y, ms = m.predict()
+    assert ms > 0
+    y64 = np.concatenate([m.forward(x[i:i + 64]) for i in range(0, 1000, 64)])
+    np.testing.assert_array_equal(y, y64)
     m.close()

@@ -285,16 +285,68 @@ def test_batch_argument_errors_before_gpu():
                   dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
         kw.update(bad)
         inst[0] = _native.Instance_t(**kw)
-        assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EINVAL
+        assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, None, 1) == _native.DMW_EINVAL
     gd.kind = 4       # packed without objects
     inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7,
                                  gamma_dt_sub=1.0, dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
-    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EINVAL
+    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, None, 1) == _native.DMW_EINVAL
     gd.kind = 0
     gd.wall = 1       # free diffusion has no wall to relax on
     inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7,
                                  gamma_dt_sub=1.0, dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
-    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, 0, 1) == _native.DMW_EUNSUPPORTED
+    assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, None, 1) == _native.DMW_EUNSUPPORTED
+
+
+def test_batch_tuning_checked_before_gpu():
+    """dmw_batch_create refuses launch shapes outside the compiled set before
+    touching a device."""
+    import ctypes as C
+    wf = ds.pgse(0.01, 0.04, 0.05, [[1, 0, 0]], 20)
+    G = np.ascontiguousarray(wf.G, np.float32)
+    wd = (_native.Waveform_t * 1)()
+    wd[0] = _native.Waveform_t(n_meas=1, n_t=20, G=_native.fptr(G), chi=None, n_echo=0, echo_idx=None)
+    gd = _native.Geometry_t(kind=0, wall=0, size=0.0, eps=0.0, nudge=0.0, rho_over_D=0.0,
+                            kappa_over_D=0.0, n_obj=0, cell=0.0, centers=None, radii=None)
+    inst = (_native.Instance_t * 1)()
+    inst[0] = _native.Instance_t(geom=gd, n_walkers=64, n_sub=1, step_len=1e-7, gamma_dt_sub=1.0,
+                                 dt_sub=1e-5, inv_T2=0.0, has_t2=0, wave_set=0)
+    lib = _native.lib()
+    h = C.c_void_p()
+    for bad, text in ((dict(walkers_per_wave=100), "walkers_per_wave"),
+                      (dict(walkers_per_wave=1024), "walkers_per_wave"),
+                      (dict(unroll=3), "unroll"), (dict(waves_per_simd=9), "waves_per_simd"),
+                      (dict(waves_per_simd=-1), "waves_per_simd")):
+        tn = _native.BatchTuning_t(**bad)
+        assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, C.byref(tn), 1) == _native.DMW_EINVAL
+        assert text in lib.dmw_last_error().decode()
+    with pytest.raises(ValueError, match="tuning"):
+        ds.BatchedWalk([dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=64)], wf,
+                       tuning="fastest")
+
+
+def test_batch_autotune_key_and_persisted_cache(tmp_path, monkeypatch):
+    """The batch tuner's cache: keyed by geometry/wall mix, measurement block and
+    n_sub class; defaults without an entry; written to and read back from the
+    per-architecture file ($DMIPY_AUTOTUNE_CACHE overrides its location)."""
+    from dmipy_sim_amd import autotune
+    k1 = autotune.batch_key([(3, 1), (2, 2), (3, 1)], [10, 17, 30], 64)
+    assert k1 == autotune.batch_key([(2, 2), (3, 1)], [17, 17], 50)
+    assert k1 != autotune.batch_key([(2, 2), (3, 1)], [17], 16)
+    assert k1 != autotune.batch_key([(2, 2)], [17], 64)
+    assert k1 != autotune.batch_key([(2, 2), (3, 1)], [500], 64)
+    path = tmp_path / "cache" / "tune.json"
+    monkeypatch.setenv("DMIPY_AUTOTUNE_CACHE", str(path))
+    assert autotune.default_path() == str(path)
+    autotune.clear()
+    autotune._loaded["done"] = False
+    assert autotune.cached_batch_shape(k1) == autotune.BATCH_DEFAULT
+    autotune._cache[k1] = {"shape": [3, 256, 2], "ms": {"4,64,1": 2.0, "3,256,2": 1.0}}
+    autotune._save_default()
+    autotune.clear()
+    assert autotune.cached_batch_shape(k1) == autotune.BATCH_DEFAULT
+    autotune._loaded["done"] = False          # a new process: the file is read on first use
+    assert autotune.cached_batch_shape(k1) == (3, 256, 2)
+    autotune.clear()
 
 
 def test_autotune_candidates_and_cache(tmp_path):
