@@ -217,10 +217,11 @@ __global__ __launch_bounds__(256) void gemm_fwd_big(int M, int N, int K, const _
 }
 
 // Layer-fused inference for width 512: one workgroup (8 wavefronts) per strip of
-// FR = 128 rows keeps the strip's activations in LDS (128 x 512 bf16, 130 KB)
-// through every hidden layer and the output layer, so activations never touch
+// ROWS = 64 or 128 rows keeps the strip's activations in LDS (ROWS x 520 bf16:
+// 65 or 130 KB, so two or one workgroups per CU) through every hidden layer and
+// the output layer, so activations never touch
 // HBM; only the input strip is read and one float per row written.  Per layer,
-// wavefront w computes features [64w, 64w + 64) of all 128 rows as the
+// wavefront w computes features [64w, 64w + 64) of all ROWS rows as the
 // transposed product Y^T = W^T X^T: A operands are rows of the transposed
 // weights W^T (streamed from L2, each wavefront's next K step prefetched into
 // registers), B operands are activation rows read from LDS, and each lane's
@@ -229,7 +230,12 @@ __global__ __launch_bounds__(256) void gemm_fwd_big(int M, int N, int K, const _
 // input).  Per output the K steps of 32 run in ascending order and the epilogue
 // bf16(GELU(acc + b)) and the output-layer dot product (per lane strided, then a
 // butterfly) are those of gemm_fwd_big / out_fwd_kernel.
-constexpr int FR = 128, FH = 512, FLD = FH + 8, FMAXL = 16;
+constexpr int FH = 512, FLD = FH + 8, FMAXL = 16;
+// Strip rows of the fused inference kernel: 64 (two workgroups per CU) or 128 (one).
+#ifndef DMW_FUSED_ROWS
+#define DMW_FUSED_ROWS 128
+#endif
+static_assert(DMW_FUSED_ROWS == 0 || DMW_FUSED_ROWS == 64 || DMW_FUSED_ROWS == 128, "fused rows: 0, 64 or 128");
 
 struct FusedArgs {
   int depth, k0;
@@ -241,13 +247,16 @@ struct FusedArgs {
   float* y;
 };
 
-__global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
-  __shared__ __attribute__((aligned(16))) __bf16 X[FR][FLD];
+template <int ROWS>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(ROWS == 64 ? 4 : 2)))
+void mlp_infer_fused(const FusedArgs a) {
+  constexpr int RT = ROWS / 16;        // row tiles of 16
+  __shared__ __attribute__((aligned(16))) __bf16 X[ROWS][FLD];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const long long r0 = (long long)blockIdx.x * FR;
+  const long long r0 = (long long)blockIdx.x * ROWS;
   {
     const int cpr = a.k0 / 8;    // 16-byte chunks per input row
-    for (int c = tid; c < FR * cpr; c += 512) {
+    for (int c = tid; c < ROWS * cpr; c += 512) {
       const int r = c / cpr, q = (c - r * cpr) * 8;
       *reinterpret_cast<bf16x8*>(&X[r][q]) = *reinterpret_cast<const bf16x8*>(a.X0 + (r0 + r) * a.k0 + q);
     }
@@ -258,11 +267,11 @@ __global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
   for (int l = 0; l < a.depth; ++l) {
     const int K = l == 0 ? a.k0 : FH;
     const __bf16* W = a.Wt + a.w_off[l];
-    f32x4 acc[4][8];
+    f32x4 acc[4][RT];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      for (int j = 0; j < RT; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const __bf16* wp[4];
     bf16x8 wf[4], wn[4];
 #pragma unroll
@@ -276,13 +285,13 @@ __global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
 #pragma unroll
         for (int i = 0; i < 4; ++i) wn[i] = *reinterpret_cast<const bf16x8*>(wp[i] + k + 32);
       }
-      bf16x8 xf[8];
+      bf16x8 xf[RT];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) xf[j] = *reinterpret_cast<const bf16x8*>(&X[16 * j + rq][k + kq]);
+      for (int j = 0; j < RT; ++j) xf[j] = *reinterpret_cast<const bf16x8*>(&X[16 * j + rq][k + kq]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 8; ++j)
+        for (int j = 0; j < RT; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wf[i], xf[j], acc[i][j], 0, 0, 0);
       if (more) {
 #pragma unroll
@@ -296,7 +305,7 @@ __global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
       const int f = f0 + 16 * i + 4 * (lane >> 4);
       const float b0 = bias[f], b1 = bias[f + 1], b2 = bias[f + 2], b3 = bias[f + 3];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < RT; ++j) {
         typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
         const bf16x4 v = {f2bf(gelu(acc[i][j][0] + b0)), f2bf(gelu(acc[i][j][1] + b1)),
                           f2bf(gelu(acc[i][j][2] + b2)), f2bf(gelu(acc[i][j][3] + b3))};
@@ -310,8 +319,8 @@ __global__ __launch_bounds__(512) void mlp_infer_fused(const FusedArgs a) {
 #pragma unroll
   for (int q = 0; q < FH / 64; ++q) w8[q] = a.P[a.wout_off + lane + 64 * q];
   const float bo = a.P[a.bout_off];
-  for (int rr = 0; rr < FR / 8; ++rr) {
-    const int row = wv * (FR / 8) + rr;
+  for (int rr = 0; rr < ROWS / 8; ++rr) {
+    const int row = wv * (ROWS / 8) + rr;
     float s = 0.0f;
 #pragma unroll
     for (int q = 0; q < FH / 64; ++q) s += bf2f(X[row][lane + 64 * q]) * w8[q];
@@ -453,7 +462,7 @@ __global__ void f32_to_bf16_kernel(long long n, const float* __restrict__ a, __b
 // ---------------------------------------------------------------------------
 struct dmw_mlp {
   int n_in = 0, k0 = 0, H = 0, depth = 0, max_batch = 0;
-  bool fused = true;               // inference through the layer-fused kernel where it applies
+  int fused_rows = DMW_FUSED_ROWS;  // rows per strip of the layer-fused inference kernel (0: per-layer kernels)
   long long n_params = 0;          // internal (padded) parameter count
   std::vector<long long> w_off, b_off;  // offsets of W_l, b_l in the flat fp32 vector
   long long wout_off = 0, bout_off = 0;
@@ -521,7 +530,8 @@ unsigned blocks(long long n, int t) { return (unsigned)((n + t - 1) / t); }
 // are not kept, and batches of 128-row tiles take the large-tile GEMM.
 int forward(dmw_mlp* m, int B, const float* targets_dev) {
   hipStream_t st = m->st;
-  if (targets_dev == nullptr && m->fused && B % dmw::FR == 0 && m->H == dmw::FH && m->depth <= dmw::FMAXL &&
+  const int fr = m->fused_rows;
+  if (targets_dev == nullptr && fr > 0 && B % fr == 0 && m->H == dmw::FH && m->depth <= dmw::FMAXL &&
       m->k0 <= dmw::FH) {
     dmw::FusedArgs fa{};
     fa.depth = m->depth;
@@ -536,7 +546,10 @@ int forward(dmw_mlp* m, int B, const float* targets_dev) {
     fa.wout_off = m->wout_off;
     fa.bout_off = m->bout_off;
     fa.y = m->out;
-    hipLaunchKernelGGL(dmw::mlp_infer_fused, dim3(B / dmw::FR), dim3(512), 0, st, fa);
+    if (fr == 64)
+      hipLaunchKernelGGL(dmw::mlp_infer_fused<64>, dim3(B / 64), dim3(512), 0, st, fa);
+    else
+      hipLaunchKernelGGL(dmw::mlp_infer_fused<128>, dim3(B / 128), dim3(512), 0, st, fa);
     MLP_TRY(hipGetLastError(), "mlp fused inference");
     return DMW_OK;
   }

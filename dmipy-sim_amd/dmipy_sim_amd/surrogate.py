@@ -14,13 +14,19 @@ Waveform embedding.  With q(t) = gamma * int_0^t G (accumulated in float64) and
 Q_k its discrete Fourier coefficients, the b-value splits over frequencies,
 b = sum_k b_k with b_k = 2 |Q_k|^2 dt / n_t (Parseval; k = 0 and Nyquist once).
 In the Gaussian-phase picture ln S = -sum_k b_k D(omega_k) with D(omega) the
-diffusion spectrum of the substrate, so the per-frequency b_k of the transverse
-(x, y) and axial (z) components are the natural inputs.  The embedding is
-31 transverse bins + 31 axial bins (the first 24 frequencies one by one, the
-rest in 7 geometric groups) + the transverse and axial b-values, in units of
-1e9 s/m^2.
+diffusion spectrum of the substrate.  The cylinders are parallel to z and the
+motion along z is free, so only the total axial b-value matters, and the
+transverse (x, y) spectrum carries the restriction: the embedding is 56
+transverse bins (the first 40 frequencies one by one, the rest in 16 geometric
+groups), the transverse and axial b-values, and six shape descriptors of the
+transverse encoding beyond its spectrum: <|q_perp|^4> / <|q_perp|^2>^2 and
+max |q_perp|^2 / <|q_perp|^2> (the non-Gaussian terms of restricted diffusion
+weigh the fourth moment of q), the x and y b-values, and the b-weighted mean
+and root-mean-square frequency of the transverse spectrum.  b-values are in
+units of 1e9 s/m^2.
 """
 import ctypes as C
+import os
 import time
 
 import numpy as np
@@ -36,9 +42,14 @@ N_IN = N_SUBSTRATE + N_WAVE
 
 # Prior over substrate instances of the training generator: the packed-cylinder
 # family (64 cylinders with Gamma-distributed radii, densely packed, permeable)
-# around 64 cylinders of mean radius 3 um, shape 4, volume fraction 0.6.
-PRIOR = dict(mean_r=(1.5e-6, 3.5e-6), shape=(2.0, 8.0), vf=(0.3, 0.65),
-             kappa=(1e-6, 1e-4), rho=(1e-7, 1e-5), D=(1.0e-9, 2.5e-9))
+# around the workload of 64 cylinders of mean radius 3 um, shape 4, volume
+# fraction 0.6, permeability 1e-5 m/s, no surface relaxivity, D = 2e-9 m^2/s.
+# kappa and rho are log-uniform; a fraction RHO_ZERO of the instances has no
+# surface relaxivity at all.
+PRIOR = dict(mean_r=(2.0e-6, 4.0e-6), shape=(3.0, 6.0), vf=(0.45, 0.65),
+             kappa=(2e-6, 5e-5), rho=(1e-7, 3e-6), D=(1.5e-9, 2.5e-9))
+RHO_ZERO = 0.25
+RHO_FLOOR = 3e-8    # log10 of max(rho, RHO_FLOOR) is the network's relaxivity input
 # Radii are clipped here.  The permeable sub-step rule (R_min / 25) makes a walk's
 # cost grow as 1 / R_min^2; at 1 um the mean generator substrate costs 3.2e5
 # sub-steps per walker over 50 ms, unclipped draws of shape 2 reach 1e7.
@@ -174,15 +185,16 @@ class MLP:
 # Inputs: substrate parameters and the waveform embedding.
 # ---------------------------------------------------------------------------
 def _bins(n_freq):
-    """Index groups of the embedding: 24 single frequencies, then 7 geometric groups."""
-    edges = [np.array([k]) for k in range(min(24, n_freq))]
-    if n_freq > 24:
-        cuts = np.unique(np.round(np.geomspace(24, n_freq, 8)).astype(int))
+    """Index groups of the transverse spectrum: 40 single frequencies, then 16
+    geometric groups up to Nyquist."""
+    edges = [np.array([k]) for k in range(min(40, n_freq))]
+    if n_freq > 40:
+        cuts = np.unique(np.round(np.geomspace(40, n_freq, 17)).astype(int))
         for a, b in zip(cuts[:-1], cuts[1:]):
             edges.append(np.arange(a, b))
-    while len(edges) < 31:
+    while len(edges) < 56:
         edges.append(np.array([], dtype=int))
-    return edges[:31]
+    return edges[:56]
 
 
 def waveform_features(waveform):
@@ -198,28 +210,39 @@ def waveform_features(waveform):
         w[-1] = 1.0
     bk = (np.abs(Q) ** 2) * (w * dt / n_t)[None, :, None]     # (n_meas, n_freq, 3), sums to b
     perp = bk[:, :, 0] + bk[:, :, 1]
-    axial = bk[:, :, 2]
-    groups = _bins(Q.shape[1])
     out = np.zeros((G.shape[0], N_WAVE))
-    for j, idx in enumerate(groups):
+    for j, idx in enumerate(_bins(Q.shape[1])):
         if idx.size:
-            out[:, j] = perp[:, idx].sum(axis=1)
-            out[:, 31 + j] = axial[:, idx].sum(axis=1)
-    out[:, 62] = perp.sum(axis=1)
-    out[:, 63] = axial.sum(axis=1)
-    return out * 1e-9
+            out[:, j] = perp[:, idx].sum(axis=1) * 1e-9
+    b_perp = perp.sum(axis=1)
+    out[:, 56] = b_perp * 1e-9
+    out[:, 57] = bk[:, :, 2].sum(axis=1) * 1e-9
+    q2 = q[:, :, 0] ** 2 + q[:, :, 1] ** 2                     # |q_perp(t)|^2
+    m2 = q2.mean(axis=1)
+    has = m2 > 0
+    safe = np.where(has, m2, 1.0)
+    out[:, 58] = np.where(has, (q2 ** 2).mean(axis=1) / safe ** 2, 0.0)
+    out[:, 59] = np.where(has, q2.max(axis=1) / safe, 0.0)
+    out[:, 60] = bk[:, :, 0].sum(axis=1) * 1e-9
+    out[:, 61] = bk[:, :, 1].sum(axis=1) * 1e-9
+    k = np.arange(Q.shape[1], dtype=np.float64)
+    bs = np.where(b_perp > 0, b_perp, 1.0)
+    out[:, 62] = np.where(b_perp > 0, (perp * k).sum(axis=1) / bs, 0.0) / 40.0
+    out[:, 63] = np.where(b_perp > 0, np.sqrt((perp * k * k).sum(axis=1) / bs), 0.0) / 40.0
+    return out
 
 
 def substrate_features(p):
     """Six normalised inputs from a parameter dict: mean radius, Gamma shape
-    (log), volume fraction, log10 permeability, log10 relaxivity, diffusivity."""
+    (log), volume fraction, log10 permeability, log10 relaxivity (floored at
+    RHO_FLOOR, so rho = 0 is an input like any other), diffusivity."""
     return np.array([
-        p["mean_r"] / 1e-6 - 2.5,
-        np.clip((np.log(p["shape"]) - np.log(4.5)) / 0.7, -3.0, 3.0),
-        (p["vf"] - 0.475) / 0.175,
-        (np.log10(p["kappa"]) + 5.0),
-        (np.log10(p["rho"]) + 6.0),
-        p["D"] / 1e-9 - 1.75,
+        p["mean_r"] / 1e-6 - 3.0,
+        np.clip((np.log(p["shape"]) - np.log(4.2)) / 0.35, -4.0, 4.0),
+        (p["vf"] - 0.55) / 0.1,
+        (np.log10(p["kappa"]) + 5.0) * 1.5,
+        (np.log10(max(p["rho"], RHO_FLOOR)) + 6.5) / 1.5,
+        p["D"] / 1e-9 - 2.0,
     ])
 
 
@@ -249,9 +272,12 @@ def features(params, waveform):
 def sample_params(rng):
     lo, hi = PRIOR["kappa"]
     rlo, rhi = PRIOR["rho"]
-    return dict(mean_r=rng.uniform(*PRIOR["mean_r"]), shape=rng.uniform(*PRIOR["shape"]),
-                vf=rng.uniform(*PRIOR["vf"]), kappa=float(np.exp(rng.uniform(np.log(lo), np.log(hi)))),
-                rho=float(np.exp(rng.uniform(np.log(rlo), np.log(rhi)))), D=rng.uniform(*PRIOR["D"]))
+    p = dict(mean_r=rng.uniform(*PRIOR["mean_r"]), shape=rng.uniform(*PRIOR["shape"]),
+             vf=rng.uniform(*PRIOR["vf"]), kappa=float(np.exp(rng.uniform(np.log(lo), np.log(hi)))),
+             rho=float(np.exp(rng.uniform(np.log(rlo), np.log(rhi)))), D=rng.uniform(*PRIOR["D"]))
+    if rng.uniform() < RHO_ZERO:
+        p["rho"] = 0.0
+    return p
 
 
 def build_substrate(p, seed, n_cyl=N_CYL):
@@ -260,7 +286,8 @@ def build_substrate(p, seed, n_cyl=N_CYL):
     rng = np.random.default_rng(seed)
     radii = np.maximum(rng.gamma(p["shape"], p["mean_r"] / p["shape"], n_cyl), R_MIN)
     centers, L, _ = geo.pack_dense(radii, p["vf"], seed=seed, dim=2, gap=0.1)
-    return geo.PackedCylinders(radii, centers, L, surface_relaxivity_t2=p["rho"],
+    return geo.PackedCylinders(radii, centers, L,
+                               surface_relaxivity_t2=p["rho"] if p["rho"] > 0 else None,
                                permeability=p["kappa"])
 
 
@@ -317,40 +344,82 @@ def walk_signals(instances, waveforms, n_walkers, seed):
     return sig.reshape(len(instances), -1)[:, :n_meas], st
 
 
+def _build_instance(args):
+    """One generator instance: substrate, realized parameters, waveforms and
+    network inputs (run in the host worker pool)."""
+    p, sub_seed, wave_seed, n_meas, n_t, T = args
+    geom = build_substrate(p, seed=sub_seed)
+    p = realized_params(p, geom)
+    wf = random_waveforms(np.random.default_rng(wave_seed), n_meas=n_meas, n_t=n_t, T=T)
+    return geom, p, wf, features(p, wf)
+
+
+def host_workers():
+    """Host processes for the generator: the cores this process may run on,
+    at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def generate(n_instances, n_walkers, seed, n_meas=N_MEAS, n_t=N_T, T=T_WAVE, chunk=256,
-             walk_seed=None, log=None):
+             walk_seed=None, log=None, keep_instances=True, workers=None):
     """Batched-MC dataset: (X (n*n_meas, 70) float32, S (n*n_meas) float64, info).
 
     Every instance gets its own substrate (parameters from PRIOR) and its own
     n_meas random waveforms; instances run ``chunk`` at a time in one batched
-    persistent launch each.
+    persistent launch each.  The host side of an instance (packing, waveforms,
+    inputs) is built in a pool of ``workers`` processes (default host_workers()),
+    each instance from its own seeds drawn in order from ``seed``, so the data
+    set does not depend on the pool.  The pool starts fresh interpreters
+    (multiprocessing's "spawn"), which import the caller's main module: a script
+    calling this needs the usual ``if __name__ == "__main__":`` guard, or
+    ``workers=1``.  ``info["instances"]`` holds the (instance, waveform) pairs
+    only with ``keep_instances``.
     """
+    import concurrent.futures as cf
+    import multiprocessing as mp
     rng = np.random.default_rng(seed)
+    todo = []
+    for _ in range(n_instances):
+        p = sample_params(rng)
+        todo.append((p, int(rng.integers(2**31)), int(rng.integers(2**63)), n_meas, n_t, T))
+    workers = host_workers() if workers is None else int(workers)
+    pool = None
+    if workers > 1 and n_instances > 8:
+        # a fresh interpreter per worker: no state of this process (its GPU
+        # context included) is inherited
+        pool = cf.ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn"))
     X, S = [], []
     gpu_ms = 0.0
     walker_steps = 0
     params = []
     instances = []
-    for c0 in range(0, n_instances, chunk):
-        insts, waves = [], []
-        for i in range(c0, min(n_instances, c0 + chunk)):
-            p = sample_params(rng)
-            geom = build_substrate(p, seed=int(rng.integers(2**31)))
-            p = realized_params(p, geom)
-            wf = random_waveforms(rng, n_meas=n_meas, n_t=n_t, T=T)
-            insts.append(dict(geometry=geom, diffusivity=p["D"]))
-            waves.append(wf)
-            params.append(p)
-            instances.append((insts[-1], wf))
-            X.append(features(p, wf))
-        sig, st = walk_signals(insts, waves, n_walkers,
-                               seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
-        S.append(sig.reshape(-1))
-        gpu_ms += st["total_ms"]
-        walker_steps += st["walker_steps"]
-        if log:
-            log(f"generate: {min(n_instances, c0 + chunk)}/{n_instances} instances, "
-                f"{st['total_ms']:.0f} ms")
+    try:
+        for c0 in range(0, n_instances, chunk):
+            part = todo[c0:c0 + chunk]
+            built = (list(pool.map(_build_instance, part, chunksize=max(1, len(part) // (4 * workers))))
+                     if pool is not None else [_build_instance(a) for a in part])
+            insts = [dict(geometry=g, diffusivity=p["D"]) for g, p, _, _ in built]
+            waves = [wf for _, _, wf, _ in built]
+            for (g, p, wf, x), inst in zip(built, insts):
+                params.append(p)
+                X.append(x)
+                if keep_instances:
+                    instances.append((inst, wf))
+            sig, st = walk_signals(insts, waves, n_walkers,
+                                   seed=(seed * 7919 + c0) if walk_seed is None else walk_seed + c0)
+            S.append(sig.reshape(-1))
+            gpu_ms += st["total_ms"]
+            walker_steps += st["walker_steps"]
+            if log:
+                log(f"generate: {min(n_instances, c0 + chunk)}/{n_instances} instances, "
+                    f"{st['total_ms']:.0f} ms")
+    finally:
+        if pool is not None:
+            pool.shutdown()
     info = dict(gpu_ms=gpu_ms, walker_steps=walker_steps, params=params, instances=instances,
                 signals=n_instances * n_meas, n_walkers=int(n_walkers))
     return np.concatenate(X), np.concatenate(S), info
@@ -427,11 +496,12 @@ class SurrogateEnsemble:
 
 
 def instance_params(geometry, diffusivity):
-    """Network parameters of a PackedCylinders instance (realized statistics)."""
-    if geometry.permeability is None or geometry.surface_relaxivity_t2 is None:
-        raise ValueError("the surrogate covers PackedCylinders with permeability and "
-                         "surface relaxivity set")
-    p = dict(kappa=float(geometry.permeability), rho=float(geometry.surface_relaxivity_t2),
+    """Network parameters of a PackedCylinders instance (realized statistics);
+    no surface relaxivity counts as rho = 0."""
+    if geometry.permeability is None:
+        raise ValueError("the surrogate covers permeable PackedCylinders")
+    rho = geometry.surface_relaxivity_t2
+    p = dict(kappa=float(geometry.permeability), rho=0.0 if rho is None else float(rho),
              D=float(diffusivity))
     return realized_params(p, geometry)
 
@@ -441,8 +511,8 @@ def predict_with_fallback(ensemble, instances, waveforms, threshold=5e-3, n_walk
     """Signals for every (instance, measurement): the ensemble mean where its
     spread is at most ``threshold``, else the batched Monte-Carlo walk.
 
-    ``instances`` are dicts with 'geometry' (PackedCylinders with permeability
-    and surface relaxivity) and 'diffusivity'; ``waveforms[i]`` is instance i's
+    ``instances`` are dicts with 'geometry' (PackedCylinders with permeability)
+    and 'diffusivity'; ``waveforms[i]`` is instance i's
     Waveform (all with the same number of measurements and n_t).  An instance
     with any flagged measurement is walked with ``n_walkers`` walkers and its
     flagged measurements are replaced.  Returns (signals (n_inst, n_meas),
@@ -476,7 +546,8 @@ def mc_ladder(n_inst, seed, ref_signals, walkers=(256, 512, 1024, 2048, 4096, 81
     RMSE."""
     rows = []
     for nw in walkers:
-        _, S, info = generate(n_inst, nw, seed=seed, chunk=n_inst, walk_seed=5000 + nw)
+        _, S, info = generate(n_inst, nw, seed=seed, chunk=n_inst, walk_seed=5000 + nw,
+                              keep_instances=False)
         rmse = float(np.sqrt(np.mean((S - ref_signals) ** 2)))
         rows.append((nw, rmse))
         if log:
@@ -488,17 +559,18 @@ def mc_ladder(n_inst, seed, ref_signals, walkers=(256, 512, 1024, 2048, 4096, 81
 
 def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_000, epochs=60,
               batch=1024, members=4, width=512, depth=8, infer_rows=65536, threshold=5e-3,
-              seed=1, target="log", lr=1e-3, log=None):
+              seed=1, target="log", lr=1e-3, log=None, keep_ensemble=False):
     """Train the ensemble on batched-MC signals and measure it against the walk.
 
     Returns a dict with the held-out RMSE of the ensemble mean, of the hybrid
     (MC where the spread exceeds ``threshold``), the surrogate's inference
     signals/s at ``infer_rows`` rows per batch (device time of all members,
     inputs resident), the walk's signals/s at the walker count that reaches the
-    same RMSE bound, and their ratio."""
+    same RMSE bound, and their ratio.  With ``keep_ensemble`` the trained
+    ensemble is returned under "ensemble" (the caller closes it)."""
     log = log or (lambda s: None)
     res = {}
-    X, S, info = generate(n_train_inst, n_walkers, seed=seed, log=log)
+    X, S, info = generate(n_train_inst, n_walkers, seed=seed, log=log, keep_instances=False)
     res["train"] = dict(signals=int(S.size), walkers=n_walkers, gpu_ms=info["gpu_ms"],
                         walker_steps=int(info["walker_steps"]))
     log(f"training set: {S.size} signals, {info['gpu_ms']:.0f} ms of walk")
@@ -540,7 +612,7 @@ def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_0
     # the walk at the accuracy bound
     nw, rmse_mc, ladder = mc_ladder(n_held, seed + 1, Sh, target=threshold, log=log)
     n_rate = max(256, min(n_train_inst, 1024))
-    _, Sr, ir = generate(n_rate, nw, seed=seed + 2)
+    _, Sr, ir = generate(n_rate, nw, seed=seed + 2, keep_instances=False)
     mc_sps = Sr.size / (ir["gpu_ms"] * 1e-3)
     res["mc"] = dict(walkers=nw, rmse=rmse_mc, ladder=ladder, signals_per_s=mc_sps,
                      instances_timed=n_rate)
@@ -554,5 +626,8 @@ def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_0
     res["gain_hybrid"] = eff / mc_sps
     log(f"walk at rmse<={threshold}: {nw} walkers, {mc_sps:.3e} signals/s; gain {res['gain']:.0f}x "
         f"(hybrid {res['gain_hybrid']:.0f}x)")
-    ens.close()
+    if keep_ensemble:
+        res["ensemble"] = ens
+    else:
+        ens.close()
     return res

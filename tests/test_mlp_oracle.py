@@ -104,24 +104,49 @@ def test_waveform_embedding_parseval_and_calc_b():
     assert f.shape == (8, 64)
     q = ds.GAMMA * np.cumsum(np.asarray(wf.G, np.float64), axis=1) * wf.dt
     b_rect = np.sum(q ** 2, axis=(1, 2)) * wf.dt * 1e-9
-    np.testing.assert_allclose(f[:, :62].sum(axis=1), b_rect, rtol=1e-10)
-    np.testing.assert_allclose(f[:, 62] + f[:, 63], b_rect, rtol=1e-10)
-    np.testing.assert_allclose(f[:, :31].sum(axis=1), f[:, 62], rtol=1e-10)
-    assert f[0].max() == 0.0                      # the b = 0 row
+    b_perp = np.sum(q[:, :, :2] ** 2, axis=(1, 2)) * wf.dt * 1e-9
+    # Parseval: the transverse bins sum to the transverse b-value, x and y split it
+    np.testing.assert_allclose(f[:, :56].sum(axis=1), b_perp, rtol=1e-10)
+    np.testing.assert_allclose(f[:, 56], b_perp, rtol=1e-10)
+    np.testing.assert_allclose(f[:, 56] + f[:, 57], b_rect, rtol=1e-10)
+    np.testing.assert_allclose(f[:, 60] + f[:, 61], f[:, 56], rtol=1e-10)
+    assert not np.any(f[0])                       # the b = 0 row
     b = np.asarray(ds.calc_b(wf), np.float64) * 1e-9
-    np.testing.assert_allclose(f[1:, 62] + f[1:, 63], b[1:], rtol=2e-2)
+    np.testing.assert_allclose(f[1:, 56] + f[1:, 57], b[1:], rtol=2e-2)
+    # 1 <= <q^4>/<q^2>^2 <= max q^2 / <q^2> (Cauchy-Schwarz, q^4 <= max q^2 . q^2)
+    assert np.all(f[1:, 58] >= 1.0) and np.all(f[1:, 58] <= f[1:, 59] * (1 + 1e-12))
     # a z-only waveform has no transverse part
     G = np.zeros_like(wf.G)
     G[:, :, 2] = wf.G[:, :, 0]
     fz = sg.waveform_features(ds.Waveform(G=G, dt=wf.dt, echo_idx=wf.echo_idx))
-    assert np.all(fz[:, :31] == 0) and np.all(fz[:, 62] == 0)
+    assert not np.any(fz[:, :57]) and not np.any(fz[:, 58:])
+
+
+def test_waveform_embedding_closed_form_of_one_sine():
+    """G_x = A sin(2 pi k t / T) over whole periods: q = c (1 - cos), so
+    <q^4>/<q^2>^2 = (35/8) / (3/2)^2, max q^2 / <q^2> = 4 / (3/2), and the
+    spectrum holds b in the ratio 2:1 at frequency 0 and k: b-weighted mean
+    frequency k/3, root-mean-square frequency k/sqrt(3)."""
+    n_t, T, k = 4000, 0.04, 7
+    dt = T / n_t
+    t = (np.arange(n_t) + 1) * dt
+    G = np.zeros((1, n_t, 3))
+    G[0, :, 0] = 0.02 * np.sin(2 * np.pi * k * t / T)
+    f = sg.waveform_features(ds.Waveform(G=G, dt=dt, echo_idx=n_t - 1))[0]
+    assert f[58] == pytest.approx((35 / 8) / 2.25, rel=2e-3)
+    assert f[59] == pytest.approx(4 / 1.5, rel=2e-3)
+    assert f[62] * 40 == pytest.approx(k / 3, rel=2e-3)
+    assert f[63] * 40 == pytest.approx(k / np.sqrt(3), rel=2e-3)
+    assert f[61] == 0.0 and f[57] == 0.0
 
 
 def test_features_shape_and_prior():
     rng = np.random.default_rng(3)
     p = sg.sample_params(rng)
     for k, (lo, hi) in sg.PRIOR.items():
-        assert lo <= p[k] <= hi
+        assert lo <= p[k] <= hi or (k == "rho" and p[k] == 0.0)
+    rhos = [sg.sample_params(rng)["rho"] for _ in range(400)]
+    assert 0.15 < np.mean(np.equal(rhos, 0.0)) < 0.35          # RHO_ZERO of the instances
     x = sg.features(p, sg.random_waveforms(rng, n_meas=5))
     assert x.shape == (5, sg.N_IN) and x.dtype == np.float32
     assert np.all(np.abs(x[:, :6]) <= 1.5)
@@ -161,3 +186,37 @@ def test_split_measurements_covers_every_measurement(n_meas):
     np.testing.assert_array_equal(cat[:n_meas], wf.G)
     assert not np.any(cat[n_meas:])
     assert all(p.dt == wf.dt for p in parts)
+
+
+def test_instance_params_without_relaxivity():
+    """The packed workload's substrate (no surface relaxivity) is an input like
+    any other: rho = 0, floored in the relaxivity feature."""
+    from dmipy_sim_amd import configs
+    g = configs.packed_cylinders_substrate()
+    p = sg.instance_params(g, 2e-9)
+    assert p["rho"] == 0.0 and p["kappa"] == 1e-5
+    assert abs(p["vf"] - 0.6) < 1e-9 and abs(p["mean_r"] - np.mean(g._radii_np)) < 1e-15
+    x = sg.substrate_features(p)
+    assert np.all(np.isfinite(x)) and np.all(np.abs(x) <= 2.0)
+
+
+def test_generator_inputs_do_not_depend_on_the_worker_pool(monkeypatch):
+    """Every instance is built from its own seeds drawn in order, so the inputs
+    (and substrates) are the same with the spawned worker pool and inline; the
+    walk is replaced by a host stub that records what it is given."""
+    seen = []
+
+    def fake_walk(instances, waveforms, n_walkers, seed):
+        seen.append([np.asarray(i["geometry"]._radii_np) for i in instances])
+        return np.zeros((len(instances), waveforms[0].G.shape[0])), {"total_ms": 0.0, "walker_steps": 0}
+
+    monkeypatch.setattr(sg, "walk_signals", fake_walk)
+    Xp, _, ip = sg.generate(12, 64, seed=5, n_meas=6, n_t=60, chunk=5, workers=3)
+    radii_pool = [r for c in seen for r in c]
+    seen.clear()
+    Xi, _, ii = sg.generate(12, 64, seed=5, n_meas=6, n_t=60, chunk=12, workers=1)
+    np.testing.assert_array_equal(Xp, Xi)
+    assert all(np.array_equal(a, b) for a, b in zip(radii_pool, seen[0]))
+    assert len(ip["instances"]) == 12 and ip["params"][3] == ii["params"][3]
+    _, _, ik = sg.generate(3, 64, seed=5, n_meas=6, n_t=60, keep_instances=False, workers=1)
+    assert ik["instances"] == [] and len(ik["params"]) == 3
