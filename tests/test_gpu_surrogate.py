@@ -1,7 +1,7 @@
 """The learned surrogate against the method without it (the batched walk).
 
 The full pipeline at its specified size: 2e5 training signals (2223 substrates
-of 64 packed permeable cylinders x 90 free waveforms of n_t = 2000 over 50 ms,
+of 64 packed permeable cylinders x 90 free waveforms of n_t = 2000 over 25 ms,
 5000 walkers each) walked by the batched launch, a 4-member ensemble of 8 x 512
 GELU networks trained on the GPU by the library's own kernels, judged against
 held-out walks of 2e5 walkers.  The test asserts what the surrogate is for: at
@@ -20,18 +20,30 @@ def test_surrogate_gain_over_walk(gpu):
     from dmipy_sim_amd import surrogate as sg
     print("surrogate: 2223 substrates x 90 signals x 5000 walkers, 4 members 8x512, 60 epochs",
           flush=True)
+    from dmipy_sim_amd import configs
     r = sg.benchmark(n_train_inst=2223, n_walkers=5000, n_held=32, held_walkers=200_000,
                      epochs=60, members=4, infer_rows=65536, seed=3,
-                     log=lambda s: print(s, flush=True))
+                     log=lambda s: print(s, flush=True), keep_ensemble=True)
+    ens = r.pop("ensemble")
     print({k: r[k] for k in ("rmse_surrogate", "rmse_hybrid", "fallback_fraction",
                              "fallback_instance_fraction", "gain", "gain_hybrid")}, flush=True)
+    # the packed-cylinder workload's own substrate (64 cylinders, VF 0.6, kappa 1e-5,
+    # no relaxivity) under 90 free waveforms of the generator's family
+    geom = configs.packed_cylinders_substrate()
+    wf = configs.random_sine_waveform(T=sg.T_WAVE, seed=21)
+    S_mc, _ = sg.walk_signals([dict(geometry=geom, diffusivity=2e-9)], [wf], 200_000, seed=11)
+    mean, _, _ = ens.predict(sg.features(sg.instance_params(geom, 2e-9), wf))
+    rmse_ws = float(np.sqrt(np.mean((mean - S_mc[0]) ** 2)))
+    print(f"packed workload substrate: rmse {rmse_ws:.4f}", flush=True)
+    ens.close()
     assert r["train"]["signals"] >= 200_000
-    assert r["rmse_hybrid"] <= 5e-3
+    assert r["rmse_surrogate"] <= 5e-3 and r["rmse_hybrid"] <= 5e-3
     assert r["mc"]["rmse"] <= 5e-3 or r["mc"]["walkers"] == 16384
     assert r["gain"] >= 100.0
     assert r["gain_hybrid"] >= 100.0
     # the ensemble spread is informative: flagged rows are a small minority
     assert r["fallback_fraction"] <= 0.05
+    assert rmse_ws <= 5e-3
 
 
 def test_fallback_replaces_flagged_measurements(gpu):

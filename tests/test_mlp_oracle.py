@@ -156,13 +156,13 @@ def test_features_shape_and_prior():
 
 def test_generator_matches_the_packed_workload_family():
     """The training generator draws the packed-cylinder workload's family: 64
-    cylinders, 90 free waveforms of n_t = 2000 over 50 ms with zero net moment
+    cylinders, 90 free waveforms of n_t = 2000 over 25 ms with zero net moment
     and max |G| in [0.05, 0.3] T/m; a volume-fraction prior containing 0.6."""
     rng = np.random.default_rng(8)
     wf = sg.random_waveforms(rng)
     G = np.asarray(wf.G, np.float64)
     assert G.shape == (90, 2000, 3)
-    assert wf.dt == pytest.approx(0.05 / 1999)
+    assert wf.dt == pytest.approx(0.025 / 1999)
     np.testing.assert_allclose(G.sum(axis=1), 0.0, atol=1e-4)
     gmax = np.linalg.norm(G, axis=2).max(axis=1)
     assert gmax[0] == 0.0 and np.all((gmax[1:] >= 0.05 * 0.999) & (gmax[1:] <= 0.3 * 1.001))
