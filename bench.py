@@ -253,7 +253,7 @@ def run_surrogate(args):
     sur = r["surrogate"]
     roofline = {"bound": "mfma", "achieved": round(sur["tflops"], 2), "peak": PEAK_BF16_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(sur["tflops"] / PEAK_BF16_TFLOPS, 4),
-                "traffic": None, "kernel": "gemm_fwd_big (forward chain)",
+                "traffic": None, "kernel": "mlp_infer_fused<128> (all layers of a 128-row strip in LDS)",
                 "kernel_ms": round(sur["ms"], 4)}
     cpu = None
     if not args.no_cpu:
