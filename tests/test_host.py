@@ -297,6 +297,19 @@ def test_batch_argument_errors_before_gpu():
     assert lib.dmw_batch_create(C.byref(h), 1, inst, 1, wd, None, 1) == _native.DMW_EUNSUPPORTED
 
 
+def test_device_arch_arguments():
+    """dmw_device_arch refuses a null or empty buffer before touching a device,
+    and without a device reports an error rather than a name."""
+    import ctypes as C
+    lib = _native.lib()
+    assert lib.dmw_device_arch(0, None, 16) == _native.DMW_EINVAL
+    buf = C.create_string_buffer(8)
+    assert lib.dmw_device_arch(0, buf, 0) == _native.DMW_EINVAL
+    if _native.device_count() == 0:
+        assert lib.dmw_device_arch(0, buf, len(buf)) != _native.DMW_OK
+        assert _native.device_arch() is None
+
+
 def test_batch_tuning_checked_before_gpu():
     """dmw_batch_create refuses launch shapes outside the compiled set before
     touching a device."""
