@@ -6,8 +6,9 @@ dmipy_sim/__init__.py:12-56): the same Python calls, with the walk executed by
 hand-written HIP kernels for gfx950 through the C ABI in include/dmipy_walk.h.
 """
 from .constants import GAMMA, DEFAULT_SLEW_RATE
-from .waveforms import (Waveform, pgse, ogse, cpmg, calc_b, set_b, rotate_waveform,
-                        tile_waveform, fibonacci_directions)
+from .waveforms import (Waveform, pgse, pgste, ogse, trapezoidal_ogse, b_trapezoidal_ogse, cpmg,
+                        calc_b, calc_btensor, btensor_invariants, ste, pte, set_b,
+                        rotate_waveform, tile_waveform, fibonacci_directions)
 from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders, Ellipsoid,
                          PermeableSlab1D, PermeableShell, MyelinatedCylinder,
                          PackedMyelinatedCylinders, pack_myelinated_cylinders,
@@ -23,7 +24,8 @@ from . import autotune
 from .surrogate import MLP, SurrogateEnsemble
 
 __all__ = [
-    "GAMMA", "DEFAULT_SLEW_RATE", "Waveform", "pgse", "ogse", "cpmg", "calc_b", "set_b",
+    "GAMMA", "DEFAULT_SLEW_RATE", "Waveform", "pgse", "pgste", "ogse", "trapezoidal_ogse",
+    "b_trapezoidal_ogse", "cpmg", "calc_b", "calc_btensor", "btensor_invariants", "ste", "pte", "set_b",
     "rotate_waveform", "tile_waveform", "fibonacci_directions", "FreeDiffusion", "Box1D",
     "Sphere", "Cylinder", "PackedCylinders", "PackedSpheres", "Ellipsoid", "PermeableSlab1D",
     "PermeableShell", "MyelinatedCylinder", "PackedMyelinatedCylinders",

@@ -6,9 +6,13 @@ rules and float32/float64 dtype behaviour so that a waveform built here carries 
 same samples as one built by dmipy_sim/waveforms.py:
 
 * ``pgse``      -- dmipy_sim/waveforms.py:93-163
+* ``pgste``     -- dmipy_sim/waveforms.py:166-262 (stimulated echo: chi_perp mask, T1 while stored)
 * ``ogse``      -- dmipy_sim/waveforms.py:265-350
+* ``trapezoidal_ogse`` / ``b_trapezoidal_ogse`` -- dmipy_sim/waveforms.py:353-487
 * ``cpmg``      -- dmipy_sim/waveforms.py:851-917
 * ``calc_b``    -- dmipy_sim/waveforms.py:490-516
+* ``calc_btensor`` / ``btensor_invariants`` -- dmipy_sim/waveforms.py:519-595
+* ``ste`` / ``pte`` -- dmipy_sim/waveforms.py:598-720 (spherical / planar tensor encoding)
 * ``set_b``     -- dmipy_sim/waveforms.py:723-772
 * ``rotate_waveform`` / ``tile_waveform`` -- dmipy_sim/waveforms.py:775-848
 
@@ -114,6 +118,40 @@ def pgse(delta, DELTA, G_magnitude, bvecs, n_t, slew_rate=DEFAULT_SLEW_RATE):
                     G_display=G_disp)
 
 
+def pgste(delta, TM, G_magnitude, bvecs, n_t, slew_rate=DEFAULT_SLEW_RATE):
+    """Pulsed-gradient stimulated echo (dmipy_sim/waveforms.py:166-262).
+
+    Total duration ``2*delta + TM`` over ``n_t`` samples: a dephasing lobe on
+    samples ``[0, n_pulse)``, the mixing time (magnetisation stored along z: no
+    gradient, ``chi_perp`` 0, only T1 acts) until ``n_pulse + round(TM/dt)``, then
+    the negated rephasing lobe.  ``n_pulse = max(1, round(delta/dt))``; the ramp
+    length follows the peak amplitude passed here.  The stimulated echo keeps half
+    the magnetisation (``stimulated_echo=True``: ``simulate`` scales by 0.5).
+    """
+    bvecs, g, n_meas = _as_meas(bvecs, G_magnitude)
+    dt = (2.0 * delta + TM) / (n_t - 1)
+    n_pulse = max(1, round(delta / dt))
+    i_recall = n_pulse + round(TM / dt)
+    _, square = resolve_slew(slew_rate)
+    n_rise = _lobe_n_rise(square, float(np.max(np.abs(g))), slew_rate, dt, n_pulse)
+    n_second = min(n_pulse, n_t - i_recall)
+    G = np.zeros((n_meas, n_t, 3), dtype=np.float32)
+    G_disp = np.zeros((n_meas, n_t, 3), dtype=np.float32)
+    for m in range(n_meas):
+        amp = g[m] * bvecs[m]
+        _fill_lobe(G, m, 0, n_pulse, amp, n_rise)
+        _fill_lobe(G, m, i_recall, n_second, -amp, n_rise)
+        _fill_lobe(G_disp, m, 0, n_pulse, amp, n_rise)
+        _fill_lobe(G_disp, m, i_recall, n_second, amp, n_rise)
+    chi_perp = np.ones(n_t, dtype=bool)
+    chi_perp[n_pulse:i_recall] = False
+    rf_events = [{'t_s': 0.0, 'label': 'Mz→Mxy', 'flip_deg': 90},
+                 {'t_s': n_pulse * dt, 'label': 'store', 'flip_deg': 90},
+                 {'t_s': i_recall * dt, 'label': 'recall', 'flip_deg': 90}]
+    return Waveform(G=G, dt=float(dt), echo_idx=n_t - 1, rf_events=rf_events,
+                    G_display=G_disp, chi_perp=chi_perp, TM=float(TM), stimulated_echo=True)
+
+
 def ogse(frequency, T_total, G_magnitude, bvecs, n_t, kind='cosine'):
     """Oscillating-gradient spin echo (dmipy_sim/waveforms.py:265-350).
 
@@ -149,6 +187,56 @@ def ogse(frequency, T_total, G_magnitude, bvecs, n_t, kind='cosine'):
     rf_events = [{'t_s': 0.0, 'label': 'Mz→Mxy', 'flip_deg': 90},
                  {'t_s': refocus * dt, 'label': 'refocus', 'flip_deg': 180}]
     return Waveform(G=G, dt=float(dt), echo_idx=n_t - 1, rf_events=rf_events)
+
+
+def trapezoidal_ogse(N, delta, DELTA, G_magnitude, bvecs, n_t, rise_time=None, slew_rate=200e3):
+    """Trapezoidal N-lobe OGSE (dmipy_sim/waveforms.py:353-455; Drobnjak et al.,
+    MRM 2016).  Two blocks of N lobes of ``n_lobe = max(1, round(delta/(N dt)))``
+    samples each, the first at sample 0 with signs +, -, +, ..., the second at
+    ``round(DELTA/dt)`` with the signs flipped (spin-echo convention); every lobe
+    ramps linearly over ``min(round(rise_time/dt), n_lobe//2)`` samples, with
+    ``rise_time = max |G| / slew_rate`` when not given.  N = 1 is a trapezoidal PGSE.
+    """
+    bvecs, g, n_meas = _as_meas(bvecs, G_magnitude)
+    g_peak = float(np.max(np.abs(g)))
+    if rise_time is None:
+        rise_time = g_peak / slew_rate if g_peak > 0 else 0.0
+    dt = (DELTA + delta) / (n_t - 1)
+    n_lobe = max(1, round(delta / (N * dt)))
+    n_rise = min(max(0, round(rise_time / dt)), n_lobe // 2)
+    n_DELTA = round(DELTA / dt)
+    G = np.zeros((n_meas, n_t, 3), dtype=np.float32)
+    for m in range(n_meas):
+        for block0, first_sign in ((0, 1.0), (n_DELTA, -1.0)):
+            for k in range(N):
+                amp = np.float32(first_sign * (-1.0) ** k) * g[m]
+                i0 = block0 + k * n_lobe
+                i1 = i0 + n_lobe
+                if n_rise > 0:
+                    up = np.linspace(0.0, amp, n_rise, endpoint=False)
+                    dn = np.linspace(amp, 0.0, n_rise, endpoint=False)
+                    G[m, i0:i0 + n_rise, :] = up[:, None] * bvecs[m]
+                    G[m, i0 + n_rise:i1 - n_rise, :] = amp * bvecs[m]
+                    G[m, i1 - n_rise:i1, :] = dn[:, None] * bvecs[m]
+                else:
+                    G[m, i0:i1, :] = amp * bvecs[m]
+    gap_mid = (n_lobe * N + n_DELTA) // 2
+    rf_events = [{'t_s': 0.0, 'label': 'Mz→Mxy', 'flip_deg': 90},
+                 {'t_s': gap_mid * dt, 'label': 'refocus', 'flip_deg': 180}]
+    return Waveform(G=G, dt=float(dt), echo_idx=n_t - 1, rf_events=rf_events)
+
+
+def b_trapezoidal_ogse(N, delta, DELTA, G, rise_time):
+    """Closed-form b-value of the trapezoidal N-lobe OGSE in s/m^2 (Eq. 2 of
+    Drobnjak et al., MRM 2016; dmipy_sim/waveforms.py:458-487): with the ramp
+    fraction x = rise_time N / delta, the within-block term
+    2 G^2 gamma^2 delta^3 / (15 N^2) (5 - 15x/2 - 5x^2/4 + 4x^3), plus for odd N the
+    net moment of a block, (delta - N rise_time) / N, acting over DELTA - delta."""
+    x = rise_time * N / delta
+    within = (2.0 * G ** 2 * GAMMA ** 2 * delta ** 3 / (15.0 * N ** 2)
+              * (5.0 - 7.5 * x - 1.25 * x ** 2 + 4.0 * x ** 3))
+    net = (1 - (-1) ** N) * (delta - N * rise_time) / (2.0 * N)
+    return float(within + G ** 2 * GAMMA ** 2 * (DELTA - delta) * net ** 2)
 
 
 def cpmg(n_echoes, TE, G_magnitude, bvecs, n_t_per_echo=100):
@@ -192,6 +280,78 @@ def calc_b(waveform):
     q2 = np.sum(q ** 2, axis=2)
     b = np.trapezoid(q2, dx=dt, axis=1)
     return np.asarray(b).astype(np.float64)
+
+
+def calc_btensor(waveform):
+    """B-tensor per measurement, B_ij = int q_i(t) q_j(t) dt in s/m^2
+    (dmipy_sim/waveforms.py:519-543): the q(t) and integration rules of calc_b,
+    so that its trace equals calc_b to float64 rounding.  (n_meas, 3, 3) float64."""
+    G = np.asarray(waveform.G, dtype=np.float32)
+    dt = waveform.dt
+    q = np.cumsum(G * dt, axis=1) * GAMMA
+    B = np.trapezoid(q[:, :, :, None] * q[:, :, None, :], dx=dt, axis=1)
+    return np.asarray(B).astype(np.float64)
+
+
+def btensor_invariants(B):
+    """(b, b_delta, b_eta) of B-tensors (n_meas, 3, 3) (dmipy_sim/waveforms.py:546-595).
+
+    b = trace; with eigenvalues l1 >= l2 >= l3, a prolate tensor (l1 farther from
+    b/3 than l3) has b_delta = (l1 - (l2 + l3)/2) / b and b_eta = (l2 - l3) / b, an
+    oblate one b_delta = (l3 - (l1 + l2)/2) / b and b_eta = (l1 - l2) / b: linear
+    encoding 1, spherical 0, planar -1/2 (Szczepankiewicz et al. 2021).  b = 0 rows
+    give zero shape parameters."""
+    lam = np.linalg.eigvalsh(np.asarray(B))[:, ::-1]
+    l1, l2, l3 = lam[:, 0], lam[:, 1], lam[:, 2]
+    b = l1 + l2 + l3
+    bs = np.where(b > 0, b, 1.0)
+    prolate = np.abs(l1 - b / 3.0) >= np.abs(l3 - b / 3.0)
+    b_delta = np.where(prolate, l1 - 0.5 * (l2 + l3), l3 - 0.5 * (l1 + l2)) / bs
+    b_eta = np.where(prolate, l2 - l3, l1 - l2) / bs
+    return b, b_delta, b_eta
+
+
+def _bipolar_axes(axes, T_total, G_magnitude, n_t, slew_rate):
+    """One bipolar pair per axis, back to back: the total time is cut into
+    2 len(axes) segments of round(n_t / (2 len(axes))) samples, axis i encodes
+    (+) on segment 2i and decodes (-) on segment 2i+1, so every axis's q returns
+    to zero before the next starts (dmipy_sim/waveforms.py:598-720)."""
+    dt = T_total / (n_t - 1)
+    n_seg = max(1, round(n_t / (2 * len(axes))))
+    _, square = resolve_slew(slew_rate)
+    n_rise = _lobe_n_rise(square, abs(float(G_magnitude)), slew_rate, dt, n_seg)
+    G = np.zeros((1, n_t, 3), dtype=np.float32)
+    for i, axis in enumerate(axes):
+        start = 2 * i * n_seg
+        mid = min((2 * i + 1) * n_seg, n_t)
+        end = min((2 * i + 2) * n_seg, n_t)
+        amp = (G_magnitude * np.asarray(axis)).astype(np.float32)
+        _fill_lobe(G, 0, start, mid - start, amp, n_rise)
+        _fill_lobe(G, 0, mid, end - mid, -amp, n_rise)
+    return Waveform(G=G, dt=float(dt), echo_idx=n_t - 1,
+                    rf_events=[{'t_s': 0.0, 'label': 'Mz→Mxy', 'flip_deg': 90}])
+
+
+def ste(delta, DELTA, G_magnitude, n_t, slew_rate=DEFAULT_SLEW_RATE):
+    """Spherical tensor encoding, B = (b/3) I (dmipy_sim/waveforms.py:598-655):
+    bipolar pairs along x, y and z in turn over DELTA + delta (only the sum is
+    used).  One measurement; scale with set_b."""
+    return _bipolar_axes(np.eye(3, dtype=np.float32), DELTA + delta, G_magnitude, n_t, slew_rate)
+
+
+def pte(delta, DELTA, G_magnitude, plane_normal, n_t, slew_rate=DEFAULT_SLEW_RATE):
+    """Planar tensor encoding, eigenvalues (b/2, b/2, 0) with the zero along
+    ``plane_normal`` (dmipy_sim/waveforms.py:658-720): bipolar pairs along two
+    orthonormal in-plane axes u, v in turn (u from Gram-Schmidt of x, or of y when
+    the normal is close to x; v = n x u)."""
+    n = np.asarray(plane_normal, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    ref = np.array([1.0, 0.0, 0.0]) if abs(n[0]) < 0.9 else np.array([0.0, 1.0, 0.0])
+    u = ref - np.dot(ref, n) * n
+    u /= np.linalg.norm(u)
+    v = np.cross(n, u)
+    v /= np.linalg.norm(v)
+    return _bipolar_axes([u, v], DELTA + delta, G_magnitude, n_t, slew_rate)
 
 
 def set_b(waveform, b_target):

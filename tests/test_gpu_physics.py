@@ -207,6 +207,46 @@ def test_stimulated_echo_half_amplitude(gpu):
     np.testing.assert_array_equal(b, a * np.float32(0.5))
 
 
+def test_pgste_t1_and_stimulated_echo(gpu):
+    """tests/test_pgste_t1.py: a PGSTE on free diffusion gives 0.5 exp(-bD)
+    exp(-TM/T1) (T1 acts only while stored), 0.5 exp(-bD) without T1, the signal
+    falls with the mixing time, and a PGSE (always transverse) is unchanged by T1
+    bit for bit.  3 Monte-Carlo SE."""
+    N, b = 40_000, 1e9
+
+    def pgste_at(TM):
+        return ds.set_b(ds.pgste(5e-3, TM, 1.0, [[1.0, 0.0, 0.0]], 600), [b])
+
+    wf = pgste_at(40e-3)
+    b_eff = ds.calc_b(wf)[0]
+    E = np.exp(-b_eff * D)
+    se = 3 * mc_se(np.exp(-4 * b_eff * D), E, N) * 0.5 + 1e-6
+    S1 = gpu.simulate(N, D, wf, ds.FreeDiffusion(), seed=3, T1=1.0)[0]
+    assert abs(S1 - 0.5 * E * np.exp(-40e-3 / 1.0)) <= se, S1
+    S0 = gpu.simulate(N, D, wf, ds.FreeDiffusion(), seed=3)[0]
+    assert abs(S0 - 0.5 * E) <= se, S0
+    Ss = [gpu.simulate(N, D, pgste_at(TM), ds.FreeDiffusion(), seed=3, T1=0.8)[0]
+          for TM in (10e-3, 40e-3, 80e-3)]
+    assert Ss[0] >= Ss[1] - 1e-3 and Ss[1] >= Ss[2] - 1e-3
+    pg = ds.set_b(ds.pgse(5e-3, 45e-3, 1.0, [[1.0, 0.0, 0.0]], 600), [b])
+    np.testing.assert_array_equal(gpu.simulate(N, D, pg, ds.FreeDiffusion(), seed=3),
+                                  gpu.simulate(N, D, pg, ds.FreeDiffusion(), seed=3, T1=1.0))
+
+
+def test_tensor_encodings_on_free_diffusion(gpu):
+    """tests/test_btensor.py MC checks: for isotropic free diffusion the signal is
+    exp(-trace(B) D) whatever the B-tensor's shape: linear, spherical and planar
+    encodings at the same b agree with exp(-bD) within 3 SE."""
+    N, b = 60_000, 1.5e9
+    wfs = [ds.set_b(ds.pgse(10e-3, 40e-3, 0.06, [[0.0, 0.0, 1.0]], 1000), [b]),
+           ds.set_b(ds.ste(10e-3, 40e-3, 0.06, 1000), [b]),
+           ds.set_b(ds.pte(10e-3, 40e-3, 0.06, [0.0, 0.0, 1.0], 1000), [b])]
+    E = np.exp(-b * D)
+    for wf in wfs:
+        S = gpu.simulate(N, D, wf, ds.FreeDiffusion(), seed=5)[0]
+        assert abs(S - E) <= 3 * mc_se(np.exp(-4 * b * D), E, N) + 1e-6, (S, E)
+
+
 def test_outputs_shapes_and_batches(gpu):
     """simulate() return tuple shapes (core.py:737-762) and walker batching:
     chunked run agrees with the single run statistically (core.py:765-818)."""
