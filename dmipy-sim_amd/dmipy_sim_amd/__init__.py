@@ -16,9 +16,9 @@ from .geometries import (FreeDiffusion, Box1D, Sphere, Cylinder, PackedCylinders
 from .physics import (make_walk_plan, permeable_sub_steps, surface_sub_steps,
                       collision_sub_steps, replay_sub_steps)
 from .core import simulate, simulate_cpmg, simulate_mixture, walk, PlannedWalk
-from .gpu import check_gpu, gpu_available
+from .gpu import check_gpu, gpu_available, free_gpu_memory, gpu_session, list_gpu_processes
 from .batch import BatchedWalk
-from .trajectories import simulate_trajectories, replay, TrajectorySet
+from .trajectories import simulate_trajectories, replay, TrajectorySet, unwrap_periodic
 from . import surrogate
 from . import autotune
 from .surrogate import MLP, SurrogateEnsemble
@@ -33,6 +33,7 @@ __all__ = [
     "pack_spheres", "pack_dense", "make_walk_plan", "permeable_sub_steps",
     "surface_sub_steps", "collision_sub_steps", "replay_sub_steps", "simulate",
     "simulate_cpmg", "simulate_mixture", "walk", "check_gpu", "gpu_available",
+    "free_gpu_memory", "gpu_session", "list_gpu_processes", "unwrap_periodic",
     "BatchedWalk", "PlannedWalk", "surrogate", "MLP", "SurrogateEnsemble", "autotune", "simulate_trajectories", "replay",
     "TrajectorySet",
 ]

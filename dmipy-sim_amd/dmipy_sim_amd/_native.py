@@ -5,6 +5,7 @@ fallback: when the library or a GPU is missing every entry point raises.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -158,6 +159,21 @@ def lib():
 
 def device_count():
     return int(lib().dmw_device_count())
+
+
+# Live library objects that own device memory (plans, batches, networks, trajectory
+# sets), so that gpu.free_gpu_memory(aggressive=True) can release them all.
+_live = weakref.WeakSet()
+
+
+def track(obj):
+    """Register an object with a close() that frees its device memory."""
+    _live.add(obj)
+    return obj
+
+
+def live_objects():
+    return list(_live)
 
 
 def device_arch(device=0):

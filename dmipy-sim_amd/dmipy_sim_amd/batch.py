@@ -115,6 +115,7 @@ class BatchedWalk:
                                              C.byref(tn), int(bool(use_graph))))
         self._h = h
         self._keep = None
+        nat.track(self)
         self.n_inst = len(instances)
         self.n_sets = len(sets_G)
         self.n_meas = n_meas

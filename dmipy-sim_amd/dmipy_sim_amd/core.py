@@ -516,6 +516,7 @@ class PlannedWalk:
         h = C.c_void_p()
         nat.check(nat.lib().dmw_plan_create(C.byref(h), gd, wd, rd))
         self._h = h
+        nat.track(self)
         self.n_meas, self.n_t, self.n_walkers = n_meas, n_t, int(n_walkers)
         self.walker_steps = int(n_walkers) * n_t * plan.n_sub
         self.kahan = bool(kahan)

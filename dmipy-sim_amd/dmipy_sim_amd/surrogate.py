@@ -84,6 +84,7 @@ class MLP:
         nat.check(nat.lib().dmw_mlp_create(C.byref(h), self.n_in, self.width, self.depth,
                                            self.max_batch))
         self._h = h
+        nat.track(self)
         self.n_params = int(nat.lib().dmw_mlp_num_params(h))
         self.n_data = 0
         self.set_params(self.init_params(seed))

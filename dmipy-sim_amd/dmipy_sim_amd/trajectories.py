@@ -20,6 +20,8 @@ the reference's signatures and return numpy arrays.
 import ctypes as C
 import math
 
+import warnings
+
 import numpy as np
 
 from . import _native as nat
@@ -100,6 +102,7 @@ class TrajectorySet:
 
     def __init__(self, handle, dt_traj, sub_steps=1, dt_sim=None, comp_kind=None):
         self._h = handle
+        nat.track(self)
         n = C.c_int64()
         t = C.c_int32()
         ch = C.c_int32()
@@ -288,6 +291,33 @@ def _prepare(G, dt_wf, dt_traj, n_t_traj, chi_perp, T2, T1):
     if T1 is not None:
         scalar -= (dt_traj / T1) * (n_t_traj - chi_r.sum(axis=1))
     return G_traj, chi_r, scalar, per_meas
+
+
+def unwrap_periodic(traj, cell_size, periodic_axes=(0, 1)):
+    """Continuous positions from positions folded into a periodic cell
+    (dmipy_sim/trajectories.py:82-132): along each periodic axis the per-save
+    displacement is taken to its minimal image, x_k - x_{k-1} - L round((x_k -
+    x_{k-1}) / L), and re-integrated from the first sample.  Time is axis -2 of
+    ``traj`` (..., n_t, 3); valid while the true per-save displacement stays below
+    L/2 (a warning is raised when one comes within 0.45 L).  float32.
+
+    The walk-once producer of this library already saves unfolded positions, so
+    this is for positions from elsewhere (e.g. a folded export)."""
+    out = np.array(traj, dtype=np.float32, copy=True)
+    L = np.float32(cell_size)
+    for ax in periodic_axes:
+        x = out[..., ax]
+        step = np.diff(x, axis=-1)
+        step = step - L * np.round(step / L)
+        if np.any(np.abs(step) > 0.45 * float(L)):
+            warnings.warn("unwrap_periodic: a per-save displacement approaches L/2 on a periodic "
+                          "axis; the save step may be too coarse to unwrap unambiguously.",
+                          RuntimeWarning, stacklevel=2)
+        cont = np.empty_like(x)
+        cont[..., 0] = x[..., 0]
+        cont[..., 1:] = x[..., :1] + np.cumsum(step, axis=-1)
+        out[..., ax] = cont
+    return out
 
 
 def simulate_trajectories(n_walkers, diffusivity, geometry, T_max, dt_save, seed=42,

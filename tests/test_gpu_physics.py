@@ -400,3 +400,18 @@ def test_ellipsoid_sphere_limit_and_anisotropy(gpu):
     assert np.all(np.abs(Se - Ss) <= 4 * np.sqrt(2 * 0.5 / n))
     Sp = gpu.simulate(n, D, wf, gpu.Ellipsoid((2e-6, 2e-6, 8e-6)), seed=3)
     assert Sp[2] < Sp[0] - 0.05 and abs(Sp[0] - Sp[1]) < 4 * np.sqrt(2 * 0.5 / n)
+
+
+def test_free_gpu_memory_closes_live_objects(gpu):
+    """free_gpu_memory(aggressive=True) closes every live planned walk, batched
+    walk, network and trajectory set (dmipy_sim/gpu.py:93-134's aggressive mode);
+    a closed object refuses further use."""
+    wf = small_pgse(n_t=20)
+    pw = gpu.PlannedWalk(ds.Sphere(3e-6), D, wf, 256)
+    bw = gpu.BatchedWalk([dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=128)], wf)
+    pw.run(seed=1)
+    assert gpu.free_gpu_memory() == 0              # both still referenced: nothing closed
+    assert gpu.free_gpu_memory(aggressive=True) >= 2
+    assert pw._h is None and bw._h is None
+    with pytest.raises((ValueError, RuntimeError, AttributeError)):
+        pw.run(seed=2)

@@ -644,3 +644,34 @@ def test_integration_stub_structs_match_header():
         assert [f for f, _ in st._fields_] == [f for f, _ in mirror._fields_], stub
         for f, _ in st._fields_:
             assert getattr(st, f).offset == getattr(mirror, f).offset, (stub, f)
+
+
+def test_unwrap_periodic_restores_a_folded_walk():
+    """trajectories.py:82-132: a random walk folded into [-L/2, L/2) on the
+    periodic axes is restored (up to the first sample's fold and float32
+    rounding); the non-periodic axis is untouched; a save step near L/2 warns."""
+    rng = np.random.default_rng(3)
+    L = 20e-6
+    steps = rng.normal(0, 0.4e-6, (5, 400, 3))
+    x = np.cumsum(steps, axis=1) + rng.uniform(-30e-6, 30e-6, (5, 1, 3))
+    folded = x.copy()
+    folded[..., :2] -= L * np.floor(folded[..., :2] / L + 0.5)
+    out = ds.unwrap_periodic(folded.astype(np.float32), L, periodic_axes=(0, 1))
+    assert out.dtype == np.float32
+    shift = x[:, :1, :2] - folded[:, :1, :2]
+    np.testing.assert_allclose(out[..., :2] + shift, x[..., :2], atol=1e-10)
+    np.testing.assert_array_equal(out[..., 2], folded[..., 2].astype(np.float32))
+    coarse = np.zeros((1, 3, 3), np.float32)
+    coarse[0, 1, 0] = 0.47 * L
+    with pytest.warns(RuntimeWarning, match="L/2"):
+        ds.unwrap_periodic(coarse, L)
+
+
+def test_gpu_memory_helpers_without_objects():
+    """gpu.py:93-185 surface: free_gpu_memory collects and returns the number of
+    objects it closed, gpu_session frees on exit, list_gpu_processes returns a
+    list (empty without a device or rocm-smi)."""
+    assert ds.free_gpu_memory() == 0
+    with ds.gpu_session(aggressive=False) as s:
+        assert isinstance(s, ds.gpu_session)
+    assert isinstance(ds.list_gpu_processes(), list)
