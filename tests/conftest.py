@@ -11,6 +11,13 @@ for p in (os.path.join(ROOT, "dmipy-sim_amd"), ROOT):
 
 D = 2e-9
 
+# The autotuner persists its choices; a test session keeps them in a file of its own
+# so that tuning runs in tests never steer later walks of the same user.
+if "DMIPY_AUTOTUNE_CACHE" not in os.environ:
+    import tempfile
+    os.environ["DMIPY_AUTOTUNE_CACHE"] = os.path.join(
+        tempfile.mkdtemp(prefix="dmipy_autotune_"), "autotune.json")
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU and the built HIP library")
