@@ -210,3 +210,31 @@ def test_large_tables_take_a_narrower_block(gpu, oracle):
     o = oracle.run_oracle(geom, D, wf, n, r0=r0, inject_normals=nrm)
     np.testing.assert_array_equal(g["final_pos"], o["final_pos"])
     assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+
+
+def test_many_measurements_span_several_passes(gpu, oracle):
+    """300 measurements exceed the widest measurement block (128): the walk runs
+    several passes over the same trajectories (the last one padded), and every
+    measurement still matches the oracle within 1e-5 with positions bit-exact."""
+    wf = small_pgse(n_t=24, n_dirs=75, b=(0.5e9, 1e9, 2e9, 3e9))
+    assert wf.G.shape[0] == 300
+    geom = ds.Cylinder(4e-6, (0, 0, 1), surface_relaxivity_t2=2e-6)
+    n = 320
+    r0 = geom.init_positions(n, 5)
+    g = gpu.walk(geom, D, wf, n, seed=5, r0=r0, outputs=("final_pos", "log_w"))
+    o = oracle.run_oracle(geom, D, wf, n, seed=5, r0=r0)
+    np.testing.assert_array_equal(g["final_pos"], o["final_pos"])
+    np.testing.assert_array_equal(g["log_w"], o["log_w"])
+    assert g["signal"].shape == (300,)
+    assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+
+
+def test_zero_gradient_and_two_samples(gpu):
+    """No gradient and no relaxation: every weight is 1 and every phase 0, so the
+    signal is exactly 1; a two-sample waveform walks (n_t = 2 is the least)."""
+    G = np.zeros((3, 2, 3), np.float32)
+    wf = ds.Waveform(G=G, dt=1e-3, echo_idx=1)
+    for geom in (ds.Sphere(3e-6), ds.Cylinder(3e-6, (0, 0, 1)), ds.FreeDiffusion()):
+        S = gpu.simulate(1000, D, wf, geom, seed=1)
+        assert S.dtype == np.float32
+        np.testing.assert_array_equal(S, np.ones(3, np.float32))
