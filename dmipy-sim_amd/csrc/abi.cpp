@@ -399,10 +399,11 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   // waveform operands.  The grid is coarsened to fit; only if even one cell does
   // not fit (and no meas_block was requested) is a narrower block taken.
   auto static_lds = [&](int mb) -> size_t { return (mb >= 32 && !P->kahan) ? 8192 + 128 * (size_t)mb : 0; };
-  auto fit_grid = [&](float step, int dim, size_t obj_bytes, Grid& G, double hyst = 0.0) -> bool {
+  auto fit_grid = [&](float step, int dim, size_t obj_bytes, Grid& G, double hyst = 0.0,
+                      int max_list = 0) -> bool {
     for (;;) {
       const size_t budget = 65536 - static_lds(P->mb);
-      G = build_grid(*geom, step, dim, budget, obj_bytes, hyst);
+      G = build_grid(*geom, step, dim, budget, obj_bytes, hyst, max_list);
       if ((size_t)geom->n_obj * obj_bytes + (G.start.size() + G.items.size()) * 4 <= budget) return true;
       if (run->meas_block != 0 || P->mb <= 16 || P->park) return false;
       P->mb = P->mb > 32 ? P->mb / 2 : 16;
@@ -456,21 +457,24 @@ int dmw_plan_create(dmw_plan** out, const dmw_geometry* geom, const dmw_waveform
   if (packed) {
     const int dim = geom->kind == DMW_PACKED_CYLINDERS ? 2 : 3;
     // Permeable packings walk the screened path: cells expanded by half a side
-    // (a walker keeps its cell's list until it leaves that box).
+    // (a walker keeps its cell's list until it leaves that box), less where the
+    // list would exceed the screen's object cache; the list starts carry each
+    // cell's box level.
     const bool screened = geom->wall == DMW_WALL_PERMEATE;
     const double hyst = screened ? 0.5 : 0.0;
     Grid G;
-    if (!fit_grid(run->step_len, dim, 16, G, hyst)) {
+    if (!fit_grid(run->step_len, dim, 16, G, hyst, screened ? dmw::SCR_MAXC : 0)) {
       delete P;
       return fail(DMW_EUNSUPPORTED, lds_msg);
     }
+    const std::vector<int> starts = G.encoded_start();
     std::vector<float> objs((size_t)geom->n_obj * 4, 0.0f);
     for (int k = 0; k < geom->n_obj; ++k) {
       for (int d = 0; d < dim; ++d) objs[k * 4 + d] = geom->centers[k * dim + d];
       objs[k * 4 + 3] = geom->radii[k];
     }
     TRY(upload(P->objs, objs.data(), objs.size() * 4, st));
-    TRY(upload(P->cstart, G.start.data(), G.start.size() * 4, st));
+    TRY(upload(P->cstart, starts.data(), starts.size() * 4, st));
     TRY(upload(P->citems, G.items.data(), std::max<size_t>(1, G.items.size()) * 4, st));
     a.n_obj = geom->n_obj;
     a.grid_n = G.n;

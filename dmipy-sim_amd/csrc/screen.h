@@ -53,6 +53,17 @@ DMW_SCR_HD bool screen_clear(float f0, float f1, float R, float m2, float c_out,
   return (fminf(f0, f1) > t_out) | (fmaxf(f0, f1) < t_in);
 }
 
+// Cell lists of the screened walk: a cell's list start carries, in the bits from
+// SCR_LEVEL_SHIFT up, the level l of the box its list was built for: the cell
+// expanded by (1 - l/2) times the walk's hysteresis width on every side (l = 0,
+// 1, 2; the grid builder shrinks the box of a cell whose list would not fit the
+// walker's object cache).
+constexpr int SCR_MAXC = 3;   // objects a walker caches for the screen
+constexpr unsigned SCR_LEVEL_SHIFT = 30;
+constexpr unsigned SCR_LIST_MASK = (1u << SCR_LEVEL_SHIFT) - 1u;
+
+DMW_SCR_HD float screen_box_width(float w, unsigned level) { return w * (1.0f - 0.5f * (float)level); }
+
 // Distance margin of the screen (host).  Covers, with a factor of at least eight:
 // the float32 rounding of positions and cached image centres (a few ulp of the
 // cell side), of f (ulp of R^2 and |x - c|^2), of the reference's root near

@@ -913,7 +913,8 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
 
 // ---------------------------------------------------------------------------
 // Screened packed permeation.  Each lane keeps the grid cell it walks in, as a
-// box expanded by a.scr_w on every side (the host lists, for that box, every
+// box expanded on every side by a.scr_w, or less for the few cells whose list
+// would not fit the cache (the host lists, for that box, every
 // object within one step of it), and, when the list holds at most SCR_MAXC
 // objects, each object's image centre, radius and f = |x - c|^2 - R^2 at the
 // walker.  A sub-step that ends inside the box and that the wall screen
@@ -922,8 +923,6 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
 // exactly as the reference's no-hit branch does (geometries.py:2099-2102).  Only
 // the other lanes run the reference's root test, over the cached cell's list.
 // ---------------------------------------------------------------------------
-constexpr int SCR_MAXC = 3;
-
 template <int DIM>
 struct PackedScreen {
   int cnt;                 // cached objects; SCR_MAXC + 1: not cached (longer list, or no cache)
@@ -946,26 +945,25 @@ struct PackedScreen {
     });
   }
 
-  // The walker's cell (as cell_of computes it), its box in the walker's frame, the
-  // list, and the cached objects' nearest images.
+  // The walker's cell (cell_of), the list and box level the cell's list start
+  // carries, its box in the walker's frame (the cell expanded by that level's
+  // width), and the cached objects' nearest images.
   __device__ __forceinline__ void refresh(const Pos& p, const WalkArgs& a, const Tables& tb) {
-    const float L = a.cell, h = a.grid_h, w = a.scr_w;
+    const float L = a.cell, h = a.grid_h;
+    const int gc = cell_of<DIM>(p, a);
+    const unsigned s0 = (unsigned)tb.start[gc];
+    beg = (int)(s0 & SCR_LIST_MASK);
+    len = (int)((unsigned)tb.start[gc + 1] & SCR_LIST_MASK) - beg;
+    const float w = screen_box_width(a.scr_w, s0 >> SCR_LEVEL_SHIFT);
     const float v[3] = {p.x, p.y, p.z};
-    int ic[3] = {0, 0, 0};
     static_for<DIM>([&](auto dc) {
       constexpr int d = decltype(dc)::value;
       const float k = floorf(v[d] * a.inv_cell + 0.5f);
-      const float fv = v[d] - a.cell * k;
-      int i = (int)floorf((fv + 0.5f * a.cell) * a.grid_inv_h);
-      i = i < 0 ? 0 : (i >= a.grid_n ? a.grid_n - 1 : i);
-      ic[d] = i;
+      const int i = grid_index(v[d], a);
       const float lo = (-0.5f * L + (float)i * h) + L * k;
       blo[d] = lo - w;
       bhi[d] = (lo + h) + w;
     });
-    const int gc = (DIM == 2) ? ic[1] * a.grid_n + ic[0] : (ic[2] * a.grid_n + ic[1]) * a.grid_n + ic[0];
-    beg = tb.start[gc];
-    len = tb.start[gc + 1] - beg;
     cnt = (a.scr_cache && len <= SCR_MAXC) ? len : SCR_MAXC + 1;
     // every slot is written (unused ones with zeros), so no earlier value stays live
     static_for<SCR_MAXC>([&](auto jc) {
@@ -1182,7 +1180,9 @@ __device__ __forceinline__ int classify(const Pos& p, const WalkArgs& a, const T
     constexpr int DIM = (GEOM == G_PCYL) ? 2 : 3;
     const int gc = cell_of<DIM>(p, a);
     int id = 0;
-    for (int i = tb.start[gc]; i < tb.start[gc + 1]; ++i) {
+    // (the screened walk's list starts carry the cell's box level above the mask)
+    const int end = (int)((unsigned)tb.start[gc + 1] & SCR_LIST_MASK);
+    for (int i = (int)((unsigned)tb.start[gc] & SCR_LIST_MASK); i < end; ++i) {
       const int k = tb.items[i];
       const float4 o = tb.objs[k];
       float qx, qy, qz;

@@ -39,7 +39,8 @@ def screen():
     h.screen_check_margin.argtypes = [C.c_double, C.c_double, C.c_double]
     h.screen_check_grid.restype = C.c_int64
     h.screen_check_grid.argtypes = [C.c_int32, _fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int32,
-                                    C.c_double, C.c_int32, C.c_uint64, C.POINTER(C.c_int64)]
+                                    C.c_double, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64)]
     return h
 
 
@@ -150,11 +151,14 @@ def test_margin_is_small_against_the_step(screen):
 
 
 @pytest.mark.parametrize("which", ["packed cylinders, 64, VF 0.6", "small cylinders", "small spheres"])
-def test_expanded_cells_list_every_reachable_object(screen, which):
+@pytest.mark.parametrize("max_list", [0, 3])
+def test_expanded_cells_list_every_reachable_object(screen, which, max_list):
     """The screened path keeps a walker on its cell's list while it stays inside
-    the cell box expanded by half a side: every point of that box within one step
-    of an object's wall must find the object in the list (the builder inflates the
-    box per coordinate, as the walk tests it)."""
+    the cell box expanded by half a side (less for a cell whose list would not fit
+    the walker's object cache of three, at the box level the list start carries):
+    every point of that box within one step of an object's wall must find the
+    object in the list (the builder inflates the box per coordinate, as the walk
+    tests it)."""
     import dmipy_sim_amd as ds
     from dmipy_sim_amd import configs
     from substrates import D, packed_cyl, packed_sph, small_pgse
@@ -169,10 +173,59 @@ def test_expanded_cells_list_every_reachable_object(screen, which):
     c = np.ascontiguousarray(g._centers_np, np.float32)
     r = np.ascontiguousarray(g._radii_np, np.float32)
     pairs = C.c_int64(0)
+    stats = (C.c_int64 * 5)()
     # a probe step of a quarter cell side: many near-wall pairs per cell
     for step in (plan.step_len, 0.05 * float(g._L_float)):
         missing = screen.screen_check_grid(len(r), c.ctypes.data_as(_fp), r.ctypes.data_as(_fp),
                                            float(np.float32(g._L_float)), step, gd.nudge, dim, 0.5,
-                                           40 if dim == 3 else 200, 7, C.byref(pairs))
+                                           max_list, 40 if dim == 3 else 200, 7, C.byref(pairs), stats)
         assert pairs.value > 1000
         assert missing == 0
+        if max_list == 0:
+            assert stats[1] == 0 and stats[2] == 0
+        if which.startswith("packed") and max_list and step == plan.step_len:
+            # the 64-cylinder packing at its own step: a few cells of 4 objects at the
+            # full box, none after the builder's shrinking, most cells at the full box
+            assert stats[1] + stats[2] >= 1 and stats[4] == 0 and stats[3] <= 3
+            assert stats[1] + stats[2] < 0.01 * stats[0]
+
+
+def test_box_levels_shrink_only_overflowing_cells(screen):
+    """Against the plain grid (no cache limit), the limited grid changes only the
+    lists that exceed three objects: those get a smaller box and a shorter list,
+    a subset of the full-box list; every other cell keeps its list and level 0."""
+    from dmipy_sim_amd import configs
+    cfg = configs.packed_cylinders_free_waveform()
+    g = cfg["geometry"]
+    import dmipy_sim_amd as ds
+    plan = ds.make_walk_plan(g, cfg["diffusivity"], cfg["waveform"].dt)
+    c = np.ascontiguousarray(g._centers_np, np.float32)
+    r = np.ascontiguousarray(g._radii_np, np.float32)
+    fn = screen.screen_check_lists
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int32, _fp, _fp, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_double,
+                   C.c_int32, _ip, _ip, _ip, C.c_int64]
+    cap = 1 << 20
+    out = [np.zeros(cap, np.int32) for _ in range(3)]
+    lens = {}
+    for max_list in (0, 3):
+        o = [np.zeros(cap, np.int32) for _ in range(3)]
+        n_cells = fn(len(r), c.ctypes.data_as(_fp), r.ctypes.data_as(_fp), float(np.float32(g._L_float)),
+                     plan.step_len, plan.geometry_desc().nudge, 2, 0.5, max_list,
+                     o[0].ctypes.data_as(_ip), o[1].ctypes.data_as(_ip), o[2].ctypes.data_as(_ip), cap)
+        assert n_cells > 0
+        start, level, items = o[0][:n_cells + 1], o[1][:n_cells], o[2][:o[0][n_cells]]
+        lens[max_list] = (start, level, items)
+    s0, l0, i0 = lens[0]
+    s3, l3, i3 = lens[3]
+    assert not l0.any()
+    changed = 0
+    for cell in range(len(l3)):
+        a = list(i0[s0[cell]:s0[cell + 1]])
+        b = list(i3[s3[cell]:s3[cell + 1]])
+        if len(a) <= 3:
+            assert l3[cell] == 0 and a == b
+        else:
+            changed += 1
+            assert l3[cell] in (1, 2) and len(b) <= 3 and set(b) <= set(a) and b == sorted(b)
+    assert changed >= 1
