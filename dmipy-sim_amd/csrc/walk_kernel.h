@@ -914,23 +914,26 @@ __device__ __forceinline__ float packed_permeate(Pos& p, float sx, float sy, flo
 // ---------------------------------------------------------------------------
 // Screened packed permeation.  Each lane keeps the grid cell it walks in, as a
 // box expanded on every side by a.scr_w, or less for the few cells whose list
-// would not fit the cache (the host lists, for that box, every
-// object within one step of it), and, when the list holds at most SCR_MAXC
-// objects, each object's image centre, radius and f = |x - c|^2 - R^2 at the
-// walker.  A sub-step that ends inside the box and that the wall screen
-// (screen.h) clears for every cached object has no valid root in the reference's
-// test (geometries.py:2067-2093, 2667-2691): its result is r + s, computed here
-// exactly as the reference's no-hit branch does (geometries.py:2099-2102).  Only
-// the other lanes run the reference's root test, over the cached cell's list.
+// would not fit the cache (the host lists, for that box, every object within one
+// step of it), and, when the list holds at most SCR_MAXC objects, each object's
+// image centre and the side of its wall the walker is clear on (screen.h: the
+// pair rs, b).  A sub-step that ends inside the box and passes the screen for
+// every cached object has no valid root in the reference's test
+// (geometries.py:2067-2093, 2667-2691): its result is r + s, computed here
+// exactly as the reference's no-hit branch does (geometries.py:2099-2102), and
+// the screen state stays as it is.  Only the other lanes run the reference's
+// root test, over the cached cell's list, and then set their state anew.
 // ---------------------------------------------------------------------------
 template <int DIM>
 struct PackedScreen {
-  int cnt;                 // cached objects; SCR_MAXC + 1: not cached (longer list, or no cache)
   int beg, len;            // the cell's list in LDS
   float blo[DIM], bhi[DIM];
-  float cx[SCR_MAXC], cy[SCR_MAXC], cz[SCR_MAXC], R[SCR_MAXC], f[SCR_MAXC];
+  // per cached object: image centre, rs = -R (clear inside) or +R, and the pass
+  // term b (screen.h); unused slots always pass (b = +inf), and slot 0 of a lane
+  // whose list is not cached never passes (b = -inf)
+  float cx[SCR_MAXC], cy[SCR_MAXC], cz[SCR_MAXC], rs[SCR_MAXC], b[SCR_MAXC];
 
-  __device__ __forceinline__ int cached() const { return cnt > SCR_MAXC ? 0 : cnt; }
+  __device__ __forceinline__ bool uncached(const WalkArgs& a) const { return !a.scr_cache || len > SCR_MAXC; }
 
   __device__ __forceinline__ bool in_box(float x, float y, float z) const {
     bool in = (x > blo[0]) & (x < bhi[0]) & (y > blo[1]) & (y < bhi[1]);
@@ -938,16 +941,27 @@ struct PackedScreen {
     return in;
   }
 
-  __device__ __forceinline__ void set_f(const Pos& p) {
+  // Slot j's state at p for an object of radius R (unused and uncached slots as above).
+  __device__ __forceinline__ void slot_state(int j, const Pos& p, float R, const WalkArgs& a) {
+    const int nc = uncached(a) ? 0 : len;
+    float r, bb;
+    screen_state(screen_d2(p.x, p.y, p.z, cx[j], cy[j], cz[j], DIM), R, a.scr_m2, a.scr_cout, a.scr_cin, r, bb);
+    rs[j] = j < nc ? r : 1.0f;
+    b[j] = j < nc ? bb : ((j == 0 && uncached(a)) ? -__builtin_inff() : __builtin_inff());
+  }
+
+  // The state of every slot at p (same cell, images and radii): after the
+  // reference's test moved, reflected or stopped the walker.
+  __device__ __forceinline__ void set_state(const Pos& p, const WalkArgs& a) {
     static_for<SCR_MAXC>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      f[j] = screen_f(p.x, p.y, p.z, cx[j], cy[j], cz[j], R[j], DIM);
+      slot_state(j, p, fabsf(rs[j]), a);
     });
   }
 
   // The walker's cell (cell_of), the list and box level the cell's list start
   // carries, its box in the walker's frame (the cell expanded by that level's
-  // width), and the cached objects' nearest images.
+  // width), the cached objects' nearest images and the walker's state.
   __device__ __forceinline__ void refresh(const Pos& p, const WalkArgs& a, const Tables& tb) {
     const float L = a.cell, h = a.grid_h;
     const int gc = cell_of<DIM>(p, a);
@@ -964,46 +978,40 @@ struct PackedScreen {
       blo[d] = lo - w;
       bhi[d] = (lo + h) + w;
     });
-    cnt = (a.scr_cache && len <= SCR_MAXC) ? len : SCR_MAXC + 1;
+    const int nc = uncached(a) ? 0 : len;
     // every slot is written (unused ones with zeros), so no earlier value stays live
     static_for<SCR_MAXC>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      const bool use = j < cached();
+      const bool use = j < nc;
       const float4 o = use ? tb.objs[tb.items[beg + j]] : float4{0.0f, 0.0f, 0.0f, 0.0f};
       cx[j] = use ? o.x + L * floorf((v[0] - o.x) * a.inv_cell + 0.5f) : 0.0f;
       cy[j] = use ? o.y + L * floorf((v[1] - o.y) * a.inv_cell + 0.5f) : 0.0f;
       cz[j] = (DIM == 3 && use) ? o.z + L * floorf((v[2] - o.z) * a.inv_cell + 0.5f) : 0.0f;
-      R[j] = o.w;
-      f[j] = screen_f(p.x, p.y, p.z, cx[j], cy[j], cz[j], R[j], DIM);
+      slot_state(j, p, o.w, a);
     });
   }
 
-  // The sub-step to (nx, ny, nz) = p + s: returns the cached objects the screen
-  // cannot clear (bit j for slot j; bit SCR_MAXC when the list is not cached: all
-  // of it needs the reference's test) and sets `leave` when the end point is
-  // outside the cell box; fn receives f at the end point.  Branch-free.
-  __device__ __forceinline__ uint32_t test(float nx, float ny, float nz, const WalkArgs& a,
-                                           float (&fn)[SCR_MAXC], bool& leave) const {
-    const int nc = cached();
-    uint32_t flags = (uint32_t)(cnt > SCR_MAXC) << SCR_MAXC;
+  // True when the sub-step to (nx, ny, nz) = p + s passes for every cached object;
+  // `leave` is set when the end point is outside the cell box.  Branch-free.
+  __device__ __forceinline__ bool pass(float nx, float ny, float nz, bool& leave) const {
+    bool ok = true;
     static_for<SCR_MAXC>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      const float f1 = screen_f(nx, ny, nz, cx[j], cy[j], cz[j], R[j], DIM);
-      fn[j] = f1;
-      const uint32_t unclear = (uint32_t)!screen_clear(f[j], f1, R[j], a.scr_m2, a.scr_cout, a.scr_cin);
-      flags |= (unclear & (uint32_t)(j < nc)) << j;
+      ok = ok & screen_pass(screen_d2(nx, ny, nz, cx[j], cy[j], cz[j], DIM), rs[j], b[j]);
     });
     leave = !in_box(nx, ny, nz);
-    return flags;
+    return ok;
   }
 
-  __device__ __forceinline__ void commit(const float (&fn)[SCR_MAXC]) {
+  // The cached objects (bit j for slot j) the sub-step to (nx, ny, nz) does not pass.
+  __device__ __forceinline__ uint32_t flags(float nx, float ny, float nz) const {
+    uint32_t f = 0u;
     static_for<SCR_MAXC>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      f[j] = fn[j];
+      f |= (uint32_t)!screen_pass(screen_d2(nx, ny, nz, cx[j], cy[j], cz[j], DIM), rs[j], b[j]) << j;
     });
+    return f;
   }
-
 };
 
 // ---------------------------------------------------------------------------
@@ -1910,36 +1918,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
         const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
         const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
         if (SCREEN) {
-          // r + s, the reference's no-hit result, unless the screen cannot clear it
+          // r + s, the reference's no-hit result, unless the screen cannot pass it
           const float nx = p.x + sx, ny = p.y + sy, nz = p.z + sz;
-          float fn[SCR_MAXC];
           bool leave;
-          const uint32_t flags = scr.test(nx, ny, nz, a, fn, leave);
-          const bool exact = flags != 0u;
+          const bool exact = !scr.pass(nx, ny, nz, leave);
           dlog = 0.0f;
           if (__ballot(exact | leave) == 0ull) {
             p.x = nx; p.y = ny; p.z = nz;
-            scr.commit(fn);
           } else if (!a.scr_cache) {
             // no cached objects (lists too long to mask): the reference's test, every lane
             dlog = packed_permeate_list<DIM>(p, sx, sy, sz, a, tb, scr.beg, scr.beg + scr.len, ps, hits,
                                              ncross, ncand);
             if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);
           } else {
-            bool moved = false;
             if (exact) {
-              // the reference's root test over the objects the screen did not clear
+              // the reference's root test over the objects the screen did not pass
               // (the whole list when it is not cached)
-              const uint32_t m = (flags >> SCR_MAXC) ? (uint32_t)((1ull << scr.len) - 1ull) : flags;
+              const uint32_t m = scr.uncached(a) ? (uint32_t)((1ull << scr.len) - 1ull) : scr.flags(nx, ny, nz);
               dlog = packed_permeate_list<DIM, true>(p, sx, sy, sz, a, tb, scr.beg, 0, ps, hits, ncross,
                                                      ncand, m);
-              moved = !((p.x == nx) & (p.y == ny) & (p.z == nz));
             } else {
               p.x = nx; p.y = ny; p.z = nz;
             }
             if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);   // a new cell
-            else if (moved) scr.set_f(p);                            // reflected
-            else scr.commit(fn);
+            else if (exact) scr.set_state(p, a);                     // moved, reflected or stopped
           }
         } else {
           dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);

@@ -4,7 +4,7 @@
 // float32 root test on many configurations.
 //   screen_check_clear: for walker p[i], step s[i], object centre c[i] and radius
 //   R[i] (p and p + s in the same frame as the centre image the kernel caches),
-//   clear[i] = 1 when the screen clears the sub-step for that object.
+//   clear[i] = 1 when the screen state set at p passes the end point p + s.
 //   screen_check_grid: builds the walk's neighbour grid (csrc/grid.h) with the
 //   expanded cells of the screened path (each cell's box at the level the builder
 //   chose for it, as the walk reads it from the encoded list starts) and counts
@@ -32,9 +32,11 @@ void screen_check_clear(int64_t n, const float* p, const float* s, const float* 
     const float* si = s + 3 * i;
     const float* ci = c + 3 * i;
     const float nx = pi[0] + si[0], ny = pi[1] + si[1], nz = pi[2] + si[2];
-    const float f0 = dmw::screen_f(pi[0], pi[1], pi[2], ci[0], ci[1], ci[2], R[i], dim);
-    const float f1 = dmw::screen_f(nx, ny, nz, ci[0], ci[1], ci[2], R[i], dim);
-    clear[i] = dmw::screen_clear(f0, f1, R[i], m2, c_out, c_in) ? 1 : 0;
+    // the state at the start point, then the pass test at the end point
+    float rs, b;
+    dmw::screen_state(dmw::screen_d2(pi[0], pi[1], pi[2], ci[0], ci[1], ci[2], dim), R[i], m2, c_out, c_in,
+                      rs, b);
+    clear[i] = dmw::screen_pass(dmw::screen_d2(nx, ny, nz, ci[0], ci[1], ci[2], dim), rs, b) ? 1 : 0;
   }
 }
 
