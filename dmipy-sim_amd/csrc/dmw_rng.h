@@ -96,11 +96,14 @@ DMW_HD void sincospi_quarter(float f, float* s, float* c) {
 // square root's argument is 0 or at least 2^-22 (z on a 2^-23 grid), inside the
 // range where sqrt_rn equals sqrtf.
 __device__ __forceinline__ void iso_direction(uint32_t a, uint32_t b, float* ux, float* uy, float* uz) {
-  const float z = 1.0f - 2.0f * u01(a);           // exact, in (-1, 1]
-  const float s = sqrt_rn(fmaxf(1.0f - z * z, 0.0f));
-  const float t = 2.0f * u01(b);                  // azimuth / pi in [0, 2)
-  const float q = rintf(2.0f * t);                // quadrant 0..4
-  const float f = t - 0.5f * q;                   // exact, |f| <= 1/4
+  // z = 1 - 2 u01(a), t = 2 u01(b) and f = t - q/2 are exact, so their single
+  // fused forms below round nothing (checked for every 24-bit input); fl(z z) <= 1,
+  // so 1 - z z >= 0 needs no clamp.
+  const float z = fmaf((float)(a >> 8), -0x1p-23f, 1.0f);   // in (-1, 1]
+  const float s = sqrt_rn(1.0f - z * z);
+  const float t = (float)(b >> 8) * 0x1p-23f;                // azimuth / pi in [0, 2)
+  const float q = rintf(2.0f * t);                           // quadrant 0..4
+  const float f = fmaf(q, -0.5f, t);                         // |f| <= 1/4
   float sf, cf;
   sincospi_quarter(f, &sf, &cf);
   // quadrant qi: (cos, sin)(pi t) = (cf, sf), (-sf, cf), (-cf, -sf), (sf, -cf) for
