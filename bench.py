@@ -60,7 +60,7 @@ TRAFFIC_BYTES = {
     "cylinder_pgse": 17_374_635 + 32_172_843,
     "free_pgse": None,
     # one 100-sample launch of the 4e6-walker job (profiles/walk_kernel_packed_cylinders.md)
-    "packed_cylinders_free_waveform": 312_439_398_400 + 307_147_776_000,
+    "packed_cylinders_free_waveform": 312_248_042_413 + 307_345_779_053,
 }
 
 
