@@ -127,10 +127,12 @@ def run_batch(args):
     log(f"first launch: {st['total_ms']:.1f} ms, units {st['n_units']}, grid {st['grid']}")
     for i in range(args.warmup):
         bw.run(seed=2 + i)
+        log(f"warmup {i + 1}/{args.warmup} done")
     kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         kms.append(bw.run(seed=100 + i)[1]["total_ms"])
+        log(f"step {i + 1}/{args.steps}: {kms[-1]:.1f} ms")   # a launch takes ~25 s
     wall = time.perf_counter() - t0
     total_ws = st["walker_steps"]
     value = total_ws * args.steps / wall

@@ -5,7 +5,8 @@
   same Philox stream over the full 2000 x 558 sub-steps is bit-exact against the
   oracle, and its phase error against the exact (float64) phase of the same
   float32 trajectory is stated for the matrix-core and the compensated paths;
-  then the full 4e6-walker launch.
+  then the full 4e6-walker launch; walkers started in the cells whose screen box
+  the grid builder shrank walk bit-exactly too.
 * Cylinder PGSE (R 5 um, rho 1e-6 m/s, 1e6 walkers, 128 measurements, n_t 1000):
   a 4096-walker slice bit-exact against the oracle with its float64 phase error
   on the matrix-core, float32 VALU and compensated paths; the full launch against
@@ -100,6 +101,61 @@ def test_packed_cylinders_full_launch(gpu):
     assert S.shape == (90,) and np.all(np.isfinite(S))
     assert np.all(np.abs(S) <= 1.0) and np.all(S > 0.0)
     assert r["n_hits"] > 0
+
+
+def _shrunk_cells(geom, plan):
+    """Cells of the screened grid whose box the builder shrank (their half-side box
+    met more objects than the walker's three-object cache), from the host build of
+    the walk's own grid code (tests/gpu_selftest/screen_check.cpp)."""
+    import ctypes as C
+    import os
+    lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_selftest",
+                              "libscreen_check.so"))
+    fp, ip = C.POINTER(C.c_float), C.POINTER(C.c_int32)
+    fn = lib.screen_check_lists
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int32, fp, fp, C.c_float, C.c_float, C.c_float, C.c_int32, C.c_double, C.c_int32,
+                   ip, ip, ip, C.c_int64]
+    c = np.ascontiguousarray(geom._centers_np, np.float32)
+    r = np.ascontiguousarray(geom._radii_np, np.float32)
+    cap = 1 << 20
+    start, level, items = (np.zeros(cap, np.int32) for _ in range(3))
+    n_cells = fn(len(r), c.ctypes.data_as(fp), r.ctypes.data_as(fp), float(np.float32(geom._L_float)),
+                 plan.step_len, plan.geometry_desc().nudge, 2, 0.5, 3, start.ctypes.data_as(ip),
+                 level.ctypes.data_as(ip), items.ctypes.data_as(ip), cap)
+    n = int(round(np.sqrt(n_cells)))
+    assert n * n == n_cells
+    return n, np.nonzero(level[:n_cells])[0]
+
+
+def test_packed_cylinders_shrunk_boxes_bit_exact(gpu, oracle):
+    """Walkers started inside the cells whose box the grid builder shrank walk in
+    and out of those boxes (40 samples x 558 sub-steps of the headline job, ~1.4 um
+    of diffusion) bit-exactly against the all-object oracle."""
+    cfg = configs.packed_cylinders_free_waveform()
+    geom, wf, Dw, seed = cfg["geometry"], cfg["waveform"], cfg["diffusivity"], cfg["seed"]
+    plan = ds.make_walk_plan(geom, Dw, wf.dt)
+    n, cells = _shrunk_cells(geom, plan)
+    assert 1 <= len(cells) < 0.01 * n * n
+    L = float(np.float32(geom._L_float))
+    h = L / n
+    rng = np.random.default_rng(11)
+    per = 64
+    r0 = np.zeros((per * len(cells), 3), np.float32)
+    for i, c in enumerate(cells):
+        cx, cy = int(c) % n, int(c) // n
+        r0[i * per:(i + 1) * per, 0] = -0.5 * L + (cx + rng.uniform(0.02, 0.98, per)) * h
+        r0[i * per:(i + 1) * per, 1] = -0.5 * L + (cy + rng.uniform(0.02, 0.98, per)) * h
+    wf40 = ds.Waveform(G=np.ascontiguousarray(wf.G[:, :40]), dt=wf.dt, echo_idx=39)
+    N = len(r0)
+    print(f"packed cylinders, shrunk boxes: cells {cells.tolist()} of {n}x{n}, {N} walkers, "
+          f"40 x 558 sub-steps, seed {seed}", flush=True)
+    g = gpu.walk(geom, Dw, wf40, N, seed=seed, r0=r0, outputs=OUTS, counters=True)
+    print("packed cylinders, shrunk boxes: oracle", flush=True)
+    o = oracle.run_oracle(geom, Dw, wf40, N, seed=seed, r0=r0)
+    _same_walk(g, o)
+    np.testing.assert_allclose(g["signal"], o["signal"], rtol=0, atol=1e-5)
+    assert g["n_hits"] > 0
 
 
 # ------------------------------------------------------------ cylinder PGSE
