@@ -277,6 +277,7 @@ def run_surrogate(args):
             "mc": {"walkers": r["mc"]["walkers"], "rmse": r["mc"]["rmse"],
                    "signals_per_s": r["mc"]["signals_per_s"]},
             "gain_over_mc": r["gain"], "gain_hybrid": r["gain_hybrid"],
+            "mc_screened": r["mc_screened"], "gain_vs_screened_walk": r["gain_vs_screened"],
             "train_ms_total": r["fit"]["train_ms"], "wall_s": round(wall, 1)}
     print(json.dumps(line), flush=True)
 

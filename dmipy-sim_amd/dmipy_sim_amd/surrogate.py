@@ -565,6 +565,33 @@ def mc_ladder(n_inst, seed, ref_signals, walkers=(256, 512, 1024, 2048, 4096, 81
     return rows[-1][0], rows[-1][1], rows
 
 
+def screened_walk_rate(instances, waveforms, n_walkers=1_000_000, n_samples=40, seed=9):
+    """Walker-steps/s of the single walk on generator instances: one substrate
+    per launch with every measurement in one pass and the packed substrates'
+    wall screen, at a walker count that fills the GPU (PlannedWalk slices of
+    ``n_samples`` samples, device time).  Returns (walker-steps/s, mean n_sub).
+
+    The batched walk that produces the training data and the walker-count
+    ladder tests every listed object on every sub-step and walks 90
+    measurements as two instances of 45; this rate is what the screened walk
+    reaches per walker-step on the same substrates."""
+    from .core import PlannedWalk
+    ws, ms, n_subs = 0, 0.0, []
+    for inst, wf in zip(instances, waveforms):
+        pw = PlannedWalk(inst["geometry"], inst["diffusivity"], wf, n_walkers)
+        try:
+            n_sub = pw.plan.n_sub
+            pw.advance(n_samples, seed=seed)          # warm-up slice of the same walk
+            r = pw.advance(n_samples, seed=seed)
+            ws += n_walkers * n_samples * n_sub
+            ms += r["kernel_ms"]
+            n_subs.append(n_sub)
+            pw.rewind()
+        finally:
+            pw.close()
+    return ws / (ms * 1e-3), float(np.mean(n_subs))
+
+
 def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_000, epochs=60,
               batch=1024, members=4, width=512, depth=8, infer_rows=65536, threshold=5e-3,
               seed=1, target="log", lr=1e-3, log=None, keep_ensemble=False):
@@ -634,6 +661,18 @@ def benchmark(n_train_inst=3125, n_walkers=10_000, n_held=32, held_walkers=200_0
     res["gain_hybrid"] = eff / mc_sps
     log(f"walk at rmse<={threshold}: {nw} walkers, {mc_sps:.3e} signals/s; gain {res['gain']:.0f}x "
         f"(hybrid {res['gain_hybrid']:.0f}x)")
+    # A conservative reference: the screened single walk's walker-steps/s on four
+    # held-out substrates, as signals/s at nw walkers per substrate (all 90
+    # measurements from one walk), as if small substrates could share the GPU
+    # at the rate a large walk reaches.
+    k = min(4, len(insts))
+    sw_rate, sw_nsub = screened_walk_rate(insts[:k], waves[:k])
+    sw_sps = sw_rate / (nw * waves[0].G.shape[1] * sw_nsub) * waves[0].G.shape[0]
+    res["mc_screened"] = dict(walker_steps_per_s=sw_rate, n_sub_mean=sw_nsub, walkers=nw,
+                              signals_per_s=sw_sps, substrates=k)
+    res["gain_vs_screened"] = sur_sps / sw_sps
+    log(f"screened single walk: {sw_rate:.3e} walker-steps/s (n_sub {sw_nsub:.0f}) -> {sw_sps:.3e} "
+        f"signals/s at {nw} walkers; gain {res['gain_vs_screened']:.0f}x")
     if keep_ensemble:
         res["ensemble"] = ens
     else:

@@ -7,7 +7,9 @@ GELU networks trained on the GPU by the library's own kernels, judged against
 held-out walks of 2e5 walkers.  The test asserts what the surrogate is for: at
 least 100x the walk's signals/s at the walker count the walk needs to reach the
 same RMSE bound (5e-3 in S/S0), with the hybrid (walk wherever the ensemble
-spread exceeds the bound) inside that bound.
+spread exceeds the bound) inside that bound; and at least 100x a conservative
+reference, the screened single walk's walker-steps/s on held-out substrates
+turned into signals/s at that walker count.
 """
 import numpy as np
 import pytest
@@ -26,7 +28,8 @@ def test_surrogate_gain_over_walk(gpu):
                      log=lambda s: print(s, flush=True), keep_ensemble=True)
     ens = r.pop("ensemble")
     print({k: r[k] for k in ("rmse_surrogate", "rmse_hybrid", "fallback_fraction",
-                             "fallback_instance_fraction", "gain", "gain_hybrid")}, flush=True)
+                             "fallback_instance_fraction", "gain", "gain_hybrid",
+                             "gain_vs_screened")}, flush=True)
     # the packed-cylinder workload's own substrate (64 cylinders, VF 0.6, kappa 1e-5,
     # no relaxivity) under 90 free waveforms of the generator's family
     geom = configs.packed_cylinders_substrate()
@@ -41,6 +44,8 @@ def test_surrogate_gain_over_walk(gpu):
     assert r["mc"]["rmse"] <= 5e-3 or r["mc"]["walkers"] == 16384
     assert r["gain"] >= 100.0
     assert r["gain_hybrid"] >= 100.0
+    # also against the screened single walk's walker-step rate, a conservative reference
+    assert r["gain_vs_screened"] >= 100.0
     # the ensemble spread is informative: flagged rows are a small minority
     assert r["fallback_fraction"] <= 0.05
     assert rmse_ws <= 5e-3
