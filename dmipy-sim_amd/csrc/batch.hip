@@ -29,6 +29,11 @@
 #include "phase16.h"
 #include "walk_kernel.h"
 
+// Occupancy budget of the screened kernel (wavefronts per SIMD).
+#ifndef DMW_SCREENED_WAVES
+#define DMW_SCREENED_WAVES 4
+#endif
+
 namespace dmw {
 
 struct InstDev {
@@ -36,7 +41,11 @@ struct InstDev {
   float inv_size;            // RN(1 / size)
   float size_sq_cut;         // least float s with sqrtf(s) >= size
   float size_sq_cut_gt;      // least float s with sqrtf(s) > size
-  float cell, inv_cell, grid_inv_h;
+  float cell, inv_cell, grid_inv_h, grid_h;
+  // screened permeable packings (walk_kernel.h PackedScreen): cache flag, box
+  // width, margin terms
+  int scr_cache;
+  float scr_w, scr_m2, scr_cout, scr_cin;
   float ax, ay, az, aux;
   int kind, wall, n_sub, has_t2;
   int grid_n, exact_minimage, n_obj;
@@ -69,7 +78,7 @@ struct BatchArgs {
   double* partials;         // (tiles, n_meas_pad, 2), row unit_begin + tile of the instance
 };
 
-template <int GEOM, int WALL, int MB, int UNROLL>
+template <int GEOM, int WALL, int MB, int UNROLL, bool SCR = false>
 __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, int tile, int unit,
                                           unsigned long long base_seed) {
   const unsigned long long seed = base_seed + I.seed_offset;
@@ -77,6 +86,9 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   const uint32_t inst = I.stream;
   constexpr bool PACKED = (GEOM == G_PCYL || GEOM == G_PSPH);
   constexpr int DIM = (GEOM == G_PSPH) ? 3 : 2;
+  // permeable packings walk the wall screen, as the single walk does (in the
+  // screened kernel, which takes exactly those instances)
+  constexpr bool SCREEN = SCR && PACKED && WALL == W_PERMEATE;
   const int lane = threadIdx.x & (kWave - 1);
   const long long widx = (long long)tile * kWave + lane;
   const bool active = widx < I.n_walkers;
@@ -85,6 +97,9 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.size = I.size; a.inv_size = I.inv_size; a.size_sq_cut = I.size_sq_cut;
   a.size_sq_cut_gt = I.size_sq_cut_gt; a.eps = I.eps; a.nudge = I.nudge; a.rhoD = I.rhoD; a.kappaD = I.kappaD;
   a.cell = I.cell; a.inv_cell = I.inv_cell; a.grid_n = I.grid_n; a.grid_inv_h = I.grid_inv_h;
+  a.grid_h = I.grid_h;
+  a.scr_cache = I.scr_cache; a.scr_w = I.scr_w; a.scr_m2 = I.scr_m2; a.scr_cout = I.scr_cout;
+  a.scr_cin = I.scr_cin;
   a.exact_minimage = I.exact_minimage; a.n_obj = I.n_obj;
   a.ax = I.ax; a.ay = I.ay; a.az = I.az; a.aux = I.aux;
   a.instance = inst; a.seed_lo = seed_lo; a.seed_hi = seed_hi;
@@ -110,6 +125,8 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
                (fsub * (p.z + L * (float)p.cz)) * I.gdt);
   else
     ph.set_ref(a, 0, lane, (fsub * p.x) * I.gdt, (fsub * p.y) * I.gdt, (fsub * p.z) * I.gdt);
+  PackedScreen<DIM> scr;
+  if (SCREEN) scr.refresh(p, a, tb);
   u32x4 rnd{0u, 0u, 0u, 0u};
   for (int t = 0; t < b.n_t; ++t) {
     ph.prefetch(a, t, 0, lane);
@@ -126,7 +143,8 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
       iso_direction((gs & 1u) ? rnd.z : rnd.x, (gs & 1u) ? rnd.w : rnd.y, &ux, &uy, &uz);
       const float sx = ux * I.step_len, sy = uy * I.step_len, sz = uz * I.step_len;
       const PowlesSrc ps{w, inst, gs, seed_lo, seed_hi, 0.0f, false};
-      float dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
+      float dlog = SCREEN ? screened_permeate<DIM>(p, sx, sy, sz, a, tb, scr, ps, hits, ncross, ncand)
+                          : wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
       if (WALL != W_REFLECT) dlog = dlog * chi;
       dlog = dlog - q2chi;
       logw = logw + dlog;
@@ -147,6 +165,7 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
     }
     ph.update(a, t, 0, lane, Px * I.gdt, Py * I.gdt, Pz * I.gdt);
     if (PACKED) fold<DIM>(p, a);
+    if (SCREEN) scr.refresh(p, a, tb);   // the folded frame: cell, box and images anew
   }
   const float wgt = expf(logw);
   ph.emit(a, 0, wgt, active, lane, b.partials + (size_t)unit * b.n_meas_pad * 2);
@@ -158,9 +177,12 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
 // Sphere/Cylinder batch runs in 1281 ms against 1354 ms at three (168 VGPRs, 68 bytes of
 // scratch) and 1652 ms at two (187 VGPRs): the per-unit instance loads and dispatch are
 // latency that more resident wavefronts hide.  Five (96 VGPRs, 372 bytes of scratch) is
-// slower again, 1314 ms.
-template <int MB, int UNROLL>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void batch_kernel(const BatchArgs b) {
+// slower again, 1314 ms.  Permeable packings walk in a kernel of their own
+// (SCREENED): the wall screen's per-walker state would raise every other geometry's
+// register pressure in a shared kernel.
+template <int MB, int UNROLL, bool SCREENED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(SCREENED ? DMW_SCREENED_WAVES : 4)))
+void batch_kernel(const BatchArgs b) {
   const unsigned long long seed = *b.seed;
   const unsigned int inc = (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u;
   for (;;) {
@@ -179,6 +201,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     for (int tile = u.y; tile < tile_end; ++tile) {
     // re-read per tile (scalar loads) rather than held across the walk
     const InstDev I = b.inst[u.x];
+    if (SCREENED) {
+      if (I.kind == G_PCYL)
+        walk_unit<G_PCYL, W_PERMEATE, MB, UNROLL, true>(b, I, tile, I.unit_begin + tile, seed);
+      else
+        walk_unit<G_PSPH, W_PERMEATE, MB, UNROLL, true>(b, I, tile, I.unit_begin + tile, seed);
+      continue;
+    }
     switch (I.kind * 3 + I.wall) {
 #define DMW_UNIT(G, W) \
       case G * 3 + W: walk_unit<G, W, MB, UNROLL>(b, I, tile, I.unit_begin + tile, seed); break;
@@ -193,10 +222,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       DMW_UNIT(G_SPH, W_PERMEATE)
       DMW_UNIT(G_PCYL, W_REFLECT)
       DMW_UNIT(G_PCYL, W_SURFACE)
-      DMW_UNIT(G_PCYL, W_PERMEATE)
       DMW_UNIT(G_PSPH, W_REFLECT)
       DMW_UNIT(G_PSPH, W_SURFACE)
-      DMW_UNIT(G_PSPH, W_PERMEATE)
       DMW_UNIT(G_ELL, W_REFLECT)
       DMW_UNIT(G_ELL, W_SURFACE)
       DMW_UNIT(G_ELL, W_PERMEATE)
@@ -214,7 +241,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   }
 }
 
-__global__ void batch_reset_kernel(unsigned int* queue) { *queue = 0u; }
+// The two queue counters (general kernel, screened kernel).
+__global__ void batch_reset_kernel(unsigned int* queue) { queue[0] = 0u; queue[1] = 0u; }
 
 // Fixed-order per-instance sum: one workgroup per instance, one thread per column.
 __global__ __launch_bounds__(256) void batch_reduce_kernel(const InstDev* __restrict__ inst,
@@ -236,6 +264,7 @@ __global__ __launch_bounds__(256) void batch_reduce_kernel(const InstDev* __rest
 // ---------------------------------------------------------------------------
 struct dmw_batch {
   int n_inst = 0, n_units = 0, n_meas = 0, n_meas_pad = 0, mb = 64, grid = 0, unroll = 1;
+  int grid_scr = 0;               // persistent grid of the screened kernel (0: no such instances)
   uint64_t walker_steps = 0;
   std::vector<void*> allocs;
   void *d_inst = nullptr, *d_seed = nullptr, *d_queue = nullptr, *d_part = nullptr, *d_out = nullptr;
@@ -245,7 +274,7 @@ struct dmw_batch {
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool use_graph = true;
-  dmw::BatchArgs args{};
+  dmw::BatchArgs args{}, args_scr{};   // general kernel, screened kernel
   ~dmw_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -272,17 +301,33 @@ hipError_t enqueue_batch(dmw_batch* B) {
   hipLaunchKernelGGL(dmw::batch_reset_kernel, dim3(1), dim3(1), 0, B->st, (unsigned int*)B->d_queue);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const dim3 g(B->grid), blk(dmw::kBlock);
-  if (B->mb == 16 && B->unroll == 2)
-    hipLaunchKernelGGL((dmw::batch_kernel<16, 2>), g, blk, 0, B->st, B->args);
-  else if (B->mb == 16)
-    hipLaunchKernelGGL((dmw::batch_kernel<16, 1>), g, blk, 0, B->st, B->args);
-  else if (B->unroll == 2)
-    hipLaunchKernelGGL((dmw::batch_kernel<64, 2>), g, blk, 0, B->st, B->args);
-  else
-    hipLaunchKernelGGL((dmw::batch_kernel<64, 1>), g, blk, 0, B->st, B->args);
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  const dim3 blk(dmw::kBlock);
+  if (B->grid > 0) {
+    const dim3 g(B->grid);
+    if (B->mb == 16 && B->unroll == 2)
+      hipLaunchKernelGGL((dmw::batch_kernel<16, 2, false>), g, blk, 0, B->st, B->args);
+    else if (B->mb == 16)
+      hipLaunchKernelGGL((dmw::batch_kernel<16, 1, false>), g, blk, 0, B->st, B->args);
+    else if (B->unroll == 2)
+      hipLaunchKernelGGL((dmw::batch_kernel<64, 2, false>), g, blk, 0, B->st, B->args);
+    else
+      hipLaunchKernelGGL((dmw::batch_kernel<64, 1, false>), g, blk, 0, B->st, B->args);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (B->grid_scr > 0) {
+    const dim3 g(B->grid_scr);
+    if (B->mb == 16 && B->unroll == 2)
+      hipLaunchKernelGGL((dmw::batch_kernel<16, 2, true>), g, blk, 0, B->st, B->args_scr);
+    else if (B->mb == 16)
+      hipLaunchKernelGGL((dmw::batch_kernel<16, 1, true>), g, blk, 0, B->st, B->args_scr);
+    else if (B->unroll == 2)
+      hipLaunchKernelGGL((dmw::batch_kernel<64, 2, true>), g, blk, 0, B->st, B->args_scr);
+    else
+      hipLaunchKernelGGL((dmw::batch_kernel<64, 1, true>), g, blk, 0, B->st, B->args_scr);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(dmw::batch_reduce_kernel, dim3(B->n_inst), dim3(256), 0, B->st,
                      (const dmw::InstDev*)B->d_inst, (const double*)B->d_part, B->n_meas_pad * 2,
                      (double*)B->d_out);
@@ -344,6 +389,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   std::vector<dmw::InstDev> hi(n_inst);
   std::vector<int2> units;
   std::vector<double> cost;
+  std::vector<char> screened_unit;   // the entry's instance walks in the screened kernel
   std::vector<float4> objs;
   std::vector<int> starts, items;
   int n_tiles = 0;
@@ -369,11 +415,16 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
     d.gh_off = (long long)I.wave_set * n_t * 2 * ((Mp + 31) / 32 * 32);  // phase_half_table's size per set
     d.gsum_off = (long long)I.wave_set * 3 * Mp;
     d.p_scale = (float)dmw::phase_p_scale(n_t, I.n_sub, I.step_len, I.gamma_dt_sub);
-    d.grid_n = 1; d.grid_inv_h = 0.0f; d.exact_minimage = 0; d.n_obj = 0;
+    d.grid_n = 1; d.grid_inv_h = 0.0f; d.grid_h = 0.0f; d.exact_minimage = 0; d.n_obj = 0;
+    d.scr_cache = 0; d.scr_w = d.scr_m2 = d.scr_cout = d.scr_cin = 0.0f;
     d.obj_off = d.start_off = d.items_off = 0;
     if (I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES) {
       const int dim = I.geom.kind == DMW_PACKED_CYLINDERS ? 2 : 3;
-      Grid G = build_grid(I.geom, I.step_len, dim);
+      // permeable packings: cells expanded by half a side for the screen, less where
+      // a list would exceed its object cache (the box level rides in the list start)
+      const bool screened = I.geom.wall == DMW_WALL_PERMEATE;
+      Grid G = build_grid(I.geom, I.step_len, dim, 60 * 1024, 16, screened ? 0.5 : 0.0,
+                          screened ? dmw::SCR_MAXC : 0);
       d.obj_off = (int)objs.size();
       d.start_off = (int)starts.size();
       d.items_off = (int)items.size();
@@ -384,18 +435,35 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
         if (dim == 3) o.z = I.geom.centers[k * dim + 2];
         objs.push_back(o);
       }
-      starts.insert(starts.end(), G.start.begin(), G.start.end());
+      const std::vector<int> enc = G.encoded_start();
+      starts.insert(starts.end(), enc.begin(), enc.end());
       items.insert(items.end(), G.items.begin(), G.items.end());
       d.grid_n = G.n;
       d.grid_inv_h = 1.0f / G.h;
+      d.grid_h = G.h;
       d.exact_minimage = G.exact_minimage ? 1 : 0;
       d.n_obj = I.geom.n_obj;
+      if (screened) {
+        double rmax = 0.0;
+        for (int k = 0; k < I.geom.n_obj; ++k) rmax = std::max(rmax, (double)I.geom.radii[k]);
+        const double m = dmw::screen_margin(I.geom.cell, rmax, I.step_len);
+        int max_len = 0;
+        for (size_t c = 0; c + 1 < G.start.size(); ++c) max_len = std::max(max_len, G.start[c + 1] - G.start[c]);
+        d.scr_cache = (G.exact_minimage || max_len > 32) ? 0 : 1;
+        d.scr_w = (float)(0.5 * G.h);
+        d.scr_m2 = (float)(2.0 * m);
+        d.scr_cin = (float)(m * m);
+        d.scr_cout = (float)(m * m + 0.25 * (double)I.step_len * (double)I.step_len * (1.0 + 0x1p-20));
+      }
     }
     d.unit_begin = n_tiles;
     const int tiles = (int)((I.n_walkers + dmw::kWave - 1) / dmw::kWave);
     const double packed_cost = (I.geom.kind >= DMW_PACKED_CYLINDERS) ? 2.0 : 1.0;
+    const bool scr_inst = (I.geom.kind == DMW_PACKED_CYLINDERS || I.geom.kind == DMW_PACKED_SPHERES) &&
+                          I.geom.wall == DMW_WALL_PERMEATE;
     for (int j = 0; j < tiles; j += tpp) {
       units.push_back(make_int2(i, j));
+      screened_unit.push_back(scr_inst ? 1 : 0);
       cost.push_back((double)std::min(tpp, tiles - j) * I.n_sub * packed_cost *
                      (I.geom.wall == DMW_WALL_PERMEATE ? 1.3 : 1.0));
     }
@@ -407,9 +475,17 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   const int n_entries = (int)units.size();
   std::vector<int> order(n_entries);
   std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+  // two queues, each in decreasing cost: the general kernel's entries, then the
+  // screened kernel's
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (screened_unit[x] != screened_unit[y]) return screened_unit[x] < screened_unit[y];
+    return cost[x] > cost[y];
+  });
   std::vector<int2> q_units(n_entries);
   for (int k = 0; k < n_entries; ++k) q_units[k] = units[order[k]];
+  int n_scr = 0;
+  for (char c : screened_unit) n_scr += c;
+  const int n_gen = n_entries - n_scr;
 
   std::vector<float> G((size_t)n_sets * n_t * Mp * 3, 0.0f);
   for (int s = 0; s < n_sets; ++s)
@@ -438,7 +514,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) n_cu = prop.multiProcessorCount;
   const int wps = tn.waves_per_simd > 0 ? tn.waves_per_simd : 4;  // the kernel's occupancy budget
-  B->grid = std::max(1, std::min(n_cu * wps, (n_entries + 3) / 4));
+  B->grid = n_gen > 0 ? std::max(1, std::min(n_cu * wps, (n_gen + 3) / 4)) : 0;
+  B->grid_scr = n_scr > 0 ? std::max(1, std::min(n_cu * DMW_SCREENED_WAVES, (n_scr + 3) / 4)) : 0;
 
   auto fail_free = [&](hipError_t err, const char* what) {
     delete B;
@@ -461,7 +538,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   BT(B->upload(&d_starts, starts.data(), starts.size() * 4), "batch upload");
   BT(B->upload(&d_items, items.data(), items.size() * 4), "batch upload");
   BT(B->upload(&B->d_seed, nullptr, 8), "batch alloc");
-  BT(B->upload(&B->d_queue, nullptr, 4), "batch alloc");
+  BT(B->upload(&B->d_queue, nullptr, 8), "batch alloc");
   BT(B->upload(&B->d_part, nullptr, (size_t)B->n_units * Mp * 2 * 8), "batch alloc");
   BT(B->upload(&B->d_out, nullptr, (size_t)n_inst * Mp * 2 * 8), "batch alloc");
   BT(hipHostMalloc((void**)&B->h_seed, 8, hipHostMallocDefault), "hipHostMalloc");
@@ -471,7 +548,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   dmw::BatchArgs& a = B->args;
   a.inst = (const dmw::InstDev*)B->d_inst;
   a.units = (const int2*)d_units;
-  a.n_units = n_entries;
+  a.n_units = n_gen;
   a.tiles_per_pop = tpp;
   a.G = (const float*)d_G;
   a.GH = (const uint2*)d_GH;
@@ -486,6 +563,10 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   a.seed = (const unsigned long long*)B->d_seed;
   a.queue = (unsigned int*)B->d_queue;
   a.partials = (double*)B->d_part;
+  B->args_scr = a;
+  B->args_scr.units = (const int2*)d_units + n_gen;
+  B->args_scr.n_units = n_scr;
+  B->args_scr.queue = (unsigned int*)B->d_queue + 1;
 
   if (B->use_graph) {
     BT(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
@@ -530,7 +611,7 @@ int dmw_batch_execute(dmw_batch* B, uint64_t seed, double* signals, dmw_batch_st
     stats->kernel_ms = ms;
     stats->walker_steps = B->walker_steps;
     stats->n_units = B->n_units;
-    stats->grid = B->grid;
+    stats->grid = std::max(B->grid, B->grid_scr);
   }
   return DMW_OK;
 }

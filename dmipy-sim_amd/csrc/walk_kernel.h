@@ -1014,6 +1014,41 @@ struct PackedScreen {
   }
 };
 
+// One screened sub-step of a permeable packing (the wavefront's lanes converged):
+// r + s, the reference's no-hit result, unless the screen cannot pass it; then the
+// reference's root test over the objects the screen did not pass and the walker's
+// new screen state.  Returns the sub-step's unit log-weight.
+template <int DIM>
+__device__ __forceinline__ float screened_permeate(Pos& p, float sx, float sy, float sz, const WalkArgs& a,
+                                                   const Tables& tb, PackedScreen<DIM>& scr,
+                                                   const PowlesSrc& ps, uint32_t& hits, uint32_t& ncross,
+                                                   uint32_t& ncand) {
+  const float nx = p.x + sx, ny = p.y + sy, nz = p.z + sz;
+  bool leave;
+  const bool exact = !scr.pass(nx, ny, nz, leave);
+  float dlog = 0.0f;
+  if (__ballot(exact | leave) == 0ull) {
+    p.x = nx; p.y = ny; p.z = nz;
+  } else if (!a.scr_cache) {
+    // no cached objects (lists too long to mask): the reference's test, every lane
+    dlog = packed_permeate_list<DIM>(p, sx, sy, sz, a, tb, scr.beg, scr.beg + scr.len, ps, hits, ncross,
+                                     ncand);
+    if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);
+  } else {
+    if (exact) {
+      // the reference's root test over the objects the screen did not pass
+      // (the whole list when it is not cached)
+      const uint32_t m = scr.uncached(a) ? (uint32_t)((1ull << scr.len) - 1ull) : scr.flags(nx, ny, nz);
+      dlog = packed_permeate_list<DIM, true>(p, sx, sy, sz, a, tb, scr.beg, 0, ps, hits, ncross, ncand, m);
+    } else {
+      p.x = nx; p.y = ny; p.z = nz;
+    }
+    if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);   // a new cell
+    else if (exact) scr.set_state(p, a);                     // moved, reflected or stopped
+  }
+  return dlog;
+}
+
 // ---------------------------------------------------------------------------
 // PackedMyelinatedCylinders (make_packed_myelin_traj_step_fn, physics.py:682-978,
 // without magnetisation transfer, as the fused forward calls it).  Walker
@@ -1918,31 +1953,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(PARK ? 4
         const float sl = (GEOM == G_MYEL) ? pick3(p.cx, a.m_step[0], a.m_step[1], a.m_step[2]) : a.step_len;
         const float sx = ux * sl, sy = uy * sl, sz = uz * sl;
         if (SCREEN) {
-          // r + s, the reference's no-hit result, unless the screen cannot pass it
-          const float nx = p.x + sx, ny = p.y + sy, nz = p.z + sz;
-          bool leave;
-          const bool exact = !scr.pass(nx, ny, nz, leave);
-          dlog = 0.0f;
-          if (__ballot(exact | leave) == 0ull) {
-            p.x = nx; p.y = ny; p.z = nz;
-          } else if (!a.scr_cache) {
-            // no cached objects (lists too long to mask): the reference's test, every lane
-            dlog = packed_permeate_list<DIM>(p, sx, sy, sz, a, tb, scr.beg, scr.beg + scr.len, ps, hits,
-                                             ncross, ncand);
-            if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);
-          } else {
-            if (exact) {
-              // the reference's root test over the objects the screen did not pass
-              // (the whole list when it is not cached)
-              const uint32_t m = scr.uncached(a) ? (uint32_t)((1ull << scr.len) - 1ull) : scr.flags(nx, ny, nz);
-              dlog = packed_permeate_list<DIM, true>(p, sx, sy, sz, a, tb, scr.beg, 0, ps, hits, ncross,
-                                                     ncand, m);
-            } else {
-              p.x = nx; p.y = ny; p.z = nz;
-            }
-            if (!scr.in_box(p.x, p.y, p.z)) scr.refresh(p, a, tb);   // a new cell
-            else if (exact) scr.set_state(p, a);                     // moved, reflected or stopped
-          }
+          dlog = screened_permeate<DIM>(p, sx, sy, sz, a, tb, scr, ps, hits, ncross, ncand);
         } else {
           dlog = wall_step<GEOM, WALL>(p, sx, sy, sz, a, tb, ps, hits, ncross, ncand);
         }
