@@ -76,6 +76,7 @@ struct BatchArgs {
   const unsigned long long* seed;  // device, read at kernel start
   unsigned int* queue;      // work counter
   double* partials;         // (tiles, n_meas_pad, 2), row unit_begin + tile of the instance
+  float4* park;             // screened kernel: parked phase rows of every resident wavefront
 };
 
 template <int GEOM, int WALL, int MB, int UNROLL, bool SCR = false>
@@ -109,10 +110,14 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   a.p_scale = I.p_scale;
   a.inv_pg_scale = I.inv_pg_scale;
   a.n_meas_pad = b.n_meas_pad;
+  a.n_meas = b.n_meas;
+  a.park = b.park;
   Tables tb{nullptr, nullptr, nullptr};
   if (PACKED) tb = Tables{b.objs + I.obj_off, b.starts + I.start_off, b.items + I.items_off};
   Pos p = sample_start<GEOM>(w, a, tb);
-  PhaseAcc<MB, false> ph;
+  // the screened kernel's 64-measurement variant parks the compensated phase in
+  // device memory between samples (its sub-step loop then holds no phase registers)
+  PhaseAcc<MB, false, SCREEN && MB >= 32> ph;
   ph.init();
   float logw = 0.0f;
   uint32_t hits = 0, ncross = 0, ncand = 0;
@@ -540,6 +545,14 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   BT(B->upload(&B->d_seed, nullptr, 8), "batch alloc");
   BT(B->upload(&B->d_queue, nullptr, 8), "batch alloc");
   BT(B->upload(&B->d_part, nullptr, (size_t)B->n_units * Mp * 2 * 8), "batch alloc");
+  // the parked phase writes partials only up to n_meas rounded to 16: the padding
+  // columns stay the zeros written here
+  BT(hipMemsetAsync(B->d_part, 0, (size_t)B->n_units * Mp * 2 * 8, B->st), "hipMemset");
+  BT(hipStreamSynchronize(B->st), "hipMemset");
+  void* d_park = nullptr;
+  if (B->grid_scr > 0)
+    BT(B->upload(&d_park, nullptr, (size_t)B->grid_scr * (dmw::kBlock / dmw::kWave) * Mp * 32 * sizeof(float4)),
+       "batch alloc");
   BT(B->upload(&B->d_out, nullptr, (size_t)n_inst * Mp * 2 * 8), "batch alloc");
   BT(hipHostMalloc((void**)&B->h_seed, 8, hipHostMallocDefault), "hipHostMalloc");
   BT(hipEventCreate(&B->ev0), "hipEventCreate");
@@ -564,6 +577,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   a.queue = (unsigned int*)B->d_queue;
   a.partials = (double*)B->d_part;
   B->args_scr = a;
+  B->args_scr.park = (float4*)d_park;
   B->args_scr.units = (const int2*)d_units + n_gen;
   B->args_scr.n_units = n_scr;
   B->args_scr.queue = (unsigned int*)B->d_queue + 1;
