@@ -322,7 +322,11 @@ hipError_t enqueue_batch(dmw_batch* B) {
   }
   if (B->grid_scr > 0) {
     const dim3 g(B->grid_scr);
-    if (B->mb == 16 && B->unroll == 2)
+    if (B->mb == 128 && B->unroll == 2)
+      hipLaunchKernelGGL((dmw::batch_kernel<128, 2, true>), g, blk, 0, B->st, B->args_scr);
+    else if (B->mb == 128)
+      hipLaunchKernelGGL((dmw::batch_kernel<128, 1, true>), g, blk, 0, B->st, B->args_scr);
+    else if (B->mb == 16 && B->unroll == 2)
       hipLaunchKernelGGL((dmw::batch_kernel<16, 2, true>), g, blk, 0, B->st, B->args_scr);
     else if (B->mb == 16)
       hipLaunchKernelGGL((dmw::batch_kernel<16, 1, true>), g, blk, 0, B->st, B->args_scr);
@@ -357,8 +361,18 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   if (unroll != 1 && unroll != 2) return dmw::batch_fail(DMW_EINVAL, "unroll must be 0 (1), 1 or 2");
   const int tpp = wpw / dmw::kWave;
   const int n_meas = sets[0].n_meas, n_t = sets[0].n_t;
-  if (n_meas < 1 || n_meas > 64 || n_t < 1)
-    return dmw::batch_fail(DMW_EINVAL, "batched launch needs 1..64 measurements");
+  // more than 64 measurements: only the screened kernel (its phase parked in device
+  // memory) takes them, so every instance must be a permeable packing
+  bool all_screened = true;
+  for (int i = 0; i < n_inst; ++i)
+    all_screened = all_screened &&
+                   (inst[i].geom.kind == DMW_PACKED_CYLINDERS || inst[i].geom.kind == DMW_PACKED_SPHERES) &&
+                   inst[i].geom.wall == DMW_WALL_PERMEATE;
+  if (n_meas < 1 || n_t < 1 || n_meas > 128)
+    return dmw::batch_fail(DMW_EINVAL, "batched launch needs 1..128 measurements");
+  if (n_meas > 64 && !all_screened)
+    return dmw::batch_fail(DMW_EINVAL,
+                           "batched launch of more than 64 measurements needs every instance to be a permeable packing");
   for (int s = 0; s < n_sets; ++s)
     if (sets[s].n_meas != n_meas || sets[s].n_t != n_t || !sets[s].G)
       return dmw::batch_fail(DMW_EINVAL, "every waveform set needs the same n_meas and n_t");
@@ -385,7 +399,7 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   dmw_batch* B = new dmw_batch();
   B->n_inst = n_inst;
   B->n_meas = n_meas;
-  B->mb = n_meas <= 16 ? 16 : 64;
+  B->mb = n_meas <= 16 ? 16 : (n_meas <= 64 ? 64 : 128);
   B->unroll = unroll;
   B->n_meas_pad = B->mb;
   B->use_graph = use_graph != 0;

@@ -3,8 +3,9 @@
 Each instance is any geometry the walk supports (free, Box1D, Cylinder, Sphere,
 PackedCylinders, PackedSpheres; reflecting, relaxing or permeable walls) with
 its own diffusivity, T2, walker count and sub-step count, and names one of a
-list of waveforms (all with the same number of measurements, <= 64, and the
-same n_t).  The result for instance i equals, in distribution, a separate
+list of waveforms (all with the same number of measurements, <= 64 -- up to 128
+when every instance is a permeable packing, whose kernel parks the phase in
+device memory -- and the same n_t).  The result for instance i equals, in distribution, a separate
 ``simulate(n_walkers_i, D_i, waveforms[k_i], geometry_i)`` call
 (dmipy_sim/core.py:193) -- the serial per-substrate loop the reference runs,
 e.g. simulate_mixture (core.py:821-858).  By default instance i uses the Philox
@@ -17,6 +18,13 @@ import numpy as np
 
 from . import _native as nat
 from .physics import make_walk_plan
+
+
+def _screened(geometry):
+    """True for the permeable packings, which walk in the screened kernel."""
+    from . import geometries as geo
+    return (isinstance(geometry, (geo.PackedCylinders, geo.PackedSpheres))
+            and getattr(geometry, "permeability", None) is not None)
 
 
 class BatchedWalk:
@@ -46,8 +54,11 @@ class BatchedWalk:
         for G in Gs:
             if G.shape != (n_meas, n_t, 3):
                 raise ValueError("every waveform needs the same (n_measurements, n_t)")
-        if n_meas > 64:
-            raise ValueError("batched walks take at most 64 measurements per waveform")
+        if n_meas > 128:
+            raise ValueError("batched walks take at most 128 measurements per waveform")
+        if n_meas > 64 and not all(_screened(s["geometry"]) for s in instances):
+            raise ValueError("batched walks of more than 64 measurements need every instance "
+                             "to be a permeable packing")
         chis = [getattr(w, 'chi_perp', None) for w in waveforms]
         chi0 = None if chis[0] is None else np.asarray(chis[0], np.float32).reshape(n_t)
         for c in chis[1:]:

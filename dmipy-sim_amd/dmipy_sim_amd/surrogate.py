@@ -64,7 +64,9 @@ N_CYL = 64
 # substrate statistics do not fix a 50-ms signal to 5e-3.  Over 25 ms they do
 # (3.4e-3 on the same held-out protocol).
 N_MEAS, N_T, T_WAVE = 90, 2000, 0.025
-BATCH_MEAS = 64     # measurements per waveform set of one batched instance
+# measurements per waveform set of one batched instance: the generator's substrates
+# are permeable packings, which the batched launch walks with up to 128
+BATCH_MEAS = 128
 
 
 # ---------------------------------------------------------------------------
@@ -572,9 +574,9 @@ def screened_walk_rate(instances, waveforms, n_walkers=1_000_000, n_samples=40, 
     ``n_samples`` samples, device time).  Returns (walker-steps/s, mean n_sub).
 
     The batched walk that produces the training data and the walker-count
-    ladder tests every listed object on every sub-step and walks 90
-    measurements as two instances of 45; this rate is what the screened walk
-    reaches per walker-step on the same substrates."""
+    ladder runs the same screen, but at the ladder's few thousand walkers per
+    substrate; this rate is what the walk reaches per walker-step when one
+    substrate fills the GPU."""
     from .core import PlannedWalk
     ws, ms, n_subs = 0, 0.0, []
     for inst, wf in zip(instances, waveforms):

@@ -303,7 +303,10 @@ typedef struct dmw_batch_tuning {
   int32_t unroll;            /* sub-step loop unrolling: 1 or 2 (0 = 1) */
 } dmw_batch_tuning;
 
-/* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored). */
+/* sets: n_sets waveforms (chi of set 0 applies to all; echo fields ignored), all with
+ * the same n_meas: 1..64, or up to 128 when every instance is a permeable packing
+ * (those walk in a kernel of their own, with the wall screen and the phase parked
+ * in device memory). */
 int dmw_batch_create(dmw_batch** batch, int32_t n_inst, const dmw_instance* inst,
                      int32_t n_sets, const dmw_waveform* sets, const dmw_batch_tuning* tuning,
                      int32_t use_graph);

@@ -56,20 +56,26 @@ def test_batch_matches_oracle_per_instance(gpu, oracle):
         assert err <= 1e-5, f"instance {i} ({s['name']}): max |dS| = {err:.3g}"
 
 
-def test_batch_screened_packings_parked_phase_match_oracle(gpu, oracle):
+@pytest.mark.parametrize("n_dirs", [10, 23])
+def test_batch_screened_packings_parked_phase_match_oracle(gpu, oracle, n_dirs):
     """Permeable packings walk in the screened kernel, and with more than 16
-    measurements its phase is parked in device memory between samples.  Mixed
-    with an instance of the general kernel (both kernels in one graph), every
-    instance matches the oracle, and a graph replay repeats the signals exactly."""
+    measurements its phase is parked in device memory between samples: 40
+    measurements (the 64-wide block, mixed with an instance of the general
+    kernel: both kernels in one graph) and 92 (the 128-wide block, which only
+    the screened kernel takes).  Every instance matches the oracle, and a graph
+    replay repeats the signals exactly."""
     from substrates import packed_cyl, packed_sph
-    wf = small_pgse(n_dirs=10)          # 40 measurements: the 64-wide block
-    assert wf.G.shape[0] == 40
+    wf = small_pgse(n_dirs=n_dirs)
+    n_meas = wf.G.shape[0]
+    assert n_meas == 4 * n_dirs
     specs = [dict(geometry=packed_cyl(kappa=5e-5, rho=1e-5), diffusivity=D, n_walkers=300),
              dict(geometry=packed_sph(kappa=5e-5), diffusivity=D, n_walkers=200),
-             dict(geometry=ds.Cylinder(3e-6, (0, 0, 1), permeability=5e-5), diffusivity=D, n_walkers=130),
              dict(geometry=packed_cyl(kappa=2e-5, seed=7), diffusivity=1.5 * D, n_walkers=257)]
+    if n_meas <= 64:
+        specs.insert(2, dict(geometry=ds.Cylinder(3e-6, (0, 0, 1), permeability=5e-5), diffusivity=D,
+                             n_walkers=130))
     seed = 31
-    print(f"batch, screened kernel: {len(specs)} instances, 40 measurements, seed {seed}", flush=True)
+    print(f"batch, screened kernel: {len(specs)} instances, {n_meas} measurements, seed {seed}", flush=True)
     bw = gpu.BatchedWalk(specs, wf)
     sig, _ = bw.run(seed)
     sig2, _ = bw.run(seed)

@@ -675,3 +675,20 @@ def test_gpu_memory_helpers_without_objects():
     with ds.gpu_session(aggressive=False) as s:
         assert isinstance(s, ds.gpu_session)
     assert isinstance(ds.list_gpu_processes(), list)
+
+
+def test_batch_over_64_measurements_needs_permeable_packings():
+    """More than 64 measurements per waveform: only permeable packings (the
+    screened kernel, phase parked in device memory) take them; any other instance
+    is refused before anything reaches the device, and so is more than 128."""
+    import dmipy_sim_amd as ds
+    from substrates import packed_cyl, small_pgse
+    wf = small_pgse(n_dirs=23)            # 92 measurements
+    with pytest.raises(ValueError, match="permeable packing"):
+        ds.BatchedWalk([dict(geometry=packed_cyl(kappa=5e-5), diffusivity=D, n_walkers=64),
+                        dict(geometry=ds.Sphere(3e-6), diffusivity=D, n_walkers=64)], wf)
+    with pytest.raises(ValueError, match="permeable packing"):
+        ds.BatchedWalk([dict(geometry=packed_cyl(), diffusivity=D, n_walkers=64)], wf)
+    with pytest.raises(ValueError, match="at most 128"):
+        ds.BatchedWalk([dict(geometry=packed_cyl(kappa=5e-5), diffusivity=D, n_walkers=64)],
+                       small_pgse(n_dirs=33))

@@ -171,17 +171,19 @@ def test_generator_matches_the_packed_workload_family():
     assert len(geom._radii_np) == 64
 
 
-@pytest.mark.parametrize("n_meas", [1, 45, 64, 65, 90, 130])
-def test_split_measurements_covers_every_measurement(n_meas):
-    """Batched instances take at most 64 measurements: a waveform is walked as
-    equal-size sets, the last padded with zero-gradient rows, which the
-    concatenation drops again."""
+@pytest.mark.parametrize("n_meas,max_meas", [(1, 64), (45, 64), (64, 64), (65, 64), (90, 64), (130, 64),
+                                             (90, 128), (128, 128), (130, 128), (300, 128)])
+def test_split_measurements_covers_every_measurement(n_meas, max_meas):
+    """Batched instances take at most max_meas measurements (64, or 128 for the
+    permeable packings the generator walks): a waveform is walked as equal-size
+    sets, the last padded with zero-gradient rows, which the concatenation drops
+    again."""
     rng = np.random.default_rng(n_meas)
     wf = sg.random_waveforms(rng, n_meas=n_meas, n_t=40)
-    parts = sg.split_measurements(wf)
+    parts = sg.split_measurements(wf, max_meas)
     k = parts[0].G.shape[0]
-    assert k <= 64 and all(p.G.shape == (k, 40, 3) for p in parts)
-    assert len(parts) == -(-n_meas // 64)
+    assert k <= max_meas and all(p.G.shape == (k, 40, 3) for p in parts)
+    assert len(parts) == -(-n_meas // max_meas)
     cat = np.concatenate([p.G for p in parts])
     np.testing.assert_array_equal(cat[:n_meas], wf.G)
     assert not np.any(cat[n_meas:])
