@@ -534,7 +534,8 @@ int dmw_batch_create(dmw_batch** out, int32_t n_inst, const dmw_instance* inst, 
   if (hipGetDeviceProperties(&prop, dev) == hipSuccess) n_cu = prop.multiProcessorCount;
   const int wps = tn.waves_per_simd > 0 ? tn.waves_per_simd : 4;  // the kernel's occupancy budget
   B->grid = n_gen > 0 ? std::max(1, std::min(n_cu * wps, (n_gen + 3) / 4)) : 0;
-  B->grid_scr = n_scr > 0 ? std::max(1, std::min(n_cu * DMW_SCREENED_WAVES, (n_scr + 3) / 4)) : 0;
+  // (the launch shape's wavefronts per SIMD size both persistent grids)
+  B->grid_scr = n_scr > 0 ? std::max(1, std::min(n_cu * wps, (n_scr + 3) / 4)) : 0;
 
   auto fail_free = [&](hipError_t err, const char* what) {
     delete B;
