@@ -10,7 +10,9 @@
 // ones fill the tail.  The sub-step loop is compiled plain and unrolled by two;
 // both knobs, and the number of resident wavefronts, are chosen by measurement
 // (autotune.py).  Packed instances keep their object tables and periodic neighbour grids
-// in one concatenated device array (read through the L1/L2 caches).  Each unit
+// in one concatenated device array (read through the L1/L2 caches); permeable
+// packings walk the single walk's wall screen in a second persistent kernel with
+// its own queue, their phase parked in device memory between samples.  Each unit
 // writes float64 partial sums; a second kernel reduces every instance's units
 // in a fixed order.  The three launches (queue reset, walk, reduce) are
 // captured once in a hipGraph and replayed.
@@ -115,8 +117,9 @@ __device__ __forceinline__ void walk_unit(const BatchArgs& b, const InstDev& I, 
   Tables tb{nullptr, nullptr, nullptr};
   if (PACKED) tb = Tables{b.objs + I.obj_off, b.starts + I.start_off, b.items + I.items_off};
   Pos p = sample_start<GEOM>(w, a, tb);
-  // the screened kernel's 64-measurement variant parks the compensated phase in
-  // device memory between samples (its sub-step loop then holds no phase registers)
+  // the screened kernel's 64- and 128-measurement variants park the compensated
+  // phase in device memory between samples (the sub-step loop then holds no phase
+  // registers)
   PhaseAcc<MB, false, SCREEN && MB >= 32> ph;
   ph.init();
   float logw = 0.0f;
