@@ -34,6 +34,16 @@ def packed_sph(rho=None, kappa=None, n=6, vf=0.25, seed=4):
     return ds.PackedSpheres(radii, c, L, surface_relaxivity_t2=rho, permeability=kappa)
 
 
+def tight_cell_cyl(kappa=5e-5, rho=1e-5):
+    """Two permeable cylinders (3 and 2 um) in a 9-um periodic cell: the largest radius
+    plus a grid cell's expanded box reaches 0.45 of the cell side, so minimum images
+    take true division and the walk's screen tests the whole cell list every sub-step
+    (no cached objects)."""
+    radii = np.array([3.0e-6, 2.0e-6])
+    centers = np.array([[0.0, 0.0], [4.5e-6, 4.5e-6]])
+    return ds.PackedCylinders(radii, centers, 9.0e-6, surface_relaxivity_t2=rho, permeability=kappa)
+
+
 # (name, geometry factory, T2, T1)
 CASES = [
     ("free", lambda: ds.FreeDiffusion(), None, None),

@@ -64,13 +64,14 @@ def test_batch_screened_packings_parked_phase_match_oracle(gpu, oracle, n_dirs):
     kernel: both kernels in one graph) and 92 (the 128-wide block, which only
     the screened kernel takes).  Every instance matches the oracle, and a graph
     replay repeats the signals exactly."""
-    from substrates import packed_cyl, packed_sph
+    from substrates import packed_cyl, packed_sph, tight_cell_cyl
     wf = small_pgse(n_dirs=n_dirs)
     n_meas = wf.G.shape[0]
     assert n_meas == 4 * n_dirs
     specs = [dict(geometry=packed_cyl(kappa=5e-5, rho=1e-5), diffusivity=D, n_walkers=300),
              dict(geometry=packed_sph(kappa=5e-5), diffusivity=D, n_walkers=200),
-             dict(geometry=packed_cyl(kappa=2e-5, seed=7), diffusivity=1.5 * D, n_walkers=257)]
+             dict(geometry=packed_cyl(kappa=2e-5, seed=7), diffusivity=1.5 * D, n_walkers=257),
+             dict(geometry=tight_cell_cyl(), diffusivity=D, n_walkers=150)]   # whole-list path
     if n_meas <= 64:
         specs.insert(2, dict(geometry=ds.Cylinder(3e-6, (0, 0, 1), permeability=5e-5), diffusivity=D,
                              n_walkers=130))

@@ -55,6 +55,20 @@ def test_walk_bit_exact(gpu, oracle, name, factory, T2, T1, inject):
         assert g["n_cross"].sum() > 0, "permeable case produced no crossings"
 
 
+def test_tight_cell_whole_list_path(gpu, oracle):
+    """A permeable packing whose cell is too small for cached nearest images: the
+    screened walk tests the whole cell list every sub-step, minimum images by true
+    division; bit-exact against the oracle."""
+    from substrates import tight_cell_cyl
+    geom = tight_cell_cyl()
+    g, o, plan = _run_pair(gpu, oracle, geom, None, None, n=320, seed=5, inject=False)
+    assert plan.wall == 2
+    for k in ("final_pos", "comp_origin", "comp_final", "n_cross", "log_w"):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    assert np.max(np.abs(g["signal"] - o["signal"])) <= 1e-5
+    assert g["n_cross"].sum() > 0
+
+
 def test_stored_magnetisation_chi_schedule(gpu, oracle):
     """chi_perp gates wall relaxation and T2 (transverse) against T1 (stored)."""
     wf = small_pgse()
