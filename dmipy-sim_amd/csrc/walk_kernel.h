@@ -950,12 +950,16 @@ struct PackedScreen {
     b[j] = j < nc ? bb : ((j == 0 && uncached(a)) ? -__builtin_inff() : __builtin_inff());
   }
 
-  // The state of every slot at p (same cell, images and radii): after the
-  // reference's test moved, reflected or stopped the walker.
+  // The state of every slot at p (same cell, images and radii) after the
+  // reference's test moved, reflected or stopped the walker.  A slot whose state
+  // the new position still passes keeps it (the position is clear on the same side
+  // of that wall); the others are set anew, skipped by the wavefront when none of
+  // its lanes needs it.
   __device__ __forceinline__ void set_state(const Pos& p, const WalkArgs& a) {
     static_for<SCR_MAXC>([&](auto jc) {
       constexpr int j = decltype(jc)::value;
-      slot_state(j, p, fabsf(rs[j]), a);
+      const bool keep = screen_pass(screen_d2(p.x, p.y, p.z, cx[j], cy[j], cz[j], DIM), rs[j], b[j]);
+      if (__ballot(!keep) != 0ull && !keep) slot_state(j, p, fabsf(rs[j]), a);
     });
   }
 
