@@ -103,10 +103,11 @@ def test_packed_cylinders_full_launch(gpu):
     assert r["n_hits"] > 0
 
 
-def _shrunk_cells(geom, plan):
+def _shrunk_cells(geom, plan, dim=2):
     """Cells of the screened grid whose box the builder shrank (their half-side box
     met more objects than the walker's three-object cache), from the host build of
-    the walk's own grid code (tests/gpu_selftest/screen_check.cpp)."""
+    the walk's own grid code (tests/gpu_selftest/screen_check.cpp), with each
+    cell's level (1: box side halved, 2: no hysteresis)."""
     import ctypes as C
     import os
     lib = C.CDLL(os.path.join(os.path.dirname(os.path.abspath(__file__)), "gpu_selftest",
@@ -121,11 +122,12 @@ def _shrunk_cells(geom, plan):
     cap = 1 << 20
     start, level, items = (np.zeros(cap, np.int32) for _ in range(3))
     n_cells = fn(len(r), c.ctypes.data_as(fp), r.ctypes.data_as(fp), float(np.float32(geom._L_float)),
-                 plan.step_len, plan.geometry_desc().nudge, 2, 0.5, 3, start.ctypes.data_as(ip),
+                 plan.step_len, plan.geometry_desc().nudge, dim, 0.5, 3, start.ctypes.data_as(ip),
                  level.ctypes.data_as(ip), items.ctypes.data_as(ip), cap)
-    n = int(round(np.sqrt(n_cells)))
-    assert n * n == n_cells
-    return n, np.nonzero(level[:n_cells])[0]
+    n = int(round(n_cells ** (1.0 / dim)))
+    assert n ** dim == n_cells
+    cells = np.nonzero(level[:n_cells])[0]
+    return (n, cells, level[cells]) if dim == 3 else (n, cells)
 
 
 def test_packed_cylinders_shrunk_boxes_bit_exact(gpu, oracle):
@@ -153,6 +155,38 @@ def test_packed_cylinders_shrunk_boxes_bit_exact(gpu, oracle):
     g = gpu.walk(geom, Dw, wf40, N, seed=seed, r0=r0, outputs=OUTS, counters=True)
     print("packed cylinders, shrunk boxes: oracle", flush=True)
     o = oracle.run_oracle(geom, Dw, wf40, N, seed=seed, r0=r0)
+    _same_walk(g, o)
+    np.testing.assert_allclose(g["signal"], o["signal"], rtol=0, atol=1e-5)
+    assert g["n_hits"] > 0
+
+
+def test_packed_spheres_shrunk_boxes_bit_exact(gpu, oracle):
+    """The box levels in three dimensions: 24 permeable spheres at VF 0.5 leave 46
+    grid cells whose half-side box meets more than three objects (one of them at
+    level 2).  Walkers started in those cells walk 30 samples x 766 sub-steps
+    bit-exactly against the all-object oracle."""
+    rng = np.random.default_rng(6)
+    radii = rng.uniform(2e-6, 3.5e-6, 24)
+    c, L0, _ = ds.pack_dense(radii, 0.5, seed=6, dim=3)
+    geom = ds.PackedSpheres(radii, c, L0, permeability=5e-5, surface_relaxivity_t2=1e-5)
+    wf = small_pgse(n_dirs=5)
+    plan = ds.make_walk_plan(geom, D, wf.dt)
+    n, cells, levels = _shrunk_cells(geom, plan, dim=3)
+    assert len(cells) >= 20 and levels.max() == 2
+    L = float(np.float32(geom._L_float))
+    h = L / n
+    per = 24
+    r0 = np.zeros((per * len(cells), 3), np.float32)
+    for i, cell in enumerate(cells):
+        ci = (int(cell) % n, (int(cell) // n) % n, int(cell) // (n * n))
+        for a in range(3):
+            r0[i * per:(i + 1) * per, a] = -0.5 * L + (ci[a] + rng.uniform(0.02, 0.98, per)) * h
+    N = len(r0)
+    print(f"packed spheres, shrunk boxes: {len(cells)} cells of {n}^3 (levels {np.bincount(levels).tolist()}), "
+          f"{N} walkers, 30 x {plan.n_sub} sub-steps", flush=True)
+    g = gpu.walk(geom, D, wf, N, seed=5, r0=r0, outputs=OUTS, counters=True)
+    print("packed spheres, shrunk boxes: oracle", flush=True)
+    o = oracle.run_oracle(geom, D, wf, N, seed=5, r0=r0)
     _same_walk(g, o)
     np.testing.assert_allclose(g["signal"], o["signal"], rtol=0, atol=1e-5)
     assert g["n_hits"] > 0
