@@ -71,7 +71,10 @@ def test_batch_screened_packings_parked_phase_match_oracle(gpu, oracle, n_dirs):
     specs = [dict(geometry=packed_cyl(kappa=5e-5, rho=1e-5), diffusivity=D, n_walkers=300),
              dict(geometry=packed_sph(kappa=5e-5), diffusivity=D, n_walkers=200),
              dict(geometry=packed_cyl(kappa=2e-5, seed=7), diffusivity=1.5 * D, n_walkers=257),
-             dict(geometry=tight_cell_cyl(), diffusivity=D, n_walkers=150)]   # whole-list path
+             dict(geometry=tight_cell_cyl(), diffusivity=D, n_walkers=150),   # whole-list path
+             # 24 spheres at VF 0.5: 46 cells with shrunk boxes, one at level 2
+             dict(geometry=packed_sph(kappa=5e-5, rho=1e-5, n=24, vf=0.5, seed=6), diffusivity=D,
+                  n_walkers=400)]
     if n_meas <= 64:
         specs.insert(2, dict(geometry=ds.Cylinder(3e-6, (0, 0, 1), permeability=5e-5), diffusivity=D,
                              n_walkers=130))
