@@ -39,6 +39,8 @@ def _pmyel():
 CASES = [
     ("pcyl_perm", lambda: packed_cyl(kappa=5e-5, rho=1e-5), None, (7, 1, 13)),
     ("psph_perm", lambda: packed_sph(kappa=5e-5), None, (15,)),
+    # 24 spheres at VF 0.5: grid cells with shrunk boxes (levels 1 and 2)
+    ("psph_dense", lambda: packed_sph(kappa=5e-5, rho=1e-5, n=24, vf=0.5, seed=6), None, (9, 4, 17)),
     ("cyl_perm", lambda: ds.Cylinder(3e-6, (0, 0, 1), permeability=5e-5), None, (1, 1, 1, 10)),
     ("sph_rho", lambda: ds.Sphere(3e-6, surface_relaxivity_t2=2e-5), 0.06, (11, 11)),
     ("pmyel", _pmyel, None, (4, 20)),
